@@ -1,0 +1,108 @@
+/* vgm — MI355X (gfx950) geometry hot path behind a C ABI.
+ *
+ * Every entry point returns 0 on success and a negative status otherwise;
+ * vgm_last_error() returns a thread-local description of the last failure.
+ * No C++ exceptions and no library types cross this boundary: plain pointers,
+ * sizes and integer handles only.
+ *
+ * Units are millimetres.  Result conventions (sentinels, frames) are those of
+ * VecGeom: a miss is kInfLength = DBL_MAX, a query from the wrong side returns
+ * a negative value, Inside returns 1 (inside), 2 (surface) or 3 (outside)
+ * (VecGeom/base/Global.h:71-88).
+ */
+#ifndef VGM_VGM_H
+#define VGM_VGM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vgm_ctx vgm_ctx;
+
+/* Query flags. */
+#define VGM_HOST_PTRS 0u   /* inputs/outputs are host memory; call is synchronous */
+#define VGM_DEVICE_PTRS 1u /* inputs/outputs are device memory on the context's device */
+#define VGM_ASYNC 2u       /* with VGM_DEVICE_PTRS: return after enqueueing on the context stream */
+#define VGM_MIXED 4u       /* FP32 bulk arithmetic, FP64 refinement and tolerance decisions */
+
+/* Status codes. */
+#define VGM_OK 0
+#define VGM_EINVAL -1
+#define VGM_EHIP -2
+#define VGM_ESTATE -3
+#define VGM_ENOMEM -4
+
+const char *vgm_last_error(void);
+
+/* ---- context ---------------------------------------------------------------
+ * A context owns one geometry and one HIP stream on one device.  `stream` may
+ * be NULL (the context creates its own) or an existing hipStream_t, in which
+ * case all work is enqueued there.  Replaces the process-wide singletons
+ * GeoManager::Instance() / CudaManager::Instance()
+ * (VecGeom/management/GeoManager.h, VecGeom/management/CudaManager.h). */
+int vgm_ctx_create(int device, void *stream, vgm_ctx **out);
+int vgm_ctx_destroy(vgm_ctx *ctx);
+int vgm_ctx_synchronize(vgm_ctx *ctx);
+/* Returns the hipStream_t the context launches on. */
+void *vgm_ctx_stream(vgm_ctx *ctx);
+
+/* ---- unplaced shapes ------------------------------------------------------
+ * Constructor arguments are those of the reference's Unplaced* classes; the
+ * library derives the cached quantities itself.  On success *shape is the id. */
+
+/* UnplacedBox(dx, dy, dz): half-lengths (VecGeom/volumes/UnplacedBox.h). */
+int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
+
+/* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
+ * VecGeom/volumes/UnplacedVolume.h). */
+int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);
+
+/* ---- volumes --------------------------------------------------------------
+ * LogicalVolume(shape) (VecGeom/volumes/LogicalVolume.h) and placed volumes
+ * carrying a Transformation3D given as 12 doubles: translation[3] followed by
+ * the rotation matrix[9] in the reference's storage order
+ * (VecGeom/base/Transformation3D.h:54-63).  `transform` may be NULL for the
+ * identity.  Adding a daughter corresponds to LogicalVolume::PlaceDaughter. */
+int vgm_lv_create(vgm_ctx *ctx, int shape, int *lv);
+int vgm_pv_create(vgm_ctx *ctx, int lv, const double *transform, int *pv);
+int vgm_lv_add_daughter(vgm_ctx *ctx, int mother_lv, int pv);
+
+/* ---- SOA3D queries on one placed volume -----------------------------------
+ * Each replaces the matching VPlacedVolume SOA3D virtual
+ * (VecGeom/volumes/PlacedVolume.h:273,299,332,372,405,421).  Points and
+ * directions are three separate coordinate arrays of length n (SOA3D<double>,
+ * VecGeom/base/SOA3D.h:36-41); directions must be unit vectors.
+ *
+ * Frames follow the reference: Contains, Inside, DistanceToIn and SafetyToIn
+ * take master-frame points and apply the placement
+ * (VecGeom/volumes/SpecializedPlacedVolImplHelper.h:186-257); DistanceToOut
+ * and SafetyToOut take local-frame points
+ * (VecGeom/volumes/PlacedVolImplHelper.h:76-124); PlacedDistanceToOut takes
+ * master-frame input (SpecializedPlacedVolImplHelper.h:148-164).
+ * `step_max` may be NULL (kInfLength everywhere). */
+int vgm_contains(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n, uint8_t *out,
+                 unsigned flags);
+int vgm_inside(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n, int32_t *out,
+               unsigned flags);
+int vgm_distance_to_in(vgm_ctx *ctx, int pv, const double *px, const double *py, const double *pz,
+                       const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                       double *out, unsigned flags);
+int vgm_distance_to_out(vgm_ctx *ctx, int pv, const double *px, const double *py, const double *pz,
+                        const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                        double *out, unsigned flags);
+int vgm_placed_distance_to_out(vgm_ctx *ctx, int pv, const double *px, const double *py, const double *pz,
+                               const double *dx, const double *dy, const double *dz, const double *step_max,
+                               size_t n, double *out, unsigned flags);
+int vgm_safety_to_in(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
+                     double *out, unsigned flags);
+int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
+                      double *out, unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VGM_VGM_H */

@@ -1,0 +1,117 @@
+"""FP64 CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of oracle/liborc.so, a scalar C++ restatement of the reference
+algorithms (see the file headers under oracle/ for the reference file:line each
+part follows).  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg use it, always as the checker and never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborc.so")
+
+CONTAINS, INSIDE, DISTANCE_TO_IN, DISTANCE_TO_OUT, SAFETY_TO_IN, SAFETY_TO_OUT, PLACED_DISTANCE_TO_OUT = range(7)
+METHODS = {
+    "Contains": CONTAINS,
+    "Inside": INSIDE,
+    "DistanceToIn": DISTANCE_TO_IN,
+    "DistanceToOut": DISTANCE_TO_OUT,
+    "SafetyToIn": SAFETY_TO_IN,
+    "SafetyToOut": SAFETY_TO_OUT,
+    "PlacedDistanceToOut": PLACED_DISTANCE_TO_OUT,
+}
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_geom_create.restype = _P
+        L.orc_geom_create.argtypes = []
+        L.orc_geom_destroy.argtypes = [_P]
+        L.orc_geom_destroy.restype = None
+        sig = {
+            "orc_box_create": [_P, _D, _D, _D],
+            "orc_shape_extent": [_P, _I, _P, _P],
+            "orc_lv_create": [_P, _I],
+            "orc_pv_create": [_P, _I, _P],
+            "orc_lv_add_daughter": [_P, _I, _I],
+            "orc_query": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _I],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _I
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Geometry:
+    """Oracle geometry: shapes, logical volumes, placed volumes."""
+
+    def __init__(self):
+        self._L = lib()
+        self.h = self._L.orc_geom_create()
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._L.orc_geom_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _chk(self, v):
+        if v < 0:
+            raise RuntimeError("oracle rejected the shape parameters")
+        return v
+
+    def box(self, dx, dy, dz):
+        return self._chk(self._L.orc_box_create(self.h, dx, dy, dz))
+
+    def extent(self, shape):
+        mn, mx = np.zeros(3), np.zeros(3)
+        self._L.orc_shape_extent(self.h, shape, mn.ctypes.data, mx.ctypes.data)
+        return mn, mx
+
+    def logical_volume(self, shape):
+        return self._L.orc_lv_create(self.h, shape)
+
+    def place(self, lv, transform=None):
+        tr = None if transform is None else np.ascontiguousarray(transform, dtype=np.float64)
+        return self._L.orc_pv_create(self.h, lv, _p(tr))
+
+    def add_daughter(self, mother_lv, pv):
+        self._L.orc_lv_add_daughter(self.h, mother_lv, pv)
+
+    def query(self, pv, method, pts, dirs=None, step_max=None, nthreads=0):
+        m = METHODS[method] if isinstance(method, str) else method
+        n = len(pts[0])
+        dt = np.uint8 if m == CONTAINS else (np.int32 if m == INSIDE else np.float64)
+        out = np.empty(n, dtype=dt)
+        px, py, pz = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        if dirs is not None:
+            dx, dy, dz = (np.ascontiguousarray(a, dtype=np.float64) for a in dirs)
+        else:
+            dx = dy = dz = None
+        st = None if step_max is None else np.ascontiguousarray(step_max, dtype=np.float64)
+        self._L.orc_query(self.h, pv, m, _p(px), _p(py), _p(pz), _p(dx), _p(dy), _p(dz), _p(st), n,
+                          out.ctypes.data, nthreads)
+        return out

@@ -1,0 +1,106 @@
+// Oracle C API (TEST INFRASTRUCTURE ONLY): geometry store + SOA3D query loops
+// mirroring the placed-volume SOA3D interface
+// (VecGeom/volumes/SpecializedPlacedVolImplHelper.h:184-257,
+//  VecGeom/volumes/PlacedVolImplHelper.h:76-124).
+#include <omp.h>
+#include <memory>
+#include "oracle.h"
+#include "geom.h"
+
+using namespace orc;
+
+extern "C" {
+
+orc_geom *orc_geom_create(void) { return reinterpret_cast<orc_geom *>(new Geometry()); }
+void orc_geom_destroy(orc_geom *g) { delete reinterpret_cast<Geometry *>(g); }
+
+static int add_shape(orc_geom *g, Shape *s)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  if (!s) return -1;
+  G->shapes.emplace_back(s);
+  return int(G->shapes.size()) - 1;
+}
+
+int orc_box_create(orc_geom *g, double dx, double dy, double dz) { return add_shape(g, make_box(dx, dy, dz)); }
+
+int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  Vec3 a, b;
+  G->shapes[shape]->Extent(a, b);
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = a[k];
+    mx[k] = b[k];
+  }
+  return 0;
+}
+
+int orc_lv_create(orc_geom *g, int shape)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  G->lvs.push_back(LogicalVolume{shape, {}});
+  return int(G->lvs.size()) - 1;
+}
+
+int orc_pv_create(orc_geom *g, int lv, const double *tr)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  PlacedVolume p;
+  p.lv = lv;
+  if (tr) {
+    for (int k = 0; k < 3; ++k) p.tr.t[k] = tr[k];
+    for (int k = 0; k < 9; ++k) p.tr.r[k] = tr[3 + k];
+  }
+  G->pvs.push_back(p);
+  return int(G->pvs.size()) - 1;
+}
+
+int orc_lv_add_daughter(orc_geom *g, int mother, int pv)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  G->lvs[mother].daughters.push_back(pv);
+  return 0;
+}
+
+int orc_query(orc_geom *g, int pv, int method, const double *px, const double *py, const double *pz,
+              const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
+              int nthreads)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  const PlacedVolume &P = G->pvs[pv];
+  const Shape &S = *G->shapes[G->lvs[P.lv].shape];
+  const Transform &T = P.tr;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long long i = 0; i < (long long)n; ++i) {
+    const Vec3 m(px[i], py[i], pz[i]);
+    switch (method) {
+    case ORC_CONTAINS: static_cast<uint8_t *>(out)[i] = S.Contains(T.Transform_(m)) ? 1 : 0; break;
+    case ORC_INSIDE: static_cast<int32_t *>(out)[i] = S.Inside(T.Transform_(m)); break;
+    case ORC_SAFETY_TO_IN: static_cast<double *>(out)[i] = S.SafetyToIn(T.Transform_(m)); break;
+    case ORC_SAFETY_TO_OUT: static_cast<double *>(out)[i] = S.SafetyToOut(m); break;
+    case ORC_DISTANCE_TO_IN: {
+      const Vec3 d(dx[i], dy[i], dz[i]);
+      static_cast<double *>(out)[i] =
+          S.DistanceToIn(T.Transform_(m), T.TransformDirection(d), step ? step[i] : kInfLength);
+      break;
+    }
+    case ORC_DISTANCE_TO_OUT: {
+      const Vec3 d(dx[i], dy[i], dz[i]);
+      static_cast<double *>(out)[i] = S.DistanceToOut(m, d, step ? step[i] : kInfLength);
+      break;
+    }
+    case ORC_PLACED_DISTANCE_TO_OUT: {
+      const Vec3 d(dx[i], dy[i], dz[i]);
+      static_cast<double *>(out)[i] =
+          S.DistanceToOut(T.Transform_(m), T.TransformDirection(d), step ? step[i] : kInfLength);
+      break;
+    }
+    default: break;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// FP64 CPU oracle — TEST INFRASTRUCTURE ONLY.
+//
+// A scalar C++ restatement of VecGeom's shape kernels and navigation, used by
+// tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg to check and
+// time the HIP library.  Nothing in the product (vecgeom_amd/) includes, links
+// or calls this code.  Built with -ffp-contract=off so that every expression
+// rounds exactly as written.
+//
+// Conventions followed (reference files, relative to the VecGeom source tree):
+//   tolerances        VecGeom/base/Math.h:27-53
+//   NonZero, InfinityLength  VecGeom/base/Global.h:157-177
+//   Inside_t values   VecGeom/base/Global.h:71-88
+//   Transformation3D  VecGeom/base/Transformation3D.h:660-673, 746-771, 808-838
+//   scalar Min/Max/Sign/Abs  VecCore scalar backend (not in the tree):
+//     Max(a,b) = a > b ? a : b, Min(a,b) = a < b ? a : b,
+//     Sign(x) = copysign(1, x), Abs = fabs.
+//     Parity for +-0 and NaN operands is unpinned; seeded inputs never
+//     produce them.
+#pragma once
+#include <cfloat>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace orc {
+
+constexpr double kTolerance = 1e-9;
+constexpr double kHalfTolerance = 0.5 * kTolerance;
+constexpr double kConeTolerance = 1e-7;
+constexpr double kAngTolerance = 1e-9;
+constexpr double kPushTolerance = 1e-6;
+constexpr double kInfLength = DBL_MAX;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2. * kPi;
+constexpr double kHalfPi = 0.5 * kPi;
+
+enum Inside { kInside = 1, kSurface = 2, kOutside = 3 };
+
+inline double Max(double a, double b) { return a > b ? a : b; }
+inline double Min(double a, double b) { return a < b ? a : b; }
+inline double Abs(double a) { return std::fabs(a); }
+inline double Sqrt(double a) { return std::sqrt(a); }
+inline double Sign(double a) { return std::copysign(1.0, a); }
+inline double NonZero(double x) { return x + std::copysign(1.0e-30, x); }
+inline double NonZeroAbs(double x) { return std::fabs(x) + 1.0e-30; }
+
+struct Vec3 {
+  double x = 0, y = 0, z = 0;
+  Vec3() = default;
+  Vec3(double a, double b, double c) : x(a), y(b), z(c) {}
+  double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  double &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+  Vec3 operator+(const Vec3 &o) const { return Vec3(x + o.x, y + o.y, z + o.z); }
+  Vec3 operator-(const Vec3 &o) const { return Vec3(x - o.x, y - o.y, z - o.z); }
+  Vec3 operator*(double s) const { return Vec3(x * s, y * s, z * s); }
+  Vec3 operator-() const { return Vec3(-x, -y, -z); }
+  Vec3 Abs() const { return Vec3(std::fabs(x), std::fabs(y), std::fabs(z)); }
+  // Vector3D::Max/Min = vecCore::math::Max(v0, v1, v2) folded left
+  double MaxC() const { return Max(Max(x, y), z); }
+  double MinC() const { return Min(Min(x, y), z); }
+  double Mag2() const { return x * x + y * y + z * z; }
+  double Mag() const { return std::sqrt(Mag2()); }
+  double Perp2() const { return x * x + y * y; }
+  double Perp() const { return std::sqrt(Perp2()); }
+};
+
+// Vector3D dot: x*x' + y*y' + z*z' in that order.
+inline double Dot(const Vec3 &a, const Vec3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 Cross(const Vec3 &a, const Vec3 &b)
+{
+  return Vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Transformation3D with the generic (unspecialised) code path.
+struct Transform {
+  double t[3] = {0, 0, 0};
+  double r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+  Vec3 Rotate(const Vec3 &m) const
+  {
+    double l0 = m.x * r[0];
+    double l1 = m.x * r[1];
+    double l2 = m.x * r[2];
+    l0 += m.y * r[3];
+    l1 += m.y * r[4];
+    l2 += m.y * r[5];
+    l0 += m.z * r[6];
+    l1 += m.z * r[7];
+    l2 += m.z * r[8];
+    return Vec3(l0, l1, l2);
+  }
+  Vec3 Transform_(const Vec3 &m) const { return Rotate(Vec3(m.x - t[0], m.y - t[1], m.z - t[2])); }
+  Vec3 TransformDirection(const Vec3 &m) const { return Rotate(m); }
+  Vec3 InverseTransform(const Vec3 &l) const
+  {
+    Vec3 m;
+    m.x = t[0];
+    m.x += l.x * r[0];
+    m.x += l.y * r[1];
+    m.x += l.z * r[2];
+    m.y = t[1];
+    m.y += l.x * r[3];
+    m.y += l.y * r[4];
+    m.y += l.z * r[5];
+    m.z = t[2];
+    m.z += l.x * r[6];
+    m.z += l.y * r[7];
+    m.z += l.z * r[8];
+    return m;
+  }
+  Vec3 InverseTransformDirection(const Vec3 &l) const
+  {
+    Vec3 m;
+    m.x = l.x * r[0];
+    m.x += l.y * r[1];
+    m.x += l.z * r[2];
+    m.y = l.x * r[3];
+    m.y += l.y * r[4];
+    m.y += l.z * r[5];
+    m.z = l.x * r[6];
+    m.z += l.y * r[7];
+    m.z += l.z * r[8];
+    return m;
+  }
+};
+
+// Abstract unplaced shape with the six scalar queries of VUnplacedVolume
+// (VecGeom/volumes/UnplacedVolume.h:41-414).
+struct Shape {
+  virtual ~Shape() = default;
+  virtual bool Contains(const Vec3 &p) const = 0;
+  virtual int Inside(const Vec3 &p) const = 0;
+  virtual double DistanceToIn(const Vec3 &p, const Vec3 &d, double step_max) const = 0;
+  virtual double DistanceToOut(const Vec3 &p, const Vec3 &d, double step_max) const = 0;
+  virtual double SafetyToIn(const Vec3 &p) const = 0;
+  virtual double SafetyToOut(const Vec3 &p) const = 0;
+  virtual void Extent(Vec3 &mn, Vec3 &mx) const = 0;
+};
+
+Shape *make_box(double dx, double dy, double dz);
+
+}  // namespace orc

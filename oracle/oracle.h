@@ -1,0 +1,41 @@
+/* FP64 CPU oracle C API — TEST INFRASTRUCTURE ONLY (tests/, smoke, bench CPU
+ * baseline).  Mirrors the shape constructors and SOA3D queries of the
+ * reference so that the HIP library can be checked against it. */
+#ifndef ORC_ORACLE_H
+#define ORC_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_geom orc_geom;
+
+enum {
+  ORC_CONTAINS = 0,
+  ORC_INSIDE = 1,
+  ORC_DISTANCE_TO_IN = 2,
+  ORC_DISTANCE_TO_OUT = 3,
+  ORC_SAFETY_TO_IN = 4,
+  ORC_SAFETY_TO_OUT = 5,
+  ORC_PLACED_DISTANCE_TO_OUT = 6
+};
+
+orc_geom *orc_geom_create(void);
+void orc_geom_destroy(orc_geom *g);
+
+int orc_box_create(orc_geom *g, double dx, double dy, double dz);
+int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx);
+int orc_lv_create(orc_geom *g, int shape);
+int orc_pv_create(orc_geom *g, int lv, const double *tr);
+int orc_lv_add_daughter(orc_geom *g, int mother, int pv);
+
+int orc_query(orc_geom *g, int pv, int method, const double *px, const double *py, const double *pz,
+              const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
+              int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

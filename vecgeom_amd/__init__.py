@@ -1,0 +1,231 @@
+"""vecgeom_amd — Python host binding of the gfx950 geometry library (libvgm.so).
+
+The library itself is C++/HIP behind the C ABI declared in include/vgm/vgm.h;
+this module is a thin ctypes layer over it for tests and benchmarks.  There is
+no CPU fallback: if libvgm.so has not been built, importing `lib()` raises.
+
+Inputs are SOA3D batches (three coordinate arrays).  Host numpy arrays use the
+synchronous host-pointer entry points; torch tensors on the GPU are passed as
+device pointers.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvgm.so")
+
+HOST_PTRS = 0
+DEVICE_PTRS = 1
+ASYNC = 2
+MIXED = 4
+
+# Inside_t (VecGeom/base/Global.h:71-88)
+kInside, kSurface, kOutside = 1, 2, 3
+kInfLength = np.finfo(np.float64).max
+
+_lib = None
+
+
+class VgmError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvgm.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VgmError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+_U = ctypes.c_uint
+
+
+def _declare(L):
+    L.vgm_last_error.restype = ctypes.c_char_p
+    L.vgm_last_error.argtypes = []
+    sig = {
+        "vgm_ctx_create": [_I, _P, ctypes.POINTER(_P)],
+        "vgm_ctx_destroy": [_P],
+        "vgm_ctx_synchronize": [_P],
+        "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_shape_extent": [_P, _I, _P, _P],
+        "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
+        "vgm_pv_create": [_P, _I, _P, ctypes.POINTER(_I)],
+        "vgm_lv_add_daughter": [_P, _I, _I],
+        "vgm_contains": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_inside": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_safety_to_in": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_safety_to_out": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_distance_to_in": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
+        "vgm_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
+        "vgm_placed_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
+    L.vgm_ctx_stream.argtypes = [_P]
+    L.vgm_ctx_stream.restype = _P
+
+
+def check(rc: int):
+    if rc != 0:
+        raise VgmError(f"vgm error {rc}: {lib().vgm_last_error().decode()}")
+
+
+def _ptr(a):
+    """Raw address of a numpy array or a torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _is_device(a) -> bool:
+    return not isinstance(a, np.ndarray) and getattr(a, "is_cuda", False)
+
+
+class SOA3D:
+    """Three coordinate arrays of equal length (SOA3D<double>)."""
+
+    def __init__(self, x, y, z):
+        self.x, self.y, self.z = x, y, z
+        self.n = len(x)
+        if not (len(y) == self.n and len(z) == self.n):
+            raise ValueError("SOA3D components differ in length")
+
+    @classmethod
+    def from_array(cls, a):
+        """From an (n,3) or (3,n) float64 numpy array."""
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim != 2:
+            raise ValueError("expected a 2-D array")
+        if a.shape[0] == 3 and a.shape[1] != 3:
+            return cls(*(np.ascontiguousarray(a[k]) for k in range(3)))
+        return cls(*(np.ascontiguousarray(a[:, k]) for k in range(3)))
+
+
+class Context:
+    """One device, one stream, one geometry (vgm_ctx)."""
+
+    def __init__(self, device: int = 0, stream: Optional[int] = None):
+        self._L = lib()
+        h = _P()
+        check(self._L.vgm_ctx_create(device, stream, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._L.vgm_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(self._L.vgm_ctx_synchronize(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return self._L.vgm_ctx_stream(self.handle)
+
+    # ---- shapes -------------------------------------------------------------
+    def _new(self, fn, *args) -> int:
+        out = _I()
+        check(fn(self.handle, *args, ctypes.byref(out)))
+        return out.value
+
+    def box(self, dx, dy, dz) -> int:
+        return self._new(self._L.vgm_box_create, dx, dy, dz)
+
+    def extent(self, shape: int):
+        mn = np.zeros(3)
+        mx = np.zeros(3)
+        check(self._L.vgm_shape_extent(self.handle, shape, mn.ctypes.data, mx.ctypes.data))
+        return mn, mx
+
+    # ---- volumes ------------------------------------------------------------
+    def logical_volume(self, shape: int) -> int:
+        return self._new(self._L.vgm_lv_create, shape)
+
+    def place(self, lv: int, transform: Optional[Sequence[float]] = None) -> int:
+        tr = None if transform is None else np.ascontiguousarray(transform, dtype=np.float64)
+        if tr is not None and tr.size != 12:
+            raise ValueError("transform is translation[3] + rotation[9]")
+        out = _I()
+        check(self._L.vgm_pv_create(self.handle, lv, _ptr(tr), ctypes.byref(out)))
+        return out.value
+
+    def add_daughter(self, mother_lv: int, pv: int):
+        check(self._L.vgm_lv_add_daughter(self.handle, mother_lv, pv))
+
+    # ---- SOA3D queries ------------------------------------------------------
+    def _flags(self, arr, mixed: bool, asynchronous: bool = False) -> int:
+        f = DEVICE_PTRS if _is_device(arr) else HOST_PTRS
+        if f == DEVICE_PTRS and asynchronous:
+            f |= ASYNC
+        if mixed:
+            f |= MIXED
+        return f
+
+    def _alloc(self, like, n, dtype):
+        if _is_device(like):
+            import torch
+            tdt = {np.uint8: torch.uint8, np.int32: torch.int32, np.float64: torch.float64}[dtype]
+            return torch.empty(n, dtype=tdt, device=like.device)
+        return np.empty(n, dtype=dtype)
+
+    def _q1(self, fn, pv, pts: SOA3D, dtype, out, mixed, asynchronous=False):
+        if out is None:
+            out = self._alloc(pts.x, pts.n, dtype)
+        check(fn(self.handle, pv, _ptr(pts.x), _ptr(pts.y), _ptr(pts.z), pts.n, _ptr(out),
+                 self._flags(pts.x, mixed, asynchronous)))
+        return out
+
+    def _q2(self, fn, pv, pts: SOA3D, dirs: SOA3D, step_max, out, mixed, asynchronous=False):
+        if dirs.n != pts.n:
+            raise ValueError("points and directions differ in length")
+        if out is None:
+            out = self._alloc(pts.x, pts.n, np.float64)
+        check(fn(self.handle, pv, _ptr(pts.x), _ptr(pts.y), _ptr(pts.z), _ptr(dirs.x), _ptr(dirs.y),
+                 _ptr(dirs.z), _ptr(step_max), pts.n, _ptr(out), self._flags(pts.x, mixed, asynchronous)))
+        return out
+
+    def contains(self, pv, pts, out=None, mixed=False, asynchronous=False):
+        return self._q1(self._L.vgm_contains, pv, pts, np.uint8, out, mixed, asynchronous)
+
+    def inside(self, pv, pts, out=None, mixed=False, asynchronous=False):
+        return self._q1(self._L.vgm_inside, pv, pts, np.int32, out, mixed, asynchronous)
+
+    def safety_to_in(self, pv, pts, out=None, mixed=False, asynchronous=False):
+        return self._q1(self._L.vgm_safety_to_in, pv, pts, np.float64, out, mixed, asynchronous)
+
+    def safety_to_out(self, pv, pts, out=None, mixed=False, asynchronous=False):
+        return self._q1(self._L.vgm_safety_to_out, pv, pts, np.float64, out, mixed, asynchronous)
+
+    def distance_to_in(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
+        return self._q2(self._L.vgm_distance_to_in, pv, pts, dirs, step_max, out, mixed, asynchronous)
+
+    def distance_to_out(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
+        return self._q2(self._L.vgm_distance_to_out, pv, pts, dirs, step_max, out, mixed, asynchronous)
+
+    def placed_distance_to_out(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
+        return self._q2(self._L.vgm_placed_distance_to_out, pv, pts, dirs, step_max, out, mixed, asynchronous)

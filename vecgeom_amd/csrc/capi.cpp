@@ -1,0 +1,357 @@
+// C-ABI layer: contexts, geometry registration/upload and SOA query dispatch.
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "../../include/vgm/vgm.h"
+#include "geometry.h"
+#include "vgm_internal.h"
+
+namespace vgm {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+// Device mirror of HostGeometry: one allocation, 256-byte aligned sections.
+struct DeviceGeometry {
+  uint64_t version = 0;
+  char *buf = nullptr;
+  size_t cap = 0;
+  ShapeRec *shapes = nullptr;
+  double *pool = nullptr;
+  LogicalRec *lvs = nullptr;
+  int32_t *daughters = nullptr;
+  PlacedRec *pvs = nullptr;
+};
+
+// Growable device scratch for host-pointer queries.
+struct Scratch {
+  char *buf = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace vgm
+
+using namespace vgm;
+
+struct vgm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  HostGeometry host;
+  DeviceGeometry dev;
+  Scratch scratch;
+};
+
+namespace {
+
+int fail(int code, const std::string &msg)
+{
+  set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+  return fail(VGM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define VGM_HIP(call)                                 \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+int ensure_scratch(vgm_ctx *c, size_t bytes)
+{
+  if (c->scratch.cap >= bytes) return VGM_OK;
+  if (c->scratch.buf) {
+    VGM_HIP(hipStreamSynchronize(c->stream));
+    VGM_HIP(hipFree(c->scratch.buf));
+    c->scratch.buf = nullptr;
+    c->scratch.cap = 0;
+  }
+  size_t cap = align256(bytes + bytes / 4);
+  VGM_HIP(hipMalloc(&c->scratch.buf, cap));
+  c->scratch.cap = cap;
+  return VGM_OK;
+}
+
+// Pack the host tables into one staging buffer and upload when stale.
+int sync_geometry(vgm_ctx *c)
+{
+  HostGeometry &h = c->host;
+  DeviceGeometry &d = c->dev;
+  if (d.version == h.version && d.buf) return VGM_OK;
+  std::vector<int32_t> dlist;
+  std::vector<LogicalRec> lvs = h.lvs;
+  for (size_t i = 0; i < lvs.size(); ++i) {
+    lvs[i].dbegin = int32_t(dlist.size());
+    lvs[i].dcount = int32_t(h.lv_daughters[i].size());
+    dlist.insert(dlist.end(), h.lv_daughters[i].begin(), h.lv_daughters[i].end());
+  }
+  const size_t s_shapes = align256(h.shapes.size() * sizeof(ShapeRec) + 1);
+  const size_t s_pool = align256(h.pool.size() * sizeof(double) + 1);
+  const size_t s_lvs = align256(lvs.size() * sizeof(LogicalRec) + 1);
+  const size_t s_dl = align256(dlist.size() * sizeof(int32_t) + 1);
+  const size_t s_pvs = align256(h.pvs.size() * sizeof(PlacedRec) + 1);
+  const size_t total = s_shapes + s_pool + s_lvs + s_dl + s_pvs;
+  std::vector<char> stage(total, 0);
+  size_t o = 0;
+  auto put = [&](const void *src, size_t n, size_t span) {
+    if (n) std::memcpy(stage.data() + o, src, n);
+    size_t at = o;
+    o += span;
+    return at;
+  };
+  const size_t o_shapes = put(h.shapes.data(), h.shapes.size() * sizeof(ShapeRec), s_shapes);
+  const size_t o_pool = put(h.pool.data(), h.pool.size() * sizeof(double), s_pool);
+  const size_t o_lvs = put(lvs.data(), lvs.size() * sizeof(LogicalRec), s_lvs);
+  const size_t o_dl = put(dlist.data(), dlist.size() * sizeof(int32_t), s_dl);
+  const size_t o_pvs = put(h.pvs.data(), h.pvs.size() * sizeof(PlacedRec), s_pvs);
+  if (d.cap < total) {
+    if (d.buf) {
+      VGM_HIP(hipStreamSynchronize(c->stream));
+      VGM_HIP(hipFree(d.buf));
+      d.buf = nullptr;
+      d.cap = 0;
+    }
+    VGM_HIP(hipMalloc(&d.buf, total));
+    d.cap = total;
+  }
+  VGM_HIP(hipMemcpyAsync(d.buf, stage.data(), total, hipMemcpyHostToDevice, c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  d.shapes = reinterpret_cast<ShapeRec *>(d.buf + o_shapes);
+  d.pool = reinterpret_cast<double *>(d.buf + o_pool);
+  d.lvs = reinterpret_cast<LogicalRec *>(d.buf + o_lvs);
+  d.daughters = reinterpret_cast<int32_t *>(d.buf + o_dl);
+  d.pvs = reinterpret_cast<PlacedRec *>(d.buf + o_pvs);
+  d.version = h.version;
+  return VGM_OK;
+}
+
+int check_pv(vgm_ctx *c, int pv)
+{
+  if (!c) return fail(VGM_EINVAL, "null context");
+  if (pv < 0 || size_t(pv) >= c->host.pvs.size()) return fail(VGM_EINVAL, "invalid placed-volume id");
+  return VGM_OK;
+}
+
+// Common driver for the seven SOA queries.
+int run_soa(vgm_ctx *c, int pv, int method, bool use_tr, const double *px, const double *py, const double *pz,
+            const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
+            size_t out_elem, unsigned flags)
+{
+  int rc = check_pv(c, pv);
+  if (rc) return rc;
+  const bool with_dir = (method == kDistanceToIn || method == kDistanceToOut);
+  if (n && (!px || !py || !pz || !out || (with_dir && (!dx || !dy || !dz))))
+    return fail(VGM_EINVAL, "null input or output pointer");
+  VGM_HIP(hipSetDevice(c->device));
+  rc = sync_geometry(c);
+  if (rc) return rc;
+  const PlacedRec &P = c->host.pvs[pv];
+  const ShapeRec &S = c->host.shapes[c->host.lvs[P.lv].shape];
+  const void *params = c->dev.pool + S.off;
+  const int precision = (flags & VGM_MIXED) ? kMixed : kFP64;
+  SoaArgs a{};
+  a.n = n;
+  if (n == 0) return VGM_OK;
+  if (flags & VGM_DEVICE_PTRS) {
+    a.px = px, a.py = py, a.pz = pz;
+    a.dx = dx, a.dy = dy, a.dz = dz;
+    a.step = step;
+    a.out = out;
+    VGM_HIP(launch_soa(c->stream, method, precision, S, params, P.tr, use_tr, a));
+    if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
+    return VGM_OK;
+  }
+  const size_t nin = with_dir ? (step ? 7 : 6) : 3;
+  const size_t bytes_in = align256(n * sizeof(double));
+  const size_t bytes_out = align256(n * out_elem);
+  rc = ensure_scratch(c, nin * bytes_in + bytes_out);
+  if (rc) return rc;
+  char *b = c->scratch.buf;
+  const double *src[7] = {px, py, pz, dx, dy, dz, step};
+  double *dst[7] = {nullptr};
+  for (size_t k = 0; k < nin; ++k) {
+    dst[k] = reinterpret_cast<double *>(b + k * bytes_in);
+    VGM_HIP(hipMemcpyAsync(dst[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  void *dout = b + nin * bytes_in;
+  a.px = dst[0], a.py = dst[1], a.pz = dst[2];
+  if (with_dir) {
+    a.dx = dst[3], a.dy = dst[4], a.dz = dst[5];
+    a.step = step ? dst[6] : nullptr;
+  }
+  a.out = dout;
+  VGM_HIP(launch_soa(c->stream, method, precision, S, params, P.tr, use_tr, a));
+  VGM_HIP(hipMemcpyAsync(out, dout, n * out_elem, hipMemcpyDeviceToHost, c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  return VGM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vgm_last_error(void) { return g_last_error.c_str(); }
+
+int vgm_ctx_create(int device, void *stream, vgm_ctx **out)
+{
+  if (!out) return fail(VGM_EINVAL, "null output pointer");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(VGM_EHIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(VGM_EINVAL, "device index out of range");
+  VGM_HIP(hipSetDevice(device));
+  vgm_ctx *c = new vgm_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return hip_fail(e, "hipStreamCreate");
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return VGM_OK;
+}
+
+int vgm_ctx_destroy(vgm_ctx *c)
+{
+  if (!c) return VGM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->dev.buf) (void)hipFree(c->dev.buf);
+  if (c->scratch.buf) (void)hipFree(c->scratch.buf);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return VGM_OK;
+}
+
+int vgm_ctx_synchronize(vgm_ctx *c)
+{
+  if (!c) return fail(VGM_EINVAL, "null context");
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  return VGM_OK;
+}
+
+void *vgm_ctx_stream(vgm_ctx *c) { return c ? static_cast<void *>(c->stream) : nullptr; }
+
+int vgm_box_create(vgm_ctx *c, double dx, double dy, double dz, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_box(c->host, dx, dy, dz, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
+int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
+{
+  if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");
+  if (shape < 0 || size_t(shape) >= c->host.shapes.size()) return fail(VGM_EINVAL, "invalid shape id");
+  const ShapeRec &r = c->host.shapes[shape];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = r.bmin[k];
+    mx[k] = r.bmax[k];
+  }
+  return VGM_OK;
+}
+
+int vgm_lv_create(vgm_ctx *c, int shape, int *lv)
+{
+  if (!c || !lv) return fail(VGM_EINVAL, "null argument");
+  if (shape < 0 || size_t(shape) >= c->host.shapes.size()) return fail(VGM_EINVAL, "invalid shape id");
+  LogicalRec r{shape, 0, 0, -1};
+  c->host.lvs.push_back(r);
+  c->host.lv_daughters.emplace_back();
+  ++c->host.version;
+  *lv = int(c->host.lvs.size()) - 1;
+  return VGM_OK;
+}
+
+int vgm_pv_create(vgm_ctx *c, int lv, const double *tr, int *pv)
+{
+  if (!c || !pv) return fail(VGM_EINVAL, "null argument");
+  if (lv < 0 || size_t(lv) >= c->host.lvs.size()) return fail(VGM_EINVAL, "invalid logical-volume id");
+  PlacedRec p{};
+  p.lv = lv;
+  p.id = int32_t(c->host.pvs.size());
+  if (tr) {
+    for (int k = 0; k < 3; ++k) p.tr.t[k] = tr[k];
+    for (int k = 0; k < 9; ++k) p.tr.r[k] = tr[3 + k];
+  } else {
+    for (int k = 0; k < 3; ++k) p.tr.t[k] = 0;
+    for (int k = 0; k < 9; ++k) p.tr.r[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  }
+  c->host.pvs.push_back(p);
+  ++c->host.version;
+  *pv = p.id;
+  return VGM_OK;
+}
+
+int vgm_lv_add_daughter(vgm_ctx *c, int mother, int pv)
+{
+  if (!c) return fail(VGM_EINVAL, "null context");
+  if (mother < 0 || size_t(mother) >= c->host.lvs.size()) return fail(VGM_EINVAL, "invalid logical-volume id");
+  if (pv < 0 || size_t(pv) >= c->host.pvs.size()) return fail(VGM_EINVAL, "invalid placed-volume id");
+  c->host.lv_daughters[mother].push_back(pv);
+  ++c->host.version;
+  return VGM_OK;
+}
+
+int vgm_contains(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, uint8_t *out,
+                 unsigned flags)
+{
+  return run_soa(c, pv, kContains, true, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 1, flags);
+}
+
+int vgm_inside(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, int32_t *out,
+               unsigned flags)
+{
+  return run_soa(c, pv, kInside, true, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 4, flags);
+}
+
+int vgm_distance_to_in(vgm_ctx *c, int pv, const double *px, const double *py, const double *pz, const double *dx,
+                       const double *dy, const double *dz, const double *step, size_t n, double *out,
+                       unsigned flags)
+{
+  return run_soa(c, pv, kDistanceToIn, true, px, py, pz, dx, dy, dz, step, n, out, 8, flags);
+}
+
+int vgm_distance_to_out(vgm_ctx *c, int pv, const double *px, const double *py, const double *pz, const double *dx,
+                        const double *dy, const double *dz, const double *step, size_t n, double *out,
+                        unsigned flags)
+{
+  return run_soa(c, pv, kDistanceToOut, false, px, py, pz, dx, dy, dz, step, n, out, 8, flags);
+}
+
+int vgm_placed_distance_to_out(vgm_ctx *c, int pv, const double *px, const double *py, const double *pz,
+                               const double *dx, const double *dy, const double *dz, const double *step, size_t n,
+                               double *out, unsigned flags)
+{
+  return run_soa(c, pv, kDistanceToOut, true, px, py, pz, dx, dy, dz, step, n, out, 8, flags);
+}
+
+int vgm_safety_to_in(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, double *out,
+                     unsigned flags)
+{
+  return run_soa(c, pv, kSafetyToIn, true, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 8, flags);
+}
+
+int vgm_safety_to_out(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, double *out,
+                      unsigned flags)
+{
+  return run_soa(c, pv, kSafetyToOut, false, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 8, flags);
+}
+
+}  // extern "C"

@@ -1,0 +1,29 @@
+// Host-side geometry model: the flat tables the device consumes, plus the
+// shape builders that derive each family's cached parameters from its
+// constructor arguments.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "vgm_types.h"
+
+namespace vgm {
+
+struct HostGeometry {
+  std::vector<ShapeRec> shapes;
+  std::vector<double> pool;  // parameter blocks, 8-byte granular
+  std::vector<LogicalRec> lvs;
+  std::vector<std::vector<int32_t>> lv_daughters;  // placed-volume ids per logical volume
+  std::vector<PlacedRec> pvs;
+  uint64_t version = 1;  // bumped on every change; the device copy follows it
+
+  // Appends a parameter block of `ndoubles` and registers a shape; returns its id.
+  int add_shape(int type, const void *block, size_t ndoubles, int n0, int n1, const double bmin[3],
+                const double bmax[3]);
+};
+
+// Shape builders.  Return the new shape id, or -1 with `err` set.
+int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err);
+
+}  // namespace vgm

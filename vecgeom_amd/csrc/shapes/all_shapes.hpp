@@ -1,0 +1,22 @@
+// Every shape family known to the kernels, and the type switch that turns a
+// runtime ShapeRec into a call of the matching templated shape.
+#pragma once
+#include "box.hpp"
+
+namespace vgm {
+
+// Call f.template operator()<Shape>() for the shape family of `type`.
+// Returns false for an unknown type.
+template <class F>
+__host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
+{
+  switch (type) {
+  case kBox:
+    f.template operator()<BoxShape>();
+    return true;
+  default:
+    return false;
+  }
+}
+
+}  // namespace vgm

@@ -1,0 +1,77 @@
+// Box kernels: VecGeom/volumes/kernel/BoxImplementation.h:76-167.
+// Parameter block: half-lengths (BoxStruct::fDimensions).
+#pragma once
+#include "../vgm_math.hpp"
+#include "box_data.h"
+
+namespace vgm {
+
+struct BoxShape {
+  using Data = BoxData;
+
+  // max(|p| - h) (BoxImplementation.h:76-90)
+  template <typename R>
+  VGM_DEV static R excess(const Data &b, V3<R> p)
+  {
+    return vmax3(vabs(p.x) - R(b.dim[0]), vabs(p.y) - R(b.dim[1]), vabs(p.z) - R(b.dim[2]));
+  }
+
+  template <typename R>
+  VGM_DEV static bool contains(const Data &b, V3<R> p)
+  {
+    return excess(b, p) < R(0);
+  }
+
+  template <typename R>
+  VGM_DEV static int inside(const Data &b, V3<R> p)
+  {
+    R dist = excess(b, p);
+    int in = dist < R(0) ? kInsideV : kOutsideV;
+    if (vabs(dist) < R(kHalfTolerance)) in = kSurfaceV;
+    return in;
+  }
+
+  // Slab method (BoxImplementation.h:109-131).
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &b, V3<R> p, V3<R> d, R /*step_max*/)
+  {
+    const R ix = R(1) / nonzero(d.x), iy = R(1) / nonzero(d.y), iz = R(1) / nonzero(d.z);
+    const R sx = vsign(d.x), sy = vsign(d.y), sz = vsign(d.z);
+    const R hx = R(b.dim[0]), hy = R(b.dim[1]), hz = R(b.dim[2]);
+    const R inx = -sx * hx - p.x, iny = -sy * hy - p.y, inz = -sz * hz - p.z;
+    const R outx = sx * hx - p.x, outy = sy * hy - p.y, outz = sz * hz - p.z;
+    const R absOrthogOut = vabs(vmin3(sx * outx, sy * outy, sz * outz));
+    const R distOut = vmin3(outx * ix, outy * iy, outz * iz);
+    R distance = vmax3(inx * ix, iny * iy, inz * iz);
+    if (distance >= distOut || distOut <= R(kHalfTolerance) || absOrthogOut <= R(kHalfTolerance))
+      distance = inf_length<R>();
+    return distance;
+  }
+
+  // BoxImplementation.h:136-151
+  template <typename R>
+  VGM_DEV static R distance_to_out(const Data &b, V3<R> p, V3<R> d, R /*step_max*/)
+  {
+    const R ix = R(1) / nonzero(d.x), iy = R(1) / nonzero(d.y), iz = R(1) / nonzero(d.z);
+    const R sx = vsign(d.x), sy = vsign(d.y), sz = vsign(d.z);
+    const R safetyIn = excess(b, p);
+    const R outx = sx * R(b.dim[0]) - p.x, outy = sy * R(b.dim[1]) - p.y, outz = sz * R(b.dim[2]) - p.z;
+    R distance = vmin3(outx * ix, outy * iy, outz * iz);
+    if (safetyIn > R(kHalfTolerance)) distance = R(-1);
+    return distance;
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_in(const Data &b, V3<R> p)
+  {
+    return excess(b, p);
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_out(const Data &b, V3<R> p)
+  {
+    return vmin3(R(b.dim[0]) - vabs(p.x), R(b.dim[1]) - vabs(p.y), R(b.dim[2]) - vabs(p.z));
+  }
+};
+
+}  // namespace vgm

@@ -1,0 +1,93 @@
+// SOA3D batch kernels: one thread per point, coalesced FP64 streams.
+// Replaces the per-point loops of SIMDSpecializedVolImplHelper
+// (VecGeom/volumes/SpecializedPlacedVolImplHelper.h:184-257) and
+// UnplacedVolumeImplHelper (VecGeom/volumes/UnplacedVolumeImplHelper.h:34-67)
+// and the CUDA benchmark kernels (test/VecGeomBenchmark/Benchmarker.cu:13-61).
+//
+// The kernels are templated on shape family and method, so nothing is
+// dispatched per point; the shape parameters are wave-uniform and come in
+// through scalar loads.
+#include "vgm_internal.h"
+#include "transform.hpp"
+#include "shapes/all_shapes.hpp"
+
+namespace vgm {
+
+constexpr int kSoaBlock = 256;
+
+template <class S, int M>
+__global__ __launch_bounds__(kSoaBlock) void soa_fp64_kernel(const typename S::Data *__restrict__ sd, Transform tr,
+                                                             int use_tr, SoaArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kSoaBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const typename S::Data &s = *sd;
+  V3<double> p{a.px[i], a.py[i], a.pz[i]};
+  if (use_tr) p = to_local_point(tr, p);
+  if constexpr (M == kContains) {
+    static_cast<uint8_t *>(a.out)[i] = S::template contains<double>(s, p) ? 1 : 0;
+  } else if constexpr (M == kInside) {
+    static_cast<int32_t *>(a.out)[i] = S::template inside<double>(s, p);
+  } else if constexpr (M == kSafetyToIn) {
+    static_cast<double *>(a.out)[i] = S::template safety_to_in<double>(s, p);
+  } else if constexpr (M == kSafetyToOut) {
+    static_cast<double *>(a.out)[i] = S::template safety_to_out<double>(s, p);
+  } else {
+    V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
+    if (use_tr) d = to_local_dir(tr, d);
+    const double step = a.step ? a.step[i] : kInfLength;
+    double r;
+    if constexpr (M == kDistanceToIn)
+      r = S::template distance_to_in<double>(s, p, d, step);
+    else
+      r = S::template distance_to_out<double>(s, p, d, step);
+    static_cast<double *>(a.out)[i] = r;
+  }
+}
+
+namespace {
+struct SoaLaunch {
+  hipStream_t stream;
+  int method;
+  const void *params;
+  const Transform &tr;
+  int use_tr;
+  const SoaArgs &a;
+  hipError_t err;
+
+  template <class S, int M>
+  void go()
+  {
+    const unsigned blocks = unsigned((a.n + kSoaBlock - 1) / kSoaBlock);
+    hipLaunchKernelGGL((soa_fp64_kernel<S, M>), dim3(blocks), dim3(kSoaBlock), 0, stream,
+                       static_cast<const typename S::Data *>(params), tr, use_tr, a);
+    err = hipGetLastError();
+  }
+
+  template <class S>
+  void operator()()
+  {
+    switch (method) {
+    case kContains: go<S, kContains>(); break;
+    case kInside: go<S, kInside>(); break;
+    case kDistanceToIn: go<S, kDistanceToIn>(); break;
+    case kDistanceToOut: go<S, kDistanceToOut>(); break;
+    case kSafetyToIn: go<S, kSafetyToIn>(); break;
+    case kSafetyToOut: go<S, kSafetyToOut>(); break;
+    default: err = hipErrorInvalidValue;
+    }
+  }
+};
+}  // namespace
+
+hipError_t launch_soa(hipStream_t stream, int method, int precision, const ShapeRec &rec, const void *params,
+                      const Transform &tr, bool use_transform, const SoaArgs &a)
+{
+  (void)precision;
+  if (a.n == 0) return hipSuccess;
+  SoaLaunch L{stream, method, params, tr, use_transform ? 1 : 0, a, hipSuccess};
+  if (!dispatch_shape_type(rec.type, L)) return hipErrorInvalidValue;
+  return L.err;
+}
+
+}  // namespace vgm

@@ -1,0 +1,32 @@
+// Internal interfaces between the C-ABI layer (capi.cpp) and the kernel
+// launchers.  Not installed; callers use include/vgm/vgm.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string>
+#include "vgm_types.h"
+
+namespace vgm {
+
+// One SOA3D batch (SOA3D<double>, VecGeom/base/SOA3D.h:36-41) plus outputs.
+struct SoaArgs {
+  const double *px, *py, *pz;
+  const double *dx, *dy, *dz;
+  const double *step;
+  size_t n;
+  void *out;
+};
+
+enum Precision : int32_t { kFP64 = 0, kMixed = 1 };
+
+// Launch one placed-volume SOA query on `stream`.  `rec` describes the shape,
+// `params` is the device address of its parameter block, `use_transform`
+// selects master-frame input (Contains/Inside/DistanceToIn/SafetyToIn and
+// PlacedDistanceToOut) versus local-frame input.
+hipError_t launch_soa(hipStream_t stream, int method, int precision, const ShapeRec &rec, const void *params,
+                      const Transform &tr, bool use_transform, const SoaArgs &a);
+
+void set_error(const std::string &msg);
+
+}  // namespace vgm

@@ -1,0 +1,57 @@
+// Device-side scalar helpers for the shape kernels.  Every kernel that must
+// bit-match the FP64 oracle is compiled with -ffp-contract=off, so the order of
+// operations written here is the order executed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+namespace vgm {
+
+// Tolerances and sentinels (VecGeom/base/Math.h:27-53, FP64 build).
+constexpr double kTolerance = 1e-9;
+constexpr double kHalfTolerance = 0.5 * 1e-9;
+constexpr double kConeTolerance = 1e-7;
+constexpr double kAngTolerance = 1e-9;
+constexpr double kPushTolerance = 1e-6;
+constexpr double kInfLength = DBL_MAX;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2. * kPi;
+constexpr double kHalfPi = 0.5 * kPi;
+constexpr double kTiny = 1e-30;
+
+#define VGM_DEV __device__ __forceinline__
+
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+
+template <typename R>
+VGM_DEV R vmax(R a, R b) { return a > b ? a : b; }
+template <typename R>
+VGM_DEV R vmin(R a, R b) { return a < b ? a : b; }
+template <typename R>
+VGM_DEV R vmax3(R a, R b, R c) { return vmax(vmax(a, b), c); }
+template <typename R>
+VGM_DEV R vmin3(R a, R b, R c) { return vmin(vmin(a, b), c); }
+VGM_DEV double vabs(double a) { return __builtin_fabs(a); }
+VGM_DEV float vabs(float a) { return __builtin_fabsf(a); }
+VGM_DEV double vsqrt(double a) { return __builtin_sqrt(a); }
+VGM_DEV float vsqrt(float a) { return __builtin_sqrtf(a); }
+// copysign(1, x): VecCore's Sign for scalar doubles.
+VGM_DEV double vsign(double x) { return __builtin_copysign(1.0, x); }
+VGM_DEV float vsign(float x) { return __builtin_copysignf(1.0f, x); }
+// NonZero(x) = x + copysign(1e-30, x)  (VecGeom/base/Global.h:170-177)
+VGM_DEV double nonzero(double x) { return x + __builtin_copysign(1.0e-30, x); }
+VGM_DEV float nonzero(float x) { return x + __builtin_copysignf(1.0e-30f, x); }
+// NonZeroAbs(x) = |x| + 1e-30 (VecGeom/base/Global.h:162-167)
+VGM_DEV double nonzero_abs(double x) { return __builtin_fabs(x) + 1.0e-30; }
+
+template <typename R>
+VGM_DEV R inf_length();
+template <>
+VGM_DEV double inf_length<double>() { return DBL_MAX; }
+template <>
+VGM_DEV float inf_length<float>() { return FLT_MAX; }
+
+}  // namespace vgm
