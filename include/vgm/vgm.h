@@ -56,6 +56,15 @@ void *vgm_ctx_stream(vgm_ctx *ctx);
 /* UnplacedBox(dx, dy, dz): half-lengths (VecGeom/volumes/UnplacedBox.h). */
 int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
 
+/* UnplacedTube(rmin, rmax, dz, sphi, dphi): radii, half-length, start and
+ * delta phi in radians (VecGeom/volumes/UnplacedTube.h:29-35). */
+int vgm_tube_create(vgm_ctx *ctx, double rmin, double rmax, double dz, double sphi, double dphi, int *shape);
+
+/* UnplacedCone(rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi): radii at -dz and
+ * +dz (VecGeom/volumes/UnplacedCone.h:51-57). */
+int vgm_cone_create(vgm_ctx *ctx, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+                    double dphi, int *shape);
+
 /* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
  * VecGeom/volumes/UnplacedVolume.h). */
 int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);

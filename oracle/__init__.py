@@ -45,6 +45,8 @@ def lib():
         L.orc_geom_destroy.restype = None
         sig = {
             "orc_box_create": [_P, _D, _D, _D],
+            "orc_tube_create": [_P, _D, _D, _D, _D, _D],
+            "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
             "orc_pv_create": [_P, _I, _P],
@@ -85,6 +87,12 @@ class Geometry:
 
     def box(self, dx, dy, dz):
         return self._chk(self._L.orc_box_create(self.h, dx, dy, dz))
+
+    def tube(self, rmin, rmax, dz, sphi, dphi):
+        return self._chk(self._L.orc_tube_create(self.h, rmin, rmax, dz, sphi, dphi))
+
+    def cone(self, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi):
+        return self._chk(self._L.orc_cone_create(self.h, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi))
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

@@ -24,6 +24,17 @@ static int add_shape(orc_geom *g, Shape *s)
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz) { return add_shape(g, make_box(dx, dy, dz)); }
 
+int orc_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi)
+{
+  return add_shape(g, make_tube(rmin, rmax, z, sphi, dphi));
+}
+
+int orc_cone_create(orc_geom *g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+                    double dphi)
+{
+  return add_shape(g, make_cone(rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi));
+}
+
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
 {
   auto *G = reinterpret_cast<Geometry *>(g);

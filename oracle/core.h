@@ -139,5 +139,7 @@ struct Shape {
 };
 
 Shape *make_box(double dx, double dy, double dz);
+Shape *make_cone(double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi);
+Shape *make_tube(double rmin, double rmax, double z, double sphi, double dphi);
 
 }  // namespace orc

@@ -155,3 +155,177 @@ BOX_KATS = [
     ((1.0, 200.0, 200.0), "DistanceToOut", (1.0 + 6.0e-15, 0, 0), VX, ("eq", 0.0)),
     ((1.0, 200.0, 200.0), "DistanceToIn", (-1.0 - 6.0e-15, 0, 0), _vecgeom431_in_m, ("eq", 0.0)),
 ]
+
+
+def mag_unit(v):
+    """v /= v.Mag() (Vector3D operator/=)."""
+    v = np.asarray(v, dtype=np.float64)
+    m = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    return tuple(float(c) for c in v / m)
+
+
+# ---- Tube: test/unit_tests/TestTube.cpp ---------------------------------------
+PI = math.pi
+T1 = (0.0, 50.0, 50.0, 0.0, 2 * PI)
+T1A = (0.0, 50.0, 50.0, 0.0, 0.5 * PI)
+T2 = (45.0, 50.0, 50.0, 0.0, 2 * PI)
+T2A = (5.0, 50.0, 50.0, 0.0, 2 * PI)
+T3 = (0.0, 50.0, 50.0, 0.5 * PI, 0.5 * PI)
+T5 = (50.0, 100.0, 50.0, 0.0, 1.5 * PI)
+TUBE6 = (750.0, 760.0, 350.0, 0.31415926535897931, 5.6548667764616276)
+TUBE7 = (2200.0, 3200.0, 2500.0, -0.68977164349384879, 3.831364227270472)
+TUBE8 = (2550.0, 2580.0, 2000.0, 0.0, 2 * PI)
+TUBE9 = (1150.0, 1180.0, 2000.0, 0.0, 2 * PI)
+DEG = PI / 180.0
+JIRA174 = (57.555599999999998, 205.55599999999998, 348.0, -5.5747247 * DEG, 5.5747247 * DEG)
+_d174 = mag_unit((0.303163, -0.492481, 0.815815))
+VMXMY = (-S2, -S2, 0.0)
+VXMY = (S2, -S2, 0.0)
+P0 = (0.0, 0.0, 0.0)
+PBX, PBMX, PBY, PBMY, PBZ, PBMZ = (100.0, 0, 0), (-100.0, 0, 0), (0, 100.0, 0), (0, -100.0, 0), (0, 0, 100.0), (0, 0, -100.0)
+PONX, PONY, PONZ = (50.0, 0, 0), (0, 50.0, 0), (0, 0, 50.0)
+
+TUBE_KATS = [
+    # TestTube.cpp:85-105 (jira 174)
+    (JIRA174, "DistanceToIn", (92.4733, 5.32907e-15, 19.943), _d174, ("eq", 0.0)),
+    (JIRA174, "DistanceToOut", (92.4733, 5.32907e-15, 19.943), _d174, ("eq", 19.499)),
+    (JIRA174, "DistanceToIn", (92.4733, -5.32907e-15, 19.943), tuple(-c for c in _d174), ("inf",)),
+    (JIRA174, "DistanceToOut", (92.4733, -5.32907e-15, 19.943), tuple(-c for c in _d174), ("eq", 0.0)),
+    # :116-121 (VECGEOM-206)
+    (T1, "DistanceToOut", (44.991, 21.816, 4.677), unit((0.16636, 0.64765, -0.74356)), ("le", -1e-300)),
+    # :124-130 (VECGEOM-222)
+    (T5, "DistanceToIn", (60.0, -60.0, 10.0), unit((-1.0, 1.0, 0.001)), ("eq", 134.8528474556)),
+    # :134-153 Inside
+    (T1, "Inside", P0, None, ("inside", 1)),
+    (T1, "Inside", PBZ, None, ("inside", 3)),
+    (T1, "Inside", PONX, None, ("inside", 2)),
+    (T1, "Inside", PONY, None, ("inside", 2)),
+    (T1, "Inside", PONZ, None, ("inside", 2)),
+    (T1A, "Inside", P0, None, ("inside", 2)),
+    (T1A, "Inside", PBZ, None, ("inside", 3)),
+    (T1A, "Inside", PONX, None, ("inside", 2)),
+    (T1A, "Inside", PONY, None, ("inside", 2)),
+    (T1A, "Inside", PONZ, None, ("inside", 2)),
+    (T2, "Inside", P0, None, ("inside", 3)),
+    (T2, "Inside", PBZ, None, ("inside", 3)),
+    (T2, "Inside", PONX, None, ("inside", 2)),
+    (T2, "Inside", PONY, None, ("inside", 2)),
+    (T2, "Inside", PONZ, None, ("inside", 3)),
+    (T2A, "Inside", P0, None, ("inside", 3)),
+    (T2A, "Inside", PONX, None, ("inside", 2)),
+    (T2A, "Inside", PONZ, None, ("inside", 3)),
+    # :179-196 SafetyToOut / DistanceToOut from the centre
+    (T1, "SafetyToOut", P0, None, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VX, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VMX, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VY, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VMY, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VZ, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VMZ, ("eq", 50)),
+    (T1, "DistanceToOut", P0, VXY, ("eq", 50)),
+    # :239-262 sector
+    (T3, "DistanceToOut", (0.0, 10.0, 0.0), VX, ("eq", 0)),
+    (T3, "Inside", (0.5, 10.0, 0.0), None, ("inside", 3)),
+    (T3, "DistanceToOut", (0.5, 10.0, 0.0), VX, ("le", -1e-300)),
+    (T3, "DistanceToOut", (-0.5, 9.0, 0.0), VX, ("eq", 0.5)),
+    (T3, "DistanceToOut", (-5.0, 9.5, 0.0), VX, ("eq", 5)),
+    (T3, "DistanceToOut", (-5.0, 9.5, 0.0), VMY, ("eq", 9.5)),
+    (T3, "DistanceToOut", (-5.0, 9.0, 0.0), VXMY, ("eq", 7.0710678)),
+    # :265-297 SafetyToIn / DistanceToIn
+    (T1, "SafetyToIn", PBX, None, ("eq", 50)),
+    (T1, "SafetyToIn", PBMX, None, ("eq", 50)),
+    (T1, "SafetyToIn", PBY, None, ("eq", 50)),
+    (T1, "SafetyToIn", PBMY, None, ("eq", 50)),
+    (T1, "SafetyToIn", PBZ, None, ("eq", 50)),
+    (T1, "SafetyToIn", PBMZ, None, ("eq", 50)),
+    (T1, "DistanceToIn", PBX, VMX, ("eq", 50)),
+    (T1, "DistanceToIn", PBMX, VX, ("eq", 50)),
+    (T1, "DistanceToIn", PBY, VMY, ("eq", 50)),
+    (T1, "DistanceToIn", PBMY, VY, ("eq", 50)),
+    (T1, "DistanceToIn", PBZ, VMZ, ("eq", 50)),
+    (T1, "DistanceToIn", PBMZ, VZ, ("eq", 50)),
+    (T1, "DistanceToIn", PBX, VXY, ("inf",)),
+    (T1A, "DistanceToIn", PBZ, VMZ, ("eq", 50)),
+    # :315-356
+    (T3, "DistanceToIn", (49.5, 5.0, 0.0), VMX, ("eq", 49.5)),
+    (T3, "DistanceToIn", (49.5, -0.5, 0.0), VMX, ("inf",)),
+    (T5, "DistanceToIn", (30.0, -20.0, 0.0), VXY, ("eq", 28.284271)),
+    (T5, "DistanceToIn", (30.0, -20.0, 0.0), VMXMY, ("eq", 42.426407)),
+    (T5, "DistanceToIn", (50.0, -20.0, 0.0), VY, ("eq", 20)),
+    (T5, "DistanceToIn", (30.0, -50.0, 0.0), VMX, ("eq", 30)),
+    # :404-428 (bugs 76, 91 and others)
+    (TUBE6, "DistanceToOut", (-388.20504321896431, -641.71398957741451, 332.85995254027955),
+     (-0.47312863350457468, -0.782046391443315, 0.40565100491504164), ("eq", 10.940583)),
+    (TUBE7, "DistanceToOut", (-2460.0, 1030.0, -2500.0),
+     (-0.086580540180167642, 0.070084247882560638, 0.9937766390194761), ("eq", 4950.348576972614)),
+    (TUBE8, "DistanceToOut", (6.71645645882942, 2579.415860329989, -1.519530725281157),
+     (-0.6305220496340839, -0.07780451841562354, 0.7722618738739774), ("eq", 1022.64931421)),
+    (TUBE9, "DistanceToOut", (2.267347771505638, 1170.164934028592, 4.820317321984064),
+     (-0.1443054266272111, -0.01508874701037938, 0.9894181489944458), ("eq", 2016.51817758)),
+    (T1A, "DistanceToOut", (0.0, 0.0, 50.0), VX, ("eq", 50)),
+    (T1A, "DistanceToOut", (0.0, 5.0, 50.0), VMY, ("eq", 5)),
+    # :475-479
+    ((45.0, 50.0, 50.0, PI / 4.0, PI / 8.0), "Inside", (33.315052227388207, 37.284142675357259, 33.366096020078537),
+     None, ("inside", 2)),
+]
+
+
+# ---- Cone: test/unit_tests/TestCons.cpp -----------------------------------------
+# Inside asserts (:183-221) are hard KATs; the OutRange(dist, wanted) checks
+# (DELTA = 0.01, :28-35) only print in the reference and are kept as soft KATs.
+C1 = (50.0, 100.0, 50.0, 100.0, 50.0, 0.0, 2 * PI)
+C2 = (50.0, 100.0, 50.0, 200.0, 50.0, -1.0, 2 * PI)
+C3 = (50.0, 100.0, 50.0, 100.0, 50.0, -PI / 6, PI / 3)
+C4 = (50.0, 100.0, 50.0, 200.0, 50.0, -PI / 6, PI / 3)
+C5 = (25.0, 50.0, 75.0, 150.0, 50.0, 0.0, 1.5 * PI)
+C6 = (0.0, 150.0, 0.0, 150.0, 50.0, 0.0, 2 * PI)
+C7 = (95.0, 100.0, 95.0, 100.0, 50.0, 0.0, 2 * PI)
+C8A = (0.0, 100.0, 0.0, 150.0, 50.0, 0.0, 2 * PI)
+C8B = (50.0, 100.0, 100.0, 150.0, 50.0, 0.0, 2 * PI)
+C8C = (100.0, 150.0, 50.0, 100.0, 50.0, 0.0, 2 * PI)
+C9 = (50.0, 60.0, 0.0, 10.0, 50.0, 0.0, 2 * PI)
+CTEST10 = (20.0, 60.0, 80.0, 140.0, 100.0, 10 * PI / 180.0, 300 * PI / 180.0)
+PPLX, PPLY, PPLZ = (120.0, 0, 0), (0, 120.0, 0), (0, 0, 120.0)
+PMIY, PMIZ = (0, -120.0, 0), (0, 0, -120.0)
+PONPHI1 = (60 * math.cos(PI / 6), -60 * math.sin(PI / 6), 0.0)
+PNEARPLZ, PNEARMIZ = (45.0, 45.0, 45.0), (45.0, 45.0, -45.0)
+PONR1 = (math.sqrt(50 * 50 / 2.0), math.sqrt(50 * 50 / 2.0), 0.0)
+PONR2 = (math.sqrt(100 * 100 / 2.0), math.sqrt(100 * 100 / 2.0), 0.0)
+
+CONE_KATS = [
+    (CTEST10, "Inside", (-691.0 - 500, 174.0, 404.0), None, ("inside", 3)),
+    (CTEST10, "Inside", (400.0 - 500, 20.9, 5.89), None, ("inside", 1)),
+    (CTEST10, "Inside", (456.0 - 500, 13.0, -14.7), None, ("inside", 1)),
+    (CTEST10, "Inside", (537.0 - 500, 1.67, -44.1), None, ("inside", 3)),
+    (CTEST10, "Inside", (537.0, 1.67, -44.1), None, ("inside", 3)),
+    (CTEST10, "Inside", (1e3, -63.5, -213.0), None, ("inside", 3)),
+    (CTEST10, "Inside", (-50.0, 0.0, 0.0), None, ("inside", 2)),
+    (CTEST10, "Inside", (6.454731216775542, -90.42080754048007, 100.0), None, ("inside", 2)),
+    (CTEST10, "Inside", (22.65282328600368, -69.34877585931267, 76.51600623610082), None, ("inside", 2)),
+    (CTEST10, "Inside", (51.28206938732319, -32.10510677306267, 35.00932544708616), None, ("inside", 3)),
+]
+
+CONE_SOFT_KATS = [  # (shape, method, p, d, wanted) with |result - wanted| <= 0.01
+    (C4, "SafetyToOut", PONPHI1, None, 0.0),
+    (C1, "SafetyToOut", PONPHI1, None, 10.0),
+    (C1, "SafetyToOut", PNEARPLZ, None, 5.0),
+    (C1, "SafetyToOut", PNEARMIZ, None, 5.0),
+    (C1, "SafetyToOut", PONR1, None, 0.0),
+    (C1, "SafetyToOut", PONR2, None, 0.0),
+    (C6, "SafetyToOut", P0, None, 50.0),
+    (C5, "SafetyToOut", (0.0, -70.0, 0.0), None, 0.0),
+    (C4, "DistanceToOut", PPLX, VMX, 70.0),
+    (C2, "DistanceToOut", PPLX, VMX, 70.0),
+    (C1, "SafetyToIn", P0, None, 50.0),
+    (C1, "SafetyToIn", PPLX, None, 20.0),
+    (C1, "SafetyToIn", PPLY, None, 20.0),
+    (C1, "SafetyToIn", PPLZ, None, 70.0),
+    (C6, "DistanceToIn", PPLZ, VMZ, 70.0),
+    (C8A, "DistanceToIn", PPLZ, VMZ, 70.0),
+    (C9, "DistanceToIn", PPLZ, VMZ, 70.0),
+    (C9, "DistanceToIn", (0.0, 0.0, 50.0), VMZ, 0.0),
+    (C6, "DistanceToIn", PMIZ, VZ, 70.0),
+    (C8A, "DistanceToIn", PMIZ, VZ, 70.0),
+    (C1, "DistanceToIn", PPLX, VMX, 20.0),
+] + [(c, "DistanceToIn", PPLZ, VMZ, KINF) for c in (C1, C2, C3, C4, C5, C7, C8B, C8C)] \
+  + [(c, "DistanceToIn", PMIZ, VZ, KINF) for c in (C1, C2, C3, C4, C5, C7, C8B, C8C, C9)]

@@ -7,8 +7,7 @@ import math
 
 import numpy as np
 
-# Rotation used to exercise the generic Transformation3D path: rotation about
-# an oblique axis, stored in the reference's 9-element order.
+PI = math.pi
 
 
 def rotation_matrix(axis, angle):
@@ -25,10 +24,26 @@ def rotation_matrix(axis, angle):
 
 
 def transform12(translation, axis=(1.0, 2.0, 3.0), angle=0.7):
+    """Translation + rotation about an oblique axis, in the reference's 12-double order."""
     return np.concatenate([np.asarray(translation, dtype=np.float64), rotation_matrix(axis, angle).reshape(-1)])
 
 
 SHAPES = {
+    # config 1
     "box_c1": ("box", (10.0, 20.0, 30.0)),
     "box_thin": ("box", (0.15, 24.707, 22.7)),
+    # config 2 tube and cone
+    "tube_c2": ("tube", (5.0, 20.0, 30.0, 0.0, 1.5 * PI)),
+    "cone_c2": ("cone", (0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * PI)),
+    # other tube and cone branches (phi <= pi, solid, thin)
+    "tube_solid": ("tube", (0.0, 50.0, 50.0, 0.0, 2 * PI)),
+    "tube_sector": ("tube", (0.0, 50.0, 50.0, 0.5 * PI, 0.5 * PI)),
+    "tube_hollow_sector": ("tube", (45.0, 50.0, 50.0, PI / 4.0, PI / 8.0)),
+    "tube_cross0": ("tube", (2200.0, 3200.0, 2500.0, -0.68977164349384879, 3.831364227270472)),
+    "cone_cut": ("cone", (50.0, 100.0, 50.0, 200.0, 50.0, -PI / 6, PI / 3)),
+    "cone_ctest10": ("cone", (20.0, 60.0, 80.0, 140.0, 100.0, 10 * PI / 180.0, 300 * PI / 180.0)),
+    "cone_thin": ("cone", (95.0, 100.0, 95.0, 100.0, 50.0, 0.0, 2 * PI)),
+    "cone_inv": ("cone", (100.0, 150.0, 50.0, 100.0, 50.0, 0.0, 2 * PI)),
+    "cone_exotic": ("cone", (50.0, 60.0, 0.0, 10.0, 50.0, 0.0, 2 * PI)),
+    "cone_c5": ("cone", (25.0, 50.0, 75.0, 150.0, 50.0, 0.0, 1.5 * PI)),
 }

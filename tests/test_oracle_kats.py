@@ -5,7 +5,8 @@ import pytest
 import oracle
 from tests.golden import kats
 
-CASES = [("box", c) for c in kats.BOX_KATS]
+CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_KATS]
+         + [("cone", c) for c in kats.CONE_KATS])
 
 
 def _run(geo, kind, params, method, p, d):
@@ -22,3 +23,35 @@ def test_kat(kind, case):
     geo = oracle.Geometry()
     r = _run(geo, kind, params, method, p, d)
     assert kats.check(r, expect), (params, method, p, d, r, expect)
+
+
+def test_tube_bug810_chain():
+    """TestTube.cpp:432-472: walk along +y through a clad/core pair of half tubes."""
+    geo = oracle.Geometry()
+    clad = geo.place(geo.logical_volume(geo.tube(90.0, 110.0, 105.0, 0.0, np.pi)))
+    core = geo.place(geo.logical_volume(geo.tube(95.0, 105.0, 100.0, 0.0, np.pi)))
+    vy = [np.array([0.0]), np.array([1.0]), np.array([0.0])]
+
+    def q(pv, m, p, d=None):
+        return geo.query(pv, m, [np.array([c]) for c in p], d)[0]
+
+    p = np.zeros(3)
+    p = p + q(clad, "DistanceToIn", p, vy) * np.array([0, 1.0, 0])
+    assert q(core, "Inside", p) == 3 and q(clad, "Inside", p) == 2
+    p = p + q(core, "DistanceToIn", p, vy) * np.array([0, 1.0, 0])
+    assert q(core, "Inside", p) == 2 and q(clad, "Inside", p) == 1
+    p = p + q(core, "DistanceToOut", p, vy) * np.array([0, 1.0, 0])
+    assert q(core, "Inside", p) == 2 and q(clad, "Inside", p) == 1
+    p = p + q(clad, "DistanceToOut", p, vy) * np.array([0, 1.0, 0])
+    assert q(core, "Inside", p) == 3 and q(clad, "Inside", p) == 2
+
+
+@pytest.mark.parametrize("case", kats.CONE_SOFT_KATS, ids=[f"cone-soft-{i}" for i in range(len(kats.CONE_SOFT_KATS))])
+def test_cone_soft_kat(case):
+    params, method, p, d, wanted = case
+    geo = oracle.Geometry()
+    r = float(_run(geo, "cone", params, method, p, d))
+    if wanted >= kats.KINF:
+        assert r >= kats.KINF, r
+    else:
+        assert abs(r - wanted) <= 0.01, (r, wanted)

@@ -62,6 +62,8 @@ def _declare(L):
         "vgm_ctx_destroy": [_P],
         "vgm_ctx_synchronize": [_P],
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_shape_extent": [_P, _I, _P, _P],
         "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
         "vgm_pv_create": [_P, _I, _P, ctypes.POINTER(_I)],
@@ -155,6 +157,12 @@ class Context:
 
     def box(self, dx, dy, dz) -> int:
         return self._new(self._L.vgm_box_create, dx, dy, dz)
+
+    def tube(self, rmin, rmax, dz, sphi, dphi) -> int:
+        return self._new(self._L.vgm_tube_create, rmin, rmax, dz, sphi, dphi)
+
+    def cone(self, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi) -> int:
+        return self._new(self._L.vgm_cone_create, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi)
 
     def extent(self, shape: int):
         mn = np.zeros(3)

@@ -255,6 +255,27 @@ int vgm_box_create(vgm_ctx *c, double dx, double dy, double dz, int *shape)
   return VGM_OK;
 }
 
+int vgm_tube_create(vgm_ctx *c, double rmin, double rmax, double dz, double sphi, double dphi, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_tube(c->host, rmin, rmax, dz, sphi, dphi, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
+int vgm_cone_create(vgm_ctx *c, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+                    double dphi, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_cone(c->host, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
 {
   if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");

@@ -1,7 +1,10 @@
 // Host geometry tables and shape parameter derivation.
 #include "geometry.h"
 #include <cstring>
+#include <cmath>
 #include "shapes/box_data.h"
+#include "shapes/tube_data.h"
+#include "shapes/cone_data.h"
 
 namespace vgm {
 
@@ -35,6 +38,149 @@ int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err
   BoxData b{{dx, dy, dz}};
   const double lo[3] = {-dx, -dy, -dz}, hi[3] = {dx, dy, dz};
   return g.add_shape(kBox, &b, sizeof(b) / sizeof(double), 0, 0, lo, hi);
+}
+
+// UnplacedTube(rmin, rmax, z, sphi, dphi) with TubeStruct::CalculateCached
+// (VecGeom/volumes/TubeStruct.h:110-176) and the wedge vectors of
+// source/Wedge_Evolution.cpp.  The bounding box is the full-phi one.
+int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)
+{
+  if (!(rmax > 0 && z > 0 && dphi > 0 && rmin < rmax)) {
+    err = "tube needs 0 <= rmin < rmax, z > 0, dphi > 0";
+    return -1;
+  }
+  const double kHalfTol = 0.5e-9;
+  TubeData t{};
+  t.rmin = rmin < 0.0 ? 0.0 : rmin;
+  t.rmax = rmax;
+  t.z = z;
+  t.sphi = sphi;
+  t.dphi = dphi;
+  t.tolIz = t.z - kHalfTol;
+  t.tolOz = t.z + kHalfTol;
+  t.rmin2 = t.rmin * t.rmin;
+  const double tolOrmin = t.rmin - kHalfTol, tolIrmin = t.rmin + kHalfTol;
+  t.tolOrmin2 = tolOrmin * tolOrmin;
+  t.tolIrmin2 = tolIrmin * tolIrmin;
+  t.rmax2 = t.rmax * t.rmax;
+  const double tolOrmax = t.rmax + kHalfTol, tolIrmax = t.rmax - kHalfTol;
+  t.tolOrmax2 = tolOrmax * tolOrmax;
+  t.tolIrmax2 = tolIrmax * tolIrmax;
+  t.along1x = std::cos(sphi);
+  t.along1y = std::sin(sphi);
+  t.along2x = std::cos(sphi + dphi);
+  t.along2y = std::sin(sphi + dphi);
+  t.normal1x = -std::sin(sphi);
+  t.normal1y = std::cos(sphi);
+  t.normal2x = std::sin(sphi + dphi);
+  t.normal2y = -std::cos(sphi + dphi);
+  t.maxval = t.rmax > t.z ? t.rmax : t.z;
+  const double kPi = 3.14159265358979323846;
+  t.phi_treat = dphi < 2. * kPi;
+  t.rmin_treat = t.rmin > 0;
+  t.le_pi = dphi <= M_PI;
+  t.lt_pi = dphi < kPi;
+  t.gt_pi = dphi > M_PI;
+  const double lo[3] = {-rmax, -rmax, -z}, hi[3] = {rmax, rmax, z};
+  return g.add_shape(kTube, &t, sizeof(t) / sizeof(double), 0, 0, lo, hi);
+}
+
+static double nonzero_h(double x) { return x + std::copysign(1.0e-30, x); }
+
+// UnplacedCone(rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi): ConeStruct
+// constructor order (VecGeom/volumes/ConeStruct.h): SetAndCheckDPhiAngle,
+// SetAndCheckSPhiAngle, CalculateCached.  The wedge keeps the raw angles.
+int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+               double dphi, std::string &err)
+{
+  if (!(dz > 0 && dphi > 0 && rmin1 >= 0 && rmin2 >= 0 && rmax1 >= rmin1 && rmax2 >= rmin2 &&
+        (rmax1 > 0 || rmax2 > 0))) {
+    err = "cone needs 0 <= rmin <= rmax, dz > 0, dphi > 0";
+    return -1;
+  }
+  const double kPi = 3.14159265358979323846, kTwoPi = 2. * kPi, kConeTol = 1e-7, kHalfTol = 0.5e-9;
+  ConeData c{};
+  c.rmin1 = rmin1;
+  c.rmax1 = rmax1;
+  c.rmin2 = rmin2;
+  c.rmax2 = rmax2;
+  c.dz = dz;
+  double fS = sphi, fD = dphi;
+  if (dphi >= kTwoPi - 0.5 * 1e-9) {
+    fD = kTwoPi;
+    fS = 0;
+  }
+  if (sphi < 0)
+    fS = kTwoPi - std::fmod(std::fabs(sphi), kTwoPi);
+  else
+    fS = std::fmod(sphi, kTwoPi);
+  if (fS + fD > kTwoPi) fS -= kTwoPi;
+  c.sphi = fS;
+  c.dphi = fD;
+  c.along1x = std::cos(fS);
+  c.along1y = std::sin(fS);
+  c.along2x = std::cos(fS + fD);
+  c.along2y = std::sin(fS + fD);
+  c.w_along1x = std::cos(sphi);
+  c.w_along1y = std::sin(sphi);
+  c.w_along2x = std::cos(sphi + dphi);
+  c.w_along2y = std::sin(sphi + dphi);
+  c.w_normal1x = -std::sin(sphi);
+  c.w_normal1y = std::cos(sphi);
+  c.w_normal2x = std::sin(sphi + dphi);
+  c.w_normal2y = -std::cos(sphi + dphi);
+  // CalculateCached
+  c.orig_rmax1 = c.rmax1;
+  c.orig_rmax2 = c.rmax2;
+  if (c.rmin1 == c.rmax1) c.rmax1 += kConeTol;
+  if (c.rmin2 == c.rmax2) c.rmax2 += kConeTol;
+  c.sq_rmin1 = c.rmin1 * c.rmin1;
+  c.sq_rmax1 = c.rmax1 * c.rmax1;
+  c.sq_rmin2 = c.rmin2 * c.rmin2;
+  c.sq_rmax2 = c.rmax2 * c.rmax2;
+  c.tan_rmin = (c.rmin2 - c.rmin1) * 0.5 / c.dz;
+  c.sec_rmin = std::sqrt(1.0 + c.tan_rmin * c.tan_rmin);
+  c.inv_sec_rmin = 1. / nonzero_h(c.sec_rmin);
+  c.tan_rmax = (c.rmax2 - c.rmax1) * 0.5 / c.dz;
+  c.sec_rmax = std::sqrt(1.0 + c.tan_rmax * c.tan_rmax);
+  c.inv_sec_rmax = 1. / nonzero_h(c.sec_rmax);
+  c.inner_slope = -(c.rmin1 - c.rmin2) / (2. * c.dz);
+  c.outer_slope = -(c.rmax1 - c.rmax2) / (2. * c.dz);
+  c.inner_offset = c.rmin2 - c.inner_slope * c.dz;
+  c.outer_offset = c.rmax2 - c.outer_slope * c.dz;
+  c.inner_tol = kConeTol * c.sec_rmin;
+  c.outer_tol = kConeTol * c.sec_rmax;
+  if (c.rmin2 > c.rmin1) {
+    c.inner_apex = 2 * c.dz * c.rmin1 / (c.rmin2 - c.rmin1);
+    c.tan_inner_apex = c.rmin2 / (2 * c.dz + c.inner_apex);
+  } else {
+    c.inner_apex = 2 * c.dz * c.rmin2 / nonzero_h(c.rmin1 - c.rmin2);
+    c.tan_inner_apex = c.rmin1 / (2 * c.dz + c.inner_apex);
+  }
+  if (c.rmin1 == 0. || c.rmin2 == 0.) c.inner_apex = 0.;
+  if (c.rmin1 == 0.) c.tan_inner_apex = c.rmin2 / (2 * c.dz);
+  if (c.rmin2 == 0.) c.tan_inner_apex = c.rmin1 / (2 * c.dz);
+  if (c.rmax2 > c.rmax1) {
+    c.outer_apex = 2 * c.dz * c.rmax1 / (c.rmax2 - c.rmax1);
+    c.tan_outer_apex = c.rmax2 / (2 * c.dz + c.outer_apex);
+  } else {
+    c.outer_apex = 2 * c.dz * c.rmax2 / nonzero_h(c.rmax1 - c.rmax2);
+    c.tan_outer_apex = c.rmax1 / (2 * c.dz + c.outer_apex);
+  }
+  if (c.rmax1 == 0. || c.rmax2 == 0.) c.outer_apex = 0.;
+  if (c.rmax1 == 0.) c.tan_outer_apex = c.rmax2 / (2 * c.dz);
+  if (c.rmax2 == 0.) c.tan_outer_apex = c.rmax1 / (2 * c.dz);
+  c.znorm_inner = c.tan_rmin / nonzero_h(c.sec_rmin);
+  c.znorm_outer = -c.tan_rmax / nonzero_h(c.sec_rmax);
+  c.tol_iz = c.dz - kHalfTol;
+  c.tol_oz = c.dz + kHalfTol;
+  c.phi_treat = fD < kTwoPi;
+  c.rmin_treat = c.rmin1 > 0 || c.rmin2 > 0;
+  c.le_pi = fD <= M_PI;
+  c.lt_pi = fD < kPi;
+  const double r = c.orig_rmax1 > c.orig_rmax2 ? c.orig_rmax1 : c.orig_rmax2;
+  const double lo[3] = {-r, -r, -dz}, hi[3] = {r, r, dz};
+  return g.add_shape(kCone, &c, sizeof(c) / sizeof(double), 0, 0, lo, hi);
 }
 
 }  // namespace vgm

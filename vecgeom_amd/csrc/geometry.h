@@ -25,5 +25,8 @@ struct HostGeometry {
 
 // Shape builders.  Return the new shape id, or -1 with `err` set.
 int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err);
+int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+               double dphi, std::string &err);
+int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err);
 
 }  // namespace vgm

@@ -2,6 +2,8 @@
 // runtime ShapeRec into a call of the matching templated shape.
 #pragma once
 #include "box.hpp"
+#include "tube.hpp"
+#include "cone.hpp"
 
 namespace vgm {
 
@@ -13,6 +15,12 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
   switch (type) {
   case kBox:
     f.template operator()<BoxShape>();
+    return true;
+  case kTube:
+    f.template operator()<TubeShape>();
+    return true;
+  case kCone:
+    f.template operator()<ConeShape>();
     return true;
   default:
     return false;

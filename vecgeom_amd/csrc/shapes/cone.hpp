@@ -1,0 +1,366 @@
+// Cone kernels, UniversalCone runtime-checked variant, scalar semantics
+// (VecGeom/volumes/kernel/ConeImplementation.h:99-517 and the scalar
+// ConeHelpers of VecGeom/volumes/ConeUtilities.h).
+#pragma once
+#include "../vgm_math.hpp"
+#include "cone_data.h"
+#include "wedge.hpp"
+
+namespace vgm {
+
+struct ConeShape {
+  using Data = ConeData;
+
+  template <typename R>
+  VGM_DEV static R plus_tol_sq(R x, double tol) { return x * (x + R(2.0 * tol)); }
+  template <typename R>
+  VGM_DEV static R minus_tol_sq(R x, double tol) { return vabs(x * (x - R(2.0 * tol))); }
+
+  // ConeUtilities::PointInCyclicalSector with kConeTolerance bands
+  template <typename R>
+  VGM_DEV static bool in_sector(const Data &c, R x, R y, bool includeSurface)
+  {
+    const R startCheck = (-x * R(c.along1y)) + (y * R(c.along1x));
+    const R endCheck = (-R(c.along2x) * y) + (R(c.along2y) * x);
+    const R tol = includeSurface ? R(-kConeTolerance) : R(kConeTolerance);
+    if (c.le_pi) return (startCheck >= tol) && (endCheck >= tol);
+    return (startCheck >= tol) || (endCheck >= tol);
+  }
+
+  template <typename R>
+  VGM_DEV static R radius_at(const Data &c, R z, bool inner)
+  {
+    if (inner) return (c.rmin1 == c.rmin2) ? R(c.rmin1) : R(c.inner_slope) * z + R(c.inner_offset);
+    return (c.orig_rmax1 == c.orig_rmax2) ? R(c.orig_rmax1) : R(c.outer_slope) * z + R(c.outer_offset);
+  }
+
+  // n . d with the (unnormalised) surface normal of ConeUtilities::GetNormal
+  template <typename R>
+  VGM_DEV static R normal_dot(const Data &c, V3<R> p, V3<R> d, bool inner)
+  {
+    const R rho = vsqrt(p.x * p.x + p.y * p.y);
+    if (inner) {
+      const R nz = ((c.rmin1 == c.rmin2) && (c.rmin1 != 0.)) ? R(0) : R(c.znorm_inner) * (rho * R(c.sec_rmin));
+      return d.x * (-p.x) + d.y * (-p.y) + d.z * nz;
+    }
+    const R nz = ((c.rmax1 == c.rmax2) && (c.rmax1 != 0.)) ? R(0) : R(c.znorm_outer) * (rho * R(c.sec_rmax));
+    return d.x * p.x + d.y * p.y + d.z * nz;
+  }
+
+  template <typename R>
+  VGM_DEV static bool on_conical_surface(const Data &c, V3<R> p, bool inner)
+  {
+    const R rho = p.x * p.x + p.y * p.y;
+    const R coneRad = radius_at<R>(c, p.z, inner);
+    const R coneRad2 = coneRad * coneRad;
+    const R tol = inner ? R(c.inner_tol) : R(c.outer_tol);
+    return (rho >= (coneRad2 - tol * coneRad)) && (rho <= (coneRad2 + tol * coneRad)) &&
+           (vabs(p.z) < (R(c.dz) + R(kConeTolerance)));
+  }
+
+  // ConeUtilities::PhiPlaneTrajectoryIntersection, PositiveDirectionOfPhiVector
+  template <typename R, bool InsectorCheck>
+  VGM_DEV static bool phi_plane(const Data &c, double ax, double ay, double nx, double ny, V3<R> pos, V3<R> dir,
+                                R &dist)
+  {
+    dist = inf_length<R>();
+    bool ok = InsectorCheck ? ((dir.x * R(nx)) + (dir.y * R(ny)) > R(0)) : ((dir.x * R(nx)) + (dir.y * R(ny)) < R(0));
+    const R dirDotXY = (dir.y * R(ax)) - (dir.x * R(ay));
+    if (dirDotXY != R(0)) dist = ((R(ay) * pos.x) - (R(ax) * pos.y)) / nonzero(dirDotXY);
+    ok = ok && (dist > R(-kConeTolerance));
+    const R hitx = pos.x + dist * dir.x;
+    const R hity = pos.y + dist * dir.y;
+    if (InsectorCheck) {
+      const R hitz = pos.z + dist * dir.z;
+      const R r2 = (hitx * hitx) + (hity * hity);
+      const R innerRadIrTol = radius_at<R>(c, hitz, true) + R(kTolerance);
+      const R outerRadIrTol = radius_at<R>(c, hitz, false) - R(kTolerance);
+      ok = ok && vabs(hitz) <= R(c.tol_iz) && (r2 >= innerRadIrTol * innerRadIrTol) &&
+           (r2 <= outerRadIrTol * outerRadIrTol);
+      ok = ok && ((hitx * R(ax)) + (hity * R(ay))) > R(0);
+    } else {
+      ok = ok && ((hitx * R(ax)) + (hity * R(ay))) >= R(0);
+    }
+    return ok;
+  }
+
+  // Scalar DetectIntersectionAndCalculateDistanceToConicalSurface
+  template <typename R, bool ForIn, bool Inner>
+  VGM_DEV static bool conical(const Data &c, V3<R> point, V3<R> direction, R &distance)
+  {
+    distance = inf_length<R>();
+    const bool onSurf = on_conical_surface<R>(c, point, Inner);
+    const R ndot = normal_dot<R>(c, point, direction, Inner);
+    if (onSurf && vabs(ndot) == R(0)) return false;
+    if (onSurf) {
+      const bool moving = ForIn ? (ndot <= R(0)) : (ndot >= R(0));
+      const bool insector = c.phi_treat ? in_sector<R>(c, point.x, point.y, true) : true;
+      if (insector && moving) {
+        distance = R(0);
+        return true;
+      }
+    }
+    bool ok = false;
+    const R pDotV2D = point.x * direction.x + point.y * direction.y;
+    R a, b, cc;
+    if (Inner) {
+      if (c.rmin1 == c.rmin2) {
+        b = pDotV2D;
+        a = direction.x * direction.x + direction.y * direction.y;
+        cc = (point.x * point.x + point.y * point.y) - R(c.rmin2) * R(c.rmin2);
+      } else {
+        const R t = R(c.tan_inner_apex);
+        const R newPz = (c.rmin2 > c.rmin1) ? (point.z + R(c.dz) + R(c.inner_apex)) * t
+                                            : (point.z - R(c.dz) - R(c.inner_apex)) * t;
+        const R dirT = direction.z * t;
+        a = (direction.x * direction.x) + (direction.y * direction.y) - dirT * dirT;
+        b = pDotV2D - (newPz * dirT);
+        cc = (point.x * point.x + point.y * point.y) - (newPz * newPz);
+      }
+      const R b2 = b * b;
+      const R ac = a * cc;
+      if (b2 < ac) return false;
+      const R d2 = b2 - ac;
+      const R delta = vsqrt(d2);
+      if (ForIn) {
+        distance = (b >= R(0)) ? (cc / nonzero(-b - delta)) : (-b + delta) / nonzero(a);
+      } else {
+        if (b == R(0) && delta == R(0)) return false;
+        distance = (b >= R(0)) ? (-b - delta) / nonzero(a) : (cc / nonzero(-b + delta));
+      }
+      if (distance < R(0)) return false;
+      const R newZ = point.z + (direction.z * distance);
+      ok = (vabs(newZ) < R(c.dz));
+    } else {
+      if (c.orig_rmax1 == c.orig_rmax2) {
+        a = direction.x * direction.x + direction.y * direction.y;
+        b = pDotV2D;
+        cc = ((point.x * point.x + point.y * point.y) - R(c.orig_rmax2) * R(c.orig_rmax2));
+      } else {
+        const R t = R(c.tan_outer_apex);
+        const R newPz = (c.orig_rmax2 > c.orig_rmax1) ? (point.z + R(c.dz) + R(c.outer_apex)) * t
+                                                      : (point.z - R(c.dz) - R(c.outer_apex)) * t;
+        const R dirT = direction.z * t;
+        a = direction.x * direction.x + direction.y * direction.y - dirT * dirT;
+        b = (pDotV2D - (newPz * dirT));
+        cc = ((point.x * point.x + point.y * point.y) - (newPz * newPz));
+      }
+      const R b2 = b * b;
+      const R ac = a * cc;
+      const R d2 = b2 - ac;
+      if (d2 < R(0)) return false;
+      const R delta = vsqrt(d2);
+      if (ForIn) {
+        if (b == R(0) && delta == R(0)) return false;
+        distance = (b > R(0)) ? (-b - delta) / nonzero(a) : (cc / nonzero(-b + delta));
+        const R newZ = point.z + (direction.z * distance);
+        ok = (vabs(newZ) < R(c.dz) + R(kHalfTolerance));
+      } else {
+        distance = (b < R(0)) ? (-b + delta) / nonzero(a) : (cc / nonzero(-b - delta));
+        ok = distance > R(0);
+      }
+      if (distance < R(0)) return false;
+    }
+    if (c.phi_treat) {
+      R hitx = R(0), hity = R(0);
+      if (distance < inf_length<R>()) {
+        hitx = point.x + distance * direction.x;
+        hity = point.y + distance * direction.y;
+      }
+      ok = ok && in_sector<R>(c, hitx, hity, true) && (distance < inf_length<R>());
+    }
+    return ok;
+  }
+
+  template <typename R, bool ForInside>
+  VGM_DEV static void generic(const Data &c, V3<R> p, bool &cin, bool &cout)
+  {
+    const R absz = vabs(p.z);
+    cout = absz > (ForInside ? R(c.dz) + R(kConeTolerance) : R(c.dz));
+    if (ForInside) cin = absz < R(c.dz) - R(kConeTolerance);
+    if (cout) return;
+    const R r2 = p.x * p.x + p.y * p.y;
+    const R rmax = (c.orig_rmax1 == c.orig_rmax2) ? R(c.orig_rmax1) : R(c.outer_slope) * p.z + R(c.outer_offset);
+    cout = cout || (r2 > (ForInside ? plus_tol_sq<R>(rmax, c.outer_tol) : rmax * rmax));
+    if (ForInside) cin = cin && (r2 < minus_tol_sq<R>(rmax, c.outer_tol));
+    if (cout) return;
+    if (c.rmin_treat) {
+      const R rmin = R(c.inner_slope) * p.z + R(c.inner_offset);
+      cout = cout || (r2 <= (ForInside ? minus_tol_sq<R>(rmin, c.inner_tol) : rmin * rmin));
+      if (ForInside) cin = cin && (r2 > plus_tol_sq<R>(rmin, c.inner_tol));
+      if (cout) return;
+    }
+    if (c.phi_treat) {
+      bool ci = false, co = false;
+      wedge_generic<R, ForInside>(c.w_along1x, c.w_along1y, c.w_along2x, c.w_along2y, c.w_normal1x, c.w_normal1y,
+                                  c.w_normal2x, c.w_normal2y, c.lt_pi, p, ci, co);
+      cout = cout || co;
+      if (ForInside) cin = cin && ci;
+    }
+  }
+
+  template <typename R>
+  VGM_DEV static bool contains(const Data &c, V3<R> p)
+  {
+    bool ci = false, co = false;
+    generic<R, false>(c, p, ci, co);
+    return !co;
+  }
+
+  template <typename R>
+  VGM_DEV static int inside(const Data &c, V3<R> p)
+  {
+    bool ci = false, co = false;
+    generic<R, true>(c, p, ci, co);
+    return ci ? kInsideV : (co ? kOutsideV : kSurfaceV);
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &c, V3<R> point, V3<R> dir, R /*step_max*/)
+  {
+    R distance = inf_length<R>();
+    R distz = vabs(point.z) - R(c.dz);
+    const bool outZ = (distz > R(kConeTolerance) && (point.z * dir.z) >= R(0)) ||
+                      (vabs(distz) < R(kConeTolerance) && (point.z * dir.z) > R(0));
+    if (outZ) return distance;
+    const R outerRad = radius_at<R>(c, point.z, false);
+    const R rsq = point.x * point.x + point.y * point.y;
+    if ((rsq > minus_tol_sq<R>(outerRad, c.outer_tol)) && (normal_dot<R>(c, point, dir, false) >= R(0)))
+      return distance;
+    bool inside = distz < R(-kConeTolerance);
+    inside = inside && (rsq < minus_tol_sq<R>(outerRad, c.outer_tol));
+    if (c.rmin_treat) inside = inside && (rsq > plus_tol_sq<R>(radius_at<R>(c, point.z, true), c.inner_tol));
+    if (c.phi_treat) inside = inside && in_sector<R>(c, point.x, point.y, false);
+    if (inside) return R(-1.0);
+    distance = inf_length<R>();
+    distz /= nonzero(vabs(dir.z));
+    const R hitx = point.x + distz * dir.x;
+    const R hity = point.y + distz * dir.y;
+    const R r2 = (hitx * hitx) + (hity * hity);
+    bool top = (point.z >= R(c.tol_iz)) && (r2 <= R(c.sq_rmax2) + R(kTolerance));
+    bool bottom = (point.z <= -R(c.tol_iz)) && (r2 <= R(c.sq_rmax1) + R(kTolerance));
+    bool okz = (top || bottom);
+    if (c.rmin_treat) {
+      top = top && (r2 >= R(c.sq_rmin2) - R(kTolerance));
+      bottom = bottom && (r2 >= R(c.sq_rmin1) - R(kTolerance));
+      okz = okz && (top || bottom);
+    }
+    if (c.phi_treat) okz = okz && in_sector<R>(c, hitx, hity, true);
+    if (okz) return distz;
+    R dist_rOuter;
+    if (conical<R, true, false>(c, point, dir, dist_rOuter) && dist_rOuter < distance) return dist_rOuter;
+    if (c.rmin_treat) {
+      R dist_rInner;
+      if (conical<R, true, true>(c, point, dir, dist_rInner) && dist_rInner < distance) distance = dist_rInner;
+    }
+    if (c.phi_treat) {
+      R dist_phi;
+      bool ok_phi = phi_plane<R, true>(c, c.along1x, c.along1y, c.w_normal1x, c.w_normal1y, point, dir, dist_phi);
+      if (ok_phi && dist_phi < distance) distance = dist_phi;
+      ok_phi = phi_plane<R, true>(c, c.along2x, c.along2y, c.w_normal2x, c.w_normal2y, point, dir, dist_phi);
+      if (ok_phi && dist_phi < distance) distance = dist_phi;
+    }
+    return distance;
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_out(const Data &c, V3<R> point, V3<R> direction, R /*step_max*/)
+  {
+    const R distz = vabs(point.z) - R(c.dz);
+    const R rsq = point.x * point.x + point.y * point.y;
+    if (distz > R(kConeTolerance)) return R(-1.0);
+    if (rsq > plus_tol_sq<R>(radius_at<R>(c, point.z, false), c.outer_tol)) return R(-1.0);
+    if (c.rmin_treat && rsq < minus_tol_sq<R>(radius_at<R>(c, point.z, true), c.inner_tol)) return R(-1.0);
+    const bool up = direction.z > R(0);
+    const bool down = direction.z < R(0);
+    if ((up && point.z > R(0) && vabs(distz) < R(kConeTolerance)) ||
+        (down && point.z < R(0) && vabs(distz) < R(kConeTolerance)))
+      return distz;
+    R distance = inf_length<R>();
+    const R dirZInv = R(1.) / nonzero(direction.z);
+    if (up) distance = (R(c.dz) - point.z) * dirZInv;
+    if (down) distance = (-R(c.dz) - point.z) * dirZInv;
+    R dist_rOuter;
+    if (conical<R, false, false>(c, point, direction, dist_rOuter) && dist_rOuter < distance) distance = dist_rOuter;
+    if (c.rmin_treat) {
+      R dist_rInner;
+      if (conical<R, false, true>(c, point, direction, dist_rInner) && dist_rInner < distance)
+        distance = dist_rInner;
+    }
+    if (c.phi_treat) {
+      const bool onStart = wedge_on_surface<R>(c.w_along1x, c.w_along1y, c.w_normal1x, c.w_normal1y, point);
+      const bool onEnd = wedge_on_surface<R>(c.w_along2x, c.w_along2y, c.w_normal2x, c.w_normal2y, point);
+      const R dn1 = direction.x * R(-c.w_normal1x) + direction.y * R(-c.w_normal1y) + direction.z * R(-0.0);
+      const R dn2 = direction.x * R(-c.w_normal2x) + direction.y * R(-c.w_normal2y) + direction.z * R(-0.0);
+      if ((onStart && dn1 > R(0)) || (onEnd && dn2 > R(0))) return R(0);
+      R dist_phi;
+      bool ok_phi =
+          phi_plane<R, false>(c, c.along1x, c.along1y, c.w_normal1x, c.w_normal1y, point, direction, dist_phi);
+      if (ok_phi && dist_phi < distance) distance = dist_phi;
+      ok_phi = phi_plane<R, false>(c, c.along2x, c.along2y, c.w_normal2x, c.w_normal2y, point, direction, dist_phi);
+      if (ok_phi && dist_phi < distance) distance = dist_phi;
+    }
+    if (distance < R(0) && vabs(distance) < R(kTolerance)) distance = R(0);
+    return distance;
+  }
+
+  template <typename R>
+  VGM_DEV static R safe_conical(const Data &c, V3<R> p, bool inner)
+  {
+    const R rho = vsqrt(p.x * p.x + p.y * p.y);
+    if (inner) {
+      const R pRMin = R(c.tan_rmin) * p.z + (R(c.rmin1) + R(c.rmin2)) * R(0.5);
+      return (rho - pRMin) * R(c.inv_sec_rmin);
+    }
+    const R pRMax = R(c.tan_rmax) * p.z + (R(c.rmax1) + R(c.rmax2)) * R(0.5);
+    return (pRMax - rho) * R(c.inv_sec_rmax);
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_in(const Data &c, V3<R> point)
+  {
+    const R distz = vabs(point.z) - R(c.dz);
+    bool inside = distz < R(-kConeTolerance);
+    const R rsq = point.x * point.x + point.y * point.y;
+    inside = inside && (rsq < minus_tol_sq<R>(radius_at<R>(c, point.z, false), c.outer_tol));
+    if (c.rmin_treat) inside = inside && (rsq > plus_tol_sq<R>(radius_at<R>(c, point.z, true), c.inner_tol));
+    if (c.phi_treat && inside) inside = in_sector<R>(c, point.x, point.y, false);
+    if (inside) return R(-1.0);
+    const R safeZ = vabs(point.z) - R(c.dz);
+    const R safeOuter = -safe_conical<R>(c, point, false);
+    const R safeInner = c.rmin_treat ? -safe_conical<R>(c, point, true) : -inf_length<R>();
+    R safety = vmax(safeZ, vmax(safeOuter, safeInner));
+    if (c.phi_treat) {
+      const R d1 = point.x * R(c.w_normal1x) + point.y * R(c.w_normal1y);
+      const R d2 = point.x * R(c.w_normal2x) + point.y * R(c.w_normal2y);
+      const R safetyPhi = c.lt_pi ? vmax(-d1, -d2) : vmin(-d1, -d2);
+      safety = vmax(safetyPhi, safety);
+    }
+    if (vabs(safety) < R(kTolerance)) safety = R(0);
+    return safety;
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_out(const Data &c, V3<R> point)
+  {
+    const R distz = vabs(point.z) - R(c.dz);
+    const R rsq = point.x * point.x + point.y * point.y;
+    bool outside = distz > R(kConeTolerance);
+    outside = outside || (rsq > plus_tol_sq<R>(radius_at<R>(c, point.z, false), c.outer_tol));
+    if (c.rmin_treat) outside = outside || (rsq < minus_tol_sq<R>(radius_at<R>(c, point.z, true), c.inner_tol));
+    if (outside) return R(-1.0);
+    const R safeZ = R(c.dz) - vabs(point.z);
+    const R safeOuter = safe_conical<R>(c, point, false);
+    const R safeInner = c.rmin_treat ? safe_conical<R>(c, point, true) : inf_length<R>();
+    R safety = vmin(safeZ, vmin(safeOuter, safeInner));
+    if (c.phi_treat) {
+      const R safetyPhi =
+          wedge_safety_to_out<R>(c.w_normal1x, c.w_normal1y, c.w_normal2x, c.w_normal2y, c.lt_pi, point);
+      safety = vmin(safetyPhi, safety);
+    }
+    if (vabs(safety) < R(kTolerance)) safety = R(0);
+    return safety;
+  }
+};
+
+}  // namespace vgm

@@ -46,6 +46,7 @@ VGM_DEV double nonzero(double x) { return x + __builtin_copysign(1.0e-30, x); }
 VGM_DEV float nonzero(float x) { return x + __builtin_copysignf(1.0e-30f, x); }
 // NonZeroAbs(x) = |x| + 1e-30 (VecGeom/base/Global.h:162-167)
 VGM_DEV double nonzero_abs(double x) { return __builtin_fabs(x) + 1.0e-30; }
+VGM_DEV float nonzero_abs(float x) { return __builtin_fabsf(x) + 1.0e-30f; }
 
 template <typename R>
 VGM_DEV R inf_length();
