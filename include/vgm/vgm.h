@@ -65,6 +65,15 @@ int vgm_tube_create(vgm_ctx *ctx, double rmin, double rmax, double dz, double sp
 int vgm_cone_create(vgm_ctx *ctx, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
                     double dphi, int *shape);
 
+/* UnplacedTrd(x1, x2, y1, y2, z): half-lengths in x and y at -z and +z
+ * (VecGeom/volumes/UnplacedTrd.h). */
+int vgm_trd_create(vgm_ctx *ctx, double x1, double x2, double y1, double y2, double z, int *shape);
+
+/* UnplacedTrapezoid(dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4,
+ * alpha2), angles in radians (VecGeom/volumes/UnplacedTrapezoid.h:56-66). */
+int vgm_trapezoid_create(vgm_ctx *ctx, double dz, double theta, double phi, double dy1, double dx1, double dx2,
+                         double alpha1, double dy2, double dx3, double dx4, double alpha2, int *shape);
+
 /* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
  * VecGeom/volumes/UnplacedVolume.h). */
 int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);

@@ -47,6 +47,8 @@ def lib():
             "orc_box_create": [_P, _D, _D, _D],
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
+            "orc_trd_create": [_P, _D, _D, _D, _D, _D],
+            "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
             "orc_pv_create": [_P, _I, _P],
@@ -93,6 +95,13 @@ class Geometry:
 
     def cone(self, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi):
         return self._chk(self._L.orc_cone_create(self.h, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi))
+
+    def trd(self, x1, x2, y1, y2, z):
+        return self._chk(self._L.orc_trd_create(self.h, x1, x2, y1, y2, z))
+
+    def trapezoid(self, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2):
+        return self._chk(self._L.orc_trapezoid_create(self.h, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3,
+                                                      dx4, alpha2))
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

@@ -35,6 +35,17 @@ int orc_cone_create(orc_geom *g, double rmin1, double rmax1, double rmin2, doubl
   return add_shape(g, make_cone(rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi));
 }
 
+int orc_trd_create(orc_geom *g, double x1, double x2, double y1, double y2, double z)
+{
+  return add_shape(g, make_trd(x1, x2, y1, y2, z));
+}
+
+int orc_trapezoid_create(orc_geom *g, double dz, double theta, double phi, double dy1, double dx1, double dx2,
+                         double alpha1, double dy2, double dx3, double dx4, double alpha2)
+{
+  return add_shape(g, make_trapezoid(dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2));
+}
+
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
 {
   auto *G = reinterpret_cast<Geometry *>(g);

@@ -329,3 +329,94 @@ CONE_SOFT_KATS = [  # (shape, method, p, d, wanted) with |result - wanted| <= 0.
     (C1, "DistanceToIn", PPLX, VMX, 20.0),
 ] + [(c, "DistanceToIn", PPLZ, VMZ, KINF) for c in (C1, C2, C3, C4, C5, C7, C8B, C8C)] \
   + [(c, "DistanceToIn", PMIZ, VZ, KINF) for c in (C1, C2, C3, C4, C5, C7, C8B, C8C, C9)]
+
+
+# ---- Trd: test/unit_tests/TestTrd.cpp -------------------------------------------
+TRD1 = (20.0, 20.0, 30.0, 30.0, 40.0)
+TRD2 = (10.0, 30.0, 20.0, 40.0, 40.0)
+TRD3 = (0.14999999999999999, 0.14999999999999999, 24.707000000000001, 24.707000000000001, 22.699999999999999)
+COSA, SINA = 4 / math.sqrt(17.0), 1 / math.sqrt(17.0)
+PONX1, PONY1, PONZ1 = (20.0, 0, 0), (0, 30.0, 0), (0, 0, 40.0)
+PONMX1, PONMY1, PONMZ1 = (-20.0, 0, 0), (0, -30.0, 0), (0, 0, -40.0)
+
+TRD_KATS = [
+    (TRD1, "Inside", P0, None, ("inside", 1)),
+    (TRD1, "Inside", PBZ, None, ("inside", 3)),
+    (TRD1, "Inside", PONX1, None, ("inside", 2)),
+    (TRD1, "Inside", PONY1, None, ("inside", 2)),
+    (TRD1, "Inside", PONZ1, None, ("inside", 2)),
+    (TRD2, "Inside", P0, None, ("inside", 1)),
+    (TRD2, "Inside", PBZ, None, ("inside", 3)),
+    (TRD2, "Inside", PONX1, None, ("inside", 2)),
+    (TRD2, "Inside", PONY1, None, ("inside", 2)),
+    (TRD2, "Inside", PONZ1, None, ("inside", 2)),
+    (TRD1, "SafetyToOut", P0, None, ("eq", 20)),
+    (TRD1, "SafetyToOut", VX, None, ("eq", 19)),
+    (TRD1, "SafetyToOut", VY, None, ("eq", 20)),
+    (TRD1, "SafetyToOut", VZ, None, ("eq", 20)),
+    (TRD2, "SafetyToOut", P0, None, ("eq", 20 * COSA)),
+    (TRD2, "SafetyToOut", VX, None, ("eq", 19 * COSA)),
+    (TRD2, "SafetyToOut", VY, None, ("eq", 20 * COSA)),
+    (TRD2, "SafetyToOut", VZ, None, ("eq", 20 * COSA + SINA)),
+    (TRD1, "DistanceToOut", P0, VX, ("eq", 20)),
+    (TRD1, "DistanceToOut", P0, VMX, ("eq", 20)),
+    (TRD1, "DistanceToOut", P0, VY, ("eq", 30)),
+    (TRD1, "DistanceToOut", P0, VMY, ("eq", 30)),
+    (TRD1, "DistanceToOut", P0, VZ, ("eq", 40)),
+    (TRD1, "DistanceToOut", P0, VMZ, ("eq", 40)),
+    (TRD1, "DistanceToOut", P0, VXY, ("eq", math.sqrt(800.0))),
+    (TRD1, "DistanceToOut", PONX1, VX, ("eq", 0)),
+    (TRD1, "DistanceToOut", PONMX1, VMX, ("eq", 0)),
+    (TRD1, "DistanceToOut", PONY1, VY, ("eq", 0)),
+    (TRD1, "DistanceToOut", PONMY1, VMY, ("eq", 0)),
+    (TRD1, "DistanceToOut", PONZ1, VZ, ("eq", 0)),
+    (TRD1, "DistanceToOut", PONMZ1, VMZ, ("eq", 0)),
+    (TRD1, "SafetyToIn", PBX, None, ("eq", 80)),
+    (TRD1, "SafetyToIn", PBMX, None, ("eq", 80)),
+    (TRD1, "SafetyToIn", PBY, None, ("eq", 70)),
+    (TRD1, "SafetyToIn", PBMY, None, ("eq", 70)),
+    (TRD1, "SafetyToIn", PBZ, None, ("eq", 60)),
+    (TRD1, "SafetyToIn", PBMZ, None, ("eq", 60)),
+    (TRD2, "SafetyToIn", PBX, None, ("eq", 80 * COSA)),
+    (TRD2, "SafetyToIn", PBMX, None, ("eq", 80 * COSA)),
+    (TRD2, "SafetyToIn", PBY, None, ("eq", 70 * COSA)),
+    (TRD2, "SafetyToIn", PBMY, None, ("eq", 70 * COSA)),
+    (TRD2, "SafetyToIn", PBZ, None, ("eq", 60)),
+    (TRD2, "SafetyToIn", PBMZ, None, ("eq", 60)),
+] + [(t, "DistanceToIn", p, d, ("eq", e)) for t in (TRD1, TRD2) for p, d, e in
+     ((PBX, VMX, 80), (PBMX, VX, 80), (PBY, VMY, 70), (PBMY, VY, 70), (PBZ, VMZ, 60), (PBMZ, VZ, 60))] \
+  + [(t, "DistanceToIn", p, VXY, ("inf",)) for t in (TRD1, TRD2) for p in (PBX, PBMX)] + [
+    (TRD3, "DistanceToIn", (0.15000000000000185, -22.048743592955137, 2.4268539333219472),
+     unit((-0.76165597579890043, 0.64364445891356026, -0.074515708658524193)), ("eq", 0.0)),
+]
+
+# ---- Trapezoid: test/unit_tests/TestTrap.cpp (box-like trap1) ---------------------
+TRAP1 = (40.0, 0.0, 0.0, 30.0, 20.0, 20.0, 0.0, 30.0, 20.0, 20.0, 0.0)
+TRAP_KATS = [
+    (TRAP1, "Inside", P0, None, ("inside", 1)),
+    (TRAP1, "Inside", PBZ, None, ("inside", 3)),
+    (TRAP1, "Inside", PONX1, None, ("inside", 2)),
+    (TRAP1, "Inside", PONY1, None, ("inside", 2)),
+    (TRAP1, "Inside", PONZ1, None, ("inside", 2)),
+    (TRAP1, "SafetyToOut", P0, None, ("eq", 20)),
+    (TRAP1, "SafetyToOut", VX, None, ("eq", 19)),
+    (TRAP1, "SafetyToOut", VY, None, ("eq", 20)),
+    (TRAP1, "SafetyToOut", VZ, None, ("eq", 20)),
+    (TRAP1, "SafetyToIn", (100.0, 100.0, 100.0), None, ("eq", 80)),
+    (TRAP1, "SafetyToIn", PBX, None, ("eq", 80)),
+    (TRAP1, "SafetyToIn", PBMX, None, ("eq", 80)),
+    (TRAP1, "SafetyToIn", PBY, None, ("eq", 70)),
+    (TRAP1, "SafetyToIn", PBMY, None, ("eq", 70)),
+    (TRAP1, "SafetyToIn", PBZ, None, ("eq", 60)),
+    (TRAP1, "SafetyToIn", PBMZ, None, ("eq", 60)),
+    (TRAP1, "SafetyToIn", (20.11, 0, 0), None, ("eq", 0.11)),
+    (TRAP1, "SafetyToOut", (20.11, 0, 0), None, ("le", 0.0)),
+    (TRAP1, "SafetyToIn", (19.89, 0, 0), None, ("le", 0.0)),
+    (TRAP1, "SafetyToOut", (19.89, 0, 0), None, ("eq", 0.11)),
+    (TRAP1, "SafetyToOut", (-19.89, 0, 0), None, ("ge", 0.0)),
+    (TRAP1, "SafetyToIn", (-20.11, 0, 0), None, ("eq", 0.11)),
+    (TRAP1, "SafetyToOut", (-20.11, 0, 0), None, ("le", 0.0)),
+    (TRAP1, "SafetyToIn", (0, 30.11, 0), None, ("eq", 0.11)),
+    (TRAP1, "SafetyToOut", (0, 30.11, 0), None, ("le", 0.0)),
+    (TRAP1, "SafetyToOut", (0, 29.89, 0), None, ("eq", 0.11)),
+]

@@ -47,3 +47,12 @@ SHAPES = {
     "cone_exotic": ("cone", (50.0, 60.0, 0.0, 10.0, 50.0, 0.0, 2 * PI)),
     "cone_c5": ("cone", (25.0, 50.0, 75.0, 150.0, 50.0, 0.0, 1.5 * PI)),
 }
+
+SHAPES.update({
+    "trd_box": ("trd", (20.0, 20.0, 30.0, 30.0, 40.0)),
+    "trd_2": ("trd", (10.0, 30.0, 20.0, 40.0, 40.0)),
+    "trd_1": ("trd", (10.0, 30.0, 20.0, 20.0, 40.0)),
+    "trap_box": ("trapezoid", (40.0, 0.0, 0.0, 30.0, 20.0, 20.0, 0.0, 30.0, 20.0, 20.0, 0.0)),
+    "trap_skew": ("trapezoid", (50.0, 0.0, 0.0, 50.0, 50.0, 50.0, PI / 4, 50.0, 50.0, 50.0, PI / 4)),
+    "trap_general": ("trapezoid", (60.0, 0.3, 0.4, 20.0, 15.0, 25.0, 0.2, 30.0, 22.0, 33.0, 0.2)),
+})

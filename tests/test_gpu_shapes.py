@@ -1,4 +1,9 @@
-"""GPU parity: every SOA3D query of every shape vs the FP64 oracle, bit-exact."""
+"""GPU parity: every SOA3D query of every shape vs the FP64 oracle, bit-exact.
+
+Inputs: uniform points in 1.5x the bounding box plus a surface band (entry and
+exit points of random rays, pushed by U(-1e-6, 1e-6) mm along the ray), with
+isotropic directions; local frame and placed with a rotated transformation.
+"""
 import numpy as np
 import pytest
 
@@ -11,55 +16,77 @@ pytestmark = pytest.mark.gpu
 
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut",
            "PlacedDistanceToOut"]
+SHIFT = np.array([3.0, -2.0, 1.5])
 
 
-def _gpu_query(ctx, pv, method, pts, dirs):
+def gpu_query(ctx, pv, method, pts, dirs):
     P = vg.SOA3D(*pts)
     D = vg.SOA3D(*dirs)
-    if method == "Contains":
-        return ctx.contains(pv, P)
-    if method == "Inside":
-        return ctx.inside(pv, P)
-    if method == "SafetyToIn":
-        return ctx.safety_to_in(pv, P)
-    if method == "SafetyToOut":
-        return ctx.safety_to_out(pv, P)
-    if method == "DistanceToIn":
-        return ctx.distance_to_in(pv, P, D)
-    if method == "DistanceToOut":
-        return ctx.distance_to_out(pv, P, D)
-    if method == "PlacedDistanceToOut":
-        return ctx.placed_distance_to_out(pv, P, D)
-    raise ValueError(method)
+    fn = {
+        "Contains": lambda: ctx.contains(pv, P),
+        "Inside": lambda: ctx.inside(pv, P),
+        "SafetyToIn": lambda: ctx.safety_to_in(pv, P),
+        "SafetyToOut": lambda: ctx.safety_to_out(pv, P),
+        "DistanceToIn": lambda: ctx.distance_to_in(pv, P, D),
+        "DistanceToOut": lambda: ctx.distance_to_out(pv, P, D),
+        "PlacedDistanceToOut": lambda: ctx.placed_distance_to_out(pv, P, D),
+    }[method]
+    return fn()
 
 
-def _setup(ctx, name, placed):
-    kind, args = SHAPES[name]
-    geo = oracle.Geometry()
-    s_o = getattr(geo, kind)(*args)
-    s_g = getattr(ctx, kind)(*args)
-    tr = transform12((3.0, -2.0, 1.5)) if placed else None
-    pv_o = geo.place(geo.logical_volume(s_o), tr)
-    pv_g = ctx.place(ctx.logical_volume(s_g), tr)
-    mn, mx = geo.extent(s_o)
-    return geo, pv_o, pv_g, mn, mx
+def surface_band(geo, pv_local, pts, dirs, seed):
+    """Entry/exit points of the rays (oracle distances), jittered by +-1e-6 mm along the ray."""
+    din = geo.query(pv_local, "DistanceToIn", pts, dirs)
+    dout = geo.query(pv_local, "DistanceToOut", pts, dirs)
+    t = np.where((din > 0) & (din < 1e30), din, np.where((dout > 0) & (dout < 1e30), dout, 0.0))
+    jit = (2.0 * sampling.uniform(seed, 9, len(t)) - 1.0) * 1e-6
+    return tuple(pts[k] + (t + jit) * dirs[k] for k in range(3))
+
+
+def make_inputs(geo, pv_local, mn, mx, n, seed=42):
+    half, cen = 0.5 * (mx - mn), 0.5 * (mx + mn)
+    pts = sampling.points_in_box(seed, 1, n, half, 1.5, cen)
+    dirs = sampling.isotropic_directions(seed, 2, n)
+    band = surface_band(geo, pv_local, pts, dirs, seed)
+    dirs2 = sampling.isotropic_directions(seed, 3, n)
+    P = tuple(np.concatenate([pts[k], band[k]]) for k in range(3))
+    D = tuple(np.concatenate([dirs[k], dirs2[k]]) for k in range(3))
+    return P, D
+
+
+def compare(ref, got):
+    ref = np.asarray(ref)
+    got = np.asarray(got)
+    same = (ref == got)
+    if ref.dtype.kind == "f":
+        same |= np.isnan(ref) & np.isnan(got)
+    return np.flatnonzero(~same)
 
 
 @pytest.mark.parametrize("placed", [False, True], ids=["local", "placed"])
 @pytest.mark.parametrize("name", sorted(SHAPES))
 def test_shape_parity(ctx, name, placed):
     print(f"[parity] shape={name} placed={placed}", flush=True)
-    geo, pv_o, pv_g, mn, mx = _setup(ctx, name, placed)
-    n = 200_000
-    half = 0.5 * (mx - mn)
-    center = 0.5 * (mx + mn)
-    pts = sampling.points_in_box(42, 1, n, half, 1.5, center)
-    dirs = sampling.isotropic_directions(42, 2, n)
+    kind, args = SHAPES[name]
+    geo = oracle.Geometry()
+    s_o = getattr(geo, kind)(*args)
+    s_g = getattr(ctx, kind)(*args)
+    lv_o, lv_g = geo.logical_volume(s_o), ctx.logical_volume(s_g)
+    pv_local = geo.place(lv_o)
+    tr = transform12(SHIFT) if placed else None
+    pv_o, pv_g = geo.place(lv_o, tr), ctx.place(lv_g, tr)
+    mn, mx = geo.extent(s_o)
+    pts, dirs = make_inputs(geo, pv_local, mn, mx, 100_000)
     if placed:
-        # put the sample in the master frame around the placed solid
-        pts = tuple(p + t for p, t in zip(pts, (3.0, -2.0, 1.5)))
+        # the same local points expressed in the master frame
+        T = transform12(SHIFT)
+        R = T[3:].reshape(3, 3)
+        loc = np.stack(pts)
+        # local = R^T-convention of Transformation3D: local_i = sum_j (m_j - t_j) r[3j+i] -> m = t + R l
+        pts = tuple(np.ascontiguousarray(c) for c in (SHIFT[:, None] + R @ loc))
+        dirs = tuple(np.ascontiguousarray(c) for c in (R @ np.stack(dirs)))
     for m in METHODS:
         ref = geo.query(pv_o, m, pts, dirs if "Distance" in m else None)
-        got = _gpu_query(ctx, pv_g, m, pts, dirs)
-        bad = np.flatnonzero(~((ref == got) | (np.isnan(ref.astype(np.float64)) & np.isnan(got.astype(np.float64)))))
+        got = gpu_query(ctx, pv_g, m, pts, dirs)
+        bad = compare(ref, got)
         assert bad.size == 0, (m, bad.size, bad[:5], ref[bad[:5]], got[bad[:5]])

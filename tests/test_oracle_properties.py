@@ -1,0 +1,83 @@
+"""Oracle properties fixed by the mathematics (CPU).
+
+* Degenerate shapes reduce to a simpler one: a trd or trapezoid with parallel
+  faces is a box, a cone with equal radii is a tube.
+* ShapeTester conventions (test/shape_tester/ShapeTester.cpp:859-1229):
+  SafetyToIn <= DistanceToIn for outside points, SafetyToOut <= DistanceToOut
+  for inside points, and the point moved by a finite distance lies on the
+  surface.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from vecgeom_amd import sampling
+from tests.shapes_spec import SHAPES
+
+N = 20000
+
+
+def _pv(geo, kind, args):
+    return geo.place(geo.logical_volume(getattr(geo, kind)(*args)))
+
+
+def _inputs(geo, kind, args, seed=7):
+    s = getattr(geo, kind)(*args)
+    mn, mx = geo.extent(s)
+    half, cen = 0.5 * (mx - mn), 0.5 * (mx + mn)
+    pts = sampling.points_in_box(seed, 1, N, half, 1.3, cen)
+    dirs = sampling.isotropic_directions(seed, 2, N)
+    return pts, dirs
+
+
+EQUIV = [
+    (("trd", (20.0, 20.0, 30.0, 30.0, 40.0)), ("box", (20.0, 30.0, 40.0))),
+    (("trapezoid", (40.0, 0.0, 0.0, 30.0, 20.0, 20.0, 0.0, 30.0, 20.0, 20.0, 0.0)), ("box", (20.0, 30.0, 40.0))),
+    (("cone", (5.0, 20.0, 5.0, 20.0, 30.0, 0.0, 2 * np.pi)), ("tube", (5.0, 20.0, 30.0, 0.0, 2 * np.pi))),
+    (("cone", (0.0, 20.0, 0.0, 20.0, 30.0, 0.3, 2.0)), ("tube", (0.0, 20.0, 30.0, 0.3, 2.0))),
+]
+
+
+@pytest.mark.parametrize("a,b", EQUIV, ids=[f"{a[0]}=={b[0]}" for a, b in EQUIV])
+def test_degenerate_equivalence(a, b):
+    geo = oracle.Geometry()
+    pa, pb = _pv(geo, *a), _pv(geo, *b)
+    pts, dirs = _inputs(geo, *b)
+    ia, ib = geo.query(pa, "Inside", pts), geo.query(pb, "Inside", pts)
+    # classification agrees away from the tolerance bands of either shape
+    firm = (ia != 2) & (ib != 2)
+    assert np.array_equal(ia[firm], ib[firm])
+    for m in ("DistanceToIn", "DistanceToOut"):
+        da = geo.query(pa, m, pts, dirs)
+        db = geo.query(pb, m, pts, dirs)
+        side = (ia == 3) if m == "DistanceToIn" else (ia == 1)
+        sel = side & firm
+        fin = sel & (da < 1e30) & (db < 1e30)
+        assert np.array_equal(da[sel] >= 1e30, db[sel] >= 1e30), m
+        assert np.max(np.abs(da[fin] - db[fin]), initial=0.0) < 1e-9, m
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_safety_bounds_and_surface_hits(name):
+    kind, args = SHAPES[name]
+    geo = oracle.Geometry()
+    pv = _pv(geo, kind, args)
+    pts, dirs = _inputs(geo, kind, args)
+    ins = geo.query(pv, "Inside", pts)
+    out, inn = ins == 3, ins == 1
+    din = geo.query(pv, "DistanceToIn", pts, dirs)
+    sin = geo.query(pv, "SafetyToIn", pts)
+    assert np.all(sin[out] <= din[out] + 1e-9)
+    dout = geo.query(pv, "DistanceToOut", pts, dirs)
+    sout = geo.query(pv, "SafetyToOut", pts)
+    assert np.all(sout[inn] <= dout[inn] + 1e-9)
+    assert np.all(dout[inn] >= 0)
+    # exit points lie on the surface
+    hit = inn & (dout < 1e30)
+    q = tuple(pts[k][hit] + dout[hit] * dirs[k][hit] for k in range(3))
+    assert np.all(geo.query(pv, "Inside", q) == 2)
+    # entry points lie on the surface
+    hit = out & (din < 1e30)
+    q = tuple(pts[k][hit] + din[hit] * dirs[k][hit] for k in range(3))
+    res = geo.query(pv, "Inside", q)
+    assert np.mean(res == 2) > 0.999, np.unique(res, return_counts=True)

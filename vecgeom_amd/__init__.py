@@ -64,6 +64,8 @@ def _declare(L):
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_trapezoid_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_shape_extent": [_P, _I, _P, _P],
         "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
         "vgm_pv_create": [_P, _I, _P, ctypes.POINTER(_I)],
@@ -163,6 +165,12 @@ class Context:
 
     def cone(self, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi) -> int:
         return self._new(self._L.vgm_cone_create, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi)
+
+    def trd(self, x1, x2, y1, y2, z) -> int:
+        return self._new(self._L.vgm_trd_create, x1, x2, y1, y2, z)
+
+    def trapezoid(self, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2) -> int:
+        return self._new(self._L.vgm_trapezoid_create, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2)
 
     def extent(self, shape: int):
         mn = np.zeros(3)

@@ -276,6 +276,28 @@ int vgm_cone_create(vgm_ctx *c, double rmin1, double rmax1, double rmin2, double
   return VGM_OK;
 }
 
+int vgm_trd_create(vgm_ctx *c, double x1, double x2, double y1, double y2, double z, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_trd(c->host, x1, x2, y1, y2, z, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
+int vgm_trapezoid_create(vgm_ctx *c, double dz, double theta, double phi, double dy1, double dx1, double dx2,
+                         double alpha1, double dy2, double dx3, double dx4, double alpha2, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  const double p[11] = {dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2};
+  int id = build_trapezoid(c->host, p, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
 {
   if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");

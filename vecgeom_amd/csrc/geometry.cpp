@@ -5,6 +5,8 @@
 #include "shapes/box_data.h"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
+#include "shapes/trd_data.h"
+#include "shapes/trapezoid_data.h"
 
 namespace vgm {
 
@@ -181,6 +183,82 @@ int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double
   const double r = c.orig_rmax1 > c.orig_rmax2 ? c.orig_rmax1 : c.orig_rmax2;
   const double lo[3] = {-r, -r, -dz}, hi[3] = {r, r, dz};
   return g.add_shape(kCone, &c, sizeof(c) / sizeof(double), 0, 0, lo, hi);
+}
+
+// UnplacedTrd(x1, x2, y1, y2, z) with TrdStruct::CalculateCached
+// (VecGeom/volumes/TrdStruct.h); fToleranceY uses the x extents as there.
+int build_trd(HostGeometry &g, double x1, double x2, double y1, double y2, double z, std::string &err)
+{
+  if (!(x1 >= 0 && x2 >= 0 && y1 >= 0 && y2 >= 0 && z > 0 && (x1 > 0 || x2 > 0) && (y1 > 0 || y2 > 0))) {
+    err = "trd needs non-negative half-lengths and z > 0";
+    return -1;
+  }
+  TrdData t{};
+  t.dx1 = x1;
+  t.dx2 = x2;
+  t.dy1 = y1;
+  t.dy2 = y2;
+  t.dz = z;
+  t.x2minusx1 = t.dx2 - t.dx1;
+  t.y2minusy1 = t.dy2 - t.dy1;
+  t.half_x1plusx2 = 0.5 * (t.dx1 + t.dx2);
+  t.half_y1plusy2 = 0.5 * (t.dy1 + t.dy2);
+  t.fx = 0.5 * (t.dx1 - t.dx2) / t.dz;
+  t.fy = 0.5 * (t.dy1 - t.dy2) / t.dz;
+  t.calfx = 1. / std::sqrt(1.0 + t.fx * t.fx);
+  t.calfy = 1. / std::sqrt(1.0 + t.fy * t.fy);
+  t.tolx = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
+  t.toly = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
+  t.vary_y = t.dy1 != t.dy2;
+  const double x = x1 > x2 ? x1 : x2, y = y1 > y2 ? y1 : y2;
+  const double lo[3] = {-x, -y, -z}, hi[3] = {x, y, z};
+  return g.add_shape(kTrd, &t, sizeof(t) / sizeof(double), 0, 0, lo, hi);
+}
+
+// UnplacedTrapezoid(dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2):
+// eight corners (source/UnplacedTrapezoid.cpp:447-465) and the four side
+// planes through them (MakeAPlane/MakePlanes, :560-660).
+int build_trapezoid(HostGeometry &g, const double p[11], std::string &err)
+{
+  const double dz = p[0], theta = p[1], phi = p[2], dy1 = p[3], dx1 = p[4], dx2 = p[5], dy2 = p[7], dx3 = p[8],
+               dx4 = p[9];
+  if (!(dz > 0 && dy1 > 0 && dx1 > 0 && dx2 > 0 && dy2 > 0 && dx3 > 0 && dx4 > 0)) {
+    err = "trapezoid needs positive half-lengths";
+    return -1;
+  }
+  const double tanA1 = std::tan(p[6]), tanA2 = std::tan(p[10]);
+  const double tthetaCphi = std::tan(theta) * std::cos(phi);
+  const double tthetaSphi = std::tan(theta) * std::sin(phi);
+  const double dxdyDy1 = tanA1 * dy1, dxdyDy2 = tanA2 * dy2;
+  const double dxdzDz = tthetaCphi * dz, dydzDz = tthetaSphi * dz;
+  double pt[8][3] = {{-dxdzDz - dxdyDy1 - dx1, -dydzDz - dy1, -dz}, {-dxdzDz - dxdyDy1 + dx1, -dydzDz - dy1, -dz},
+                     {-dxdzDz + dxdyDy1 - dx2, -dydzDz + dy1, -dz}, {-dxdzDz + dxdyDy1 + dx2, -dydzDz + dy1, -dz},
+                     {+dxdzDz - dxdyDy2 - dx3, +dydzDz - dy2, +dz}, {+dxdzDz - dxdyDy2 + dx3, +dydzDz - dy2, +dz},
+                     {+dxdzDz + dxdyDy2 - dx4, +dydzDz + dy2, +dz}, {+dxdzDz + dxdyDy2 + dx4, +dydzDz + dy2, +dz}};
+  TrapezoidData t{};
+  t.dz = dz;
+  const int quads[4][4] = {{0, 1, 5, 4}, {2, 6, 7, 3}, {0, 4, 6, 2}, {1, 3, 7, 5}};
+  for (int i = 0; i < 4; ++i) {
+    const double *p1 = pt[quads[i][0]], *p2 = pt[quads[i][1]], *p3 = pt[quads[i][2]], *p4 = pt[quads[i][3]];
+    const double u[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+    const double v[3] = {p4[0] - p2[0], p4[1] - p2[1], p4[2] - p2[2]};
+    double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const double inv = 1. / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; ++k) n[k] = n[k] * inv;
+    double cen[3];
+    for (int k = 0; k < 3; ++k) cen[k] = 0.25 * (p1[k] + p2[k] + p3[k] + p4[k]);
+    t.a[i] = n[0];
+    t.b[i] = n[1];
+    t.c[i] = n[2];
+    t.d[i] = -(n[0] * cen[0] + n[1] * cen[1] + n[2] * cen[2]);
+  }
+  double lo[3] = {pt[0][0], pt[0][1], pt[0][2]}, hi[3] = {pt[0][0], pt[0][1], pt[0][2]};
+  for (int i = 1; i < 8; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = pt[i][k] < lo[k] ? pt[i][k] : lo[k];
+      hi[k] = pt[i][k] > hi[k] ? pt[i][k] : hi[k];
+    }
+  return g.add_shape(kTrapezoid, &t, sizeof(t) / sizeof(double), 0, 0, lo, hi);
 }
 
 }  // namespace vgm

@@ -4,6 +4,8 @@
 #include "box.hpp"
 #include "tube.hpp"
 #include "cone.hpp"
+#include "trd.hpp"
+#include "trapezoid.hpp"
 
 namespace vgm {
 
@@ -21,6 +23,12 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kCone:
     f.template operator()<ConeShape>();
+    return true;
+  case kTrd:
+    f.template operator()<TrdShape>();
+    return true;
+  case kTrapezoid:
+    f.template operator()<TrapezoidShape>();
     return true;
   default:
     return false;

@@ -1,0 +1,131 @@
+// Trapezoid kernels with PlaneShell<4> side planes, scalar semantics
+// (VecGeom/volumes/kernel/TrapezoidImplementation.h:98-395,
+//  VecGeom/base/PlaneShell.h:130-290).
+#pragma once
+#include "../vgm_math.hpp"
+#include "trapezoid_data.h"
+
+namespace vgm {
+
+struct TrapezoidShape {
+  using Data = TrapezoidData;
+
+  template <typename R>
+  VGM_DEV static R pdist(const Data &t, int i, V3<R> p)
+  {
+    return R(t.a[i]) * p.x + R(t.b[i]) * p.y + R(t.c[i]) * p.z + R(t.d[i]);
+  }
+  template <typename R>
+  VGM_DEV static R proj(const Data &t, int i, V3<R> d)
+  {
+    return R(t.a[i]) * d.x + R(t.b[i]) * d.y + R(t.c[i]) * d.z;
+  }
+
+  template <typename R, bool ForInside>
+  VGM_DEV static void generic(const Data &t, V3<R> p, bool &cin, bool &cout)
+  {
+    cout = vabs(p.z) > (ForInside ? R(t.dz) + R(kHalfTolerance) : R(t.dz));
+    if (ForInside) cin = vabs(p.z) < R(t.dz) - R(kHalfTolerance);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const R dist = pdist<R>(t, i, p);
+      cout = cout || (dist > (ForInside ? R(kHalfTolerance) : R(0)));
+      if (ForInside) cin = cin && (dist < R(-kHalfTolerance));
+    }
+  }
+
+  template <typename R>
+  VGM_DEV static bool contains(const Data &t, V3<R> p)
+  {
+    bool ci = false, co = false;
+    generic<R, false>(t, p, ci, co);
+    return !co;
+  }
+
+  template <typename R>
+  VGM_DEV static int inside(const Data &t, V3<R> p)
+  {
+    bool ci = false, co = false;
+    generic<R, true>(t, p, ci, co);
+    return ci ? kInsideV : (co ? kOutsideV : kSurfaceV);
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &t, V3<R> point, V3<R> dir, R /*step_max*/)
+  {
+    const R signZdir = vsign(dir.z);
+    const R max = signZdir * R(t.dz) - point.z;
+    if (signZdir * max < R(kHalfTolerance)) return inf_length<R>();
+    const R invdir = R(1.0) / nonzero(dir.z);
+    R smax = max * invdir;
+    R smin = -(signZdir * R(t.dz) + point.z) * invdir;
+    R pd[4], pj[4], vd[4];
+    bool done = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      pd[i] = pdist<R>(t, i, point);
+      pj[i] = proj<R>(t, i, dir);
+      vd[i] = -pd[i] / nonzero(pj[i]);
+      done = done || (pd[i] > R(kHalfTolerance) && pj[i] >= R(0));
+      done = done || (pd[i] > R(-kHalfTolerance) && pj[i] > R(0));
+    }
+    if (done) return inf_length<R>();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool posPoint = pd[i] > R(-kHalfTolerance);
+      const bool posDir = pj[i] > R(0);
+      const bool fromIn = !posPoint && posDir;
+      const bool fromOut = posPoint && !posDir;
+      if ((fromIn && vd[i] < smin) || (fromOut && vd[i] > smax)) return inf_length<R>();
+      if (fromOut && vd[i] > smin) smin = vd[i];
+      if (fromIn && vd[i] < smax) smax = vd[i];
+    }
+    return (smin <= smax) ? smin : inf_length<R>();
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_out(const Data &t, V3<R> point, V3<R> dir, R /*step_max*/)
+  {
+    if (vabs(point.z) > R(t.dz) + R(kHalfTolerance)) return R(-1.0);
+    R distance = inf_length<R>();
+    const R distz = (vsign(dir.z) * R(t.dz) - point.z) / nonzero(dir.z);
+    if (dir.z != R(0)) distance = distz;
+    R distOut = inf_length<R>();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const R pd = pdist<R>(t, i, point);
+      const R pj = proj<R>(t, i, dir);
+      const R vd = -pd / nonzero(pj);
+      if (pd > R(kHalfTolerance)) distOut = R(-1.);
+      if (pj > R(0) && vd < distOut) distOut = vd;
+    }
+    if (distOut < distance) distance = distOut;
+    return distance;
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_in(const Data &t, V3<R> point)
+  {
+    R safety = vabs(point.z) - R(t.dz);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const R d = pdist<R>(t, i, point);
+      if (d > safety) safety = d;
+    }
+    return safety;
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_out(const Data &t, V3<R> point)
+  {
+    R safety = R(t.dz) - vabs(point.z);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const R d = -(pdist<R>(t, i, point));
+      if (d < safety) safety = d;
+    }
+    return safety;
+  }
+};
+
+}  // namespace vgm
