@@ -74,6 +74,11 @@ int vgm_trd_create(vgm_ctx *ctx, double x1, double x2, double y1, double y2, dou
 int vgm_trapezoid_create(vgm_ctx *ctx, double dz, double theta, double phi, double dy1, double dx1, double dx2,
                          double alpha1, double dy2, double dx3, double dx4, double alpha2, int *shape);
 
+/* UnplacedPolycone(phistart, deltaphi, Nz, z, rmin, rmax): nz planes with
+ * their inner and outer radii (VecGeom/volumes/UnplacedPolycone.h:79-89). */
+int vgm_polycone_create(vgm_ctx *ctx, double sphi, double dphi, int nz, const double *z, const double *rmin,
+                        const double *rmax, int *shape);
+
 /* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
  * VecGeom/volumes/UnplacedVolume.h). */
 int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);

@@ -48,6 +48,7 @@ def lib():
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
+            "orc_polycone_create": [_P, _D, _D, _I, _P, _P, _P],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
@@ -102,6 +103,10 @@ class Geometry:
     def trapezoid(self, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2):
         return self._chk(self._L.orc_trapezoid_create(self.h, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3,
                                                       dx4, alpha2))
+
+    def polycone(self, sphi, dphi, z, rmin, rmax):
+        z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
+        return self._chk(self._L.orc_polycone_create(self.h, sphi, dphi, len(z), _p(z), _p(rmin), _p(rmax)))
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

@@ -46,6 +46,12 @@ int orc_trapezoid_create(orc_geom *g, double dz, double theta, double phi, doubl
   return add_shape(g, make_trapezoid(dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2));
 }
 
+int orc_polycone_create(orc_geom *g, double sphi, double dphi, int nz, const double *z, const double *rmin,
+                        const double *rmax)
+{
+  return add_shape(g, make_polycone(sphi, dphi, nz, z, rmin, rmax));
+}
+
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
 {
   auto *G = reinterpret_cast<Geometry *>(g);

@@ -32,6 +32,8 @@ int orc_cone_create(orc_geom *g, double rmin1, double rmax1, double rmin2, doubl
 int orc_trd_create(orc_geom *g, double x1, double x2, double y1, double y2, double z);
 int orc_trapezoid_create(orc_geom *g, double dz, double theta, double phi, double dy1, double dx1, double dx2,
                          double alpha1, double dy2, double dx3, double dx4, double alpha2);
+int orc_polycone_create(orc_geom *g, double sphi, double dphi, int nz, const double *z, const double *rmin,
+                        const double *rmax);
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx);
 int orc_lv_create(orc_geom *g, int shape);
 int orc_pv_create(orc_geom *g, int lv, const double *tr);

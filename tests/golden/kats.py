@@ -48,6 +48,10 @@ def check(result, expect):
         return float(result) >= expect[1]
     if kind == "inf":
         return float(result) >= KINF
+    if kind == "exact":
+        return float(result) == float(expect[1])
+    if kind == "near":
+        return abs(float(result) - expect[1]) < expect[2]
     raise ValueError(kind)
 
 
@@ -419,4 +423,74 @@ TRAP_KATS = [
     (TRAP1, "SafetyToIn", (0, 30.11, 0), None, ("eq", 0.11)),
     (TRAP1, "SafetyToOut", (0, 30.11, 0), None, ("le", 0.0)),
     (TRAP1, "SafetyToOut", (0, 29.89, 0), None, ("eq", 0.11)),
+]
+
+
+# ---- Polycone: test/unit_tests/TestPolycone.cpp -----------------------------------
+# shapes are (sphi, dphi, z, rmin, rmax); poly1 asserts at :57-88 are exact (==).
+POLY1 = (0.0, 360.0 * PI / 180.0, (-1.0, -0.5, 0.5, 2.0), (0.1, 0.0, 0.0, 0.4), (1.0, 2.0, 2.0, 1.5))
+PSIMPLE = (0.0, 360.0 * PI / 180.0, (-10.0, 0.0, 10.0), (0.0, 0.0, 0.0), (70.0, 70.0, 80.0))
+MYPCONE = (-10.0 * PI / 180.0, 10.0 * PI / 180.0, (-20.0, -10.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0),
+           (30.0, 30.0, 0.0, 0.0, 0.0, 0.0, 40.0, 40.0), (70.0, 70.0, 70.0, 40.0, 40.0, 80.0, 80.0, 60.0))
+
+
+def _exact(v):
+    return ("exact", v)
+
+
+POLYCONE_KATS = [
+    (POLY1, "Contains", (0.0, 0.0, 0.0), None, _exact(1)),
+    (POLY1, "Contains", (0.0, 0.0, -2.0), None, _exact(0)),
+    (POLY1, "Contains", (0.0, 0.0, -0.8), None, _exact(0)),
+    (POLY1, "Contains", (0.0, 0.0, -1.8), None, _exact(0)),
+    (POLY1, "Contains", (0.0, 0.0, 10.0), None, _exact(0)),
+    (POLY1, "Contains", (0.0, 0.0, 1.8), None, _exact(0)),
+    (POLY1, "DistanceToIn", (0.0, 0.0, -3.0), (0.0, 0.0, 1.0), _exact(2.5)),
+    (POLY1, "DistanceToIn", (0.0, 0.0, -2.0), (0.0, 0.0, -1.0), _exact(KINF)),
+    (POLY1, "DistanceToIn", (0.0, 0.0, 3.0), (0.0, 0.0, -1.0), _exact(2.5)),
+    (POLY1, "DistanceToIn", (0.0, 0.0, 3.0), (0.0, 0.0, 1.0), _exact(KINF)),
+    (POLY1, "DistanceToIn", (3.0, 0.0, 0.0), (-1.0, 0.0, 0.0), _exact(1.0)),
+    (POLY1, "DistanceToIn", (0.0, 0.0, 1.9999999), (1.0, 0.0, 0.0), ("near", 0.4, 1e-6)),
+    (POLY1, "SafetyToIn", (0.0, 0.0, -3.0), None, _exact(2.0)),
+    (POLY1, "SafetyToIn", (0.5, 0.0, -1.0), None, _exact(0.0)),
+    (POLY1, "SafetyToIn", (0.0, 0.0, 3.0), None, _exact(1.0)),
+    (POLY1, "SafetyToIn", (2.0, 0.0, 0.1), None, _exact(0.0)),
+    (POLY1, "SafetyToOut", (0.0, 0.0, 0.0), None, _exact(0.5)),
+    (POLY1, "SafetyToOut", (0.0, 0.0, 0.5), None, _exact(0.0)),
+    (POLY1, "SafetyToOut", (1.9, 0.0, 0.0), None, ("near", 0.1, 1e-6)),
+    (POLY1, "SafetyToOut", (0.2, 0.0, -1.0), None, _exact(0.0)),
+    (POLY1, "SafetyToOut", (1.4, 0.0, 2.0), None, _exact(0.0)),
+    (POLY1, "DistanceToOut", (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), _exact(0.5)),
+    (POLY1, "DistanceToOut", (0.0, 0.0, 0.0), (0.0, 0.0, -1.0), _exact(0.5)),
+    (POLY1, "DistanceToOut", (2.0, 0.0, 0.0), (1.0, 0.0, 0.0), _exact(0.0)),
+    (POLY1, "DistanceToOut", (2.0, 0.0, 0.0), (-1.0, 0.0, 0.0), _exact(4.0)),
+    (POLY1, "DistanceToOut", (1.0, 0.0, 2.0), (0.0, 0.0, 1.0), _exact(0.0)),
+    (POLY1, "DistanceToOut", (0.5, 0.0, -1.0), (0.0, 0.0, -1.0), _exact(0.0)),
+    (POLY1, "DistanceToOut", (0.5, 0.0, -1.0), (0.0, 0.0, 1.0), _exact(3.0)),
+    # :158-171 Inside of the simple tube+cone
+    (PSIMPLE, "Inside", (0.0, 0.0, 0.0), None, ("inside", 1)),
+    (PSIMPLE, "Inside", (0.0, 0.0, 100.0), None, ("inside", 3)),
+    (PSIMPLE, "Inside", (100.0, 0.0, 0.0), None, ("inside", 3)),
+    (PSIMPLE, "Inside", (0.0, 100.0, 0.0), None, ("inside", 3)),
+    (PSIMPLE, "Inside", (70.0, 0.0, -5.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (0.0, 70.0, -5.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (70.0, 0.0, 10.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (-70.0, 0.0, -5.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (0.0, -70.0, -5.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (0.0, 0.0, -10.0), None, ("inside", 2)),
+    (PSIMPLE, "Inside", (0.0, 25.0, 0.0), None, ("inside", 1)),
+    (PSIMPLE, "Inside", (4.0, 25.0, 0.0), None, ("inside", 1)),
+    # :209-216 SafetyToOut, :236-251 DistanceToIn
+    (PSIMPLE, "SafetyToOut", (5.0, 5.0, -5.0), None, ("eq", 5)),
+    (PSIMPLE, "SafetyToOut", (5.0, 5.0, 7.0), None, ("eq", 3)),
+    (PSIMPLE, "SafetyToOut", (69.0, 0.0, -5.0), None, ("eq", 1)),
+    (PSIMPLE, "SafetyToOut", (-3.0, -3.0, 8.0), None, ("eq", 2)),
+    (PSIMPLE, "DistanceToIn", (100.0, 0.0, -1.0), VMX, ("eq", 30)),
+    (PSIMPLE, "DistanceToIn", (-100.0, 0.0, -1.0), VX, ("eq", 30)),
+    (PSIMPLE, "DistanceToIn", (0.0, 100.0, -5.0), VMY, ("eq", 30)),
+    (PSIMPLE, "DistanceToIn", (0.0, -100.0, -5.0), VY, ("eq", 30)),
+    (PSIMPLE, "DistanceToIn", (0.0, 0.0, 100.0), VMZ, ("eq", 90)),
+    (PSIMPLE, "DistanceToIn", (0.0, 0.0, -100.0), VZ, ("eq", 90)),
+    (PSIMPLE, "DistanceToIn", (100.0, 0.0, 0.0), VXY, ("inf",)),
+    (PSIMPLE, "DistanceToIn", (-100.0, 0.0, 0.0), VMXY, ("inf",)),
 ]

@@ -56,3 +56,17 @@ SHAPES.update({
     "trap_skew": ("trapezoid", (50.0, 0.0, 0.0, 50.0, 50.0, 50.0, PI / 4, 50.0, 50.0, 50.0, PI / 4)),
     "trap_general": ("trapezoid", (60.0, 0.3, 0.4, 20.0, 15.0, 25.0, 0.2, 30.0, 22.0, 33.0, 0.2)),
 })
+
+SHAPES.update({
+    "pcon_poly1": ("polycone", (0.0, 2 * PI, (-1.0, -0.5, 0.5, 2.0), (0.1, 0.0, 0.0, 0.4), (1.0, 2.0, 2.0, 1.5))),
+    "pcon_simple": ("polycone", (0.0, 2 * PI, (-10.0, 0.0, 10.0), (0.0, 0.0, 0.0), (70.0, 70.0, 80.0))),
+    "pcon_mypcone": ("polycone", (-10.0 * PI / 180.0, 10.0 * PI / 180.0,
+                                  (-20.0, -10.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0),
+                                  (30.0, 30.0, 0.0, 0.0, 0.0, 0.0, 40.0, 40.0),
+                                  (70.0, 70.0, 70.0, 40.0, 40.0, 80.0, 80.0, 60.0))),
+    "pcon_trak": ("polycone", (0.0, 2 * PI, (-2935., -1899., -1899., -1899., 1899., 1899., 1899., 2935.),
+                               (74., 34., 31., 31., 31., 31., 34., 74.), (1233.,) * 8)),
+    "pcon_10planes": ("polycone", (0.3, 4.5, tuple(float(z) for z in range(-50, 50, 10)),
+                                   (5.0, 6.0, 4.0, 8.0, 8.0, 3.0, 2.0, 5.0, 6.0, 4.0),
+                                   (20.0, 25.0, 22.0, 30.0, 28.0, 26.0, 35.0, 30.0, 24.0, 20.0))),
+})

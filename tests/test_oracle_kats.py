@@ -7,7 +7,7 @@ from tests.golden import kats
 
 CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_KATS]
          + [("cone", c) for c in kats.CONE_KATS] + [("trd", c) for c in kats.TRD_KATS]
-         + [("trapezoid", c) for c in kats.TRAP_KATS])
+         + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS])
 
 
 def _run(geo, kind, params, method, p, d):

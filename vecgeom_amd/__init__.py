@@ -65,6 +65,7 @@ def _declare(L):
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_polycone_create": [_P, _D, _D, _I, _P, _P, _P, ctypes.POINTER(_I)],
         "vgm_trapezoid_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_shape_extent": [_P, _I, _P, _P],
         "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
@@ -171,6 +172,12 @@ class Context:
 
     def trapezoid(self, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2) -> int:
         return self._new(self._L.vgm_trapezoid_create, dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2)
+
+    def polycone(self, sphi, dphi, z, rmin, rmax) -> int:
+        z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
+        if not (len(z) == len(rmin) == len(rmax)):
+            raise ValueError("z, rmin and rmax differ in length")
+        return self._new(self._L.vgm_polycone_create, sphi, dphi, len(z), _ptr(z), _ptr(rmin), _ptr(rmax))
 
     def extent(self, shape: int):
         mn = np.zeros(3)

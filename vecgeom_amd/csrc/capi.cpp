@@ -298,6 +298,17 @@ int vgm_trapezoid_create(vgm_ctx *c, double dz, double theta, double phi, double
   return VGM_OK;
 }
 
+int vgm_polycone_create(vgm_ctx *c, double sphi, double dphi, int nz, const double *z, const double *rmin,
+                        const double *rmax, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_polycone(c->host, sphi, dphi, nz, z, rmin, rmax, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
 {
   if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");

@@ -92,16 +92,16 @@ static double nonzero_h(double x) { return x + std::copysign(1.0e-30, x); }
 // UnplacedCone(rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi): ConeStruct
 // constructor order (VecGeom/volumes/ConeStruct.h): SetAndCheckDPhiAngle,
 // SetAndCheckSPhiAngle, CalculateCached.  The wedge keeps the raw angles.
-int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
-               double dphi, std::string &err)
+bool make_cone_data(ConeData &c, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+                    double dphi, std::string &err)
 {
   if (!(dz > 0 && dphi > 0 && rmin1 >= 0 && rmin2 >= 0 && rmax1 >= rmin1 && rmax2 >= rmin2 &&
         (rmax1 > 0 || rmax2 > 0))) {
     err = "cone needs 0 <= rmin <= rmax, dz > 0, dphi > 0";
-    return -1;
+    return false;
   }
   const double kPi = 3.14159265358979323846, kTwoPi = 2. * kPi, kConeTol = 1e-7, kHalfTol = 0.5e-9;
-  ConeData c{};
+  c = ConeData{};
   c.rmin1 = rmin1;
   c.rmax1 = rmax1;
   c.rmin2 = rmin2;
@@ -180,6 +180,14 @@ int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double
   c.rmin_treat = c.rmin1 > 0 || c.rmin2 > 0;
   c.le_pi = fD <= M_PI;
   c.lt_pi = fD < kPi;
+  return true;
+}
+
+int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+               double dphi, std::string &err)
+{
+  ConeData c;
+  if (!make_cone_data(c, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi, err)) return -1;
   const double r = c.orig_rmax1 > c.orig_rmax2 ? c.orig_rmax1 : c.orig_rmax2;
   const double lo[3] = {-r, -r, -dz}, hi[3] = {r, r, dz};
   return g.add_shape(kCone, &c, sizeof(c) / sizeof(double), 0, 0, lo, hi);

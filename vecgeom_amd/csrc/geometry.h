@@ -23,12 +23,19 @@ struct HostGeometry {
                 const double bmax[3]);
 };
 
+struct ConeData;
+// Cone parameter derivation shared by the cone and polycone builders.
+bool make_cone_data(ConeData &c, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
+                    double dphi, std::string &err);
+
 // Shape builders.  Return the new shape id, or -1 with `err` set.
 int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err);
 int build_cone(HostGeometry &g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
                double dphi, std::string &err);
 int build_trd(HostGeometry &g, double x1, double x2, double y1, double y2, double z, std::string &err);
 int build_trapezoid(HostGeometry &g, const double p[11], std::string &err);
+int build_polycone(HostGeometry &g, double sphi, double dphi, int nz, const double *z, const double *rmin,
+                   const double *rmax, std::string &err);
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err);
 
 }  // namespace vgm

@@ -6,6 +6,7 @@
 #include "cone.hpp"
 #include "trd.hpp"
 #include "trapezoid.hpp"
+#include "polycone.hpp"
 
 namespace vgm {
 
@@ -29,6 +30,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kTrapezoid:
     f.template operator()<TrapezoidShape>();
+    return true;
+  case kPolycone:
+    f.template operator()<PolyconeShape>();
     return true;
   default:
     return false;
