@@ -1,9 +1,17 @@
-"""Benchmark driver (one JSON line on stdout).
+"""Benchmark driver: track-steps/sec of the batched heterogeneous transport.
 
-Workload: box SOA3D queries (half-lengths 10,20,30 mm; 1,000,000 points uniform
-in the box scaled x1.5; isotropic directions, seed 42; stepMax 1e30).  One step
-= Inside + DistanceToIn + DistanceToOut (inside points) + SafetyToIn +
-SafetyToOut over the batch, inputs resident in HBM.
+Workload (synthetic, built by vecgeom_amd.workloads): 8 solids per shape
+family with log-uniform parameters, per-solid track counts log-uniform in
+[1k, 4M] rescaled to --tracks (default 40M), points uniform in 1.3x each
+solid's bounding box, isotropic directions, seed 42.  One bench step = one
+persistent launch advancing every track by --track-steps (default 50)
+track-steps.  Inputs are resident in HBM when timing starts.
+
+Both precision paths run in the same process on identical initial states:
+plain FP64 (bit-identical to the CPU oracle) and mixed (float filter, FP64
+boundary steps).  `value` is the mixed path's whole-job track-steps/s; the
+FP64 figure and the ratio are reported beside it.  The CPU baseline is the
+FP64 oracle on the host cores over a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -16,117 +24,151 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "track_step_flops.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def available_families():
+    from vecgeom_amd import workloads
+    import vecgeom_amd as vg
+
+    L = vg.lib()
+    fams = []
+    for f in workloads.FAMILIES:
+        if hasattr(L, f"vgm_{f}_create"):
+            fams.append(f)
+    return tuple(fams)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed bench steps (launches)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tracks", type=int, default=40_000_000)
+    ap.add_argument("--track-steps", type=int, default=50, help="track-steps per track per launch")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-tracks", type=int, default=400_000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py measures one GPU; runs on several GPUs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
     import vecgeom_amd as vg
-    from vecgeom_amd import sampling
+    from vecgeom_amd import workloads
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     ctx = vg.Context(0, stream=stream.cuda_stream)
-    n = 1_000_000
-    half = (10.0, 20.0, 30.0)
-    pts = sampling.points_in_box(42, 1, n, half, 1.5)
-    dirs = sampling.isotropic_directions(42, 2, n)
-    shape = ctx.box(*half)
-    pv = ctx.place(ctx.logical_volume(shape))
-    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    P = vg.SOA3D(*(T(a) for a in pts))
-    D = vg.SOA3D(*(T(a) for a in dirs))
-    step = T(np.full(n, 1e30))
-    ins = ctx.inside(pv, P)
-    sel = (ins == vg.kInside)
-    Pin = vg.SOA3D(P.x[sel].contiguous(), P.y[sel].contiguous(), P.z[sel].contiguous())
-    Din = vg.SOA3D(D.x[sel].contiguous(), D.y[sel].contiguous(), D.z[sel].contiguous())
-    stepin = step[sel].contiguous()
-    n_in = Pin.n
-    o_i = torch.empty(n, dtype=torch.int32, device=dev)
-    o_d = torch.empty(n, dtype=torch.float64, device=dev)
-    o_do = torch.empty(n_in, dtype=torch.float64, device=dev)
-    o_s = torch.empty(n, dtype=torch.float64, device=dev)
-
-    ev = {}
-
-    def one_step(record=False):
-        def rec(name, fn):
-            if record:
-                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                fn()
-                b.record(stream)
-                ev.setdefault(name, []).append((a, b))
-            else:
-                fn()
-        rec("Inside", lambda: ctx.inside(pv, P, out=o_i, asynchronous=True))
-        rec("DistanceToIn", lambda: ctx.distance_to_in(pv, P, D, step, out=o_d, asynchronous=True))
-        rec("DistanceToOut", lambda: ctx.distance_to_out(pv, Pin, Din, stepin, out=o_do, asynchronous=True))
-        rec("SafetyToIn", lambda: ctx.safety_to_in(pv, P, out=o_s, asynchronous=True))
-        rec("SafetyToOut", lambda: ctx.safety_to_out(pv, P, out=o_s, asynchronous=True))
-
-    for _ in range(args.warmup):
-        one_step()
-    torch.cuda.synchronize()
+    fams = available_families()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step(record=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    queries = args.steps * (4 * n + n_in)
-    kern_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
-    # dominant kernel: DistanceToIn, 24 B point + 24 B dir + 8 B step_max in, 8 B out
-    bytes_q = 64.0
-    t_dti = kern_ms["DistanceToIn"] * 1e-3
-    achieved = bytes_q * n / t_dti / 1e9
+    specs, counts = workloads.batched_solids(args.tracks, families=fams, per_family=8, seed=42)
+    shapes, offsets, steps, arr = workloads.build(ctx, specs, counts, seed=42)
+    n = int(offsets[-1])
+    log(f"[bench] {len(specs)} solids ({', '.join(fams)}), {n} tracks, built in {time.perf_counter() - t0:.1f}s")
+    init = [torch.from_numpy(a).to(dev) for a in arr]
+    del arr
+    side0 = torch.zeros(n, dtype=torch.int8, device=dev)
+    batch = vg.Batch(ctx, shapes, offsets, steps)
+    batch.classify(init[0], init[1], init[2], side0)
+    S = args.track_steps
+
+    def run(mixed):
+        state = [t.clone() for t in init]
+        side = side0.clone()
+        cr = torch.zeros(n, dtype=torch.int32, device=dev)
+        if not args.no_autotune:
+            t = time.perf_counter()
+            shape = batch.autotune(*state, side, nsteps=5, mixed=mixed)
+            log(f"[bench] autotune {'mixed' if mixed else 'fp64'}: block/grid/tile = {shape} "
+                f"({time.perf_counter() - t:.1f}s)")
+        step0 = 0
+        for _ in range(args.warmup):
+            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True)
+            step0 += S
+        torch.cuda.synchronize()
+        evs = []
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True)
+            b.record(stream)
+            evs.append((a, b))
+            step0 += S
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        kern = float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
+        nfp64 = 0
+        if mixed:
+            st2 = [x.clone() for x in init]
+            nfp64 = batch.transport(*st2, side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev), S,
+                                    step0=0, seed=7, mixed=True, count_fp64=True)
+        return wall, kern, nfp64, batch.launch()
+
+    wall64, kern64, _, launch64 = run(False)
+    rate64 = args.steps * n * S / wall64
+    log(f"[bench] fp64: {rate64 / 1e9:.2f} G track-steps/s (kernel {kern64 * 1e3:.2f} ms/launch)")
+    wallmx, kernmx, nfp64, launchmx = run(True)
+    ratemx = args.steps * n * S / wallmx
+    log(f"[bench] mixed: {ratemx / 1e9:.2f} G track-steps/s (kernel {kernmx * 1e3:.2f} ms/launch), "
+        f"fp64 redo fraction {nfp64 / (n * S):.4f}")
+
+    roofline = None
+    if os.path.exists(FLOPS_FIXTURE):
+        flops = json.load(open(FLOPS_FIXTURE))
+        per_step = float(flops["fp64_flops_per_track_step"])
+        achieved = per_step * n * S / kern64 / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                    "kernel": "transport_kernel<FP64>", "flops_per_track_step": per_step}
 
     cpu = None
     if not args.no_cpu:
         import oracle
+
         geo = oracle.Geometry()
-        pvo = geo.place(geo.logical_volume(geo.box(*half)))
-        m = 200_000
-        sp = tuple(a[:m] for a in pts)
-        sd = tuple(a[:m] for a in dirs)
-        nth = os.cpu_count() or 1
-        best = 1e30
-        for _ in range(3):
-            c0 = time.perf_counter()
-            geo.query(pvo, "DistanceToIn", sp, sd, np.full(m, 1e30), nthreads=nth)
-            best = min(best, time.perf_counter() - c0)
-        cpu = {"value": m / best, "unit": "DistanceToIn queries/s", "cores": nth, "kind": "port",
-               "sample": f"{m} box DistanceToIn queries, best of 3"}
+        cspecs, ccounts = workloads.batched_solids(args.cpu_tracks, families=fams, per_family=8, seed=42)
+        cs, co, cst, carr = workloads.build(geo, cspecs, ccounts, seed=42)
+        cn = int(co[-1])
+        side = np.zeros(cn, np.int8)
+        cr = np.zeros(cn, np.int32)
+        nth = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        t = time.perf_counter()
+        geo.batch_transport(cs, co, cst, *carr, side, cr, S, step0=0, seed=7, classify=True, nthreads=nth)
+        dt = time.perf_counter() - t
+        cpu = {"value": cn * S / dt, "unit": "track-steps/s", "cores": nth, "kind": "port",
+               "sample": f"{cn} tracks x {S} track-steps, same solids and generator, FP64 oracle, OpenMP"}
 
     line = {
-        "metric": "box SOA3D queries/s (Inside, DistanceToIn, DistanceToOut, SafetyToIn, SafetyToOut)",
-        "value": queries / wall,
-        "unit": "queries/s",
+        "metric": "track-steps/sec on 1 MI355X (mixed+learned vs plain FP64), 0 safety violations",
+        "value": ratemx,
+        "unit": "track-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wallmx / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "box 10x20x30 mm, 1M points in 1.5x box, seed 42", "points": n},
-        "kernel_ms": kern_ms,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "config": {"workload": f"batched heterogeneous solids: {len(specs)} solids ({'/'.join(fams)}), "
+                               f"{n} tracks, {S} track-steps per launch",
+                   "tracks": n, "track_steps_per_launch": S, "families": list(fams)},
+        "fp64_value": rate64,
+        "mixed_over_fp64": ratemx / rate64,
+        "fp64_redo_fraction": nfp64 / (n * S),
+        "kernel_ms": {"fp64": kern64 * 1e3, "mixed": kernmx * 1e3},
+        "launch": {"fp64": launch64, "mixed": launchmx},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

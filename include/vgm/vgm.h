@@ -124,6 +124,39 @@ int vgm_safety_to_in(vgm_ctx *ctx, int pv, const double *x, const double *y, con
 int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
                       double *out, unsigned flags);
 
+/* ---- batched heterogeneous transport ---------------------------------------
+ * No counterpart call exists in the reference: it replaces a caller's loop of
+ * per-solid SOA3D calls (VPlacedVolume::SafetyToIn/Out, DistanceToIn/Out,
+ * VecGeom/volumes/PlacedVolume.h:332-421) over many solids.  Solid i owns the
+ * tracks [offsets[i], offsets[i+1]) of one set of SOA3D arrays, given in the
+ * solid's local frame.  One track-step is: safety on the current side; a
+ * physics step l ~ Exp(mean_step[i]) from a counter-based generator keyed by
+ * (seed, track, step0 + k); if l exceeds the safety, the boundary distance
+ * decides between moving l or crossing (boundary + 1e-6 mm push, side flips).
+ * A step that does not cross ends in an interaction with a new isotropic
+ * direction; outside the solid, coordinates wrap periodically within 1.3x
+ * the solid's bounding box.  Positions and directions are updated in place.
+ * All pointers are device memory.  Tracks are processed by a persistent,
+ * shape-sorted work queue; vgm_batch_autotune picks block, grid and tile size.
+ * VGM_MIXED selects float filtering with FP64 boundary steps. */
+typedef struct vgm_batch vgm_batch;
+int vgm_batch_create(vgm_ctx *ctx, int nsolids, const int *shapes, const int64_t *offsets, const double *mean_step,
+                     vgm_batch **out);
+int vgm_batch_destroy(vgm_batch *batch);
+/* side[i] = Contains(point i) of its solid. */
+int vgm_batch_classify(vgm_batch *batch, const double *x, const double *y, const double *z, int8_t *side);
+/* Advances every track by nsteps track-steps; crossings[i] is
+ * incremented per boundary crossed.  fp64_steps (host, may be NULL) receives the
+ * number of steps a VGM_MIXED run redid in FP64. */
+int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, double *dx, double *dy, double *dz,
+                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
+                        unsigned flags, unsigned long long *fp64_steps);
+int vgm_batch_set_launch(vgm_batch *batch, int block, int grid, int tile_tracks);
+int vgm_batch_get_launch(vgm_batch *batch, int *shape3);
+int vgm_batch_autotune(vgm_batch *batch, const double *x, const double *y, const double *z, const double *dx,
+                       const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,
+                       int *best3);
+
 #ifdef __cplusplus
 }
 #endif

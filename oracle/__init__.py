@@ -60,6 +60,11 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = _I
+        L.orc_batch_transport.argtypes = [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I,
+                                          ctypes.c_uint64, _I, _I]
+        L.orc_batch_transport.restype = _I
+        L.orc_physics_step.argtypes = [_D, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_physics_step.restype = _D
         _lib = L
     return _lib
 
@@ -122,6 +127,16 @@ class Geometry:
 
     def add_daughter(self, mother_lv, pv):
         self._L.orc_lv_add_daughter(self.h, mother_lv, pv)
+
+    def batch_transport(self, shapes, offsets, mean_step, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0,
+                        seed=1, classify=False, nthreads=0):
+        """Track-step loop over solid i's tracks [offsets[i], offsets[i+1]); arrays updated in place."""
+        shapes = np.ascontiguousarray(shapes, dtype=np.int32)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        mean_step = np.ascontiguousarray(mean_step, dtype=np.float64)
+        self._L.orc_batch_transport(self.h, len(shapes), _p(shapes), _p(offsets), _p(mean_step), _p(x), _p(y), _p(z),
+                                    _p(dx), _p(dy), _p(dz), _p(side), _p(crossings), nsteps, step0, seed,
+                                    1 if classify else 0, nthreads)
 
     def query(self, pv, method, pts, dirs=None, step_max=None, nthreads=0):
         m = METHODS[method] if isinstance(method, str) else method

@@ -43,6 +43,13 @@ int orc_query(orc_geom *g, int pv, int method, const double *px, const double *p
               const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
               int nthreads);
 
+/* Batched track-step transport (project-defined loop, see transport.cpp). */
+double orc_physics_step(double mean, uint64_t seed, uint64_t track, uint32_t step);
+int orc_batch_transport(orc_geom *g, int nsolids, const int *shapes, const int64_t *offsets, const double *mean_step,
+                        double *x, double *y, double *z, const double *dx, const double *dy, const double *dz,
+                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed, int classify,
+                        int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

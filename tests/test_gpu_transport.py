@@ -1,0 +1,81 @@
+"""GPU batched transport vs the CPU oracle.
+
+FP64 mode must reproduce the oracle's trajectories bit for bit (positions,
+directions, sides, crossing counts).  Mixed mode must follow the FP64
+trajectory except where float and double disagree beyond the guard band;
+the number of such tracks is reported and must stay tiny.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import vecgeom_amd as vg
+from vecgeom_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+FAMILIES = ("box", "tube", "cone", "trd", "trapezoid", "polycone")
+
+
+def _setup(ctx, total, nsteps):
+    import torch
+
+    specs, counts = workloads.batched_solids(total, families=FAMILIES, per_family=4, seed=11)
+    geo = oracle.Geometry()
+    shapes_o, offs, steps, arr = workloads.build(geo, specs, counts, seed=11)
+    shapes_g, offs_g, steps_g, _ = workloads.build(ctx, specs, counts, seed=11)
+    assert np.array_equal(offs, offs_g) and np.array_equal(steps, steps_g)
+    n = int(offs[-1])
+    ref = [a.copy() for a in arr]
+    side_o = np.zeros(n, np.int8)
+    cr_o = np.zeros(n, np.int32)
+    geo.batch_transport(shapes_o, offs, steps, *ref, side_o, cr_o, nsteps, step0=0, seed=5, classify=True)
+    dev = [torch.from_numpy(a.copy()).cuda() for a in arr]
+    side = torch.zeros(n, dtype=torch.int8, device="cuda")
+    cr = torch.zeros(n, dtype=torch.int32, device="cuda")
+    b = vg.Batch(ctx, shapes_g, offs_g, steps_g)
+    b.classify(dev[0], dev[1], dev[2], side)
+    return b, dev, side, cr, ref, side_o, cr_o, arr
+
+
+@pytest.mark.parametrize("launch", [None, (128, 64, 512), (512, 1024, 8192)], ids=["default", "small", "large"])
+def test_transport_fp64_bit_exact(ctx, launch):
+    print(f"[transport] fp64 launch={launch}", flush=True)
+    b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
+    if launch:
+        b.set_launch(*launch)
+    b.transport(*dev, side, cr, 30, step0=0, seed=5)
+    for k in range(6):
+        got = dev[k].cpu().numpy()
+        bad = np.flatnonzero(got != ref[k])
+        assert bad.size == 0, (k, bad.size, bad[:5])
+    assert np.array_equal(side.cpu().numpy(), side_o)
+    assert np.array_equal(cr.cpu().numpy(), cr_o)
+    b.close()
+
+
+def test_transport_split_runs_equal_single_run(ctx):
+    """Two launches of 15 steps (step0 = 0, 15) equal one launch of 30."""
+    print("[transport] split", flush=True)
+    b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 60_000, 30)
+    b.transport(*dev, side, cr, 15, step0=0, seed=5)
+    b.transport(*dev, side, cr, 15, step0=15, seed=5)
+    for k in range(6):
+        assert np.array_equal(dev[k].cpu().numpy(), ref[k]), k
+    assert np.array_equal(cr.cpu().numpy(), cr_o)
+    b.close()
+
+
+def test_transport_mixed_follows_fp64(ctx):
+    print("[transport] mixed", flush=True)
+    b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
+    nfp64 = b.transport(*dev, side, cr, 30, step0=0, seed=5, mixed=True, count_fp64=True)
+    pos = np.stack([dev[k].cpu().numpy() for k in range(3)])
+    rpos = np.stack(ref[:3])
+    diverged = np.any(pos != rpos, axis=0) | (side.cpu().numpy() != side_o)
+    frac = diverged.mean()
+    n = pos.shape[1]
+    print(f"[transport] mixed: diverged {diverged.sum()} / {n}, fp64 steps {nfp64} / {30 * n}", flush=True)
+    assert frac < 1e-3
+    assert 0 < nfp64 < 30 * n
+    b.close()

@@ -85,6 +85,20 @@ def _declare(L):
         f.restype = _I
     L.vgm_ctx_stream.argtypes = [_P]
     L.vgm_ctx_stream.restype = _P
+    bsig = {
+        "vgm_batch_create": [_P, _I, _P, _P, _P, ctypes.POINTER(_P)],
+        "vgm_batch_destroy": [_P],
+        "vgm_batch_classify": [_P, _P, _P, _P, _P],
+        "vgm_batch_transport": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, ctypes.c_uint64, _U,
+                                ctypes.POINTER(ctypes.c_ulonglong)],
+        "vgm_batch_set_launch": [_P, _I, _I, _I],
+        "vgm_batch_get_launch": [_P, _P],
+        "vgm_batch_autotune": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _U, _P],
+    }
+    for name, args in bsig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
 
 
 def check(rc: int):
@@ -252,3 +266,59 @@ class Context:
 
     def placed_distance_to_out(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
         return self._q2(self._L.vgm_placed_distance_to_out, pv, pts, dirs, step_max, out, mixed, asynchronous)
+
+
+class Batch:
+    """Batched heterogeneous track-step transport over many solids (vgm_batch).
+
+    Track state lives in device tensors (float64 x, y, z, dx, dy, dz; int8 side;
+    int32 crossings), solid i owning tracks [offsets[i], offsets[i+1]).
+    """
+
+    def __init__(self, ctx: Context, shapes, offsets, mean_step):
+        self.ctx = ctx
+        self._L = ctx._L
+        self._shapes = np.ascontiguousarray(shapes, dtype=np.int32)
+        self._offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._steps = np.ascontiguousarray(mean_step, dtype=np.float64)
+        h = _P()
+        check(self._L.vgm_batch_create(ctx.handle, len(self._shapes), _ptr(self._shapes), _ptr(self._offsets),
+                                       _ptr(self._steps), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._L.vgm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def classify(self, x, y, z, side):
+        check(self._L.vgm_batch_classify(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(side)))
+
+    def transport(self, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0, seed=1, mixed=False,
+                  asynchronous=False, count_fp64=False):
+        flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0)
+        cnt = ctypes.c_ulonglong(0)
+        check(self._L.vgm_batch_transport(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                          _ptr(side), _ptr(crossings), nsteps, step0, seed, flags,
+                                          ctypes.byref(cnt) if count_fp64 else None))
+        return cnt.value
+
+    def set_launch(self, block, grid, tile_tracks):
+        check(self._L.vgm_batch_set_launch(self.handle, block, grid, tile_tracks))
+
+    def launch(self):
+        out = np.zeros(3, dtype=np.int32)
+        check(self._L.vgm_batch_get_launch(self.handle, out.ctypes.data))
+        return tuple(int(v) for v in out)
+
+    def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False):
+        out = np.zeros(3, dtype=np.int32)
+        check(self._L.vgm_batch_autotune(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                         _ptr(side), nsteps, MIXED if mixed else 0, out.ctypes.data))
+        return tuple(int(v) for v in out)

@@ -1,0 +1,295 @@
+// Batched heterogeneous transport: work-queue construction, launch-shape
+// autotuner and the C-ABI entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "context.h"
+#include "transport.h"
+
+using namespace vgm;
+
+struct vgm_batch {
+  vgm_ctx *ctx = nullptr;
+  std::vector<SolidRec> solids;
+  std::vector<Tile> tiles;
+  int64_t ntracks = 0;
+  LaunchShape shape{256, 0, 2048};
+  bool tiles_dirty = true;
+  char *dbuf = nullptr;  // solids | tiles | counter | fp64 step counter
+  size_t dcap = 0;
+  SolidRec *d_solids = nullptr;
+  Tile *d_tiles = nullptr;
+  int *d_counter = nullptr;
+  unsigned long long *d_fp64 = nullptr;
+};
+
+namespace {
+
+// Tiles of `tile_tracks` consecutive tracks, ordered by shape family and then
+// by solid so that neighbouring workgroups share code and parameters.
+void build_tiles(vgm_batch *b)
+{
+  std::vector<int> order(b->solids.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return b->solids[a].type < b->solids[c].type; });
+  b->tiles.clear();
+  const int64_t tt = b->shape.tile_tracks;
+  for (int s : order) {
+    const SolidRec &r = b->solids[s];
+    for (int64_t t = r.begin; t < r.end; t += tt) {
+      Tile tl;
+      tl.solid = s;
+      tl.pad = 0;
+      tl.begin = t;
+      tl.end = std::min<int64_t>(t + tt, r.end);
+      b->tiles.push_back(tl);
+    }
+  }
+}
+
+int upload(vgm_batch *b)
+{
+  vgm_ctx *c = b->ctx;
+  int rc = sync_geometry(c);
+  if (rc) return rc;
+  if (!b->tiles_dirty && b->dbuf) return VGM_OK;
+  build_tiles(b);
+  const size_t s1 = align256(b->solids.size() * sizeof(SolidRec) + 1);
+  const size_t s2 = align256(b->tiles.size() * sizeof(Tile) + 1);
+  const size_t total = s1 + s2 + 256;
+  if (b->dcap < total) {
+    if (b->dbuf) {
+      VGM_HIP(hipStreamSynchronize(c->stream));
+      VGM_HIP(hipFree(b->dbuf));
+    }
+    VGM_HIP(hipMalloc(&b->dbuf, total));
+    b->dcap = total;
+  }
+  b->d_solids = reinterpret_cast<SolidRec *>(b->dbuf);
+  b->d_tiles = reinterpret_cast<Tile *>(b->dbuf + s1);
+  b->d_counter = reinterpret_cast<int *>(b->dbuf + s1 + s2);
+  b->d_fp64 = reinterpret_cast<unsigned long long *>(b->dbuf + s1 + s2 + 64);
+  VGM_HIP(hipMemcpyAsync(b->d_solids, b->solids.data(), b->solids.size() * sizeof(SolidRec), hipMemcpyHostToDevice,
+                         c->stream));
+  VGM_HIP(hipMemcpyAsync(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                         c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  b->tiles_dirty = false;
+  return VGM_OK;
+}
+
+TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *dx, double *dy, double *dz,
+                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed)
+{
+  TransportArgs a{};
+  a.pool = b->ctx->dev.pool;
+  a.solids = b->d_solids;
+  a.tiles = b->d_tiles;
+  a.ntiles = int32_t(b->tiles.size());
+  a.nsteps = nsteps;
+  a.step0 = step0;
+  a.seed = seed;
+  a.x = x, a.y = y, a.z = z;
+  a.dx = dx, a.dy = dy, a.dz = dz;
+  a.side = side;
+  a.crossings = crossings;
+  a.counter = b->d_counter;
+  a.fp64_steps = b->d_fp64;
+  return a;
+}
+
+LaunchShape resolved(vgm_batch *b)
+{
+  LaunchShape ls = b->shape;
+  if (ls.grid <= 0) ls.grid = std::max(1, b->ctx->num_cus) * 4;
+  // never more workgroups than tiles: the surplus would only spin on the queue
+  ls.grid = int(std::min<size_t>(size_t(ls.grid), std::max<size_t>(1, b->tiles.size())));
+  return ls;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *offsets, const double *mean_step,
+                     vgm_batch **out)
+{
+  if (!c || !shapes || !offsets || !mean_step || !out || nsolids <= 0) return fail(VGM_EINVAL, "null argument");
+  vgm_batch *b = new vgm_batch();
+  b->ctx = c;
+  for (int i = 0; i < nsolids; ++i) {
+    if (shapes[i] < 0 || size_t(shapes[i]) >= c->host.shapes.size() || offsets[i + 1] < offsets[i] ||
+        !(mean_step[i] > 0)) {
+      delete b;
+      return fail(VGM_EINVAL, "invalid shape id, track range or mean step");
+    }
+    const ShapeRec &r = c->host.shapes[shapes[i]];
+    SolidRec s{};
+    s.type = r.type;
+    s.off = r.off;
+    s.begin = offsets[i];
+    s.end = offsets[i + 1];
+    s.lambda = mean_step[i];
+    double d2 = 0;
+    for (int k = 0; k < 3; ++k) d2 += (r.bmax[k] - r.bmin[k]) * (r.bmax[k] - r.bmin[k]);
+    s.scale = std::sqrt(d2);
+    for (int k = 0; k < 3; ++k) {
+      s.wc[k] = 0.5 * (r.bmin[k] + r.bmax[k]);
+      s.wh[k] = 1.3 * (0.5 * (r.bmax[k] - r.bmin[k]));
+    }
+    b->solids.push_back(s);
+  }
+  b->ntracks = offsets[nsolids];
+  *out = b;
+  return VGM_OK;
+}
+
+int vgm_batch_destroy(vgm_batch *b)
+{
+  if (!b) return VGM_OK;
+  if (b->dbuf) {
+    (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->dbuf);
+  }
+  delete b;
+  return VGM_OK;
+}
+
+int vgm_batch_set_launch(vgm_batch *b, int block, int grid, int tile_tracks)
+{
+  if (!b || block <= 0 || block > 1024 || block % 64 || tile_tracks <= 0)
+    return fail(VGM_EINVAL, "block must be a multiple of 64 up to 1024, tile_tracks > 0");
+  if (tile_tracks != b->shape.tile_tracks) b->tiles_dirty = true;
+  b->shape = LaunchShape{block, grid, tile_tracks};
+  return VGM_OK;
+}
+
+int vgm_batch_get_launch(vgm_batch *b, int *shape3)
+{
+  if (!b || !shape3) return fail(VGM_EINVAL, "null argument");
+  const LaunchShape ls = resolved(b);
+  shape3[0] = ls.block;
+  shape3[1] = ls.grid;
+  shape3[2] = ls.tile_tracks;
+  return VGM_OK;
+}
+
+int vgm_batch_classify(vgm_batch *b, const double *x, const double *y, const double *z, int8_t *side)
+{
+  if (!b || !x || !y || !z || !side) return fail(VGM_EINVAL, "null argument");
+  VGM_HIP(hipSetDevice(b->ctx->device));
+  int rc = upload(b);
+  if (rc) return rc;
+  TransportArgs a = make_args(b, const_cast<double *>(x), const_cast<double *>(y), const_cast<double *>(z), nullptr,
+                              nullptr, nullptr, side, nullptr, 0, 0, 0);
+  VGM_HIP(launch_transport(b->ctx->stream, kTransportClassify, a, resolved(b)));
+  VGM_HIP(hipStreamSynchronize(b->ctx->stream));
+  return VGM_OK;
+}
+
+int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *dx, double *dy, double *dz,
+                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
+                        unsigned flags, unsigned long long *fp64_steps)
+{
+  if (!b || !x || !y || !z || !dx || !dy || !dz || !side || !crossings || nsteps < 0)
+    return fail(VGM_EINVAL, "null argument or negative step count");
+  VGM_HIP(hipSetDevice(b->ctx->device));
+  int rc = upload(b);
+  if (rc) return rc;
+  TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed);
+  const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
+  VGM_HIP(launch_transport(b->ctx->stream, mode, a, resolved(b)));
+  if (fp64_steps) {
+    VGM_HIP(hipMemcpyAsync(fp64_steps, b->d_fp64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+    VGM_HIP(hipStreamSynchronize(b->ctx->stream));
+  } else if (!(flags & VGM_ASYNC)) {
+    VGM_HIP(hipStreamSynchronize(b->ctx->stream));
+  }
+  return VGM_OK;
+}
+
+// Times every candidate launch shape on scratch copies of the track state
+// (the caller's buffers are left untouched) and keeps the fastest.
+int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const double *z, const double *dx,
+                       const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,
+                       int *best3)
+{
+  if (!b || !x || !y || !z || !dx || !dy || !dz || !side || nsteps <= 0) return fail(VGM_EINVAL, "null argument");
+  vgm_ctx *c = b->ctx;
+  VGM_HIP(hipSetDevice(c->device));
+  const size_t n = size_t(b->ntracks);
+  char *work = nullptr;
+  const size_t bytes = align256(6 * n * sizeof(double)) + align256(n) + align256(n * sizeof(int32_t));
+  VGM_HIP(hipMalloc(&work, bytes));
+  double *wx = reinterpret_cast<double *>(work);
+  double *wy = wx + n, *wz = wy + n, *wdx = wz + n, *wdy = wdx + n, *wdz = wdy + n;
+  int8_t *ws = reinterpret_cast<int8_t *>(work + align256(6 * n * sizeof(double)));
+  int32_t *wc = reinterpret_cast<int32_t *>(work + align256(6 * n * sizeof(double)) + align256(n));
+  hipEvent_t e0, e1;
+  VGM_HIP(hipEventCreate(&e0));
+  VGM_HIP(hipEventCreate(&e1));
+  const int blocks[] = {128, 256, 512};
+  const int tiles[] = {512, 2048, 8192};
+  const int per_cu[] = {2, 4, 8};
+  const LaunchShape keep = b->shape;
+  float best = 1e30f;
+  LaunchShape best_shape = keep;
+  int rc = VGM_OK;
+  for (int bl : blocks)
+    for (int tt : tiles)
+      for (int pc : per_cu) {
+        if (tt < bl) continue;
+        vgm_batch_set_launch(b, bl, pc * std::max(1, c->num_cus), tt);
+        if ((rc = upload(b))) goto done;
+        for (int rep = 0; rep < 2; ++rep) {
+          if (hipMemcpyAsync(wx, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(wy, y, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(wz, z, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(wdx, dx, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(wdy, dy, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(wdz, dz, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemcpyAsync(ws, side, n, hipMemcpyDeviceToDevice, c->stream) ||
+              hipMemsetAsync(wc, 0, n * sizeof(int32_t), c->stream)) {
+            rc = fail(VGM_EHIP, "autotune staging copy failed");
+            goto done;
+          }
+          TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1);
+          const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
+          (void)hipEventRecord(e0, c->stream);
+          if (launch_transport(c->stream, mode, a, resolved(b)) != hipSuccess) {
+            rc = fail(VGM_EHIP, "autotune launch failed");
+            goto done;
+          }
+          (void)hipEventRecord(e1, c->stream);
+          (void)hipEventSynchronize(e1);
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep == 1 && ms < best) {
+            best = ms;
+            best_shape = LaunchShape{bl, pc * std::max(1, c->num_cus), tt};
+          }
+        }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(work);
+  if (rc) {
+    b->shape = keep;
+    b->tiles_dirty = true;
+    return rc;
+  }
+  vgm_batch_set_launch(b, best_shape.block, best_shape.grid, best_shape.tile_tracks);
+  if (best3) {
+    best3[0] = best_shape.block;
+    best3[1] = best_shape.grid;
+    best3[2] = best_shape.tile_tracks;
+  }
+  return VGM_OK;
+}
+
+}  // extern "C"

@@ -5,46 +5,18 @@
 #include <string>
 #include <vector>
 #include "../../include/vgm/vgm.h"
-#include "geometry.h"
-#include "vgm_internal.h"
+#include "context.h"
 
 namespace vgm {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-// Device mirror of HostGeometry: one allocation, 256-byte aligned sections.
-struct DeviceGeometry {
-  uint64_t version = 0;
-  char *buf = nullptr;
-  size_t cap = 0;
-  ShapeRec *shapes = nullptr;
-  double *pool = nullptr;
-  LogicalRec *lvs = nullptr;
-  int32_t *daughters = nullptr;
-  PlacedRec *pvs = nullptr;
-};
-
-// Growable device scratch for host-pointer queries.
-struct Scratch {
-  char *buf = nullptr;
-  size_t cap = 0;
-};
-
 }  // namespace vgm
 
 using namespace vgm;
 
-struct vgm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  HostGeometry host;
-  DeviceGeometry dev;
-  Scratch scratch;
-};
-
-namespace {
+namespace vgm {
 
 int fail(int code, const std::string &msg)
 {
@@ -56,12 +28,6 @@ int hip_fail(hipError_t e, const char *what)
 {
   return fail(VGM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define VGM_HIP(call)                                 \
-  do {                                                \
-    hipError_t e_ = (call);                           \
-    if (e_ != hipSuccess) return hip_fail(e_, #call); \
-  } while (0)
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -132,6 +98,10 @@ int sync_geometry(vgm_ctx *c)
   d.version = h.version;
   return VGM_OK;
 }
+
+}  // namespace vgm
+
+namespace {
 
 int check_pv(vgm_ctx *c, int pv)
 {
@@ -210,6 +180,8 @@ int vgm_ctx_create(int device, void *stream, vgm_ctx **out)
   VGM_HIP(hipSetDevice(device));
   vgm_ctx *c = new vgm_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    c->num_cus = 256;
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
   } else {

@@ -1,0 +1,51 @@
+// Library context: one device, one stream, one geometry and its device copy.
+// Shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "../../include/vgm/vgm.h"
+#include "geometry.h"
+#include "vgm_internal.h"
+
+namespace vgm {
+
+// Device mirror of HostGeometry: one allocation, 256-byte aligned sections.
+struct DeviceGeometry {
+  uint64_t version = 0;
+  char *buf = nullptr;
+  size_t cap = 0;
+  ShapeRec *shapes = nullptr;
+  double *pool = nullptr;
+  LogicalRec *lvs = nullptr;
+  int32_t *daughters = nullptr;
+  PlacedRec *pvs = nullptr;
+};
+
+// Growable device scratch for host-pointer queries.
+struct Scratch {
+  char *buf = nullptr;
+  size_t cap = 0;
+};
+
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+int sync_geometry(vgm_ctx *c);
+size_t align256(size_t x);
+
+}  // namespace vgm
+
+struct vgm_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  vgm::HostGeometry host;
+  vgm::DeviceGeometry dev;
+  vgm::Scratch scratch;
+};
+
+#define VGM_HIP(call)                                           \
+  do {                                                          \
+    hipError_t e_ = (call);                                     \
+    if (e_ != hipSuccess) return ::vgm::hip_fail(e_, #call);    \
+  } while (0)

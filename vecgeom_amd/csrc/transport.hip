@@ -1,0 +1,211 @@
+// Batched heterogeneous track-step transport: many solids of different
+// families, uneven track counts, one persistent launch.
+//
+// Work is cut into tiles of consecutive tracks of one solid; tiles are sorted
+// by shape family so that concurrently resident waves run the same code and
+// the same parameter block.  Workgroups pull tile indices from a device
+// counter until the queue is drained, so the grid is sized to the machine
+// (a multiple of the CU count), not to the work.  Each track keeps its state in
+// registers for all steps and touches HBM once on entry and once on exit.
+//
+// Precision modes:
+//   FP64   every operation in double, bit-identical to the CPU oracle.
+//   MIXED  safety and boundary distance evaluated in float as a filter; any
+//          step that could reach a boundary is redone with the FP64 kernels,
+//          so positions and crossings follow the FP64 trajectory except where
+//          float and double disagree by more than the guard band.
+#include "vgm_internal.h"
+#include "transport.h"
+#include "stepping.hpp"
+#include "shapes/all_shapes.hpp"
+
+namespace vgm {
+
+namespace {
+
+struct TrackState {
+  V3<double> p, d;
+  bool in;
+  int32_t crossings;
+};
+
+// One track-step in FP64 (see stepping.hpp for the definition).  Returns
+// true when the step ended on a boundary crossing.
+template <class S>
+VGM_DEV bool step_fp64(const typename S::Data &s, TrackState &t, double l)
+{
+  const double saf = t.in ? S::template safety_to_out<double>(s, t.p) : S::template safety_to_in<double>(s, t.p);
+  if (l <= saf) {
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  double db = t.in ? S::template distance_to_out<double>(s, t.p, t.d, kInfLength)
+                   : S::template distance_to_in<double>(s, t.p, t.d, kInfLength);
+  if (db < 0.0) db = 0.0;
+  if (l < db) {
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  const double m = db + kBoundaryPush;
+  t.p.x = t.p.x + m * t.d.x;
+  t.p.y = t.p.y + m * t.d.y;
+  t.p.z = t.p.z + m * t.d.z;
+  t.in = !t.in;
+  t.crossings++;
+  return true;
+}
+
+// Absolute guard for float results: a few float ulps of the largest
+// coordinate magnitude in play, plus a relative margin on the distance.
+VGM_DEV float guard_band(const V3<float> &p, float scale)
+{
+  return 4.0e-6f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + scale);
+}
+
+template <class S>
+VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, float scale, uint32_t &fp64_steps)
+{
+  const V3<float> pf{float(t.p.x), float(t.p.y), float(t.p.z)};
+  const V3<float> df{float(t.d.x), float(t.d.y), float(t.d.z)};
+  const float lf = float(l);
+  const float g = guard_band(pf, scale);
+  const float saf = t.in ? S::template safety_to_out<float>(s, pf) : S::template safety_to_in<float>(s, pf);
+  bool move_free = lf + g <= saf;
+  if (!move_free) {
+    const float db = t.in ? S::template distance_to_out<float>(s, pf, df, FLT_MAX)
+                          : S::template distance_to_in<float>(s, pf, df, FLT_MAX);
+    move_free = (db < FLT_MAX) ? (lf + g + 1.0e-3f * db < db) : (!t.in);
+  }
+  if (move_free) {
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  fp64_steps++;
+  return step_fp64<S>(s, t, l);
+}
+
+template <class S, int Mode>
+VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const Tile &tl, uint32_t &fp64_steps)
+{
+  const typename S::Data &s = *reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
+  const float scale = float(sr.scale);
+  for (int64_t i = tl.begin + threadIdx.x; i < tl.end; i += blockDim.x) {
+    TrackState t;
+    t.p = V3<double>{a.x[i], a.y[i], a.z[i]};
+    t.d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
+    t.in = a.side[i] != 0;
+    t.crossings = 0;
+    for (int k = 0; k < a.nsteps; ++k) {
+      const uint32_t step = uint32_t(a.step0 + k);
+      const double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
+      const bool crossed =
+          (Mode == kTransportFP64) ? step_fp64<S>(s, t, l) : step_mixed<S>(s, t, l, scale, fp64_steps);
+      if (!crossed) t.d = new_direction(a.seed, uint64_t(i), step, t.d);
+      if (!t.in) {
+        t.p.x = wrap(t.p.x, sr.wc[0], sr.wh[0]);
+        t.p.y = wrap(t.p.y, sr.wc[1], sr.wh[1]);
+        t.p.z = wrap(t.p.z, sr.wc[2], sr.wh[2]);
+      }
+    }
+    a.x[i] = t.p.x;
+    a.y[i] = t.p.y;
+    a.z[i] = t.p.z;
+    a.dx[i] = t.d.x;
+    a.dy[i] = t.d.y;
+    a.dz[i] = t.d.z;
+    a.side[i] = t.in ? 1 : 0;
+    a.crossings[i] += t.crossings;
+  }
+}
+
+template <int Mode>
+struct TileRunner {
+  const TransportArgs &a;
+  const SolidRec &sr;
+  const Tile &tl;
+  uint32_t &fp64_steps;
+  template <class S>
+  __device__ void operator()()
+  {
+    run_tile<S, Mode>(a, sr, tl, fp64_steps);
+  }
+};
+
+}  // namespace
+
+template <int Mode>
+__global__ void transport_kernel(TransportArgs a)
+{
+  __shared__ int tile_s;
+  uint32_t fp64_steps = 0;
+  for (;;) {
+    if (threadIdx.x == 0) tile_s = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int t = tile_s;
+    __syncthreads();
+    if (t >= a.ntiles) break;
+    const Tile tl = a.tiles[t];
+    const SolidRec sr = a.solids[tl.solid];
+    TileRunner<Mode> r{a, sr, tl, fp64_steps};
+    dispatch_shape_type(sr.type, r);
+  }
+  if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
+}
+
+namespace {
+struct Classifier {
+  const TransportArgs &a;
+  const SolidRec &sr;
+  const Tile &tl;
+  template <class S>
+  __device__ void operator()()
+  {
+    const typename S::Data &s = *reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
+    for (int64_t i = tl.begin + threadIdx.x; i < tl.end; i += blockDim.x)
+      a.side[i] = S::template contains<double>(s, V3<double>{a.x[i], a.y[i], a.z[i]}) ? 1 : 0;
+  }
+};
+}  // namespace
+
+// Initial inside/outside flag of every track: Contains on its solid.
+__global__ void classify_kernel(TransportArgs a)
+{
+  __shared__ int tile_s;
+  for (;;) {
+    if (threadIdx.x == 0) tile_s = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int t = tile_s;
+    __syncthreads();
+    if (t >= a.ntiles) break;
+    const Tile tl = a.tiles[t];
+    const SolidRec sr = a.solids[tl.solid];
+    Classifier c{a, sr, tl};
+    dispatch_shape_type(sr.type, c);
+  }
+}
+
+__global__ void reset_counter_kernel(int *counter, unsigned long long *fp64_steps)
+{
+  *counter = 0;
+  if (fp64_steps) *fp64_steps = 0;
+}
+
+hipError_t launch_transport(hipStream_t stream, int mode, const TransportArgs &a, const LaunchShape &ls)
+{
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, a.counter, a.fp64_steps);
+  if (mode == kTransportClassify)
+    hipLaunchKernelGGL(classify_kernel, dim3(ls.grid), dim3(ls.block), 0, stream, a);
+  else if (mode == kTransportMixed)
+    hipLaunchKernelGGL(transport_kernel<kTransportMixed>, dim3(ls.grid), dim3(ls.block), 0, stream, a);
+  else
+    hipLaunchKernelGGL(transport_kernel<kTransportFP64>, dim3(ls.grid), dim3(ls.block), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace vgm
