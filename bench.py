@@ -65,8 +65,9 @@ def main():
     from vecgeom_amd import workloads
 
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    ctx = vg.Context(0, stream=stream.cuda_stream)
+    ctx = vg.Context(0)
+    # Launches go to the library's own stream; events are recorded there too.
+    stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
     fams = available_families()
     t0 = time.perf_counter()
     specs, counts = workloads.batched_solids(args.tracks, families=fams, per_family=8, seed=42)
@@ -77,6 +78,7 @@ def main():
     del arr
     side0 = torch.zeros(n, dtype=torch.int8, device=dev)
     batch = vg.Batch(ctx, shapes, offsets, steps)
+    torch.cuda.synchronize()
     batch.classify(init[0], init[1], init[2], side0)
     S = args.track_steps
 
@@ -84,6 +86,7 @@ def main():
         state = [t.clone() for t in init]
         side = side0.clone()
         cr = torch.zeros(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
         if not args.no_autotune:
             t = time.perf_counter()
             shape = batch.autotune(*state, side, nsteps=5, mixed=mixed)
@@ -109,7 +112,9 @@ def main():
         nfp64 = 0
         if mixed:
             st2 = [x.clone() for x in init]
-            nfp64 = batch.transport(*st2, side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev), S,
+            s2, c2 = side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev)
+            torch.cuda.synchronize()
+            nfp64 = batch.transport(*st2, s2, c2, S,
                                     step0=0, seed=7, mixed=True, count_fp64=True)
         return wall, kern, nfp64, batch.launch()
 

@@ -49,6 +49,7 @@ def lib():
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
             "orc_polycone_create": [_P, _D, _D, _I, _P, _P, _P],
+            "orc_polyhedron_create": [_P, _D, _D, _I, _I, _P, _P, _P],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
@@ -112,6 +113,11 @@ class Geometry:
     def polycone(self, sphi, dphi, z, rmin, rmax):
         z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
         return self._chk(self._L.orc_polycone_create(self.h, sphi, dphi, len(z), _p(z), _p(rmin), _p(rmax)))
+
+    def polyhedron(self, sphi, dphi, nsides, z, rmin, rmax):
+        z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
+        return self._chk(self._L.orc_polyhedron_create(self.h, sphi, dphi, nsides, len(z), _p(z), _p(rmin),
+                                                       _p(rmax)))
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

@@ -52,6 +52,12 @@ int orc_polycone_create(orc_geom *g, double sphi, double dphi, int nz, const dou
   return add_shape(g, make_polycone(sphi, dphi, nz, z, rmin, rmax));
 }
 
+int orc_polyhedron_create(orc_geom *g, double sphi, double dphi, int nsides, int nz, const double *z,
+                          const double *rmin, const double *rmax)
+{
+  return add_shape(g, make_polyhedron(sphi, dphi, nsides, nz, z, rmin, rmax));
+}
+
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
 {
   auto *G = reinterpret_cast<Geometry *>(g);

@@ -144,6 +144,8 @@ Shape *make_trd(double x1, double x2, double y1, double y2, double z);
 Shape *make_trapezoid(double dz, double theta, double phi, double dy1, double dx1, double dx2, double alpha1,
                       double dy2, double dx3, double dx4, double alpha2);
 Shape *make_polycone(double sphi, double dphi, int nz, const double *z, const double *rmin, const double *rmax);
+Shape *make_polyhedron(double phiStart, double phiDelta, int sideCount, int nz, const double *z, const double *rmin,
+                       const double *rmax);
 Shape *make_tube(double rmin, double rmax, double z, double sphi, double dphi);
 
 }  // namespace orc

@@ -34,6 +34,8 @@ int orc_trapezoid_create(orc_geom *g, double dz, double theta, double phi, doubl
                          double alpha1, double dy2, double dx3, double dx4, double alpha2);
 int orc_polycone_create(orc_geom *g, double sphi, double dphi, int nz, const double *z, const double *rmin,
                         const double *rmax);
+int orc_polyhedron_create(orc_geom *g, double sphi, double dphi, int nsides, int nz, const double *z,
+                          const double *rmin, const double *rmax);
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx);
 int orc_lv_create(orc_geom *g, int shape);
 int orc_pv_create(orc_geom *g, int lv, const double *tr);

@@ -494,3 +494,60 @@ POLYCONE_KATS = [
     (PSIMPLE, "DistanceToIn", (100.0, 0.0, 0.0), VXY, ("inf",)),
     (PSIMPLE, "DistanceToIn", (-100.0, 0.0, 0.0), VMXY, ("inf",)),
 ]
+
+# ---- Polyhedron: test/unit_tests/TestPolyhedra.cpp ---------------------------
+# shapes are (sphi, dphi, nsides, z, rmin, rmax).  The asserts at :183-224 are
+# `fabs(result - value) < kTolerance` ("near" with 1e-9) or exact Inside codes.
+MYPGON = (0.0, PI / 4.0, 5, (-30.0, -20.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0),
+          (30.0, 30.0, 0.0, 0.0, 0.0, 0.0, 40.0, 40.0), (70.0, 70.0, 70.0, 40.0, 40.0, 80.0, 80.0, 60.0))
+# :97-101 SimplePolyhedron "Hexagonal prism" from the (r, z) contour
+# {(15,10), (15,-10), (0,-10), (0,10)}.  The contour constructor takes corner
+# radii and converts them to side radii r * cos(sidePhi / 2)
+# (VecGeom/volumes/PolyhedronStruct.h:204-222): z planes (-10, 10), rmin 0,
+# rmax 15 cos(pi/6), six sides, full phi.
+_HEX_R = 15.0 * math.cos(0.5 * (2.0 * PI / 6))
+HEXPRISM = (0.0, 2.0 * PI, 6, (-10.0, 10.0), (0.0, 0.0), (_HEX_R, _HEX_R))
+_PG_TOL = 1e-9
+_PG_P1, _PG_P2, _PG_P3 = (0.0, 0.0, -5.0), (50.0, 0.0, 40.0), (5.0, 1.0, 20.0)
+_PG_P4, _PG_P5, _PG_P6 = (45.0, 5.0, 30.0), (0.0, 0.0, 30.0), (41.0, 0.0, 10.0)
+
+
+def _near(v):
+    return ("near", v, _PG_TOL)
+
+
+POLYHEDRON_KATS = [
+    (MYPGON, "Inside", _PG_P1, None, ("inside", 2)),
+    (MYPGON, "Inside", _PG_P2, None, ("inside", 2)),
+    (MYPGON, "Inside", _PG_P3, None, ("inside", 1)),
+    (MYPGON, "Inside", _PG_P4, None, ("inside", 1)),
+    (MYPGON, "Inside", _PG_P5, None, ("inside", 3)),
+    (MYPGON, "Inside", _PG_P6, None, ("inside", 3)),
+    (HEXPRISM, "Inside", O, None, ("inside", 1)),
+    (HEXPRISM, "Inside", (15.0, 0.0, 0.0), None, ("inside", 2)),
+    (MYPGON, "Contains", (20.0 + 1e-9, 20.0, 5.0), None, _exact(1)),
+    (MYPGON, "Inside", (20.0 + 1e-9, 20.0, 5.0), None, ("inside", 1)),
+    (MYPGON, "Contains", (20.0, 20.0, 5.0), None, _exact(0)),
+    (MYPGON, "Inside", (20.0, 20.0, 5.0), None, ("inside", 2)),
+    (MYPGON, "Contains", (20.0 - 1e-9, 20.0, 5.0), None, _exact(0)),
+    (MYPGON, "Inside", (20.0 - 1e-9, 20.0, 5.0), None, ("inside", 3)),
+    (MYPGON, "DistanceToIn", _PG_P1, VX, _near(0.0)),
+    (MYPGON, "DistanceToIn", _PG_P1, VMY, _near(0.0)),
+    (MYPGON, "DistanceToIn", _PG_P5, VX, _near(40.12368793931)),
+    (MYPGON, "DistanceToIn", (41.0, 0.0, 10.0), VMX, _near(0.87631206069)),
+    (MYPGON, "DistanceToIn", (41.0, 0.0, 10.0), VZ, _near(0.218402670765)),
+    (MYPGON, "DistanceToOut", _PG_P1, VMX, _near(0.0)),
+    (MYPGON, "DistanceToOut", _PG_P3, VMY, _near(1.0)),
+    (MYPGON, "DistanceToOut", _PG_P3, VZ, _near(1.27382374146)),
+    (MYPGON, "DistanceToOut", _PG_P4, VZ, _near(10.0)),
+    (MYPGON, "DistanceToOut", _PG_P4, VX, _near(34.8538673445)),
+    (MYPGON, "DistanceToOut", _PG_P4, VY, _near(40.0)),
+    (HEXPRISM, "DistanceToOut", O, VX, ("ge", 1e-300)),
+    (HEXPRISM, "DistanceToOut", O, VY, ("ge", 1e-300)),
+    (HEXPRISM, "DistanceToOut", O, VZ, ("ge", 1e-300)),
+]
+# :187 DistanceToIn(p2, +y) is either ~0 or beyond 1e10.
+POLYHEDRON_EITHER = [(MYPGON, "DistanceToIn", _PG_P2, VY)]
+# :170-173 Extent == bounding box == ((0,0,-30), (80/cos(h), 40 sqrt(2)/cos(h), 40)), h = dphi/10.
+_H = 0.5 * (PI / 4.0) / 5.0
+POLYHEDRON_EXTENT = [(MYPGON, (0.0, 0.0, -30.0), (80.0 / math.cos(_H), 40.0 * math.sqrt(2.0) / math.cos(_H), 40.0))]

@@ -70,3 +70,17 @@ SHAPES.update({
                                    (5.0, 6.0, 4.0, 8.0, 8.0, 3.0, 2.0, 5.0, 6.0, 4.0),
                                    (20.0, 25.0, 22.0, 30.0, 28.0, 26.0, 35.0, 30.0, 24.0, 20.0))),
 })
+
+# Polyhedra: (sphi, dphi, nsides, z, rmin, rmax); radii are to the sides.
+SHAPES.update({
+    "pgon_mypgon": ("polyhedron", (0.0, PI / 4.0, 5, (-30.0, -20.0, -10.0, 0.0, 10.0, 20.0, 30.0, 40.0),
+                                   (30.0, 30.0, 0.0, 0.0, 0.0, 0.0, 40.0, 40.0),
+                                   (70.0, 70.0, 70.0, 40.0, 40.0, 80.0, 80.0, 60.0))),
+    "pgon_hex": ("polyhedron", (0.0, 2 * PI, 6, (-10.0, 10.0), (0.0, 0.0), (12.99, 12.99))),
+    "pgon_8x6": ("polyhedron", (0.1, 2 * PI, 8, (-40.0, -25.0, -5.0, 5.0, 25.0, 40.0),
+                                (5.0, 8.0, 8.0, 12.0, 6.0, 6.0), (20.0, 30.0, 25.0, 35.0, 30.0, 18.0))),
+    "pgon_bigcut": ("polyhedron", (1.0, 4.0, 7, (-20.0, 0.0, 30.0), (0.0, 5.0, 10.0), (25.0, 30.0, 20.0))),
+    # a zero-height middle segment (two planes at z = 0): the border case
+    "pgon_smallcut": ("polyhedron", (-0.4, 1.2, 3, (-15.0, 0.0, 0.0, 20.0), (0.0, 2.0, 6.0, 0.0),
+                                     (20.0, 25.0, 30.0, 15.0))),
+})

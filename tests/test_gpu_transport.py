@@ -34,6 +34,7 @@ def _setup(ctx, total, nsteps):
     side = torch.zeros(n, dtype=torch.int8, device="cuda")
     cr = torch.zeros(n, dtype=torch.int32, device="cuda")
     b = vg.Batch(ctx, shapes_g, offs_g, steps_g)
+    torch.cuda.synchronize()  # torch's fills run on its stream, the library on its own
     b.classify(dev[0], dev[1], dev[2], side)
     return b, dev, side, cr, ref, side_o, cr_o, arr
 

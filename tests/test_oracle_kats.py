@@ -7,7 +7,8 @@ from tests.golden import kats
 
 CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_KATS]
          + [("cone", c) for c in kats.CONE_KATS] + [("trd", c) for c in kats.TRD_KATS]
-         + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS])
+         + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS]
+         + [("polyhedron", c) for c in kats.POLYHEDRON_KATS])
 
 
 def _run(geo, kind, params, method, p, d):
@@ -56,3 +57,20 @@ def test_cone_soft_kat(case):
         assert r >= kats.KINF, r
     else:
         assert abs(r - wanted) <= 0.01, (r, wanted)
+
+
+def test_polyhedron_either_kat():
+    """TestPolyhedra.cpp:187: DistanceToIn(p2, +y) is ~0 or beyond 1e10."""
+    geo = oracle.Geometry()
+    for params, method, p, d in kats.POLYHEDRON_EITHER:
+        r = float(_run(geo, "polyhedron", params, method, p, d))
+        assert abs(r) < 1e-9 or abs(r) > 1e10, r
+
+
+def test_polyhedron_extent_kat():
+    """TestPolyhedra.cpp:170-173: Extent equals the expected corner box."""
+    geo = oracle.Geometry()
+    for params, lo, hi in kats.POLYHEDRON_EXTENT:
+        mn, mx = geo.extent(geo.polyhedron(*params))
+        for a, b in zip(list(mn) + list(mx), list(lo) + list(hi)):
+            assert kats.approx_equal(float(a), float(b)), (mn, mx, lo, hi)

