@@ -79,6 +79,12 @@ int vgm_trapezoid_create(vgm_ctx *ctx, double dz, double theta, double phi, doub
 int vgm_polycone_create(vgm_ctx *ctx, double sphi, double dphi, int nz, const double *z, const double *rmin,
                         const double *rmax, int *shape);
 
+/* UnplacedPolyhedron(phiStart, phiDelta, sideCount, zPlaneCount, zPlanes,
+ * rMin, rMax): radii are distances to the sides, not to the corners; z planes
+ * must be non-decreasing (VecGeom/volumes/UnplacedPolyhedron.h:119-121). */
+int vgm_polyhedron_create(vgm_ctx *ctx, double sphi, double dphi, int nsides, int nz, const double *z,
+                          const double *rmin, const double *rmax, int *shape);
+
 /* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
  * VecGeom/volumes/UnplacedVolume.h). */
 int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);

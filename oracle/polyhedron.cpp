@@ -411,8 +411,11 @@ struct Polyhedron final : Shape {
     return distance;
   }
 
+  // A zero-height end segment has no phi quads: the reference would read
+  // past the shell there (undefined); the restatement says "not in the wedge".
   bool InPhiCutoutWedge(const Seg &s, const Vec3 &p) const
   {
+    if (s.phi.n < 2) return false;
     bool s0 = Dot(p, s.phi.normal[0]) + s.phi.dist[0] >= 0;
     bool s1 = Dot(p, s.phi.normal[1]) + s.phi.dist[1] >= 0;
     if (fHasLargePhiCutout) return s0 || s1;
@@ -567,6 +570,7 @@ struct Polyhedron final : Shape {
     const Seg &s = seg[zIndex];
     int phiIndex = FindPhiSegment(p);
     if (phiIndex < 0) return kOutside;
+    if (!s.outer.n && !s.inner.n) return kOutside;  // no shell to take a radius from
     const Vec3 &vout = s.outer.n ? s.outer.sideN[0][phiIndex] : s.inner.sideN[0][phiIndex];
     double rdv = Abs(Dot(p, vout));
     bool in1 = (rdv > fRMin[zIndex] - kTolerance) && (rdv < fRMax[zIndex] + kTolerance);
@@ -589,8 +593,11 @@ struct Polyhedron final : Shape {
     if (dz > kHalfTolerance) return kOutside;
     if (fSameZ[zIndex]) return InsideSegBorder(p, zIndex);
     const Seg &s = seg[zIndex];
-    int io = s.outer.InsideOne(p, phiIndex);
-    if (io != kInside) return io;
+    // (a zero-height end segment may have no outer shell: skipped)
+    if (s.outer.n > 0) {
+      int io = s.outer.InsideOne(p, phiIndex);
+      if (io != kInside) return io;
+    }
     if (s.hasInner()) {
       int ii = s.inner.InsideOne(p, phiIndex);
       if (ii == kInside) return kOutside;
@@ -613,6 +620,7 @@ struct Polyhedron final : Shape {
     if (fSameZ[zIndex]) {
       int phiIndex = FindPhiSegment(p);
       if (phiIndex < 0) return false;
+      if (!s.outer.n && !s.inner.n) return false;  // no shell to take a radius from
       const Vec3 &vout = s.outer.n ? s.outer.sideN[0][phiIndex] : s.inner.sideN[0][phiIndex];
       double rdv = Abs(Dot(p, vout));
       bool in1 = (rdv >= fRMin[zIndex]) && (rdv <= fRMax[zIndex]);

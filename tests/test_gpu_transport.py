@@ -14,7 +14,7 @@ from vecgeom_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-FAMILIES = ("box", "tube", "cone", "trd", "trapezoid", "polycone")
+FAMILIES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron")
 
 
 def _setup(ctx, total, nsteps):

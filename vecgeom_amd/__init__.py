@@ -75,6 +75,7 @@ def _declare(L):
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_polycone_create": [_P, _D, _D, _I, _P, _P, _P, ctypes.POINTER(_I)],
+        "vgm_polyhedron_create": [_P, _D, _D, _I, _I, _P, _P, _P, ctypes.POINTER(_I)],
         "vgm_trapezoid_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_shape_extent": [_P, _I, _P, _P],
         "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
@@ -201,6 +202,11 @@ class Context:
         if not (len(z) == len(rmin) == len(rmax)):
             raise ValueError("z, rmin and rmax differ in length")
         return self._new(self._L.vgm_polycone_create, sphi, dphi, len(z), _ptr(z), _ptr(rmin), _ptr(rmax))
+
+    def polyhedron(self, sphi, dphi, nsides, z, rmin, rmax) -> int:
+        z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
+        return self._new(self._L.vgm_polyhedron_create, sphi, dphi, nsides, len(z), _ptr(z), _ptr(rmin),
+                         _ptr(rmax))
 
     def extent(self, shape: int):
         mn = np.zeros(3)

@@ -281,6 +281,17 @@ int vgm_polycone_create(vgm_ctx *c, double sphi, double dphi, int nz, const doub
   return VGM_OK;
 }
 
+int vgm_polyhedron_create(vgm_ctx *c, double sphi, double dphi, int nsides, int nz, const double *z,
+                          const double *rmin, const double *rmax, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_polyhedron(c->host, sphi, dphi, nsides, nz, z, rmin, rmax, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
 {
   if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");

@@ -45,14 +45,10 @@ int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err
 // UnplacedTube(rmin, rmax, z, sphi, dphi) with TubeStruct::CalculateCached
 // (VecGeom/volumes/TubeStruct.h:110-176) and the wedge vectors of
 // source/Wedge_Evolution.cpp.  The bounding box is the full-phi one.
-int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)
+void make_tube_data(TubeData &t, double rmin, double rmax, double z, double sphi, double dphi)
 {
-  if (!(rmax > 0 && z > 0 && dphi > 0 && rmin < rmax)) {
-    err = "tube needs 0 <= rmin < rmax, z > 0, dphi > 0";
-    return -1;
-  }
   const double kHalfTol = 0.5e-9;
-  TubeData t{};
+  t = TubeData{};
   t.rmin = rmin < 0.0 ? 0.0 : rmin;
   t.rmax = rmax;
   t.z = z;
@@ -83,6 +79,16 @@ int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi,
   t.le_pi = dphi <= M_PI;
   t.lt_pi = dphi < kPi;
   t.gt_pi = dphi > M_PI;
+}
+
+int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)
+{
+  if (!(rmax > 0 && z > 0 && dphi > 0 && rmin < rmax)) {
+    err = "tube needs 0 <= rmin < rmax, z > 0, dphi > 0";
+    return -1;
+  }
+  TubeData t;
+  make_tube_data(t, rmin, rmax, z, sphi, dphi);
   const double lo[3] = {-rmax, -rmax, -z}, hi[3] = {rmax, rmax, z};
   return g.add_shape(kTube, &t, sizeof(t) / sizeof(double), 0, 0, lo, hi);
 }

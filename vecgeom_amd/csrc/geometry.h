@@ -24,6 +24,9 @@ struct HostGeometry {
 };
 
 struct ConeData;
+struct TubeData;
+// TubeStruct cached values (tube builder, and the polyhedron's bounding tube).
+void make_tube_data(TubeData &t, double rmin, double rmax, double z, double sphi, double dphi);
 // Cone parameter derivation shared by the cone and polycone builders.
 bool make_cone_data(ConeData &c, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
                     double dphi, std::string &err);
@@ -37,5 +40,7 @@ int build_trapezoid(HostGeometry &g, const double p[11], std::string &err);
 int build_polycone(HostGeometry &g, double sphi, double dphi, int nz, const double *z, const double *rmin,
                    const double *rmax, std::string &err);
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err);
+int build_polyhedron(HostGeometry &g, double sphi, double dphi, int nsides, int nz, const double *z,
+                     const double *rmin, const double *rmax, std::string &err);
 
 }  // namespace vgm

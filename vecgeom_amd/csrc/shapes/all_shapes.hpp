@@ -7,6 +7,7 @@
 #include "trd.hpp"
 #include "trapezoid.hpp"
 #include "polycone.hpp"
+#include "polyhedron.hpp"
 
 namespace vgm {
 
@@ -33,6 +34,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kPolycone:
     f.template operator()<PolyconeShape>();
+    return true;
+  case kPolyhedron:
+    f.template operator()<PolyhedronShape>();
     return true;
   default:
     return false;

@@ -1,0 +1,596 @@
+// Polyhedron kernels (VecGeom/volumes/kernel/PolyhedronImplementation.h:300-1200,
+// the generic inner-radius / phi-cutout treatment that every factory
+// specialisation computes identically) over the quadrilateral shells of
+// Quadrilaterals.h / Planes.h.  Scalar-backend semantics, FP64 or FP32 by R.
+//
+// Each z segment holds up to three shells of quads (outer, inner, two phi
+// cut planes).  A lane walks segments in z order and scans one shell at a
+// time; the quads of one solid sit together in the pool, so the scans hit
+// L1/L2 after the first lane of a wave touches them.
+#pragma once
+#include "tube.hpp"
+#include "wedge.hpp"
+#include "polyhedron_data.h"
+
+namespace vgm {
+
+struct PolyhedronShape {
+  using Data = PolyhedronData;
+  using Quad = PolyhedronQuad;
+
+  template <typename R>
+  VGM_DEV static R pdot(const double *n, V3<R> p)
+  {
+    return p.x * R(n[0]) + p.y * R(n[1]) + p.z * R(n[2]);
+  }
+  template <typename R>
+  VGM_DEV static R plane(const Quad &q, V3<R> p)
+  {
+    return pdot<R>(q.n, p) + R(q.d);
+  }
+  template <typename R>
+  VGM_DEV static bool side_ok(const Quad &q, int j, V3<R> x, R tol)
+  {
+    return pdot<R>(q.sn[j], x) + R(q.sd[j]) >= tol;
+  }
+
+  // Planes::Contains: strictly behind every plane of the shell.
+  template <typename R>
+  VGM_DEV static bool shell_contains(const Quad *q, int n, V3<R> p)
+  {
+    for (int i = 0; i < n; ++i)
+      if (!(plane<R>(q[i], p) < R(0))) return false;
+    return true;
+  }
+
+  // Planes::Inside(point, i)
+  template <typename R>
+  VGM_DEV static int quad_inside(const Quad &q, V3<R> p)
+  {
+    const R dr = R(q.n[0]) * p.x + R(q.n[1]) * p.y + R(q.n[2]) * p.z + R(q.d);
+    int r = kInsideV;
+    if (dr > R(kHalfTolerance)) r = kOutsideV;
+    if (r == kInsideV && dr > R(-kHalfTolerance)) r = kSurfaceV;
+    return r;
+  }
+
+  // Planes::Inside(point): the first plane that puts the point outside ends the scan.
+  template <typename R>
+  VGM_DEV static int shell_inside(const Quad *q, int n, V3<R> p)
+  {
+    int r = kInsideV;
+    for (int i = 0; i < n; ++i) {
+      const R dr = R(q[i].n[0]) * p.x + R(q[i].n[1]) * p.y + R(q[i].n[2]) * p.z + R(q[i].d);
+      if (dr > R(kHalfTolerance)) r = kOutsideV;
+      if (r == kInsideV && dr > R(-kHalfTolerance)) r = kSurfaceV;
+      if (r == kOutsideV) break;
+    }
+    return r;
+  }
+
+  // Quadrilaterals::DistanceToIn<behindPlanesT> (Quadrilaterals.h:320-350):
+  // the first quad hit from its front (or back) side wins.
+  template <typename R, bool Behind>
+  VGM_DEV static R shell_distance_to_in(const Quad *q, int n, V3<R> p, V3<R> d)
+  {
+    R best = inf_length<R>();
+    for (int i = 0; i < n; ++i) {
+      R distance = plane<R>(q[i], p);
+      if (!((Behind ? -distance : distance) > R(-kTolerance))) continue;
+      const R proj = pdot<R>(q[i].n, d);
+      if (!((Behind ? proj : -proj) > R(0))) continue;
+      distance /= -(proj + vcopysign(R(1e-20), proj));
+      const V3<R> x{p.x + d.x * distance, p.y + d.y * distance, p.z + d.z * distance};
+      bool valid = true;
+      for (int j = 0; j < 4; ++j) {
+        valid &= side_ok<R>(q[i], j, x, R(-kTolerance));
+        if (!valid) break;
+      }
+      if (valid) best = distance;
+      if (best < inf_length<R>()) break;
+    }
+    return vmax(R(0), best);
+  }
+
+  // Quadrilaterals::DistanceToOut(point, direction, zMin, zMax) (:420-455)
+  template <typename R>
+  VGM_DEV static R shell_distance_to_out(const Quad *q, int n, V3<R> p, V3<R> d, R zMin, R zMax)
+  {
+    R best = inf_length<R>();
+    for (int i = 0; i < n; ++i) {
+      R dt = plane<R>(q[i], p);
+      if (!(dt < R(kTolerance))) continue;
+      const R proj = pdot<R>(q[i].n, d);
+      if (!(proj > R(0))) continue;
+      dt /= -proj;
+      if (!(dt < best)) continue;
+      bool valid;
+      if (zMin == zMax) {
+        const V3<R> x{p.x + d.x * dt, p.y + d.y * dt, p.z + d.z * dt};
+        valid = true;
+        for (int j = 0; j < 4; ++j) {
+          valid &= side_ok<R>(q[i], j, x, R(-kTolerance));
+          if (!valid) break;
+        }
+      } else {
+        const R zp = p.z + dt * d.z;
+        valid = (zp >= zMin - R(kTolerance)) && (zp < zMax + R(kTolerance));
+      }
+      if (!valid) continue;
+      best = dt;
+    }
+    if (best > R(-kTolerance)) best = vmax(best, R(0));
+    return best;
+  }
+
+  // DistanceToLineSegmentSquared1 (GenericKernels.h:190-215)
+  template <typename R>
+  VGM_DEV static R segment_dist2(const double *c0, const double *c1, V3<R> p)
+  {
+    const R lx = R(c1[0]) - R(c0[0]), ly = R(c1[1]) - R(c0[1]), lz = R(c1[2]) - R(c0[2]);
+    const R px = p.x - R(c0[0]), py = p.y - R(c0[1]), pz = p.z - R(c0[2]);
+    const R dot0 = px * lx + py * ly + pz * lz;
+    if (dot0 <= R(0)) return px * px + py * py + pz * pz;
+    const R dot1 = lx * lx + ly * ly + lz * lz;
+    if (dot1 <= dot0) {
+      const R qx = p.x - R(c1[0]), qy = p.y - R(c1[1]), qz = p.z - R(c1[2]);
+      return qx * qx + qy * qy + qz * qz;
+    }
+    const R t = dot0 / dot1;
+    const R qx = R(c0[0]) + lx * t, qy = R(c0[1]) + ly * t, qz = R(c0[2]) + lz * t;
+    const R ex = qx - p.x, ey = qy - p.y, ez = qz - p.z;
+    return ex * ex + ey * ey + ez * ez;
+  }
+
+  // Quadrilaterals::ScalarDistanceSquared (:460-490): exact squared
+  // distance to the quad, through its plane or its nearest edge.
+  template <typename R>
+  VGM_DEV static R quad_dist2(const Quad &q, V3<R> p)
+  {
+    const R distance = plane<R>(q, p);
+    const V3<R> x{p.x - R(q.n[0]) * distance, p.y - R(q.n[1]) * distance, p.z - R(q.n[2]) * distance};
+    bool within[4];
+    for (int j = 0; j < 4; ++j) within[j] = side_ok<R>(q, j, x, R(0));
+    if (within[0] && within[1] && within[2] && within[3]) return distance * distance;
+    R dsq = inf_length<R>();
+    for (int j = 0; j < 4; ++j) {
+      if (!within[j]) {
+        const R e = segment_dist2<R>(q.c[j], q.c[(j + 1) & 3], p);
+        if (e < dsq) dsq = e;
+      }
+    }
+    return dsq;
+  }
+
+  // FindZSegmentKernel (PolyhedronImplementation.h:313-327)
+  template <typename R>
+  VGM_DEV static int find_z_segment(const Data &h, R pz)
+  {
+    const double *zp = h.zplanes();
+    const int nz = h.nseg + 1;
+    int index = -1, k = 0;
+    while (k < nz - 1 && pz - R(kTolerance) > R(zp[k])) {
+      ++index;
+      ++k;
+    }
+    if (pz + R(kTolerance) > R(zp[k])) return index + 1;
+    return index;
+  }
+
+  // FindPhiSegment (:344-364)
+  template <typename R>
+  VGM_DEV static int find_phi_segment(const Data &h, V3<R> p)
+  {
+    const double *ps = h.phisec();
+    R first = p.x * R(ps[0]) + p.y * R(ps[1]) + p.z * R(ps[2]);
+    for (int i = 1; i < h.nsides + 1; ++i) {
+      const R second = p.x * R(ps[3 * i]) + p.y * R(ps[3 * i + 1]) + p.z * R(ps[3 * i + 2]);
+      if (first > R(-kTolerance) && second < R(kTolerance)) return i - 1;
+      first = second;
+    }
+    return -1;
+  }
+
+  template <typename R>
+  VGM_DEV static bool bounds_contain(const Data &h, V3<R> p)
+  {
+    return TubeShape::contains<R>(h.btube, V3<R>{p.x, p.y, p.z - R(h.bt_offset)});
+  }
+
+  // DistanceToInZSegment (:368-400)
+  template <typename R>
+  VGM_DEV static R distance_to_in_zseg(const Data &h, int si, V3<R> p, V3<R> d)
+  {
+    const PolyhedronSeg &s = h.segs()[si];
+    const Quad *q = h.quads();
+    R distance = shell_distance_to_in<R, false>(q + s.outer, s.n_outer, p, d);
+    if (distance < inf_length<R>()) return distance;
+    if (h.phi_cutout) distance = shell_distance_to_in<R, false>(q + s.phi, s.n_phi, p, d);
+    if (distance < inf_length<R>()) return distance;
+    if (s.n_inner > 0) distance = shell_distance_to_in<R, true>(q + s.inner, s.n_inner, p, d);
+    return distance;
+  }
+
+  // DistanceToOutZSegment (:403-445)
+  template <typename R>
+  VGM_DEV static R distance_to_out_zseg(const Data &h, int si, R zMin, R zMax, V3<R> p, V3<R> d)
+  {
+    const PolyhedronSeg &s = h.segs()[si];
+    const Quad *q = h.quads();
+    bool done = false;
+    R distance = inf_length<R>();
+    if (s.n_inner > 0) {
+      distance = shell_distance_to_in<R, false>(q + s.inner, s.n_inner, p, d);
+      done = distance < inf_length<R>();
+      if (done) return distance;
+    }
+    if (h.phi_cutout) {
+      const R distphi = shell_distance_to_in<R, true>(q + s.phi, s.n_phi, p, d);
+      if (!done && distance > R(-kTolerance)) distance = distphi;
+      done = distance > R(-kTolerance) && distance < inf_length<R>();
+      if (done) return distance;
+    }
+    const R distout = shell_distance_to_out<R>(q + s.outer, s.n_outer, p, d, zMin, zMax);
+    if (!done) distance = distout;
+    return distance;
+  }
+
+  // InPhiCutoutWedge (:659-671).  A zero-height end segment has no phi
+  // quads; the reference then reads past the shell (undefined), here the
+  // point counts as outside the cutout.
+  template <typename R>
+  VGM_DEV static bool in_phi_cutout_wedge(const Data &h, const PolyhedronSeg &s, V3<R> p)
+  {
+    if (s.n_phi < 2) return false;
+    const Quad *q = h.quads() + s.phi;
+    const bool s0 = pdot<R>(q[0].n, p) + R(q[0].d) >= R(0);
+    const bool s1 = pdot<R>(q[1].n, p) + R(q[1].d) >= R(0);
+    if (h.large_cutout) return s0 || s1;
+    return s0 && s1;
+  }
+
+  // ScalarSafetyToZSegmentSquared (:448-560)
+  template <typename R>
+  VGM_DEV static R safety_zseg2(const Data &h, int si, int &phiIndex, V3<R> p, bool pt_inside)
+  {
+    const PolyhedronSeg &s = h.segs()[si];
+    const Quad *q = h.quads();
+    const bool in_cutout = phiIndex < 0;
+    R safetySquared = inf_length<R>();
+    if (h.phi_cutout && s.n_phi == 2) {
+      if (pt_inside || in_cutout) {
+        safetySquared = quad_dist2<R>(q[s.phi], p);
+        const R saf = quad_dist2<R>(q[s.phi + 1], p);
+        if (saf < safetySquared) safetySquared = saf;
+        if (in_cutout) {
+          if (h.large_cutout && s.n_inner > 0) {
+            R sq = quad_dist2<R>(q[s.inner], p);
+            if (sq < safetySquared) {
+              phiIndex = 0;
+              safetySquared = sq;
+            }
+            if (s.n_inner > 1) {
+              sq = quad_dist2<R>(q[s.inner + s.n_inner - 1], p);
+              if (sq < safetySquared) {
+                phiIndex = s.n_inner - 1;
+                safetySquared = sq;
+              }
+            }
+          }
+          return safetySquared;
+        }
+      }
+    }
+    if (in_cutout && si > 0 && si < h.nseg - 1 && h.zplanes()[si] == h.zplanes()[si + 1]) {
+      if (s.n_outer > 0) {
+        R so = quad_dist2<R>(q[s.outer], p);
+        if (so < safetySquared) {
+          phiIndex = 0;
+          safetySquared = so;
+        }
+        if (s.n_outer > 1) {
+          so = quad_dist2<R>(q[s.outer + s.n_outer - 1], p);
+          if (so < safetySquared) {
+            phiIndex = s.n_outer - 1;
+            safetySquared = so;
+          }
+        }
+      }
+      if (s.n_inner > 0) {
+        R sq = quad_dist2<R>(q[s.inner], p);
+        if (sq < safetySquared) {
+          phiIndex = 0;
+          safetySquared = sq;
+        }
+        if (s.n_inner > 1) {
+          sq = quad_dist2<R>(q[s.inner + s.n_inner - 1], p);
+          if (sq < safetySquared) {
+            phiIndex = s.n_inner - 1;
+            safetySquared = sq;
+          }
+        }
+      }
+      return safetySquared;
+    }
+    // A phi-cutout point reaches here only for a zero-height end segment,
+    // where the reference indexes the shells with -1 (out of bounds).
+    if (phiIndex < 0) return safetySquared;
+    R so = inf_length<R>();
+    if (s.n_outer > 0) so = quad_dist2<R>(q[s.outer + phiIndex], p);
+    R sq = inf_length<R>();
+    if (s.n_inner > 0) sq = quad_dist2<R>(q[s.inner + phiIndex], p);
+    if (sq < safetySquared) safetySquared = sq;
+    if (so < safetySquared) safetySquared = so;
+    return safetySquared;
+  }
+
+  // ScalarDistanceToEndcaps<pointInsideT> (:565-615)
+  template <typename R>
+  VGM_DEV static void distance_to_endcaps(const Data &h, bool pointInside, V3<R> p, V3<R> d, R &distance)
+  {
+    const int last = h.nseg - 1;
+    const bool first = pointInside ? (d.z < R(0)) : !(d.z < R(0));
+    const PolyhedronSeg &s = h.segs()[first ? 0 : last];
+    const R zPlane = R(h.zplanes()[first ? 0 : h.nseg]);
+    const R dt = (zPlane - p.z) / nonzero(d.z);
+    if (dt < R(-kTolerance) || dt >= distance) return;
+    const V3<R> x{p.x + d.x * dt, p.y + d.y * dt, p.z + d.z * dt};
+    const Quad *q = h.quads();
+    if (!shell_contains<R>(q + s.outer, s.n_outer, x)) return;
+    if (s.n_inner > 0 && shell_contains<R>(q + s.inner, s.n_inner, x)) return;
+    if (h.phi_cutout && in_phi_cutout_wedge<R>(h, s, x)) return;
+    distance = dt;
+  }
+
+  // ScalarSafetyToEndcapsSquared (:620-650)
+  template <typename R>
+  VGM_DEV static void safety_to_endcaps2(const Data &h, V3<R> p, R &distanceSquared)
+  {
+    const R firstDistance = R(h.zplanes()[0]) - p.z;
+    const R lastDistance = R(h.zplanes()[h.nseg]) - p.z;
+    const bool isFirst = vabs(firstDistance) < vabs(lastDistance);
+    const PolyhedronSeg &s = h.segs()[isFirst ? 0 : h.nseg - 1];
+    const R dt = isFirst ? firstDistance : lastDistance;
+    const R dt2 = dt * dt;
+    if (dt2 >= distanceSquared) return;
+    const V3<R> x{p.x, p.y, p.z + dt};
+    const Quad *q = h.quads();
+    if (!shell_contains<R>(q + s.outer, s.n_outer, x)) return;
+    if (s.n_inner > 0 && shell_contains<R>(q + s.inner, s.n_inner, x)) return;
+    if (h.phi_cutout && in_phi_cutout_wedge<R>(h, s, x)) return;
+    distanceSquared = dt2;
+  }
+
+  template <typename R>
+  VGM_DEV static int wedge_inside(const Data &h, V3<R> p)
+  {
+    bool ci = false, co = false;
+    wedge_generic<R, true>(h.w_a1x, h.w_a1y, h.w_a2x, h.w_a2y, h.w_n1x, h.w_n1y, h.w_n2x, h.w_n2y, h.w_lt_pi != 0,
+                           p, ci, co);
+    int r = kSurfaceV;
+    if (co) r = kOutsideV;
+    if (ci) r = kInsideV;
+    return r;
+  }
+
+  // Side-plane normal 0 of the segment's outer (else inner) quad at phiIndex:
+  // the radial direction used for the "radius" of a point on a z border.
+  template <typename R>
+  VGM_DEV static bool border_radius(const Data &h, const PolyhedronSeg &s, int phiIndex, V3<R> p, R &rdv)
+  {
+    const Quad *q = h.quads();
+    const Quad *qq = s.n_outer ? &q[s.outer + phiIndex] : (s.n_inner ? &q[s.inner + phiIndex] : nullptr);
+    if (!qq) return false;
+    rdv = vabs(pdot<R>(qq->sn[0], p));
+    return true;
+  }
+
+  // ScalarInsideSegBorder (:797-834)
+  template <typename R>
+  VGM_DEV static int inside_seg_border(const Data &h, V3<R> p, int zIndex)
+  {
+    const PolyhedronSeg &s = h.segs()[zIndex];
+    const int phiIndex = find_phi_segment<R>(h, p);
+    if (phiIndex < 0) return kOutsideV;
+    R rdv;
+    if (!border_radius<R>(h, s, phiIndex, p, rdv)) return kOutsideV;
+    const double *rn = h.rmin(), *rx = h.rmax();
+    const bool in1 = (rdv > R(rn[zIndex]) - R(kTolerance)) && (rdv < R(rx[zIndex]) + R(kTolerance));
+    const bool in2 = (rdv > R(rn[zIndex + 1]) - R(kTolerance)) && (rdv < R(rx[zIndex + 1]) + R(kTolerance));
+    if (in1 && in2) {
+      if ((rdv < R(rn[zIndex]) + R(kTolerance)) || (rdv > R(rx[zIndex]) - R(kTolerance)) ||
+          (rdv < R(rn[zIndex + 1]) + R(kTolerance)) || (rdv > R(rx[zIndex + 1]) - R(kTolerance)))
+        return kSurfaceV;
+      if (h.phi_cutout) return wedge_inside<R>(h, p);
+      return kInsideV;
+    }
+    if (!in1 && !in2) return kOutsideV;
+    return kSurfaceV;
+  }
+
+  // ScalarInsideSegPhi (:837-885)
+  template <typename R>
+  VGM_DEV static int inside_seg_phi(const Data &h, V3<R> p, int zIndex, int phiIndex)
+  {
+    if (phiIndex < 0) return kOutsideV;
+    const R dz = vabs(p.z - R(h.bt_offset)) - R(h.half_len);
+    if (dz > R(kHalfTolerance)) return kOutsideV;
+    if (h.samez()[zIndex] != 0.0) return inside_seg_border<R>(h, p, zIndex);
+    const PolyhedronSeg &s = h.segs()[zIndex];
+    const Quad *q = h.quads();
+    if (s.n_outer > 0) {
+      const int io = quad_inside<R>(q[s.outer + phiIndex], p);
+      if (io != kInsideV) return io;
+    }
+    if (s.n_inner > 0) {
+      const int ii = quad_inside<R>(q[s.inner + phiIndex], p);
+      if (ii == kInsideV) return kOutsideV;
+      if (ii == kSurfaceV) return kSurfaceV;
+    }
+    if (h.phi_cutout) {
+      const int ip = wedge_inside<R>(h, p);
+      if (ip != kInsideV) return ip;
+    }
+    if (vabs(dz) < R(kHalfTolerance)) return kSurfaceV;
+    return kInsideV;
+  }
+
+  // ScalarContainsKernel (:675-733)
+  template <typename R>
+  VGM_DEV static bool contains(const Data &h, V3<R> p)
+  {
+    if (!bounds_contain<R>(h, p)) return false;
+    const int zIndex = find_z_segment<R>(h, p.z);
+    if (!((zIndex >= 0) && (zIndex < h.nseg))) return false;
+    const PolyhedronSeg &s = h.segs()[zIndex];
+    const Quad *q = h.quads();
+    if (h.samez()[zIndex] != 0.0) {
+      const int phiIndex = find_phi_segment<R>(h, p);
+      if (phiIndex < 0) return false;
+      R rdv;
+      if (!border_radius<R>(h, s, phiIndex, p, rdv)) return false;
+      const double *rn = h.rmin(), *rx = h.rmax();
+      const bool in1 = (rdv >= R(rn[zIndex])) && (rdv <= R(rx[zIndex]));
+      const bool in2 = (rdv >= R(rn[zIndex + 1])) && (rdv <= R(rx[zIndex + 1]));
+      return in1 | in2;
+    }
+    if (!shell_contains<R>(q + s.outer, s.n_outer, p)) return false;
+    if (s.n_inner > 0 && shell_contains<R>(q + s.inner, s.n_inner, p)) return false;
+    if (h.phi_cutout && !shell_contains<R>(q + s.phi, s.n_phi, p)) return false;
+    return true;
+  }
+
+  // ScalarInsideKernel (:737-793)
+  template <typename R>
+  VGM_DEV static int inside(const Data &h, V3<R> p)
+  {
+    if (!bounds_contain<R>(h, p)) return kOutsideV;
+    int zIndex = find_z_segment<R>(h, p.z);
+    if (zIndex > h.nseg - 1) zIndex = h.nseg - 1;
+    if (zIndex < 0) zIndex = 0;
+    if (h.samez()[zIndex] != 0.0) return inside_seg_border<R>(h, p, zIndex);
+    const PolyhedronSeg &s = h.segs()[zIndex];
+    const Quad *q = h.quads();
+    const int io = shell_inside<R>(q + s.outer, s.n_outer, p);
+    if (io != kInsideV) return io;
+    if (s.n_inner > 0) {
+      const int ii = shell_inside<R>(q + s.inner, s.n_inner, p);
+      if (ii == kInsideV) return kOutsideV;
+      if (ii == kSurfaceV) return kSurfaceV;
+    }
+    if (h.phi_cutout) {
+      const int ip = shell_inside<R>(q + s.phi, s.n_phi, p);
+      if (ip != kInsideV) return ip;
+    }
+    const R dz = vabs(vabs(p.z - R(h.bt_offset)) - R(h.half_len));
+    if (dz < R(kTolerance)) return kSurfaceV;
+    return kInsideV;
+  }
+
+  // DistanceToIn (:895-960)
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &h, V3<R> p, V3<R> d, R step_max)
+  {
+    const double *zp = h.zplanes();
+    if ((p.z < R(zp[0]) + R(kTolerance)) && d.z <= R(0)) return inf_length<R>();
+    if ((p.z > R(zp[h.nseg]) - R(kTolerance)) && d.z >= R(0)) return inf_length<R>();
+    if (inside<R>(h, p) == kInsideV) return R(-1);
+    R tubeDistance = R(0);
+    {
+      const V3<R> bp{p.x, p.y, p.z - R(h.bt_offset)};
+      if (!TubeShape::contains<R>(h.btube, bp)) {
+        tubeDistance = TubeShape::distance_to_in<R>(h.btube, bp, d, step_max);
+        if (tubeDistance == inf_length<R>()) return inf_length<R>();
+      }
+    }
+    int zIndex = find_z_segment<R>(h, p.z);
+    const int zMax = h.nseg;
+    zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
+    R distance = inf_length<R>();
+    if (d.z >= R(0)) {
+      for (; zIndex < zMax; ++zIndex) {
+        distance = distance_to_in_zseg<R>(h, zIndex, p, d);
+        if (distance >= R(0) && distance < inf_length<R>()) break;
+      }
+    } else {
+      for (; zIndex >= 0; --zIndex) {
+        distance = distance_to_in_zseg<R>(h, zIndex, p, d);
+        if (distance >= R(0) && distance < inf_length<R>()) break;
+      }
+    }
+    distance_to_endcaps<R>(h, false, p, d, distance);
+    return (distance >= tubeDistance - R(1E-6)) ? distance : inf_length<R>();
+  }
+
+  // SafetyToIn / SafetyToOut (:965-1075): walk z segments outwards from the
+  // point's segment while the z gap is below the best squared distance.
+  template <typename R>
+  VGM_DEV static R safety(const Data &h, V3<R> p, bool pt_inside)
+  {
+    R s2 = inf_length<R>();
+    const int zMax = h.nseg;
+    int zIndex = find_z_segment<R>(h, p.z);
+    zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
+    int phiIndex = find_phi_segment<R>(h, p);
+    const int in = inside_seg_phi<R>(h, p, zIndex, phiIndex);
+    if (in == kSurfaceV) return R(0);
+    const bool contains = (in == kInsideV);
+    if (contains ^ pt_inside) return R(-1);
+    const double *zp = h.zplanes();
+    for (int z = zIndex; z < zMax;) {
+      s2 = vmin(s2, safety_zseg2<R>(h, z, phiIndex, p, pt_inside));
+      ++z;
+      const R dz = R(zp[z]) - p.z;
+      if (dz * dz > s2) break;
+    }
+    for (int z = zIndex - 1; z >= 0; --z) {
+      s2 = vmin(s2, safety_zseg2<R>(h, z, phiIndex, p, pt_inside));
+      const R dz = p.z - R(zp[z]);
+      if (dz * dz > s2) break;
+    }
+    safety_to_endcaps2<R>(h, p, s2);
+    return vsqrt(s2);
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_in(const Data &h, V3<R> p)
+  {
+    return safety<R>(h, p, false);
+  }
+  template <typename R>
+  VGM_DEV static R safety_to_out(const Data &h, V3<R> p)
+  {
+    return safety<R>(h, p, true);
+  }
+
+  // DistanceToOut (:1080-1140)
+  template <typename R>
+  VGM_DEV static R distance_to_out(const Data &h, V3<R> p, V3<R> d, R /*step_max*/)
+  {
+    const int zMax = h.nseg;
+    const double *zp = h.zplanes();
+    if ((p.z < R(zp[0]) - R(kTolerance)) || (p.z > R(zp[zMax]) + R(kTolerance))) return R(-1);
+    if (inside<R>(h, p) == kOutsideV) return R(-1);
+    int zIndex = find_z_segment<R>(h, p.z);
+    zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
+    R distance = inf_length<R>();
+    if (d.z >= R(0)) {
+      for (; zIndex < zMax; ++zIndex) {
+        distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
+        if (distance >= R(0) && distance < inf_length<R>()) break;
+        if (R(zp[zIndex]) - p.z > distance) break;
+      }
+    } else {
+      for (; zIndex >= 0; --zIndex) {
+        distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
+        if (distance >= R(0) && distance < inf_length<R>()) break;
+        if (p.z - R(zp[zIndex]) > distance) break;
+      }
+    }
+    distance_to_endcaps<R>(h, true, p, d, distance);
+    if (distance >= inf_length<R>()) distance = R(0);
+    return distance;
+  }
+};
+
+}  // namespace vgm

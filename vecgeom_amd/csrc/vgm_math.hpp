@@ -19,7 +19,7 @@ constexpr double kTwoPi = 2. * kPi;
 constexpr double kHalfPi = 0.5 * kPi;
 constexpr double kTiny = 1e-30;
 
-#define VGM_DEV __device__ __forceinline__
+#define VGM_DEV __host__ __device__ __forceinline__
 
 template <typename R>
 struct V3 {
@@ -41,6 +41,8 @@ VGM_DEV float vsqrt(float a) { return __builtin_sqrtf(a); }
 // copysign(1, x): VecCore's Sign for scalar doubles.
 VGM_DEV double vsign(double x) { return __builtin_copysign(1.0, x); }
 VGM_DEV float vsign(float x) { return __builtin_copysignf(1.0f, x); }
+VGM_DEV double vcopysign(double a, double b) { return __builtin_copysign(a, b); }
+VGM_DEV float vcopysign(float a, float b) { return __builtin_copysignf(a, b); }
 // NonZero(x) = x + copysign(1e-30, x)  (VecGeom/base/Global.h:170-177)
 VGM_DEV double nonzero(double x) { return x + __builtin_copysign(1.0e-30, x); }
 VGM_DEV float nonzero(float x) { return x + __builtin_copysignf(1.0e-30f, x); }

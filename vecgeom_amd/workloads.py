@@ -73,7 +73,9 @@ def solid_spec(family: str, seed: int, index: int):
         z = [-0.5 * nz * dz + i * dz for i in range(nz)]
         rmax = [_lu(nx(), 5.0, 100.0) for _ in range(nz)]
         rmin = [r * 0.4 * nx() if i % 2 else 0.0 for i, r in enumerate(rmax)]
-        dphi = two_pi if nx() < 0.5 else 1.5 + (two_pi - 1.6) * nx()
+        # full phi, or a cut below 180 degrees: above it the reference's Inside
+        # treats the two phi planes as an intersection (non-convex wedge)
+        dphi = two_pi if nx() < 0.5 else 0.8 + 2.3 * nx()
         return ("polyhedron", (0.0, dphi, 8, tuple(z), tuple(rmin), tuple(rmax)))
     if family == "tessellated":
         # convex hull of points on an ellipsoid; triangle count log-uniform in [1k, 20k]
