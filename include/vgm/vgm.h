@@ -142,8 +142,13 @@ int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, co
  * A step that does not cross ends in an interaction with a new isotropic
  * direction; outside the solid, coordinates wrap periodically within 1.3x
  * the solid's bounding box.  Positions and directions are updated in place.
- * All pointers are device memory.  Tracks are processed by a persistent,
- * shape-sorted work queue; vgm_batch_autotune picks block, grid and tile size.
+ * All pointers are device memory.  Tracks are cut into tiles sorted by shape
+ * family; each family runs as its own kernel, the families concurrently.
+ * vgm_batch_autotune picks block, grid and tile size. */
+/* Launch shape: block = threads per workgroup (multiple of 64, at most 256);
+ * grid > 0 = persistent launch with that many workgroups in total, shared
+ * between families by cost, pulling tiles from a queue; grid <= 0 = one
+ * workgroup per tile; tile_tracks = tracks per tile.
  * VGM_MIXED selects float filtering with FP64 boundary steps. */
 typedef struct vgm_batch vgm_batch;
 int vgm_batch_create(vgm_ctx *ctx, int nsolids, const int *shapes, const int64_t *offsets, const double *mean_step,

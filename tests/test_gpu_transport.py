@@ -39,7 +39,8 @@ def _setup(ctx, total, nsteps):
     return b, dev, side, cr, ref, side_o, cr_o, arr
 
 
-@pytest.mark.parametrize("launch", [None, (128, 64, 512), (512, 1024, 8192)], ids=["default", "small", "large"])
+@pytest.mark.parametrize("launch", [None, (128, 64, 512), (256, 1024, 8192), (64, 0, 2048)],
+                         ids=["default", "small", "large", "per-tile"])
 def test_transport_fp64_bit_exact(ctx, launch):
     print(f"[transport] fp64 launch={launch}", flush=True)
     b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)

@@ -9,19 +9,27 @@
 
 using namespace vgm;
 
+// Families run concurrently on this many streams of their own, forked from
+// and joined back into the context stream.
+constexpr int kFamilyStreams = 4;
+
 struct vgm_batch {
   vgm_ctx *ctx = nullptr;
   std::vector<SolidRec> solids;
   std::vector<Tile> tiles;
+  std::vector<FamilyLaunch> fams;  // one per family present, in tile order
+  double weight[kNumShapeTypes] = {};  // relative cost of each family's tiles
   int64_t ntracks = 0;
   LaunchShape shape{256, 0, 2048};
   bool tiles_dirty = true;
-  char *dbuf = nullptr;  // solids | tiles | counter | fp64 step counter
+  char *dbuf = nullptr;  // solids | tiles | counters | fp64 step counter
   size_t dcap = 0;
   SolidRec *d_solids = nullptr;
   Tile *d_tiles = nullptr;
-  int *d_counter = nullptr;
+  int *d_counters = nullptr;  // one per family, then one for classify
   unsigned long long *d_fp64 = nullptr;
+  hipStream_t fam_stream[kFamilyStreams] = {};
+  hipEvent_t fork = nullptr, join[kFamilyStreams] = {};
 };
 
 namespace {
@@ -35,9 +43,12 @@ void build_tiles(vgm_batch *b)
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int c) { return b->solids[a].type < b->solids[c].type; });
   b->tiles.clear();
+  b->fams.clear();
   const int64_t tt = b->shape.tile_tracks;
   for (int s : order) {
     const SolidRec &r = b->solids[s];
+    if (r.end > r.begin && (b->fams.empty() || b->fams.back().type != r.type))
+      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr});
     for (int64_t t = r.begin; t < r.end; t += tt) {
       Tile tl;
       tl.solid = s;
@@ -46,7 +57,19 @@ void build_tiles(vgm_batch *b)
       tl.end = std::min<int64_t>(t + tt, r.end);
       b->tiles.push_back(tl);
     }
+    if (!b->fams.empty()) b->fams.back().t1 = int(b->tiles.size());
   }
+}
+
+int ensure_streams(vgm_batch *b)
+{
+  if (b->fork) return VGM_OK;
+  for (int i = 0; i < kFamilyStreams; ++i) {
+    VGM_HIP(hipStreamCreateWithFlags(&b->fam_stream[i], hipStreamNonBlocking));
+    VGM_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+  }
+  VGM_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  return VGM_OK;
 }
 
 int upload(vgm_batch *b)
@@ -58,7 +81,7 @@ int upload(vgm_batch *b)
   build_tiles(b);
   const size_t s1 = align256(b->solids.size() * sizeof(SolidRec) + 1);
   const size_t s2 = align256(b->tiles.size() * sizeof(Tile) + 1);
-  const size_t total = s1 + s2 + 256;
+  const size_t total = s1 + s2 + 512;
   if (b->dcap < total) {
     if (b->dbuf) {
       VGM_HIP(hipStreamSynchronize(c->stream));
@@ -69,8 +92,8 @@ int upload(vgm_batch *b)
   }
   b->d_solids = reinterpret_cast<SolidRec *>(b->dbuf);
   b->d_tiles = reinterpret_cast<Tile *>(b->dbuf + s1);
-  b->d_counter = reinterpret_cast<int *>(b->dbuf + s1 + s2);
-  b->d_fp64 = reinterpret_cast<unsigned long long *>(b->dbuf + s1 + s2 + 64);
+  b->d_counters = reinterpret_cast<int *>(b->dbuf + s1 + s2);
+  b->d_fp64 = reinterpret_cast<unsigned long long *>(b->dbuf + s1 + s2 + 256);
   VGM_HIP(hipMemcpyAsync(b->d_solids, b->solids.data(), b->solids.size() * sizeof(SolidRec), hipMemcpyHostToDevice,
                          c->stream));
   VGM_HIP(hipMemcpyAsync(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
@@ -95,7 +118,7 @@ TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *d
   a.dx = dx, a.dy = dy, a.dz = dz;
   a.side = side;
   a.crossings = crossings;
-  a.counter = b->d_counter;
+  a.counter = b->d_counters + kNumShapeTypes;
   a.fp64_steps = b->d_fp64;
   return a;
 }
@@ -103,10 +126,57 @@ TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *d
 LaunchShape resolved(vgm_batch *b)
 {
   LaunchShape ls = b->shape;
-  if (ls.grid <= 0) ls.grid = std::max(1, b->ctx->num_cus) * 4;
-  // never more workgroups than tiles: the surplus would only spin on the queue
-  ls.grid = int(std::min<size_t>(size_t(ls.grid), std::max<size_t>(1, b->tiles.size())));
+  // persistent: never more workgroups than tiles, the surplus would only spin
+  if (ls.grid > 0) ls.grid = int(std::min<size_t>(size_t(ls.grid), std::max<size_t>(1, b->tiles.size())));
   return ls;
+}
+
+// All families of one transport call: counters reset on the context stream,
+// then each family's kernel on one of the family streams (heaviest families
+// first, each to the least loaded stream), joined back into the context
+// stream.  A single family runs on the context stream directly.
+hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const LaunchShape &ls)
+{
+  hipStream_t main = b->ctx->stream;
+  hipError_t e = launch_reset(main, b->d_counters, kNumShapeTypes + 1, a.fp64_steps);
+  if (e != hipSuccess) return e;
+  double wsum = 0;
+  for (const FamilyLaunch &f : b->fams) wsum += b->weight[f.type];
+  std::vector<FamilyLaunch> fl = b->fams;
+  for (FamilyLaunch &f : fl) {
+    const int nt = f.t1 - f.t0;
+    if (ls.grid > 0) {
+      const double share = wsum > 0 ? b->weight[f.type] / wsum : 1.0 / double(fl.size());
+      f.grid = std::max(1, std::min(nt, int(std::lround(share * ls.grid))));
+      f.counter = b->d_counters + f.type;
+    } else {
+      f.grid = nt;
+      f.counter = nullptr;
+    }
+  }
+  if (fl.size() == 1) return launch_family(main, mode, a, fl[0], ls.block);
+  std::sort(fl.begin(), fl.end(),
+            [&](const FamilyLaunch &x, const FamilyLaunch &y) { return b->weight[x.type] > b->weight[y.type]; });
+  double load[kFamilyStreams] = {};
+  bool used[kFamilyStreams] = {};
+  if ((e = hipEventRecord(b->fork, main)) != hipSuccess) return e;
+  for (const FamilyLaunch &f : fl) {
+    int s = 0;
+    for (int i = 1; i < kFamilyStreams; ++i)
+      if (load[i] < load[s]) s = i;
+    if (!used[s]) {
+      if ((e = hipStreamWaitEvent(b->fam_stream[s], b->fork, 0)) != hipSuccess) return e;
+      used[s] = true;
+    }
+    load[s] += b->weight[f.type];
+    if ((e = launch_family(b->fam_stream[s], mode, a, f, ls.block)) != hipSuccess) return e;
+  }
+  for (int s = 0; s < kFamilyStreams; ++s) {
+    if (!used[s]) continue;
+    if ((e = hipEventRecord(b->join[s], b->fam_stream[s])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(main, b->join[s], 0)) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
@@ -142,6 +212,7 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
     b->solids.push_back(s);
   }
   b->ntracks = offsets[nsolids];
+  for (const SolidRec &s : b->solids) b->weight[s.type] += double(s.end - s.begin);
   *out = b;
   return VGM_OK;
 }
@@ -149,18 +220,21 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
 int vgm_batch_destroy(vgm_batch *b)
 {
   if (!b) return VGM_OK;
-  if (b->dbuf) {
-    (void)hipStreamSynchronize(b->ctx->stream);
-    (void)hipFree(b->dbuf);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->dbuf) (void)hipFree(b->dbuf);
+  for (int i = 0; i < kFamilyStreams; ++i) {
+    if (b->fam_stream[i]) (void)hipStreamDestroy(b->fam_stream[i]);
+    if (b->join[i]) (void)hipEventDestroy(b->join[i]);
   }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return VGM_OK;
 }
 
 int vgm_batch_set_launch(vgm_batch *b, int block, int grid, int tile_tracks)
 {
-  if (!b || block <= 0 || block > 1024 || block % 64 || tile_tracks <= 0)
-    return fail(VGM_EINVAL, "block must be a multiple of 64 up to 1024, tile_tracks > 0");
+  if (!b || block <= 0 || block > kTransportMaxBlock || block % 64 || tile_tracks <= 0)
+    return fail(VGM_EINVAL, "block must be a multiple of 64 up to 256, tile_tracks > 0");
   if (tile_tracks != b->shape.tile_tracks) b->tiles_dirty = true;
   b->shape = LaunchShape{block, grid, tile_tracks};
   return VGM_OK;
@@ -184,7 +258,9 @@ int vgm_batch_classify(vgm_batch *b, const double *x, const double *y, const dou
   if (rc) return rc;
   TransportArgs a = make_args(b, const_cast<double *>(x), const_cast<double *>(y), const_cast<double *>(z), nullptr,
                               nullptr, nullptr, side, nullptr, 0, 0, 0);
-  VGM_HIP(launch_transport(b->ctx->stream, kTransportClassify, a, resolved(b)));
+  VGM_HIP(launch_reset(b->ctx->stream, b->d_counters, kNumShapeTypes + 1, nullptr));
+  const int grid = int(std::min<size_t>(b->tiles.size(), size_t(4 * std::max(1, b->ctx->num_cus))));
+  VGM_HIP(launch_classify(b->ctx->stream, a, std::max(1, grid), 256));
   VGM_HIP(hipStreamSynchronize(b->ctx->stream));
   return VGM_OK;
 }
@@ -200,7 +276,8 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
   if (rc) return rc;
   TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed);
   const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
-  VGM_HIP(launch_transport(b->ctx->stream, mode, a, resolved(b)));
+  if ((rc = ensure_streams(b))) return rc;
+  VGM_HIP(launch_families(b, mode, a, resolved(b)));
   if (fp64_steps) {
     VGM_HIP(hipMemcpyAsync(fp64_steps, b->d_fp64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->ctx->stream));
@@ -211,8 +288,12 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
   return VGM_OK;
 }
 
-// Times every candidate launch shape on scratch copies of the track state
-// (the caller's buffers are left untouched) and keeps the fastest.
+// Autotuner.  Works on scratch copies of the track state (the caller's
+// buffers are left untouched).  First each family is timed alone, which
+// gives the weights that split the persistent grid and balance the family
+// streams; then every candidate launch shape is timed on the whole batch and
+// the fastest is kept.  Candidates: workgroup size, tile size, and either one
+// workgroup per tile or a persistent grid of 2, 4 or 8 workgroups per CU.
 int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const double *z, const double *dx,
                        const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,
                        int *best3)
@@ -220,6 +301,8 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   if (!b || !x || !y || !z || !dx || !dy || !dz || !side || nsteps <= 0) return fail(VGM_EINVAL, "null argument");
   vgm_ctx *c = b->ctx;
   VGM_HIP(hipSetDevice(c->device));
+  int rc = ensure_streams(b);
+  if (rc) return rc;
   const size_t n = size_t(b->ntracks);
   char *work = nullptr;
   const size_t bytes = align256(6 * n * sizeof(double)) + align256(n) + align256(n * sizeof(int32_t));
@@ -228,54 +311,79 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   double *wy = wx + n, *wz = wy + n, *wdx = wz + n, *wdy = wdx + n, *wdz = wdy + n;
   int8_t *ws = reinterpret_cast<int8_t *>(work + align256(6 * n * sizeof(double)));
   int32_t *wc = reinterpret_cast<int32_t *>(work + align256(6 * n * sizeof(double)) + align256(n));
-  hipEvent_t e0, e1;
-  VGM_HIP(hipEventCreate(&e0));
-  VGM_HIP(hipEventCreate(&e1));
-  const int blocks[] = {128, 256, 512};
-  const int tiles[] = {512, 2048, 8192};
-  const int per_cu[] = {2, 4, 8};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
   const LaunchShape keep = b->shape;
+  const int cus = std::max(1, c->num_cus);
   float best = 1e30f;
   LaunchShape best_shape = keep;
-  int rc = VGM_OK;
-  for (int bl : blocks)
-    for (int tt : tiles)
-      for (int pc : per_cu) {
-        if (tt < bl) continue;
-        vgm_batch_set_launch(b, bl, pc * std::max(1, c->num_cus), tt);
-        if ((rc = upload(b))) goto done;
-        for (int rep = 0; rep < 2; ++rep) {
-          if (hipMemcpyAsync(wx, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(wy, y, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(wz, z, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(wdx, dx, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(wdy, dy, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(wdz, dz, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemcpyAsync(ws, side, n, hipMemcpyDeviceToDevice, c->stream) ||
-              hipMemsetAsync(wc, 0, n * sizeof(int32_t), c->stream)) {
-            rc = fail(VGM_EHIP, "autotune staging copy failed");
-            goto done;
-          }
+  double weight[kNumShapeTypes] = {};
+
+  auto stage = [&]() -> bool {
+    return !(hipMemcpyAsync(wx, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(wy, y, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(wz, z, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(wdx, dx, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(wdy, dy, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(wdz, dz, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemcpyAsync(ws, side, n, hipMemcpyDeviceToDevice, c->stream) ||
+             hipMemsetAsync(wc, 0, n * sizeof(int32_t), c->stream));
+  };
+  // time `launch` on freshly staged state, second of two repetitions
+  auto timed = [&](auto &&launch, float &ms) -> int {
+    for (int rep = 0; rep < 2; ++rep) {
+      if (!stage()) return fail(VGM_EHIP, "autotune staging copy failed");
+      (void)hipEventRecord(e0, c->stream);
+      if (launch() != hipSuccess) return fail(VGM_EHIP, "autotune launch failed");
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
+    }
+    return VGM_OK;
+  };
+
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(VGM_EHIP, "hipEventCreate failed");
+    goto done;
+  }
+  // 1. family weights: each family alone, one workgroup of 256 per 2048-track tile
+  vgm_batch_set_launch(b, 256, 0, 2048);
+  if ((rc = upload(b))) goto done;
+  for (const FamilyLaunch &f0 : b->fams) {
+    FamilyLaunch f = f0;
+    f.grid = f.t1 - f.t0;
+    f.counter = nullptr;
+    TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1);
+    float ms = 0;
+    if ((rc = timed([&]() { return launch_family(c->stream, mode, a, f, 256); }, ms))) goto done;
+    weight[f.type] = std::max(1e-3, double(ms));
+  }
+  for (int t = 0; t < kNumShapeTypes; ++t)
+    if (weight[t] > 0) b->weight[t] = weight[t];
+  // 2. launch shapes on the whole batch
+  {
+    const int blocks[] = {64, 128, 256};
+    const int tiles[] = {512, 2048, 8192};
+    const int per_cu[] = {0, 2, 4, 8};
+    for (int bl : blocks)
+      for (int tt : tiles)
+        for (int pc : per_cu) {
+          if (tt < bl) continue;
+          vgm_batch_set_launch(b, bl, pc * cus, tt);
+          if ((rc = upload(b))) goto done;
           TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1);
-          const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
-          (void)hipEventRecord(e0, c->stream);
-          if (launch_transport(c->stream, mode, a, resolved(b)) != hipSuccess) {
-            rc = fail(VGM_EHIP, "autotune launch failed");
-            goto done;
-          }
-          (void)hipEventRecord(e1, c->stream);
-          (void)hipEventSynchronize(e1);
+          const LaunchShape ls = resolved(b);
           float ms = 0;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (rep == 1 && ms < best) {
+          if ((rc = timed([&]() { return launch_families(b, mode, a, ls); }, ms))) goto done;
+          if (ms < best) {
             best = ms;
-            best_shape = LaunchShape{bl, pc * std::max(1, c->num_cus), tt};
+            best_shape = LaunchShape{bl, pc * cus, tt};
           }
         }
-      }
+  }
 done:
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(work);
   if (rc) {

@@ -44,12 +44,30 @@ struct TransportArgs {
   unsigned long long *fp64_steps;
 };
 
+// Upper bound of the workgroup size of the transport kernels.  It is also
+// their launch bound: 256 threads let the compiler give a wave up to 256
+// VGPRs, which the FP64 shape paths need to stay out of scratch.
+constexpr int kTransportMaxBlock = 256;
+
 struct LaunchShape {
   int block;        // threads per workgroup
   int grid;         // persistent workgroups
   int tile_tracks;  // tracks per tile
 };
 
-hipError_t launch_transport(hipStream_t stream, int mode, const TransportArgs &a, const LaunchShape &ls);
+// One family's share of a transport launch: tiles [t0, t1) of the sorted
+// queue.  counter != nullptr selects the persistent form with `grid`
+// workgroups; otherwise one workgroup runs one tile.
+struct FamilyLaunch {
+  int type;
+  int t0, t1;
+  int grid;
+  int *counter;
+};
+
+hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps);
+hipError_t launch_family(hipStream_t stream, int mode, const TransportArgs &a, const FamilyLaunch &f, int block);
+// Initial inside flags (persistent over all tiles, uses a.counter).
+hipError_t launch_classify(hipStream_t stream, const TransportArgs &a, int grid, int block);
 
 }  // namespace vgm

@@ -1,11 +1,14 @@
 // Batched heterogeneous track-step transport: many solids of different
-// families, uneven track counts, one persistent launch.
+// families, uneven track counts.
 //
 // Work is cut into tiles of consecutive tracks of one solid; tiles are sorted
-// by shape family so that concurrently resident waves run the same code and
-// the same parameter block.  Workgroups pull tile indices from a device
-// counter until the queue is drained, so the grid is sized to the machine
-// (a multiple of the CU count), not to the work.  Each track keeps its state in
+// by shape family, and each family's range of tiles is run by a kernel
+// instantiated for that family alone, so its register allocation is that of
+// its own code path (a box step needs a third of the registers of a
+// polyhedron step).  The family kernels run concurrently on separate streams.
+// Either one workgroup runs one tile, or the launch is persistent: a grid
+// sized to the family's share of the machine pulls tiles from a device
+// counter until its range is drained.  Each track keeps its state in
 // registers for all steps and touches HBM once on entry and once on exit.
 //
 // Precision modes:
@@ -124,36 +127,36 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const Tile &tl
   }
 }
 
-template <int Mode>
-struct TileRunner {
-  const TransportArgs &a;
-  const SolidRec &sr;
-  const Tile &tl;
-  uint32_t &fp64_steps;
-  template <class S>
-  __device__ void operator()()
-  {
-    run_tile<S, Mode>(a, sr, tl, fp64_steps);
-  }
-};
-
 }  // namespace
 
-template <int Mode>
-__global__ void transport_kernel(TransportArgs a)
+// One shape family, one precision mode.  Tiles [t0, t1) of the type-sorted
+// queue belong to this family.  With a counter the launch is persistent:
+// the grid is sized to the family's share of the machine and workgroups pull
+// tiles until the range is drained.  Without one, workgroup b runs tile t0+b.
+// Every family has its own kernel, so each gets the register allocation of
+// its own code path rather than the largest one of all families.
+template <class S, int Mode>
+__global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(TransportArgs a, int t0, int t1,
+                                                                              int *counter)
 {
-  __shared__ int tile_s;
   uint32_t fp64_steps = 0;
-  for (;;) {
-    if (threadIdx.x == 0) tile_s = atomicAdd(a.counter, 1);
-    __syncthreads();
-    const int t = tile_s;
-    __syncthreads();
-    if (t >= a.ntiles) break;
-    const Tile tl = a.tiles[t];
-    const SolidRec sr = a.solids[tl.solid];
-    TileRunner<Mode> r{a, sr, tl, fp64_steps};
-    dispatch_shape_type(sr.type, r);
+  if (!counter) {
+    const int t = t0 + int(blockIdx.x);
+    if (t < t1) {
+      const Tile tl = a.tiles[t];
+      run_tile<S, Mode>(a, a.solids[tl.solid], tl, fp64_steps);
+    }
+  } else {
+    __shared__ int tile_s;
+    for (;;) {
+      if (threadIdx.x == 0) tile_s = t0 + atomicAdd(counter, 1);
+      __syncthreads();
+      const int t = tile_s;
+      __syncthreads();
+      if (t >= t1) break;
+      const Tile tl = a.tiles[t];
+      run_tile<S, Mode>(a, a.solids[tl.solid], tl, fp64_steps);
+    }
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
 }
@@ -174,7 +177,7 @@ struct Classifier {
 }  // namespace
 
 // Initial inside/outside flag of every track: Contains on its solid.
-__global__ void classify_kernel(TransportArgs a)
+__global__ __launch_bounds__(kTransportMaxBlock) void classify_kernel(TransportArgs a)
 {
   __shared__ int tile_s;
   for (;;) {
@@ -190,21 +193,51 @@ __global__ void classify_kernel(TransportArgs a)
   }
 }
 
-__global__ void reset_counter_kernel(int *counter, unsigned long long *fp64_steps)
+__global__ void reset_counters_kernel(int *counters, int n, unsigned long long *fp64_steps)
 {
-  *counter = 0;
-  if (fp64_steps) *fp64_steps = 0;
+  const int i = int(threadIdx.x);
+  if (i < n) counters[i] = 0;
+  if (i == 0 && fp64_steps) *fp64_steps = 0;
 }
 
-hipError_t launch_transport(hipStream_t stream, int mode, const TransportArgs &a, const LaunchShape &ls)
+namespace {
+struct FamilyLauncher {
+  hipStream_t stream;
+  int mode;
+  const TransportArgs &a;
+  const FamilyLaunch &f;
+  int block;
+  template <class S>
+  void operator()()
+  {
+    const int grid = f.counter ? f.grid : (f.t1 - f.t0);
+    if (grid <= 0) return;
+    if (mode == kTransportMixed)
+      hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed>), dim3(grid), dim3(block), 0, stream, a, f.t0,
+                         f.t1, f.counter);
+    else
+      hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64>), dim3(grid), dim3(block), 0, stream, a, f.t0,
+                         f.t1, f.counter);
+  }
+};
+}  // namespace
+
+hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps)
 {
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, a.counter, a.fp64_steps);
-  if (mode == kTransportClassify)
-    hipLaunchKernelGGL(classify_kernel, dim3(ls.grid), dim3(ls.block), 0, stream, a);
-  else if (mode == kTransportMixed)
-    hipLaunchKernelGGL(transport_kernel<kTransportMixed>, dim3(ls.grid), dim3(ls.block), 0, stream, a);
-  else
-    hipLaunchKernelGGL(transport_kernel<kTransportFP64>, dim3(ls.grid), dim3(ls.block), 0, stream, a);
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n, fp64_steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_family(hipStream_t stream, int mode, const TransportArgs &a, const FamilyLaunch &f, int block)
+{
+  FamilyLauncher l{stream, mode, a, f, block};
+  if (!dispatch_shape_type(f.type, l)) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_classify(hipStream_t stream, const TransportArgs &a, int grid, int block)
+{
+  hipLaunchKernelGGL(classify_kernel, dim3(grid), dim3(block), 0, stream, a);
   return hipGetLastError();
 }
 
