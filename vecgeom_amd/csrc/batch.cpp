@@ -48,7 +48,8 @@ void build_tiles(vgm_batch *b)
   for (int s : order) {
     const SolidRec &r = b->solids[s];
     if (r.end > r.begin && (b->fams.empty() || b->fams.back().type != r.type))
-      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr});
+      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr, 0});
+    if (r.end > r.begin) b->fams.back().stage_nd = std::max(b->fams.back().stage_nd, r.nd);
     for (int64_t t = r.begin; t < r.end; t += tt) {
       Tile tl;
       tl.solid = s;
@@ -199,6 +200,8 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
     SolidRec s{};
     s.type = r.type;
     s.off = r.off;
+    s.nd = r.nd;
+    s.pad = 0;
     s.begin = offsets[i];
     s.end = offsets[i + 1];
     s.lambda = mean_step[i];

@@ -18,6 +18,8 @@ int HostGeometry::add_shape(int type, const void *block, size_t ndoubles, int n0
   r.off = int32_t(pool.size());
   r.n0 = n0;
   r.n1 = n1;
+  r.nd = int32_t(ndoubles);
+  r.pad = 0;
   for (int k = 0; k < 3; ++k) {
     r.bmin[k] = bmin[k];
     r.bmax[k] = bmax[k];

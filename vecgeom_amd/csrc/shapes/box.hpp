@@ -8,6 +8,7 @@ namespace vgm {
 
 struct BoxShape {
   using Data = BoxData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
 
   // max(|p| - h) (BoxImplementation.h:76-90)
   template <typename R>

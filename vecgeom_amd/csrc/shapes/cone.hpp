@@ -10,6 +10,7 @@ namespace vgm {
 
 struct ConeShape {
   using Data = ConeData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
 
   template <typename R>
   VGM_DEV static R plus_tol_sq(R x, double tol) { return x * (x + R(2.0 * tol)); }

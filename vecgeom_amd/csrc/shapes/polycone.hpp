@@ -10,6 +10,7 @@ namespace vgm {
 
 struct PolyconeShape {
   using Data = PolyconeData;
+  static constexpr bool kStage = true;  // parameter block staged in LDS
 
   VGM_DEV static int section_index(const Data &d, double zp)
   {

@@ -16,6 +16,7 @@ namespace vgm {
 
 struct PolyhedronShape {
   using Data = PolyhedronData;
+  static constexpr bool kStage = true;  // parameter block staged in LDS
   using Quad = PolyhedronQuad;
 
   template <typename R>
@@ -149,12 +150,14 @@ struct PolyhedronShape {
   {
     const R distance = plane<R>(q, p);
     const V3<R> x{p.x - R(q.n[0]) * distance, p.y - R(q.n[1]) * distance, p.z - R(q.n[2]) * distance};
-    bool within[4];
-    for (int j = 0; j < 4; ++j) within[j] = side_ok<R>(q, j, x, R(0));
-    if (within[0] && within[1] && within[2] && within[3]) return distance * distance;
+    unsigned outside = 0;  // bit j: the foot point is outside side plane j
+    for (int j = 0; j < 4; ++j) outside |= side_ok<R>(q, j, x, R(0)) ? 0u : (1u << j);
+    if (!outside) return distance * distance;
     R dsq = inf_length<R>();
+    // one call site for the edge distance keeps the kernel small
+#pragma unroll 1
     for (int j = 0; j < 4; ++j) {
-      if (!within[j]) {
+      if (outside & (1u << j)) {
         const R e = segment_dist2<R>(q.c[j], q.c[(j + 1) & 3], p);
         if (e < dsq) dsq = e;
       }
@@ -249,78 +252,70 @@ struct PolyhedronShape {
     return s0 && s1;
   }
 
-  // ScalarSafetyToZSegmentSquared (:448-560)
+  // ScalarSafetyToZSegmentSquared (:448-560).  The reference's branches pick
+  // at most four quads and keep the smallest squared distance (some of them
+  // also record the phi index of the winner).  Here the branches only fill a
+  // list of (quad, phi index to record) and one loop evaluates it, so the
+  // quad distance is compiled once; the comparisons and their order are the
+  // reference's.
   template <typename R>
   VGM_DEV static R safety_zseg2(const Data &h, int si, int &phiIndex, V3<R> p, bool pt_inside)
   {
     const PolyhedronSeg &s = h.segs()[si];
-    const Quad *q = h.quads();
     const bool in_cutout = phiIndex < 0;
+    uint64_t cq = 0;   // quad indices, 16 bits each
+    uint32_t cp = 0;   // phi index to record + 1, 8 bits each (0: none)
+    int nc = 0;
+    bool first_assigns = false;  // the first candidate is assigned, not compared
+    auto add = [&](int quad, int phi_rec) {
+      cq |= uint64_t(uint32_t(quad) & 0xffffu) << (16 * nc);
+      cp |= uint32_t(phi_rec + 1) << (8 * nc);
+      ++nc;
+    };
+    bool done = false;
+    if (h.phi_cutout && s.n_phi == 2 && (pt_inside || in_cutout)) {
+      first_assigns = true;
+      add(s.phi, -1);
+      add(s.phi + 1, -1);
+      if (in_cutout) {
+        if (h.large_cutout && s.n_inner > 0) {
+          add(s.inner, 0);
+          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1);
+        }
+        done = true;
+      }
+    }
+    if (!done) {
+      if (in_cutout && si > 0 && si < h.nseg - 1 && h.zplanes()[si] == h.zplanes()[si + 1]) {
+        if (s.n_outer > 0) {
+          add(s.outer, 0);
+          if (s.n_outer > 1) add(s.outer + s.n_outer - 1, s.n_outer - 1);
+        }
+        if (s.n_inner > 0) {
+          add(s.inner, 0);
+          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1);
+        }
+      } else if (phiIndex >= 0) {
+        // A phi-cutout point reaches the shells only for a zero-height end
+        // segment, where the reference indexes them with -1 (out of bounds);
+        // it is skipped here.  Inner is compared before outer.
+        if (s.n_inner > 0) add(s.inner + phiIndex, -1);
+        if (s.n_outer > 0) add(s.outer + phiIndex, -1);
+      }
+    }
+    const Quad *q = h.quads();
     R safetySquared = inf_length<R>();
-    if (h.phi_cutout && s.n_phi == 2) {
-      if (pt_inside || in_cutout) {
-        safetySquared = quad_dist2<R>(q[s.phi], p);
-        const R saf = quad_dist2<R>(q[s.phi + 1], p);
-        if (saf < safetySquared) safetySquared = saf;
-        if (in_cutout) {
-          if (h.large_cutout && s.n_inner > 0) {
-            R sq = quad_dist2<R>(q[s.inner], p);
-            if (sq < safetySquared) {
-              phiIndex = 0;
-              safetySquared = sq;
-            }
-            if (s.n_inner > 1) {
-              sq = quad_dist2<R>(q[s.inner + s.n_inner - 1], p);
-              if (sq < safetySquared) {
-                phiIndex = s.n_inner - 1;
-                safetySquared = sq;
-              }
-            }
-          }
-          return safetySquared;
-        }
+#pragma unroll 1
+    for (int c = 0; c < nc; ++c) {
+      const R v = quad_dist2<R>(q[int((cq >> (16 * c)) & 0xffffu)], p);
+      if (c == 0 && first_assigns) {
+        safetySquared = v;
+      } else if (v < safetySquared) {
+        safetySquared = v;
+        const int rec = int((cp >> (8 * c)) & 0xffu) - 1;
+        if (rec >= 0) phiIndex = rec;
       }
     }
-    if (in_cutout && si > 0 && si < h.nseg - 1 && h.zplanes()[si] == h.zplanes()[si + 1]) {
-      if (s.n_outer > 0) {
-        R so = quad_dist2<R>(q[s.outer], p);
-        if (so < safetySquared) {
-          phiIndex = 0;
-          safetySquared = so;
-        }
-        if (s.n_outer > 1) {
-          so = quad_dist2<R>(q[s.outer + s.n_outer - 1], p);
-          if (so < safetySquared) {
-            phiIndex = s.n_outer - 1;
-            safetySquared = so;
-          }
-        }
-      }
-      if (s.n_inner > 0) {
-        R sq = quad_dist2<R>(q[s.inner], p);
-        if (sq < safetySquared) {
-          phiIndex = 0;
-          safetySquared = sq;
-        }
-        if (s.n_inner > 1) {
-          sq = quad_dist2<R>(q[s.inner + s.n_inner - 1], p);
-          if (sq < safetySquared) {
-            phiIndex = s.n_inner - 1;
-            safetySquared = sq;
-          }
-        }
-      }
-      return safetySquared;
-    }
-    // A phi-cutout point reaches here only for a zero-height end segment,
-    // where the reference indexes the shells with -1 (out of bounds).
-    if (phiIndex < 0) return safetySquared;
-    R so = inf_length<R>();
-    if (s.n_outer > 0) so = quad_dist2<R>(q[s.outer + phiIndex], p);
-    R sq = inf_length<R>();
-    if (s.n_inner > 0) sq = quad_dist2<R>(q[s.inner + phiIndex], p);
-    if (sq < safetySquared) safetySquared = sq;
-    if (so < safetySquared) safetySquared = so;
     return safetySquared;
   }
 
@@ -507,16 +502,10 @@ struct PolyhedronShape {
     const int zMax = h.nseg;
     zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
     R distance = inf_length<R>();
-    if (d.z >= R(0)) {
-      for (; zIndex < zMax; ++zIndex) {
-        distance = distance_to_in_zseg<R>(h, zIndex, p, d);
-        if (distance >= R(0) && distance < inf_length<R>()) break;
-      }
-    } else {
-      for (; zIndex >= 0; --zIndex) {
-        distance = distance_to_in_zseg<R>(h, zIndex, p, d);
-        if (distance >= R(0) && distance < inf_length<R>()) break;
-      }
+    const int step = (d.z >= R(0)) ? 1 : -1;  // walk segments along the ray's z
+    for (; zIndex >= 0 && zIndex < zMax; zIndex += step) {
+      distance = distance_to_in_zseg<R>(h, zIndex, p, d);
+      if (distance >= R(0) && distance < inf_length<R>()) break;
     }
     distance_to_endcaps<R>(h, false, p, d, distance);
     return (distance >= tubeDistance - R(1E-6)) ? distance : inf_length<R>();
@@ -537,16 +526,29 @@ struct PolyhedronShape {
     const bool contains = (in == kInsideV);
     if (contains ^ pt_inside) return R(-1);
     const double *zp = h.zplanes();
-    for (int z = zIndex; z < zMax;) {
+    // upwards from the point's segment while the next plane is nearer than
+    // the best distance, then downwards from the segment below it
+    int z = zIndex;
+    bool up = true;
+    while (z >= 0) {
+      if (up && z >= zMax) {
+        up = false;
+        z = zIndex - 1;
+        continue;
+      }
       s2 = vmin(s2, safety_zseg2<R>(h, z, phiIndex, p, pt_inside));
-      ++z;
-      const R dz = R(zp[z]) - p.z;
-      if (dz * dz > s2) break;
-    }
-    for (int z = zIndex - 1; z >= 0; --z) {
-      s2 = vmin(s2, safety_zseg2<R>(h, z, phiIndex, p, pt_inside));
-      const R dz = p.z - R(zp[z]);
-      if (dz * dz > s2) break;
+      if (up) {
+        ++z;
+        const R dz = R(zp[z]) - p.z;
+        if (dz * dz > s2) {
+          up = false;
+          z = zIndex - 1;
+        }
+      } else {
+        const R dz = p.z - R(zp[z]);
+        if (dz * dz > s2) break;
+        --z;
+      }
     }
     safety_to_endcaps2<R>(h, p, s2);
     return vsqrt(s2);
@@ -574,18 +576,11 @@ struct PolyhedronShape {
     int zIndex = find_z_segment<R>(h, p.z);
     zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
     R distance = inf_length<R>();
-    if (d.z >= R(0)) {
-      for (; zIndex < zMax; ++zIndex) {
-        distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
-        if (distance >= R(0) && distance < inf_length<R>()) break;
-        if (R(zp[zIndex]) - p.z > distance) break;
-      }
-    } else {
-      for (; zIndex >= 0; --zIndex) {
-        distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
-        if (distance >= R(0) && distance < inf_length<R>()) break;
-        if (p.z - R(zp[zIndex]) > distance) break;
-      }
+    const bool up = d.z >= R(0);
+    for (; zIndex >= 0 && zIndex < zMax; zIndex += up ? 1 : -1) {
+      distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
+      if (distance >= R(0) && distance < inf_length<R>()) break;
+      if ((up ? R(zp[zIndex]) - p.z : p.z - R(zp[zIndex])) > distance) break;
     }
     distance_to_endcaps<R>(h, true, p, d, distance);
     if (distance >= inf_length<R>()) distance = R(0);

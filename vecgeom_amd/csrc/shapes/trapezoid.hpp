@@ -9,6 +9,7 @@ namespace vgm {
 
 struct TrapezoidShape {
   using Data = TrapezoidData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
 
   template <typename R>
   VGM_DEV static R pdist(const Data &t, int i, V3<R> p)

@@ -8,6 +8,7 @@ namespace vgm {
 
 struct TrdShape {
   using Data = TrdData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

@@ -11,6 +11,7 @@ namespace vgm {
 
 struct TubeShape {
   using Data = TubeData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>

@@ -10,18 +10,26 @@
 #include "vgm_internal.h"
 #include "transform.hpp"
 #include "shapes/all_shapes.hpp"
+#include "staging.hpp"
 
 namespace vgm {
 
 constexpr int kSoaBlock = 256;
 
-template <class S, int M>
-__global__ __launch_bounds__(kSoaBlock) void soa_fp64_kernel(const typename S::Data *__restrict__ sd, Transform tr,
-                                                             int use_tr, SoaArgs a)
+template <class S, int M, bool Staged>
+__global__ __launch_bounds__(kSoaBlock) void soa_fp64_kernel(const typename S::Data *__restrict__ sd, int nd,
+                                                             Transform tr, int use_tr, SoaArgs a)
 {
+  extern __shared__ double lds_block[];
+  const typename S::Data *sp = sd;
+  if constexpr (Staged) {
+    stage_block(lds_block, reinterpret_cast<const double *>(sd), nd);
+    __syncthreads();
+    sp = reinterpret_cast<const typename S::Data *>(lds_block);
+  }
   const size_t i = size_t(blockIdx.x) * kSoaBlock + threadIdx.x;
   if (i >= a.n) return;
-  const typename S::Data &s = *sd;
+  const typename S::Data &s = *sp;
   V3<double> p{a.px[i], a.py[i], a.pz[i]};
   if (use_tr) p = to_local_point(tr, p);
   if constexpr (M == kContains) {
@@ -49,6 +57,7 @@ namespace {
 struct SoaLaunch {
   hipStream_t stream;
   int method;
+  int nd;
   const void *params;
   const Transform &tr;
   int use_tr;
@@ -59,8 +68,13 @@ struct SoaLaunch {
   void go()
   {
     const unsigned blocks = unsigned((a.n + kSoaBlock - 1) / kSoaBlock);
-    hipLaunchKernelGGL((soa_fp64_kernel<S, M>), dim3(blocks), dim3(kSoaBlock), 0, stream,
-                       static_cast<const typename S::Data *>(params), tr, use_tr, a);
+    const auto *p = static_cast<const typename S::Data *>(params);
+    if (S::kStage && nd > 0 && size_t(nd) * sizeof(double) <= size_t(kMaxStageBytes))
+      hipLaunchKernelGGL((soa_fp64_kernel<S, M, S::kStage>), dim3(blocks), dim3(kSoaBlock),
+                         size_t(nd) * sizeof(double), stream, p, nd, tr, use_tr, a);
+    else
+      hipLaunchKernelGGL((soa_fp64_kernel<S, M, false>), dim3(blocks), dim3(kSoaBlock), 0, stream, p, nd, tr,
+                         use_tr, a);
     err = hipGetLastError();
   }
 
@@ -85,7 +99,7 @@ hipError_t launch_soa(hipStream_t stream, int method, int precision, const Shape
 {
   (void)precision;
   if (a.n == 0) return hipSuccess;
-  SoaLaunch L{stream, method, params, tr, use_transform ? 1 : 0, a, hipSuccess};
+  SoaLaunch L{stream, method, rec.nd, params, tr, use_transform ? 1 : 0, a, hipSuccess};
   if (!dispatch_shape_type(rec.type, L)) return hipErrorInvalidValue;
   return L.err;
 }

@@ -13,6 +13,8 @@ enum TransportMode : int32_t { kTransportFP64 = 0, kTransportMixed = 1, kTranspo
 struct SolidRec {
   int32_t type;
   int32_t off;  // parameter block offset in the pool (doubles)
+  int32_t nd;   // parameter block size (doubles)
+  int32_t pad;
   int64_t begin, end;
   double lambda;  // mean physics step (mm)
   double scale;   // bounding-box diagonal (mm), sets the float guard band
@@ -63,6 +65,7 @@ struct FamilyLaunch {
   int t0, t1;
   int grid;
   int *counter;
+  int stage_nd;  // > 0: stage parameter blocks of up to this many doubles in LDS
 };
 
 hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps);

@@ -21,6 +21,7 @@
 #include "transport.h"
 #include "stepping.hpp"
 #include "shapes/all_shapes.hpp"
+#include "staging.hpp"
 
 namespace vgm {
 
@@ -94,9 +95,9 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
 }
 
 template <class S, int Mode>
-VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const Tile &tl, uint32_t &fp64_steps)
+VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s, const Tile &tl,
+                      uint32_t &fp64_steps)
 {
-  const typename S::Data &s = *reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
   const float scale = float(sr.scale);
   for (int64_t i = tl.begin + threadIdx.x; i < tl.end; i += blockDim.x) {
     TrackState t;
@@ -135,27 +136,38 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const Tile &tl
 // tiles until the range is drained.  Without one, workgroup b runs tile t0+b.
 // Every family has its own kernel, so each gets the register allocation of
 // its own code path rather than the largest one of all families.
-template <class S, int Mode>
+template <class S, int Mode, bool Staged>
 __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(TransportArgs a, int t0, int t1,
                                                                               int *counter)
 {
+  extern __shared__ double lds_block[];
+  __shared__ int tile_s;
   uint32_t fp64_steps = 0;
-  if (!counter) {
-    const int t = t0 + int(blockIdx.x);
-    if (t < t1) {
-      const Tile tl = a.tiles[t];
-      run_tile<S, Mode>(a, a.solids[tl.solid], tl, fp64_steps);
-    }
-  } else {
-    __shared__ int tile_s;
-    for (;;) {
+  int staged_solid = -1;
+  for (int k = 0;; ++k) {
+    int t;
+    if (!counter) {
+      if (k > 0) break;
+      t = t0 + int(blockIdx.x);
+    } else {
       if (threadIdx.x == 0) tile_s = t0 + atomicAdd(counter, 1);
       __syncthreads();
-      const int t = tile_s;
+      t = tile_s;
       __syncthreads();
-      if (t >= t1) break;
-      const Tile tl = a.tiles[t];
-      run_tile<S, Mode>(a, a.solids[tl.solid], tl, fp64_steps);
+    }
+    if (t >= t1) break;
+    const Tile tl = a.tiles[t];
+    const SolidRec sr = a.solids[tl.solid];
+    if constexpr (Staged) {
+      if (tl.solid != staged_solid) {  // uniform across the workgroup
+        __syncthreads();
+        stage_block(lds_block, a.pool + sr.off, sr.nd);
+        __syncthreads();
+        staged_solid = tl.solid;
+      }
+      run_tile<S, Mode>(a, sr, *reinterpret_cast<const typename S::Data *>(lds_block), tl, fp64_steps);
+    } else {
+      run_tile<S, Mode>(a, sr, *reinterpret_cast<const typename S::Data *>(a.pool + sr.off), tl, fp64_steps);
     }
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
@@ -212,12 +224,23 @@ struct FamilyLauncher {
   {
     const int grid = f.counter ? f.grid : (f.t1 - f.t0);
     if (grid <= 0) return;
-    if (mode == kTransportMixed)
-      hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed>), dim3(grid), dim3(block), 0, stream, a, f.t0,
-                         f.t1, f.counter);
-    else
-      hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64>), dim3(grid), dim3(block), 0, stream, a, f.t0,
-                         f.t1, f.counter);
+    const bool staged = S::kStage && f.stage_nd > 0 && size_t(f.stage_nd) * sizeof(double) <= size_t(kMaxStageBytes);
+    const size_t lds = staged ? size_t(f.stage_nd) * sizeof(double) : 0;
+    if (mode == kTransportMixed) {
+      if (staged)
+        hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed, S::kStage>), dim3(grid), dim3(block), lds,
+                           stream, a, f.t0, f.t1, f.counter);
+      else
+        hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed, false>), dim3(grid), dim3(block), 0, stream,
+                           a, f.t0, f.t1, f.counter);
+    } else {
+      if (staged)
+        hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64, S::kStage>), dim3(grid), dim3(block), lds,
+                           stream, a, f.t0, f.t1, f.counter);
+      else
+        hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64, false>), dim3(grid), dim3(block), 0, stream,
+                           a, f.t0, f.t1, f.counter);
+    }
   }
 };
 }  // namespace

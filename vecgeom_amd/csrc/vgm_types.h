@@ -30,6 +30,11 @@ enum Method : int32_t {
   kNumMethods = 6
 };
 
+// Parameter blocks up to this size are staged in LDS by the kernels of the
+// families whose lanes index their tables divergently (polycone sections,
+// polyhedron quadrilaterals); larger ones are read from global memory.
+constexpr int kMaxStageBytes = 64 * 1024;
+
 // Inside_t values (VecGeom/base/Global.h:71-88).
 enum : int32_t { kInsideV = 1, kSurfaceV = 2, kOutsideV = 3 };
 
@@ -46,6 +51,8 @@ struct ShapeRec {
   int32_t off;   // offset of the type's parameter block in the pool (doubles)
   int32_t n0;    // type-specific counts (sections, segments, facets ...)
   int32_t n1;
+  int32_t nd;    // size of the parameter block (doubles)
+  int32_t pad;
   double bmin[3];  // local bounding box (Extent)
   double bmax[3];
 };
