@@ -85,6 +85,14 @@ int vgm_polycone_create(vgm_ctx *ctx, double sphi, double dphi, int nz, const do
 int vgm_polyhedron_create(vgm_ctx *ctx, double sphi, double dphi, int nsides, int nz, const double *z,
                           const double *rmin, const double *rmax, int *shape);
 
+/* UnplacedTessellated with triangular facets only
+ * (TessellatedStruct::AddTriangularFacet, VecGeom/volumes/TessellatedStruct.h:596-610,
+ * then Close()): triangles holds nfacets x 3 vertices x 3 coordinates, each
+ * facet anti-clockwise seen from outside; degenerate facets are dropped.
+ * test_dir is the direction of the Contains/Inside test ray (fTestDir, drawn
+ * at random by the reference); NULL selects a fixed default. */
+int vgm_tessellated_create(vgm_ctx *ctx, int nfacets, const double *triangles, const double *test_dir, int *shape);
+
 /* Axis-aligned local bounding box of a shape (VUnplacedVolume::Extent,
  * VecGeom/volumes/UnplacedVolume.h). */
 int vgm_shape_extent(vgm_ctx *ctx, int shape, double *min3, double *max3);

@@ -12,6 +12,10 @@ import os
 
 import numpy as np
 
+# Direction of the Inside/Contains test ray of tessellated solids, the same
+# default as the library's (sin 1.1 cos 0.7, sin 1.1 sin 0.7, cos 1.1).
+TESS_TEST_DIR = (0.681632986593423, 0.5741315443479861, 0.4535961214255773)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liborc.so")
 
@@ -50,6 +54,7 @@ def lib():
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
             "orc_polycone_create": [_P, _D, _D, _I, _P, _P, _P],
             "orc_polyhedron_create": [_P, _D, _D, _I, _I, _P, _P, _P],
+            "orc_tessellated_create": [_P, _I, _P, _P, _I],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
@@ -118,6 +123,12 @@ class Geometry:
         z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
         return self._chk(self._L.orc_polyhedron_create(self.h, sphi, dphi, nsides, len(z), _p(z), _p(rmin),
                                                        _p(rmax)))
+
+    def tessellated(self, triangles, test_dir=None, brute_force=False):
+        """Triangles (n, 3, 3): vertices anti-clockwise seen from outside."""
+        tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
+        td = np.ascontiguousarray(TESS_TEST_DIR if test_dir is None else test_dir, dtype=np.float64)
+        return self._chk(self._L.orc_tessellated_create(self.h, len(tri), _p(tri), _p(td), int(brute_force)))
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

@@ -58,6 +58,12 @@ int orc_polyhedron_create(orc_geom *g, double sphi, double dphi, int nsides, int
   return add_shape(g, make_polyhedron(sphi, dphi, nsides, nz, z, rmin, rmax));
 }
 
+int orc_tessellated_create(orc_geom *g, int nfacets, const double *triangles, const double *test_dir,
+                           int brute_force)
+{
+  return add_shape(g, make_tessellated(nfacets, triangles, test_dir, brute_force != 0));
+}
+
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx)
 {
   auto *G = reinterpret_cast<Geometry *>(g);

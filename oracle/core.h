@@ -147,5 +147,7 @@ Shape *make_polycone(double sphi, double dphi, int nz, const double *z, const do
 Shape *make_polyhedron(double phiStart, double phiDelta, int sideCount, int nz, const double *z, const double *rmin,
                        const double *rmax);
 Shape *make_tube(double rmin, double rmax, double z, double sphi, double dphi);
+// triangles: nfacets x 3 vertices x 3 coordinates, anti-clockwise seen from outside
+Shape *make_tessellated(int nfacets, const double *triangles, const double *test_dir, bool brute_force);
 
 }  // namespace orc

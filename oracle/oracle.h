@@ -36,6 +36,9 @@ int orc_polycone_create(orc_geom *g, double sphi, double dphi, int nz, const dou
                         const double *rmax);
 int orc_polyhedron_create(orc_geom *g, double sphi, double dphi, int nsides, int nz, const double *z,
                           const double *rmin, const double *rmax);
+/* Triangular facets (anti-clockwise from outside), fTestDir for Inside. */
+int orc_tessellated_create(orc_geom *g, int nfacets, const double *triangles, const double *test_dir,
+                           int brute_force);
 int orc_shape_extent(orc_geom *g, int shape, double *mn, double *mx);
 int orc_lv_create(orc_geom *g, int shape);
 int orc_pv_create(orc_geom *g, int lv, const double *tr);

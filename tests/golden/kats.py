@@ -551,3 +551,123 @@ POLYHEDRON_EITHER = [(MYPGON, "DistanceToIn", _PG_P2, VY)]
 # :170-173 Extent == bounding box == ((0,0,-30), (80/cos(h), 40 sqrt(2)/cos(h), 40)), h = dphi/10.
 _H = 0.5 * (PI / 4.0) / 5.0
 POLYHEDRON_EXTENT = [(MYPGON, (0.0, 0.0, -30.0), (80.0 / math.cos(_H), 40.0 * math.sqrt(2.0) / math.cos(_H), 40.0))]
+
+# ---- Tessellated: test/unit_tests/TestTessellated.cpp ------------------------
+def trd_like_triangles(x1, x2, y1, y2, z):
+    """CreateTrdLikeTessellated (TestTessellated.cpp:15-37): six quadrilateral
+    facets, each added as the triangles (v0, v1, v2) and (v0, v2, v3)
+    (TessellatedStruct::AddQuadrilateralFacet, VecGeom/volumes/TessellatedStruct.h:626-650)."""
+    quads = [
+        ((-x2, y2, z), (-x2, -y2, z), (x2, -y2, z), (x2, y2, z)),
+        ((-x1, y1, -z), (x1, y1, -z), (x1, -y1, -z), (-x1, -y1, -z)),
+        ((-x2, -y2, z), (-x1, -y1, -z), (x1, -y1, -z), (x2, -y2, z)),
+        ((x2, -y2, z), (x1, -y1, -z), (x1, y1, -z), (x2, y2, z)),
+        ((x2, y2, z), (x1, y1, -z), (-x1, y1, -z), (-x2, y2, z)),
+        ((-x2, y2, z), (-x1, y1, -z), (-x1, -y1, -z), (-x2, -y2, z)),
+    ]
+    tris = []
+    for a, b, c, d in quads:
+        tris.append((a, b, c))
+        tris.append((a, c, d))
+    return np.array(tris, dtype=np.float64)
+
+
+TSL1 = (trd_like_triangles(20.0, 20.0, 30.0, 30.0, 40.0),)
+TSL2 = (trd_like_triangles(10.0, 30.0, 20.0, 40.0, 40.0),)
+TSL3 = (trd_like_triangles(0.14999999999999999, 0.14999999999999999, 24.707000000000001, 24.707000000000001,
+                           22.699999999999999),)
+_COSA, _SINA = 4 / math.sqrt(17.0), 1 / math.sqrt(17.0)
+_PBIG = {"x": (100.0, 0.0, 0.0), "y": (0.0, 100.0, 0.0), "z": (0.0, 0.0, 100.0),
+         "mx": (-100.0, 0.0, 0.0), "my": (0.0, -100.0, 0.0), "mz": (0.0, 0.0, -100.0)}
+
+TESS_KATS = [
+    # :55-77 Inside
+    (TSL1, "Inside", O, None, ("inside", 1)),
+    (TSL1, "Inside", _PBIG["z"], None, ("inside", 3)),
+    (TSL1, "Inside", (20.0, 0, 0), None, ("inside", 2)),
+    (TSL1, "Inside", (0, 30.0, 0), None, ("inside", 2)),
+    (TSL1, "Inside", (0, 0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (20.0, 30.0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (-20.0, 30.0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (20.0, -30.0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (20.0, 30.0, -40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (20.0, 30.0, 0.0), None, ("inside", 2)),
+    (TSL1, "Inside", (0.0, 30.0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (20.0, 0.0, 40.0), None, ("inside", 2)),
+    (TSL1, "Inside", (-20.0, -30.0, -40.0), None, ("inside", 2)),
+    (TSL2, "Inside", O, None, ("inside", 1)),
+    (TSL2, "Inside", _PBIG["z"], None, ("inside", 3)),
+    (TSL2, "Inside", (20.0, 0, 0), None, ("inside", 2)),
+    (TSL2, "Inside", (0, 30.0, 0), None, ("inside", 2)),
+    (TSL2, "Inside", (0, 0, 40.0), None, ("inside", 2)),
+    # :97-113 SafetyToOut
+    (TSL1, "SafetyToOut", O, None, ("eq", 20.0)),
+    (TSL1, "SafetyToOut", VX, None, ("eq", 19.0)),
+    (TSL1, "SafetyToOut", VY, None, ("eq", 20.0)),
+    (TSL1, "SafetyToOut", VZ, None, ("eq", 20.0)),
+    (TSL2, "SafetyToOut", O, None, ("eq", 20 * _COSA)),
+    (TSL2, "SafetyToOut", VX, None, ("eq", 19 * _COSA)),
+    (TSL2, "SafetyToOut", VY, None, ("eq", 20 * _COSA)),
+    (TSL2, "SafetyToOut", VZ, None, ("eq", 20 * _COSA + _SINA)),
+    # :115-165 DistanceToOut
+    (TSL1, "DistanceToOut", O, VX, ("eq", 20.0)),
+    (TSL1, "DistanceToOut", O, VMX, ("eq", 20.0)),
+    (TSL1, "DistanceToOut", O, VY, ("eq", 30.0)),
+    (TSL1, "DistanceToOut", O, VMY, ("eq", 30.0)),
+    (TSL1, "DistanceToOut", O, VZ, ("eq", 40.0)),
+    (TSL1, "DistanceToOut", O, VMZ, ("eq", 40.0)),
+    (TSL1, "DistanceToOut", O, VXY, ("eq", math.sqrt(800.0))),
+    (TSL1, "DistanceToOut", (20.0, 0, 0), VX, ("eq", 0.0)),
+    (TSL1, "DistanceToOut", (-20.0, 0, 0), VMX, ("eq", 0.0)),
+    (TSL1, "DistanceToOut", (0, 30.0, 0), VY, ("eq", 0.0)),
+    (TSL1, "DistanceToOut", (0, -30.0, 0), VMY, ("eq", 0.0)),
+    (TSL1, "DistanceToOut", (0, 0, 40.0), VZ, ("eq", 0.0)),
+    (TSL1, "DistanceToOut", (0, 0, -40.0), VMZ, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", O, VX, ("eq", 20.0)),
+    (TSL2, "DistanceToOut", O, VMX, ("eq", 20.0)),
+    (TSL2, "DistanceToOut", O, VY, ("eq", 30.0)),
+    (TSL2, "DistanceToOut", O, VMY, ("eq", 30.0)),
+    (TSL2, "DistanceToOut", O, VZ, ("eq", 40.0)),
+    (TSL2, "DistanceToOut", O, VMZ, ("eq", 40.0)),
+    (TSL2, "DistanceToOut", O, VXY, ("eq", math.sqrt(800.0))),
+    (TSL2, "DistanceToOut", (20.0, 0, 0), VX, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", (-20.0, 0, 0), VMX, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", (0, 30.0, 0), VY, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", (0, -30.0, 0), VMY, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", (0, 0, 40.0), VZ, ("eq", 0.0)),
+    (TSL2, "DistanceToOut", (0, 0, -40.0), VMZ, ("eq", 0.0)),
+    # :166-190 SafetyToIn
+    (TSL1, "SafetyToIn", _PBIG["x"], None, ("eq", 80.0)),
+    (TSL1, "SafetyToIn", _PBIG["mx"], None, ("eq", 80.0)),
+    (TSL1, "SafetyToIn", _PBIG["y"], None, ("eq", 70.0)),
+    (TSL1, "SafetyToIn", _PBIG["my"], None, ("eq", 70.0)),
+    (TSL1, "SafetyToIn", _PBIG["z"], None, ("eq", 60.0)),
+    (TSL1, "SafetyToIn", _PBIG["mz"], None, ("eq", 60.0)),
+    (TSL2, "SafetyToIn", _PBIG["x"], None, ("eq", 80 * _COSA)),
+    (TSL2, "SafetyToIn", _PBIG["mx"], None, ("eq", 80 * _COSA)),
+    (TSL2, "SafetyToIn", _PBIG["y"], None, ("eq", 70 * _COSA)),
+    (TSL2, "SafetyToIn", _PBIG["my"], None, ("eq", 70 * _COSA)),
+    (TSL2, "SafetyToIn", _PBIG["z"], None, ("eq", 60.0)),
+    (TSL2, "SafetyToIn", _PBIG["mz"], None, ("eq", 60.0)),
+    # :191-230 DistanceToIn
+    (TSL1, "DistanceToIn", _PBIG["x"], VMX, ("eq", 80.0)),
+    (TSL1, "DistanceToIn", _PBIG["mx"], VX, ("eq", 80.0)),
+    (TSL1, "DistanceToIn", _PBIG["y"], VMY, ("eq", 70.0)),
+    (TSL1, "DistanceToIn", _PBIG["my"], VY, ("eq", 70.0)),
+    (TSL1, "DistanceToIn", _PBIG["z"], VMZ, ("eq", 60.0)),
+    (TSL1, "DistanceToIn", _PBIG["mz"], VZ, ("eq", 60.0)),
+    (TSL1, "DistanceToIn", _PBIG["x"], VXY, ("eq", KINF)),
+    (TSL1, "DistanceToIn", _PBIG["mx"], VXY, ("eq", KINF)),
+    (TSL2, "DistanceToIn", _PBIG["x"], VMX, ("eq", 80.0)),
+    (TSL2, "DistanceToIn", _PBIG["mx"], VX, ("eq", 80.0)),
+    (TSL2, "DistanceToIn", _PBIG["y"], VMY, ("eq", 70.0)),
+    (TSL2, "DistanceToIn", _PBIG["my"], VY, ("eq", 70.0)),
+    (TSL2, "DistanceToIn", _PBIG["z"], VMZ, ("eq", 60.0)),
+    (TSL2, "DistanceToIn", _PBIG["mz"], VZ, ("eq", 60.0)),
+    (TSL2, "DistanceToIn", _PBIG["x"], VXY, ("eq", KINF)),
+    (TSL2, "DistanceToIn", _PBIG["mx"], VXY, ("eq", KINF)),
+    (TSL3, "DistanceToIn", (0.15000000000000185, -22.048743592955137, 2.4268539333219472),
+     unit((-0.76165597579890043, 0.64364445891356026, -0.074515708658524193)), ("eq", 0.0)),
+]
+# :232-244 Extent
+TESS_EXTENT = [(TSL1, (-20.0, -30.0, -40.0), (20.0, 30.0, 40.0)), (TSL2, (-30.0, -40.0, -40.0), (30.0, 40.0, 40.0))]

@@ -84,3 +84,21 @@ SHAPES.update({
     "pgon_smallcut": ("polyhedron", (-0.4, 1.2, 3, (-15.0, 0.0, 0.0, 20.0), (0.0, 2.0, 6.0, 0.0),
                                      (20.0, 25.0, 30.0, 15.0))),
 })
+
+
+def _tess(kind, *args):
+    from tests.golden import kats
+    from vecgeom_amd import workloads
+    if kind == "trd":
+        return (kats.trd_like_triangles(*args),)
+    return (workloads.ellipsoid_hull(*args),)
+
+
+# Tessellated solids: the reference's trd-like KAT constructions and convex
+# hulls of points on an ellipsoid.
+SHAPES.update({
+    "tess_box": ("tessellated", _tess("trd", 20.0, 20.0, 30.0, 30.0, 40.0)),
+    "tess_trd": ("tessellated", _tess("trd", 10.0, 30.0, 20.0, 40.0, 40.0)),
+    "tess_hull_300": ("tessellated", _tess("hull", 300, (30.0, 20.0, 45.0), 3)),
+    "tess_hull_2k": ("tessellated", _tess("hull", 2000, (12.0, 50.0, 30.0), 4)),
+})

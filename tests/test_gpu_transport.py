@@ -14,7 +14,7 @@ from vecgeom_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-FAMILIES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron")
+FAMILIES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated")
 
 
 def _setup(ctx, total, nsteps):

@@ -8,7 +8,8 @@ from tests.golden import kats
 CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_KATS]
          + [("cone", c) for c in kats.CONE_KATS] + [("trd", c) for c in kats.TRD_KATS]
          + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS]
-         + [("polyhedron", c) for c in kats.POLYHEDRON_KATS])
+         + [("polyhedron", c) for c in kats.POLYHEDRON_KATS]
+         + [("tessellated", c) for c in kats.TESS_KATS])
 
 
 def _run(geo, kind, params, method, p, d):
@@ -74,3 +75,12 @@ def test_polyhedron_extent_kat():
         mn, mx = geo.extent(geo.polyhedron(*params))
         for a, b in zip(list(mn) + list(mx), list(lo) + list(hi)):
             assert kats.approx_equal(float(a), float(b)), (mn, mx, lo, hi)
+
+
+@pytest.mark.parametrize("brute", [False, True], ids=["tree", "brute"])
+def test_tessellated_extent_kat(brute):
+    """TestTessellated.cpp:232-244."""
+    geo = oracle.Geometry()
+    for params, lo, hi in kats.TESS_EXTENT:
+        mn, mx = geo.extent(geo.tessellated(*params, brute_force=brute))
+        assert np.allclose(mn, lo) and np.allclose(mx, hi)

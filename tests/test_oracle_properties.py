@@ -38,6 +38,9 @@ EQUIV = [
     (("polycone", (0.0, 2 * np.pi, (-40.0, 40.0), (0.0, 5.0), (10.0, 25.0))),
      ("cone", (0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * np.pi))),
     (("polycone", (0.2, 2.5, (-30.0, 30.0), (5.0, 5.0), (20.0, 20.0))), ("tube", (5.0, 20.0, 30.0, 0.2, 2.5))),
+    # the reference's trd-like tessellated construction (TestTessellated.cpp:15-37)
+    (("tessellated", SHAPES["tess_box"][1]), ("box", (20.0, 30.0, 40.0))),
+    (("tessellated", SHAPES["tess_trd"][1]), ("trd", (10.0, 30.0, 20.0, 40.0, 40.0))),
     # four sides starting at -45 degrees: the side planes are x = +-r, y = +-r
     (("polyhedron", (-np.pi / 4, 2 * np.pi, 4, (-40.0, 40.0), (0.0, 0.0), (20.0, 20.0))), ("box", (20.0, 20.0, 40.0))),
 ]
@@ -101,3 +104,38 @@ def test_safety_bounds_and_surface_hits(name):
     q = tuple(pts[k][hit] + din[hit] * dirs[k][hit] for k in range(3))
     res = geo.query(pv, "Inside", q)
     assert np.mean(res == 2) > 0.999, np.unique(res, return_counts=True)
+
+
+TESS = [n for n in sorted(SHAPES) if SHAPES[n][0] == "tessellated"]
+
+
+@pytest.mark.parametrize("name", TESS)
+def test_tessellated_tree_equals_brute_force(name):
+    """The oracle's bounding-volume tree prunes without changing any result."""
+    _, (tris,) = SHAPES[name]
+    geo = oracle.Geometry()
+    a = geo.place(geo.logical_volume(geo.tessellated(tris)))
+    b = geo.place(geo.logical_volume(geo.tessellated(tris, brute_force=True)))
+    pts, dirs = _inputs(geo, "tessellated", (tris,))
+    for m in ("Contains", "Inside", "SafetyToIn", "SafetyToOut", "DistanceToIn", "DistanceToOut"):
+        d = dirs if m.startswith("Distance") else None
+        ra, rb = geo.query(a, m, pts, d), geo.query(b, m, pts, d)
+        assert np.array_equal(ra, rb), m
+
+
+def test_tessellated_box_safety_is_euclidean():
+    """Outside a tessellated box the safety is the Euclidean distance to the
+    box (point-triangle distances), which bounds the box's own Chebyshev
+    SafetyToIn from above; inside both equal the distance to the nearest face."""
+    geo = oracle.Geometry()
+    t = _pv(geo, "tessellated", SHAPES["tess_box"][1])
+    pts, _ = _inputs(geo, "box", (20.0, 30.0, 40.0))
+    h = np.array([20.0, 30.0, 40.0])
+    q = np.abs(np.stack(pts, axis=1))
+    out = np.sqrt(np.sum(np.maximum(q - h, 0.0) ** 2, axis=1))
+    inn = np.min(h - q, axis=1)
+    ins = geo.query(t, "Inside", pts)
+    si = geo.query(t, "SafetyToIn", pts)
+    so = geo.query(t, "SafetyToOut", pts)
+    assert np.allclose(si[ins == 3], out[ins == 3], atol=1e-9)
+    assert np.allclose(so[ins == 1], inn[ins == 1], atol=1e-9)

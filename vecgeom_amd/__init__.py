@@ -76,6 +76,7 @@ def _declare(L):
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_polycone_create": [_P, _D, _D, _I, _P, _P, _P, ctypes.POINTER(_I)],
         "vgm_polyhedron_create": [_P, _D, _D, _I, _I, _P, _P, _P, ctypes.POINTER(_I)],
+        "vgm_tessellated_create": [_P, _I, _P, _P, ctypes.POINTER(_I)],
         "vgm_trapezoid_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_shape_extent": [_P, _I, _P, _P],
         "vgm_lv_create": [_P, _I, ctypes.POINTER(_I)],
@@ -207,6 +208,12 @@ class Context:
         z, rmin, rmax = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, rmin, rmax))
         return self._new(self._L.vgm_polyhedron_create, sphi, dphi, nsides, len(z), _ptr(z), _ptr(rmin),
                          _ptr(rmax))
+
+    def tessellated(self, triangles, test_dir=None) -> int:
+        """Triangles (n, 3, 3), anti-clockwise seen from outside."""
+        tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
+        td = None if test_dir is None else np.ascontiguousarray(test_dir, dtype=np.float64)
+        return self._new(self._L.vgm_tessellated_create, len(tri), _ptr(tri), _ptr(td))
 
     def extent(self, shape: int):
         mn = np.zeros(3)

@@ -292,6 +292,16 @@ int vgm_polyhedron_create(vgm_ctx *c, double sphi, double dphi, int nsides, int 
   return VGM_OK;
 }
 
+int vgm_tessellated_create(vgm_ctx *c, int nfacets, const double *triangles, const double *test_dir, int *shape)
+{
+  if (!c || !shape || !triangles) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_tessellated(c->host, nfacets, triangles, test_dir, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_shape_extent(vgm_ctx *c, int shape, double *mn, double *mx)
 {
   if (!c || !mn || !mx) return fail(VGM_EINVAL, "null argument");

@@ -40,6 +40,9 @@ int build_trapezoid(HostGeometry &g, const double p[11], std::string &err);
 int build_polycone(HostGeometry &g, double sphi, double dphi, int nz, const double *z, const double *rmin,
                    const double *rmax, std::string &err);
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err);
+// triangles: nfacets x 9 doubles; test_dir may be NULL (library default).
+int build_tessellated(HostGeometry &g, int nfacets, const double *triangles, const double *test_dir,
+                      std::string &err);
 int build_polyhedron(HostGeometry &g, double sphi, double dphi, int nsides, int nz, const double *z,
                      const double *rmin, const double *rmax, std::string &err);
 

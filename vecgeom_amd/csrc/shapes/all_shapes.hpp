@@ -8,6 +8,7 @@
 #include "trapezoid.hpp"
 #include "polycone.hpp"
 #include "polyhedron.hpp"
+#include "tessellated.hpp"
 
 namespace vgm {
 
@@ -37,6 +38,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kPolyhedron:
     f.template operator()<PolyhedronShape>();
+    return true;
+  case kTessellated:
+    f.template operator()<TessellatedShape>();
     return true;
   default:
     return false;

@@ -1,0 +1,316 @@
+// Tessellated kernels (VecGeom/volumes/kernel/TessellatedImplementation.h:62-318)
+// with the per-facet arithmetic of TessellatedCluster (DistanceToCluster,
+// InsideCluster, SafetySq; VecGeom/volumes/TessellatedCluster.h:228-545) and
+// the facet setup of Tile<3> (VecGeom/volumes/Tile.h:75-150).
+//
+// The reference walks float boxes of facet clusters in distance order and
+// keeps the strict minimum; its results are the minimum over all facets.
+// Here a bounding-volume tree over the facets (boxes enlarged by a margin,
+// so pruning is conservative) is walked with a per-lane stack; equal
+// distances go to the facet defined first.  Facet data stay in global
+// memory: a 20k-facet solid is 3.8 MB, served from L2 / infinity cache.
+#pragma once
+#include "../vgm_math.hpp"
+#include "tessellated_data.h"
+
+namespace vgm {
+
+struct TessellatedShape {
+  using Data = TessData;
+  static constexpr bool kStage = false;  // parameter block staged in LDS
+  static constexpr int kStack = 64;      // tree depth is below 32 (median splits)
+
+  template <typename R>
+  VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)
+  {
+    bool inside = true;
+    for (int i = 0; i < 3; ++i)
+      inside &= (px - R(f.v[i][0])) * R(f.sv[i][0]) + (py - R(f.v[i][1])) * R(f.sv[i][1]) +
+                    (pz - R(f.v[i][2])) * R(f.sv[i][2]) >
+                R(-kTolerance);
+    return inside;
+  }
+
+  // DistanceToLineSegmentSquared2, one lane with mask = true
+  template <typename R>
+  VGM_DEV static R segment_dist2(const double *c0, const double *c1, V3<R> p)
+  {
+    const R lx = R(c1[0]) - R(c0[0]), ly = R(c1[1]) - R(c0[1]), lz = R(c1[2]) - R(c0[2]);
+    const R px = p.x - R(c0[0]), py = p.y - R(c0[1]), pz = p.z - R(c0[2]);
+    const R dot0 = px * lx + py * ly + pz * lz;
+    if (dot0 <= R(0)) return px * px + py * py + pz * pz;
+    const R dot1 = lx * lx + ly * ly + lz * lz;
+    R result = inf_length<R>();
+    if (dot1 <= dot0) {
+      const R qx = p.x - R(c1[0]), qy = p.y - R(c1[1]), qz = p.z - R(c1[2]);
+      result = qx * qx + qy * qy + qz * qz;
+    }
+    if (result < inf_length<R>()) return result;
+    const R t = dot0 / dot1;
+    const R ex = (R(c0[0]) + lx * t) - p.x, ey = (R(c0[1]) + ly * t) - p.y, ez = (R(c0[2]) + lz * t) - p.z;
+    return ex * ex + ey * ey + ez * ez;
+  }
+
+  // TessellatedCluster::SafetySq for one facet
+  template <typename R, bool ToIn>
+  VGM_DEV static R facet_safety2(const TessFacet &f, V3<R> p)
+  {
+    R s = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
+    const R xx = p.x - R(f.n[0]) * s, xy = p.y - R(f.n[1]) * s, xz = p.z - R(f.n[2]) * s;
+    bool within = facet_contains<R>(f, xx, xy, xz);
+    if (ToIn)
+      within &= s > R(-kTolerance);
+    else
+      within &= s < R(kTolerance);
+    s *= s;
+    if (within) return s;
+    R e = inf_length<R>();
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const R ei = segment_dist2<R>(f.v[i], f.v[i == 2 ? 0 : i + 1], p);
+      e = (i == 0) ? ei : vmin(e, ei);
+    }
+    return e;
+  }
+
+  template <typename R>
+  VGM_DEV static R box_entry(const TessNode &nd, V3<R> p, V3<R> inv)
+  {
+    R a = (R(nd.lo[0]) - p.x) * inv.x, b = (R(nd.hi[0]) - p.x) * inv.x;
+    R t0 = vmin(a, b), t1 = vmax(a, b);
+    a = (R(nd.lo[1]) - p.y) * inv.y;
+    b = (R(nd.hi[1]) - p.y) * inv.y;
+    t0 = vmax(t0, vmin(a, b));
+    t1 = vmin(t1, vmax(a, b));
+    a = (R(nd.lo[2]) - p.z) * inv.z;
+    b = (R(nd.hi[2]) - p.z) * inv.z;
+    t0 = vmax(t0, vmin(a, b));
+    t1 = vmin(t1, vmax(a, b));
+    return (t0 <= t1) ? t0 : inf_length<R>();
+  }
+
+  template <typename R>
+  VGM_DEV static R box_dist2(const TessNode &nd, V3<R> p)
+  {
+    const R ex = vmax(vmax(R(nd.lo[0]) - p.x, p.x - R(nd.hi[0])), R(0));
+    const R ey = vmax(vmax(R(nd.lo[1]) - p.y, p.y - R(nd.hi[1])), R(0));
+    const R ez = vmax(vmax(R(nd.lo[2]) - p.z, p.z - R(nd.hi[2])), R(0));
+    return ex * ex + ey * ey + ez * ez;
+  }
+
+  // Minimum entering and exiting facet hits (DistanceToCluster over all
+  // facets).  The walk is bounded by the minimum of the kind selected by
+  // to_in; the other kind is only needed below that bound.
+  template <typename R>
+  struct Hits {
+    R dIn, dOut;
+    int iIn, iOut;      // storage slot of the facet, -1 if none
+    int oIn, oOut;      // definition index (tie-break)
+  };
+
+  template <typename R>
+  VGM_DEV static void ray_hits(const Data &t, V3<R> p, V3<R> d, bool to_in, Hits<R> &h)
+  {
+    h.dIn = h.dOut = inf_length<R>();
+    h.iIn = h.iOut = h.oIn = h.oOut = -1;
+    const TessFacet *fa = t.facets();
+    const TessNode *nodes = t.nodes();
+    const V3<R> inv{R(1) / nonzero(d.x), R(1) / nonzero(d.y), R(1) / nonzero(d.z)};
+    int stack[kStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const TessNode &nd = nodes[stack[--sp]];
+      const R bound = to_in ? (h.iIn >= 0 ? h.dIn : inf_length<R>()) : (h.iOut >= 0 ? h.dOut : inf_length<R>());
+      const R te = box_entry<R>(nd, p, inv);
+      if (te == inf_length<R>() || te > bound) continue;
+      if (nd.left >= 0) {
+        stack[sp++] = nd.left;
+        stack[sp++] = nd.right;
+        continue;
+      }
+      for (int k = nd.first; k < nd.first + nd.count; ++k) {
+        const TessFacet &f = fa[k];
+        const R ndd = nonzero(d.x * R(f.n[0]) + d.y * R(f.n[1]) + d.z * R(f.n[2]));
+        const R saf = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
+        const bool vIn = ndd < R(0) && saf > R(-kTolerance);
+        const bool vOut = ndd > R(0) && saf < R(kTolerance);
+        if (!vIn && !vOut) continue;
+        const R dist = -saf / ndd;
+        if (!facet_contains<R>(f, p.x + d.x * dist, p.y + d.y * dist, p.z + d.z * dist)) continue;
+        if (vIn) {
+          if (dist < h.dIn || (dist == h.dIn && f.orig < h.oIn)) {
+            h.dIn = dist;
+            h.iIn = k;
+            h.oIn = f.orig;
+          }
+        } else if (dist < h.dOut || (dist == h.dOut && f.orig < h.oOut)) {
+          h.dOut = dist;
+          h.iOut = k;
+          h.oOut = f.orig;
+        }
+      }
+    }
+  }
+
+  template <typename R, bool ToIn>
+  VGM_DEV static R safety2(const Data &t, V3<R> p)
+  {
+    const TessFacet *fa = t.facets();
+    const TessNode *nodes = t.nodes();
+    R best = inf_length<R>();
+    int stack[kStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const TessNode &nd = nodes[stack[--sp]];
+      if (box_dist2<R>(nd, p) > best) continue;
+      if (nd.left >= 0) {
+        stack[sp++] = nd.left;
+        stack[sp++] = nd.right;
+        continue;
+      }
+      for (int k = nd.first; k < nd.first + nd.count; ++k) {
+        const R s = facet_safety2<R, ToIn>(fa[k], p);
+        if (s < best) best = s;
+      }
+    }
+    return best;
+  }
+
+  // BoxImplementation::IntersectCachedKernel2 over the extent, t0 = -kTolerance,
+  // t1 = kInfLength (BoxImplementation.h:326-368)
+  template <typename R>
+  VGM_DEV static R bbox_intersect(const Data &t, V3<R> p, V3<R> d)
+  {
+    const R ix = R(1.0) / nonzero(d.x), iy = R(1.0) / nonzero(d.y), iz = R(1.0) / nonzero(d.z);
+    const bool sx = ix < R(0), sy = iy < R(0), sz = iz < R(0);
+    R tmin = (R(sx ? t.mx[0] : t.mn[0]) - p.x) * ix;
+    const R tymax = (R(sy ? t.mn[1] : t.mx[1]) - p.y) * iy;
+    if (tmin > tymax) return inf_length<R>();
+    R tmax = (R(sx ? t.mn[0] : t.mx[0]) - p.x) * ix;
+    const R tymin = (R(sy ? t.mx[1] : t.mn[1]) - p.y) * iy;
+    if (tymin > tmax) return inf_length<R>();
+    tmin = vmax(tmin, tymin);
+    tmax = vmin(tmax, tymax);
+    const R tzmin = (R(sz ? t.mx[2] : t.mn[2]) - p.z) * iz;
+    const R tzmax = (R(sz ? t.mn[2] : t.mx[2]) - p.z) * iz;
+    if ((tmin > tzmax) || (tzmin > tmax)) return inf_length<R>();
+    tmin = vmax(tmin, tzmin);
+    tmax = vmin(tmax, tzmax);
+    if (!((tmin <= inf_length<R>() + R(kTolerance)) && (tmax > R(-kTolerance) - R(kTolerance))))
+      return inf_length<R>();
+    return tmin;
+  }
+
+  // DistanceToSolid<ToIn> (TessellatedImplementation.h:176-279)
+  template <typename R, bool ToIn>
+  VGM_DEV static R distance_to_solid(const Data &t, V3<R> p, V3<R> d, R step_max, Hits<R> &h, int &isurf)
+  {
+    isurf = -1;
+    R distance;
+    if (ToIn) {
+      distance = bbox_intersect<R>(t, p, d);
+      if (distance >= step_max) {
+        h.iIn = h.iOut = -1;
+        return distance;
+      }
+    }
+    ray_hits<R>(t, p, d, ToIn, h);
+    const TessFacet *fa = t.facets();
+    distance = inf_length<R>();
+    if (ToIn) {
+      if (h.iIn >= 0) {
+        isurf = h.iIn;
+        distance = h.dIn;
+      }
+      if (h.iIn < 0) {
+        if (h.iOut >= 0 && h.dOut * (d.x * R(fa[h.iOut].n[0]) + d.y * R(fa[h.iOut].n[1]) + d.z * R(fa[h.iOut].n[2])) >
+                               R(kTolerance))
+          distance = R(-1);
+      } else {
+        if (h.iOut >= 0 && h.dOut > R(kTolerance) && h.dOut < h.dIn) distance = R(-1);
+      }
+    } else {
+      if (h.iOut >= 0) {
+        isurf = h.iOut;
+        distance = h.dOut;
+      }
+      if (h.iOut < 0) {
+        distance = R(-1);
+      } else if (h.iIn >= 0 && h.dIn < h.dOut &&
+                 h.dIn * (d.x * R(fa[h.iIn].n[0]) + d.y * R(fa[h.iIn].n[1]) + d.z * R(fa[h.iIn].n[2])) <
+                     R(-kTolerance)) {
+        distance = R(-1);
+        isurf = -1;
+      }
+    }
+    return distance;
+  }
+
+  template <typename R>
+  VGM_DEV static V3<R> test_dir(const Data &t)
+  {
+    return V3<R>{R(t.test_dir[0]), R(t.test_dir[1]), R(t.test_dir[2])};
+  }
+
+  template <typename R>
+  VGM_DEV static bool contains(const Data &t, V3<R> p)
+  {
+    Hits<R> h;
+    int isurf;
+    distance_to_solid<R, false>(t, p, test_dir<R>(t), inf_length<R>(), h, isurf);
+    return isurf >= 0;
+  }
+
+  template <typename R>
+  VGM_DEV static int inside(const Data &t, V3<R> p)
+  {
+    Hits<R> h;
+    int iOut;
+    const V3<R> td = test_dir<R>(t);
+    const R dOut = distance_to_solid<R, false>(t, p, td, inf_length<R>(), h, iOut);
+    if (iOut < 0) return kOutsideV;
+    const TessFacet *fa = t.facets();
+    if (dOut < R(0) || dOut * (td.x * R(fa[iOut].n[0]) + td.y * R(fa[iOut].n[1]) + td.z * R(fa[iOut].n[2])) <
+                           R(kTolerance))
+      return kSurfaceV;
+    if (h.iIn < 0 || dOut < h.dIn) return kInsideV;
+    if (h.dIn < R(0) ||
+        h.dIn * (td.x * R(fa[h.iIn].n[0]) + td.y * R(fa[h.iIn].n[1]) + td.z * R(fa[h.iIn].n[2])) > R(-kTolerance))
+      return kSurfaceV;
+    return kOutsideV;
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &t, V3<R> p, V3<R> d, R step_max)
+  {
+    Hits<R> h;
+    int isurf;
+    return distance_to_solid<R, true>(t, p, d, step_max, h, isurf);
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_out(const Data &t, V3<R> p, V3<R> d, R step_max)
+  {
+    Hits<R> h;
+    int isurf;
+    return distance_to_solid<R, false>(t, p, d, step_max, h, isurf);
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_in(const Data &t, V3<R> p)
+  {
+    if (contains<R>(t, p)) return R(-1);
+    return vsqrt(safety2<R, true>(t, p));
+  }
+
+  template <typename R>
+  VGM_DEV static R safety_to_out(const Data &t, V3<R> p)
+  {
+    if (!contains<R>(t, p)) return R(-1);
+    return vsqrt(safety2<R, false>(t, p));
+  }
+};
+
+}  // namespace vgm
