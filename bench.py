@@ -88,8 +88,9 @@ def main():
         cr = torch.zeros(n, dtype=torch.int32, device=dev)
         torch.cuda.synchronize()
         if not args.no_autotune:
+            log(f"[bench] autotune {'mixed' if mixed else 'fp64'} ...")
             t = time.perf_counter()
-            shape = batch.autotune(*state, side, nsteps=5, mixed=mixed)
+            shape = batch.autotune(*state, side, nsteps=2, mixed=mixed)
             log(f"[bench] autotune {'mixed' if mixed else 'fp64'}: block/grid/tile = {shape} "
                 f"({time.perf_counter() - t:.1f}s)")
         step0 = 0

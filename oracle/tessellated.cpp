@@ -273,6 +273,9 @@ struct Tessellated final : Shape {
       if (te == kInfLength || te > bound) continue;
       if (nd.left < 0) {
         for (int k = nd.first; k < nd.first + nd.count; ++k) visit(order[k]);
+      } else if (box_entry(nodes[nd.left], p, inv) <= box_entry(nodes[nd.right], p, inv)) {
+        stack[sp++] = nd.right;  // nearer child last: visited first
+        stack[sp++] = nd.left;
       } else {
         stack[sp++] = nd.left;
         stack[sp++] = nd.right;
@@ -302,6 +305,9 @@ struct Tessellated final : Shape {
           const double s = facet_safety2<ToIn>(f[order[k]], p);
           if (s < best) best = s;
         }
+      } else if (box_dist2(nodes[nd.left], p) <= box_dist2(nodes[nd.right], p)) {
+        stack[sp++] = nd.right;  // nearer child last: visited first
+        stack[sp++] = nd.left;
       } else {
         stack[sp++] = nd.left;
         stack[sp++] = nd.right;

@@ -295,8 +295,9 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
 // buffers are left untouched).  First each family is timed alone, which
 // gives the weights that split the persistent grid and balance the family
 // streams; then every candidate launch shape is timed on the whole batch and
-// the fastest is kept.  Candidates: workgroup size, tile size, and either one
-// workgroup per tile or a persistent grid of 2, 4 or 8 workgroups per CU.
+// the fastest is kept.  Candidates: workgroup size (64, 128, 256), tile size
+// (512, 2048 tracks), and either one workgroup per tile or a persistent grid
+// of 4 workgroups per CU.
 int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const double *z, const double *dx,
                        const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,
                        int *best3)
@@ -366,8 +367,8 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   // 2. launch shapes on the whole batch
   {
     const int blocks[] = {64, 128, 256};
-    const int tiles[] = {512, 2048, 8192};
-    const int per_cu[] = {0, 2, 4, 8};
+    const int tiles[] = {512, 2048};
+    const int per_cu[] = {0, 4};
     for (int bl : blocks)
       for (int tt : tiles)
         for (int pc : per_cu) {

@@ -116,6 +116,8 @@ struct TessellatedShape {
     const TessFacet *fa = t.facets();
     const TessNode *nodes = t.nodes();
     const V3<R> inv{R(1) / nonzero(d.x), R(1) / nonzero(d.y), R(1) / nonzero(d.z)};
+    // nearest-first walk: the nearer child is pushed last; every node is
+    // re-tested against the (shrinking) bound when it is popped
     int stack[kStack];
     int sp = 0;
     stack[sp++] = 0;
@@ -125,8 +127,17 @@ struct TessellatedShape {
       const R te = box_entry<R>(nd, p, inv);
       if (te == inf_length<R>() || te > bound) continue;
       if (nd.left >= 0) {
-        stack[sp++] = nd.left;
-        stack[sp++] = nd.right;
+        const R tl = box_entry<R>(nodes[nd.left], p, inv), tr = box_entry<R>(nodes[nd.right], p, inv);
+        const bool l_ok = !(tl == inf_length<R>() || tl > bound), r_ok = !(tr == inf_length<R>() || tr > bound);
+        if (l_ok && r_ok) {
+          const bool left_first = tl <= tr;
+          stack[sp++] = left_first ? nd.right : nd.left;
+          stack[sp++] = left_first ? nd.left : nd.right;
+        } else if (l_ok) {
+          stack[sp++] = nd.left;
+        } else if (r_ok) {
+          stack[sp++] = nd.right;
+        }
         continue;
       }
       for (int k = nd.first; k < nd.first + nd.count; ++k) {
@@ -165,9 +176,18 @@ struct TessellatedShape {
     while (sp) {
       const TessNode &nd = nodes[stack[--sp]];
       if (box_dist2<R>(nd, p) > best) continue;
-      if (nd.left >= 0) {
-        stack[sp++] = nd.left;
-        stack[sp++] = nd.right;
+      if (nd.left >= 0) {  // nearer child last, so it is visited first
+        const R dl = box_dist2<R>(nodes[nd.left], p), dr = box_dist2<R>(nodes[nd.right], p);
+        const bool l_ok = !(dl > best), r_ok = !(dr > best);
+        if (l_ok && r_ok) {
+          const bool left_first = dl <= dr;
+          stack[sp++] = left_first ? nd.right : nd.left;
+          stack[sp++] = left_first ? nd.left : nd.right;
+        } else if (l_ok) {
+          stack[sp++] = nd.left;
+        } else if (r_ok) {
+          stack[sp++] = nd.right;
+        }
         continue;
       }
       for (int k = nd.first; k < nd.first + nd.count; ++k) {
