@@ -138,6 +138,46 @@ int vgm_safety_to_in(vgm_ctx *ctx, int pv, const double *x, const double *y, con
 int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
                       double *out, unsigned flags);
 
+/* ---- navigation -------------------------------------------------------------
+ * The drop-in for VNavigator::ComputeStepAndPropagatedState
+ * (VecGeom/navigation/VNavigator.h:51-55, BVHNavigator over every logical
+ * volume with daughters), GlobalLocator::LocateGlobalPoint
+ * (VecGeom/navigation/GlobalLocator.h:35) and VSafetyEstimator::ComputeSafety
+ * (VecGeom/navigation/VSafetyEstimator.h:45) on SOA3D batches of global
+ * points and directions.
+ *
+ * A navigation state is a uint32 index (NavStateIndex): the node of the
+ * placement tree reached by a depth-first walk from the world, 1 = world,
+ * 0 = outside.  A track carries (state, last_exited, boundary) like
+ * NavStateIndex's fNavInd / fLastExited / fOnBoundary.  vgm_nav_node decodes
+ * a state into its placed volume, level and mother state.
+ *
+ * vgm_nav_close builds the tables for the tree under world_pv (ABBoxes with
+ * a 1e-3 mm margin, the reference's surface-area BVH per logical volume,
+ * cached global matrices); call it again after changing the geometry. */
+int vgm_nav_close(vgm_ctx *ctx, int world_pv, int *nnodes);
+int vgm_nav_node(vgm_ctx *ctx, uint32_t state, int *pv, int *level, int *mother);
+/* state[i] = located state of global point i (0 if outside the world). */
+int vgm_nav_locate(vgm_ctx *ctx, const double *x, const double *y, const double *z, size_t n, uint32_t *state,
+                   unsigned flags);
+/* One ComputeStepAndPropagatedState per track: step[i] = geometric step
+ * (bounded by step_limit[i]; step_limit may be NULL for no limit); state,
+ * last and boundary are replaced by the propagated state. */
+int vgm_nav_step(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+                 const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                 uint8_t *boundary, double *step, unsigned flags);
+/* nsteps straight-line steps per track (no physics limit): each step moves
+ * the global point by the geometric step and takes the propagated state; a
+ * track that leaves the world stops.  steps[i] is incremented by the number
+ * of steps taken. */
+int vgm_nav_transport(vgm_ctx *ctx, double *x, double *y, double *z, const double *dx, const double *dy,
+                      const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
+                      int32_t *steps, unsigned flags);
+/* out[i] = safety of global point i in its state (mother SafetyToOut, then
+ * the daughters through the BVH); 0 for state 0. */
+int vgm_nav_safety(vgm_ctx *ctx, const double *x, const double *y, const double *z, size_t n,
+                   const uint32_t *state, double *out, unsigned flags);
+
 /* ---- batched heterogeneous transport ---------------------------------------
  * No counterpart call exists in the reference: it replaces a caller's loop of
  * per-solid SOA3D calls (VPlacedVolume::SafetyToIn/Out, DistanceToIn/Out,

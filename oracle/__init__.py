@@ -55,6 +55,12 @@ def lib():
             "orc_polycone_create": [_P, _D, _D, _I, _P, _P, _P],
             "orc_polyhedron_create": [_P, _D, _D, _I, _I, _P, _P, _P],
             "orc_tessellated_create": [_P, _I, _P, _P, _I],
+            "orc_nav_close": [_P, _I, _I],
+            "orc_nav_node": [_P, ctypes.c_uint, _P, _P, _P],
+            "orc_nav_locate": [_P, _P, _P, _P, ctypes.c_size_t, _P, _I],
+            "orc_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _P, _I],
+            "orc_nav_transport": [_P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _I, _P, _I],
+            "orc_nav_safety": [_P, _P, _P, _P, ctypes.c_size_t, _P, _P, _I],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
@@ -129,6 +135,43 @@ class Geometry:
         tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
         td = np.ascontiguousarray(TESS_TEST_DIR if test_dir is None else test_dir, dtype=np.float64)
         return self._chk(self._L.orc_tessellated_create(self.h, len(tri), _p(tri), _p(td), int(brute_force)))
+
+    # ---- navigation -----------------------------------------------------------
+    def nav_close(self, world_pv, use_bvh=False):
+        """Build navigation tables; returns the number of state nodes."""
+        return self._chk(self._L.orc_nav_close(self.h, world_pv, int(use_bvh)))
+
+    def nav_node(self, node):
+        pv, lvl, mo = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        if self._L.orc_nav_node(self.h, node, ctypes.byref(pv), ctypes.byref(lvl), ctypes.byref(mo)) != 0:
+            raise ValueError("invalid node")
+        return pv.value, lvl.value, mo.value
+
+    def nav_locate(self, pts, nthreads=0):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        st = np.zeros(len(x), np.uint32)
+        self._L.orc_nav_locate(self.h, _p(x), _p(y), _p(z), len(x), _p(st), nthreads)
+        return st
+
+    def nav_step(self, pts, dirs, state, last, boundary, step_limit=None, nthreads=0):
+        """One ComputeStepAndPropagatedState per track; updates the state arrays, returns steps."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        dx, dy, dz = (np.ascontiguousarray(a, dtype=np.float64) for a in dirs)
+        sl = None if step_limit is None else np.ascontiguousarray(step_limit, dtype=np.float64)
+        out = np.zeros(len(x))
+        self._L.orc_nav_step(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), None if sl is None else _p(sl),
+                             len(x), _p(state), _p(last), _p(boundary), _p(out), nthreads)
+        return out
+
+    def nav_transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, nthreads=0):
+        self._L.orc_nav_transport(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), len(x), _p(state), _p(last),
+                                  _p(boundary), nsteps, _p(steps), nthreads)
+
+    def nav_safety(self, pts, state, nthreads=0):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        out = np.zeros(len(x))
+        self._L.orc_nav_safety(self.h, _p(x), _p(y), _p(z), len(x), _p(state), _p(out), nthreads)
+        return out
 
     def extent(self, shape):
         mn, mx = np.zeros(3), np.zeros(3)

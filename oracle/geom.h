@@ -21,6 +21,15 @@ struct Geometry {
   std::vector<std::unique_ptr<Shape>> shapes;
   std::vector<LogicalVolume> lvs;
   std::vector<PlacedVolume> pvs;
+  void *nav = nullptr;  // navigation tables (navigation.cpp), built by orc_nav_close
+  void (*nav_deleter)(void *) = nullptr;
+  Geometry() = default;
+  Geometry(const Geometry &) = delete;
+  Geometry &operator=(const Geometry &) = delete;
+  ~Geometry()
+  {
+    if (nav && nav_deleter) nav_deleter(nav);
+  }
 };
 
 }  // namespace orc

@@ -48,6 +48,22 @@ int orc_query(orc_geom *g, int pv, int method, const double *px, const double *p
               const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
               int nthreads);
 
+/* Navigation (navigation.cpp): states are node numbers of a depth-first
+ * walk of the placement tree from the world (1 = world, 0 = outside).
+ * orc_nav_close returns the number of nodes. */
+int orc_nav_close(orc_geom *g, int world_pv, int use_bvh);
+int orc_nav_node(orc_geom *g, unsigned node, int *pv, int *level, int *mother);
+int orc_nav_locate(orc_geom *g, const double *x, const double *y, const double *z, size_t n, uint32_t *state,
+                   int nthreads);
+int orc_nav_step(orc_geom *g, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+                 const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                 uint8_t *boundary, double *step, int nthreads);
+int orc_nav_transport(orc_geom *g, double *x, double *y, double *z, const double *dx, const double *dy,
+                      const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
+                      int32_t *steps, int nthreads);
+int orc_nav_safety(orc_geom *g, const double *x, const double *y, const double *z, size_t n, const uint32_t *state,
+                   double *out, int nthreads);
+
 /* Batched track-step transport (project-defined loop, see transport.cpp). */
 double orc_physics_step(double mean, uint64_t seed, uint64_t track, uint32_t step);
 int orc_batch_transport(orc_geom *g, int nsolids, const int *shapes, const int64_t *offsets, const double *mean_step,

@@ -96,6 +96,18 @@ def _declare(L):
         f.restype = _I
     L.vgm_ctx_stream.argtypes = [_P]
     L.vgm_ctx_stream.restype = _P
+    nsig = {
+        "vgm_nav_close": [_P, _I, ctypes.POINTER(_I)],
+        "vgm_nav_node": [_P, ctypes.c_uint32, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)],
+        "vgm_nav_locate": [_P, _P, _P, _P, _SZ, _P, _U],
+        "vgm_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
+        "vgm_nav_transport": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _I, _P, _U],
+        "vgm_nav_safety": [_P, _P, _P, _P, _SZ, _P, _P, _U],
+    }
+    for name, args in nsig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
     bsig = {
         "vgm_batch_create": [_P, _I, _P, _P, _P, ctypes.POINTER(_P)],
         "vgm_batch_destroy": [_P],
@@ -288,6 +300,43 @@ class Context:
 
     def placed_distance_to_out(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
         return self._q2(self._L.vgm_placed_distance_to_out, pv, pts, dirs, step_max, out, mixed, asynchronous)
+
+
+class Navigation:
+    """Navigation on a closed geometry (vgm_nav_*): states are uint32 node
+    indices of the placement tree (1 = world, 0 = outside)."""
+
+    def __init__(self, ctx: "Context", world_pv: int):
+        self.ctx = ctx
+        self._L = ctx._L
+        n = _I()
+        check(self._L.vgm_nav_close(ctx.handle, world_pv, ctypes.byref(n)))
+        self.nnodes = n.value
+
+    def node(self, state: int):
+        pv, lvl, mo = _I(), _I(), _I()
+        check(self._L.vgm_nav_node(self.ctx.handle, state, ctypes.byref(pv), ctypes.byref(lvl), ctypes.byref(mo)))
+        return pv.value, lvl.value, mo.value
+
+    def _fl(self, a, asynchronous=False):
+        return (DEVICE_PTRS if _is_device(a) else HOST_PTRS) | (ASYNC if asynchronous else 0)
+
+    def locate(self, x, y, z, state):
+        check(self._L.vgm_nav_locate(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), len(x), _ptr(state), self._fl(x)))
+
+    def step(self, x, y, z, dx, dy, dz, state, last, boundary, step, step_limit=None):
+        check(self._L.vgm_nav_step(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                   _ptr(step_limit), len(x), _ptr(state), _ptr(last), _ptr(boundary), _ptr(step),
+                                   self._fl(x)))
+
+    def transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, asynchronous=False):
+        check(self._L.vgm_nav_transport(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                        len(x), _ptr(state), _ptr(last), _ptr(boundary), nsteps, _ptr(steps),
+                                        self._fl(x, asynchronous)))
+
+    def safety(self, x, y, z, state, out):
+        check(self._L.vgm_nav_safety(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), len(x), _ptr(state), _ptr(out),
+                                     self._fl(x)))
 
 
 class Batch:

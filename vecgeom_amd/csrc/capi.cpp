@@ -64,7 +64,12 @@ int sync_geometry(vgm_ctx *c)
   const size_t s_lvs = align256(lvs.size() * sizeof(LogicalRec) + 1);
   const size_t s_dl = align256(dlist.size() * sizeof(int32_t) + 1);
   const size_t s_pvs = align256(h.pvs.size() * sizeof(PlacedRec) + 1);
-  const size_t total = s_shapes + s_pool + s_lvs + s_dl + s_pvs;
+  const size_t s_nn = align256(h.nav_nodes.size() * sizeof(NavNode) + 1);
+  const size_t s_nc = align256(h.nav_children.size() * sizeof(int32_t) + 1);
+  const size_t s_bv = align256(h.bvhs.size() * sizeof(BvhRec) + 1);
+  const size_t s_ip = align256(h.nav_ipool.size() * sizeof(int32_t) + 1);
+  const size_t s_bp = align256(h.nav_bpool.size() * sizeof(double) + 1);
+  const size_t total = s_shapes + s_pool + s_lvs + s_dl + s_pvs + s_nn + s_nc + s_bv + s_ip + s_bp;
   std::vector<char> stage(total, 0);
   size_t o = 0;
   auto put = [&](const void *src, size_t n, size_t span) {
@@ -78,6 +83,11 @@ int sync_geometry(vgm_ctx *c)
   const size_t o_lvs = put(lvs.data(), lvs.size() * sizeof(LogicalRec), s_lvs);
   const size_t o_dl = put(dlist.data(), dlist.size() * sizeof(int32_t), s_dl);
   const size_t o_pvs = put(h.pvs.data(), h.pvs.size() * sizeof(PlacedRec), s_pvs);
+  const size_t o_nn = put(h.nav_nodes.data(), h.nav_nodes.size() * sizeof(NavNode), s_nn);
+  const size_t o_nc = put(h.nav_children.data(), h.nav_children.size() * sizeof(int32_t), s_nc);
+  const size_t o_bv = put(h.bvhs.data(), h.bvhs.size() * sizeof(BvhRec), s_bv);
+  const size_t o_ip = put(h.nav_ipool.data(), h.nav_ipool.size() * sizeof(int32_t), s_ip);
+  const size_t o_bp = put(h.nav_bpool.data(), h.nav_bpool.size() * sizeof(double), s_bp);
   if (d.cap < total) {
     if (d.buf) {
       VGM_HIP(hipStreamSynchronize(c->stream));
@@ -95,6 +105,11 @@ int sync_geometry(vgm_ctx *c)
   d.lvs = reinterpret_cast<LogicalRec *>(d.buf + o_lvs);
   d.daughters = reinterpret_cast<int32_t *>(d.buf + o_dl);
   d.pvs = reinterpret_cast<PlacedRec *>(d.buf + o_pvs);
+  d.nav_nodes = reinterpret_cast<NavNode *>(d.buf + o_nn);
+  d.nav_children = reinterpret_cast<int32_t *>(d.buf + o_nc);
+  d.bvhs = reinterpret_cast<BvhRec *>(d.buf + o_bv);
+  d.nav_ipool = reinterpret_cast<int32_t *>(d.buf + o_ip);
+  d.nav_bpool = reinterpret_cast<double *>(d.buf + o_bp);
   d.version = h.version;
   return VGM_OK;
 }

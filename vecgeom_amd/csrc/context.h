@@ -19,6 +19,11 @@ struct DeviceGeometry {
   LogicalRec *lvs = nullptr;
   int32_t *daughters = nullptr;
   PlacedRec *pvs = nullptr;
+  NavNode *nav_nodes = nullptr;
+  int32_t *nav_children = nullptr;
+  BvhRec *bvhs = nullptr;
+  int32_t *nav_ipool = nullptr;
+  double *nav_bpool = nullptr;
 };
 
 // Growable device scratch for host-pointer queries.
@@ -30,6 +35,7 @@ struct Scratch {
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 int sync_geometry(vgm_ctx *c);
+int ensure_scratch(vgm_ctx *c, size_t bytes);
 size_t align256(size_t x);
 
 }  // namespace vgm

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 #include "vgm_types.h"
+#include "nav_data.h"
 
 namespace vgm {
 
@@ -16,6 +17,13 @@ struct HostGeometry {
   std::vector<LogicalRec> lvs;
   std::vector<std::vector<int32_t>> lv_daughters;  // placed-volume ids per logical volume
   std::vector<PlacedRec> pvs;
+  // navigation tables, built by close_navigation
+  int world_pv = -1;
+  std::vector<NavNode> nav_nodes;      // [0] unused: node 0 is "outside"
+  std::vector<int32_t> nav_children;
+  std::vector<BvhRec> bvhs;
+  std::vector<int32_t> nav_ipool;
+  std::vector<double> nav_bpool;
   uint64_t version = 1;  // bumped on every change; the device copy follows it
 
   // Appends a parameter block of `ndoubles` and registers a shape; returns its id.
@@ -30,6 +38,9 @@ void make_tube_data(TubeData &t, double rmin, double rmax, double z, double sphi
 // Cone parameter derivation shared by the cone and polycone builders.
 bool make_cone_data(ConeData &c, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
                     double dphi, std::string &err);
+
+// Navigation tables for the tree under `world_pv`; returns the node count.
+int close_navigation(HostGeometry &g, int world_pv, std::string &err);
 
 // Shape builders.  Return the new shape id, or -1 with `err` set.
 int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err);

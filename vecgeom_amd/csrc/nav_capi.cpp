@@ -1,0 +1,156 @@
+// Navigation entry points of the C ABI (include/vgm/vgm.h).
+#include <cstring>
+#include <initializer_list>
+#include "context.h"
+#include "nav_kernels.h"
+
+using namespace vgm;
+
+namespace {
+
+// One array argument of a navigation call.  Host arrays are staged through
+// the context's device scratch; device arrays are used in place.  `slot` is
+// the NavArgs field that receives the device address.
+struct Arr {
+  const void *user;
+  size_t bytes;
+  bool in, out;
+  void **slot;
+  bool optional = false;  // may be null
+};
+
+int run_nav(vgm_ctx *c, int kind, NavArgs &a, std::initializer_list<Arr> arrs, unsigned flags)
+{
+  if (!c) return fail(VGM_EINVAL, "null context");
+  if (c->host.nav_nodes.size() < 2) return fail(VGM_ESTATE, "navigation tables not built: call vgm_nav_close");
+  for (const Arr &r : arrs)
+    if (!r.user && !r.optional && a.n) return fail(VGM_EINVAL, "null array argument");
+  VGM_HIP(hipSetDevice(c->device));
+  int rc = sync_geometry(c);
+  if (rc) return rc;
+  const DeviceGeometry &d = c->dev;
+  a.g = GeomView{d.shapes, d.pool, d.lvs, d.daughters, d.pvs, d.nav_nodes, d.nav_children, d.bvhs,
+                 d.nav_ipool, d.nav_bpool, c->host.world_pv};
+  if (a.n == 0) return VGM_OK;
+  if (flags & VGM_DEVICE_PTRS) {
+    for (const Arr &r : arrs) *r.slot = const_cast<void *>(r.user);
+    VGM_HIP(launch_nav(c->stream, kind, a));
+    if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
+    return VGM_OK;
+  }
+  size_t total = 0;
+  for (const Arr &r : arrs)
+    if (r.user) total += align256(r.bytes);
+  if ((rc = ensure_scratch(c, total))) return rc;
+  char *b = c->scratch.buf;
+  for (const Arr &r : arrs) {
+    if (!r.user) {
+      *r.slot = nullptr;
+      continue;
+    }
+    *r.slot = b;
+    if (r.in) VGM_HIP(hipMemcpyAsync(b, r.user, r.bytes, hipMemcpyHostToDevice, c->stream));
+    b += align256(r.bytes);
+  }
+  VGM_HIP(launch_nav(c->stream, kind, a));
+  for (const Arr &r : arrs)
+    if (r.user && r.out)
+      VGM_HIP(hipMemcpyAsync(const_cast<void *>(r.user), *r.slot, r.bytes, hipMemcpyDeviceToHost, c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  return VGM_OK;
+}
+
+// address of a NavArgs pointer field, whatever its pointee type
+template <typename T>
+void **slot(T *&p)
+{
+  return (void **)&p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vgm_nav_close(vgm_ctx *c, int world_pv, int *nnodes)
+{
+  if (!c) return fail(VGM_EINVAL, "null context");
+  std::string err;
+  const int n = close_navigation(c->host, world_pv, err);
+  if (n < 0) return fail(VGM_EINVAL, err);
+  if (nnodes) *nnodes = n;
+  return VGM_OK;
+}
+
+int vgm_nav_node(vgm_ctx *c, uint32_t node, int *pv, int *level, int *mother)
+{
+  if (!c || !pv || !level || !mother) return fail(VGM_EINVAL, "null argument");
+  if (node == 0 || node >= c->host.nav_nodes.size()) return fail(VGM_EINVAL, "invalid state index");
+  const NavNode &N = c->host.nav_nodes[node];
+  *pv = N.pv;
+  *level = N.level;
+  *mother = N.mother;
+  return VGM_OK;
+}
+
+int vgm_nav_locate(vgm_ctx *c, const double *x, const double *y, const double *z, size_t n, uint32_t *state,
+                   unsigned flags)
+{
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavLocate, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {state, b4, false, true, slot(a.state)}},
+                 flags);
+}
+
+int vgm_nav_step(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+                 const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                 uint8_t *boundary, double *step, unsigned flags)
+{
+  if (n && (!state || !last || !boundary || !step)) return fail(VGM_EINVAL, "null state or output array");
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavStep, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit), true},
+                  {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
+                  {boundary, n, true, true, slot(a.boundary)}, {step, b8, false, true, slot(a.step)}},
+                 flags);
+}
+
+int vgm_nav_transport(vgm_ctx *c, double *x, double *y, double *z, const double *dx, const double *dy,
+                      const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
+                      int32_t *steps, unsigned flags)
+{
+  if (n && (!state || !last || !boundary || !steps)) return fail(VGM_EINVAL, "null state or counter array");
+  if (nsteps < 0) return fail(VGM_EINVAL, "negative step count");
+  NavArgs a{};
+  a.n = n;
+  a.nsteps = nsteps;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavTransport, a,
+                 {{x, b8, true, true, slot(a.x)}, {y, b8, true, true, slot(a.y)}, {z, b8, true, true, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {state, b4, true, true, slot(a.state)},
+                  {last, b4, true, true, slot(a.last)}, {boundary, n, true, true, slot(a.boundary)},
+                  {steps, b4, true, true, slot(a.steps)}},
+                 flags);
+}
+
+int vgm_nav_safety(vgm_ctx *c, const double *x, const double *y, const double *z, size_t n, const uint32_t *state,
+                   double *out, unsigned flags)
+{
+  if (n && !out) return fail(VGM_EINVAL, "null output array");
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavSafety, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {state, b4, true, false, slot(a.state)}, {out, b8, false, true, slot(a.step)}},
+                 flags);
+}
+
+}  // extern "C"

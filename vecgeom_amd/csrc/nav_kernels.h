@@ -1,0 +1,28 @@
+// Interface between the navigation C-ABI (nav_capi.cpp) and the navigation
+// kernels (nav_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "navigation.hpp"
+
+namespace vgm {
+
+enum NavKind : int32_t { kNavLocate = 0, kNavStep = 1, kNavTransport = 2, kNavSafety = 3 };
+
+struct NavArgs {
+  GeomView g;
+  double *x, *y, *z;
+  const double *dx, *dy, *dz;
+  const double *step_limit;  // per track, may be null (unlimited)
+  uint32_t *state, *last;
+  uint8_t *boundary;
+  double *step;    // step: output step; safety: output safety
+  int32_t *steps;  // transport: steps taken (accumulated)
+  size_t n;
+  int32_t nsteps;
+};
+
+hipError_t launch_nav(hipStream_t stream, int kind, const NavArgs &a);
+
+}  // namespace vgm

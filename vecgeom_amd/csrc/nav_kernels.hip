@@ -1,0 +1,93 @@
+// Navigation kernels: one lane per track.
+//   locate     GlobalLocator::LocateGlobalPoint from the world (top = true)
+//   step       one ComputeStepAndPropagatedState per track
+//   transport  nsteps of it with an unlimited physics step, moving the global
+//              point by each step; a track that leaves the world stops
+//   safety     VSafetyEstimator::ComputeSafety (mother SafetyToOut, then the
+//              daughters through the BVH)
+// States and geometry tables live in HBM; a track's point, direction and
+// state stay in registers for the whole transport launch.
+#include "nav_kernels.h"
+#include "navigation.hpp"
+
+namespace vgm {
+
+constexpr int kNavBlock = 128;
+
+__global__ __launch_bounds__(kNavBlock) void nav_locate_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  Navigator nav{a.g};
+  a.state[i] = nav.locate_global(V3<double>{a.x[i], a.y[i], a.z[i]});
+  if (a.last) a.last[i] = 0;
+  if (a.boundary) a.boundary[i] = 0;
+}
+
+__global__ __launch_bounds__(kNavBlock) void nav_step_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  if (!a.state[i]) {
+    a.step[i] = 0.0;
+    return;
+  }
+  Navigator nav{a.g};
+  const NavState in{a.state[i], a.last[i], a.boundary[i] != 0};
+  NavState out;
+  a.step[i] = nav.step(V3<double>{a.x[i], a.y[i], a.z[i]}, V3<double>{a.dx[i], a.dy[i], a.dz[i]},
+                       a.step_limit ? a.step_limit[i] : kInfLength, in, out);
+  a.state[i] = out.ind;
+  a.last[i] = out.last;
+  a.boundary[i] = out.boundary ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kNavBlock) void nav_transport_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  Navigator nav{a.g};
+  NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
+  V3<double> p{a.x[i], a.y[i], a.z[i]};
+  const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
+  int k = 0;
+  for (; k < a.nsteps && s.ind; ++k) {
+    NavState out;
+    const double st = nav.step(p, d, kInfLength, s, out);
+    p.x = p.x + st * d.x;
+    p.y = p.y + st * d.y;
+    p.z = p.z + st * d.z;
+    s = out;
+  }
+  a.x[i] = p.x;
+  a.y[i] = p.y;
+  a.z[i] = p.z;
+  a.state[i] = s.ind;
+  a.last[i] = s.last;
+  a.boundary[i] = s.boundary ? 1 : 0;
+  a.steps[i] += k;
+}
+
+__global__ __launch_bounds__(kNavBlock) void nav_safety_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  Navigator nav{a.g};
+  a.step[i] = nav.safety(V3<double>{a.x[i], a.y[i], a.z[i]}, a.state[i]);
+}
+
+hipError_t launch_nav(hipStream_t stream, int kind, const NavArgs &a)
+{
+  if (a.n == 0) return hipSuccess;
+  const unsigned blocks = unsigned((a.n + kNavBlock - 1) / kNavBlock);
+  switch (kind) {
+  case kNavLocate: hipLaunchKernelGGL(nav_locate_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavStep: hipLaunchKernelGGL(nav_step_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavTransport: hipLaunchKernelGGL(nav_transport_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace vgm

@@ -1,0 +1,317 @@
+// Navigation on the device: one lane per track, FP64.
+//   ComputeStepAndPropagatedState of VNavigatorHelper<BVHNavigator<false>>
+//     (VecGeom/navigation/VNavigator.h:436-492): global -> local with the
+//     state's cached matrix, DistanceToOut of the mother clamped at 0
+//     (TreatDistanceToMother :241-249), daughters through the BVH
+//     (BVH::CheckDaughterIntersections, VecGeom/base/BVH.h:96-146),
+//     PrepareOutState (:192-235), and relocation of the point pushed 1e-6 mm
+//     past the boundary (MovePointAfterBoundary :168-173, Relocate :1070-1091)
+//     with GlobalLocator::LocateGlobalPoint / LocateGlobalPointExclVolume /
+//     RelocatePointFromPathForceDifferent (VecGeom/navigation/GlobalLocator.h:35-200)
+//     and BVH::LevelLocate (VecGeom/base/BVH.h:219-280).
+//   Safety: BVHSafetyEstimator::ComputeSafetyForLocalPoint
+//     (VecGeom/navigation/BVHSafetyEstimator.h:39-47) with BVH::ComputeSafety
+//     (VecGeom/base/BVH.h:172-209).
+// Shapes are reached through one out-of-line dispatcher per query, so the
+// navigation code is compiled once rather than once per shape family.
+#pragma once
+#include "vgm_math.hpp"
+#include "transform.hpp"
+#include "nav_data.h"
+#include "shapes/all_shapes.hpp"
+
+namespace vgm {
+
+struct GeomView {
+  const ShapeRec *shapes;
+  const double *pool;
+  const LogicalRec *lvs;
+  const int32_t *daughters;
+  const PlacedRec *pvs;
+  const NavNode *nodes;
+  const int32_t *children;
+  const BvhRec *bvhs;
+  const int32_t *ipool;
+  const double *bpool;
+  int32_t world_pv;
+};
+
+enum ShapeQuery : int32_t { kQContains = 0, kQDistIn = 1, kQDistOut = 2, kQSafIn = 3, kQSafOut = 4 };
+
+struct ShapeCall {
+  const double *params;
+  int query;
+  V3<double> p, d;
+  double step;
+  double result;
+  template <class S>
+  __host__ __device__ void operator()()
+  {
+    const auto &s = *reinterpret_cast<const typename S::Data *>(params);
+    switch (query) {
+    case kQContains: result = S::template contains<double>(s, p) ? 1.0 : 0.0; break;
+    case kQDistIn: result = S::template distance_to_in<double>(s, p, d, step); break;
+    case kQDistOut: result = S::template distance_to_out<double>(s, p, d, step); break;
+    case kQSafIn: result = S::template safety_to_in<double>(s, p); break;
+    default: result = S::template safety_to_out<double>(s, p); break;
+    }
+  }
+};
+
+__host__ __device__ __noinline__ double shape_query(const ShapeRec *shapes, const double *pool, int shape, int query,
+                                           V3<double> p, V3<double> d, double step)
+{
+  const ShapeRec &r = shapes[shape];
+  ShapeCall c{pool + r.off, query, p, d, step, 0.0};
+  dispatch_shape_type(r.type, c);
+  return c.result;
+}
+
+struct Navigator {
+  const GeomView &g;
+
+  __host__ __device__ int shape_of_pv(int pv) const { return g.lvs[g.pvs[pv].lv].shape; }
+
+  // ---- AABB (VecGeom/base/AABB.h) ---------------------------------------------
+  __host__ __device__ static bool box_contains(const double *b, V3<double> p)
+  {
+    return p.x >= b[0] && p.x <= b[3] && p.y >= b[1] && p.y <= b[4] && p.z >= b[2] && p.z <= b[5];
+  }
+  __host__ __device__ static void box_intersect(const double *b, V3<double> p, V3<double> inv, double &tmin, double &tmax)
+  {
+    double a0 = (b[0] - p.x) * inv.x, a1 = (b[1] - p.y) * inv.y, a2 = (b[2] - p.z) * inv.z;
+    double c0 = (b[3] - p.x) * inv.x, c1 = (b[4] - p.y) * inv.y, c2 = (b[5] - p.z) * inv.z;
+    if (a0 > c0) {
+      const double t = a0;
+      a0 = c0;
+      c0 = t;
+    }
+    if (a1 > c1) {
+      const double t = a1;
+      a1 = c1;
+      c1 = t;
+    }
+    if (a2 > c2) {
+      const double t = a2;
+      a2 = c2;
+      c2 = t;
+    }
+    tmin = vmax(vmax(a0, a1), a2);
+    tmax = vmin(vmin(c0, c1), c2) * (1. + 2 * 2.220446049250313e-16);
+  }
+  __host__ __device__ static double box_safety(const double *b, V3<double> p)
+  {
+    const double cx = 0.5 * (b[3] + b[0]), cy = 0.5 * (b[4] + b[1]), cz = 0.5 * (b[5] + b[2]);
+    const double sx = b[3] - b[0], sy = b[4] - b[1], sz = b[5] - b[2];
+    return vmax(vmax(vabs(p.x - cx) - 0.5 * sx, vabs(p.y - cy) - 0.5 * sy), vabs(p.z - cz) - 0.5 * sz);
+  }
+
+  // ---- daughters through the BVH --------------------------------------------------
+  // BVH::CheckDaughterIntersections: step and hit (daughter index) updated
+  __host__ __device__ void check_daughters(int lv, V3<double> p, V3<double> d, double &step, int last_pv, int &hit) const
+  {
+    const LogicalRec &L = g.lvs[lv];
+    if (L.bvh < 0) return;
+    const BvhRec &B = g.bvhs[L.bvh];
+    const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
+    const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
+    const V3<double> inv{1.0 / nonzero(d.x), 1.0 / nonzero(d.y), 1.0 / nonzero(d.z)};
+    unsigned stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    do {
+      const unsigned id = stack[--sp];
+      const int nc = nchild[id];
+      if (nc >= 0) {
+        for (int i = 0; i < nc; ++i) {
+          const int k = prim[offset[id] + i];
+          double tmin, tmax;
+          box_intersect(pbox + 6 * k, p, inv, tmin, tmax);
+          if (tmin <= tmax && tmax >= 0.0 && tmin < step) {
+            const int v = g.daughters[L.dbegin + k];
+            const Transform &T = g.pvs[v].tr;
+            const double dist = shape_query(g.shapes, g.pool, shape_of_pv(v), kQDistIn, to_local_point(T, p),
+                                            to_local_dir(T, d), step);
+            if (dist < step && !(dist <= 0.0 && v == last_pv)) {
+              step = dist;
+              hit = k;
+            }
+          }
+        }
+      } else {
+        const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
+        double tminL, tmaxL, tminR, tmaxR;
+        box_intersect(nbox + 6 * cl, p, inv, tminL, tmaxL);
+        box_intersect(nbox + 6 * cr, p, inv, tminR, tmaxR);
+        const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
+        const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
+        if (tminR < tminL) {
+          if (tr) stack[sp++] = cr;
+          if (tl) stack[sp++] = cl;
+        } else {
+          if (tl) stack[sp++] = cl;
+          if (tr) stack[sp++] = cr;
+        }
+      }
+    } while (sp > 0);
+  }
+
+  // BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint): first
+  // daughter in tree order (skipping excl_pv) that contains the point
+  __host__ __device__ int level_locate(int lv, int excl_pv, V3<double> p, V3<double> &dp) const
+  {
+    const LogicalRec &L = g.lvs[lv];
+    if (L.bvh < 0) return -1;
+    const BvhRec &B = g.bvhs[L.bvh];
+    const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
+    const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
+    unsigned stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    do {
+      const unsigned id = stack[--sp];
+      const int nc = nchild[id];
+      if (nc >= 0) {
+        for (int i = 0; i < nc; ++i) {
+          const int k = prim[offset[id] + i];
+          if (!box_contains(pbox + 6 * k, p)) continue;
+          const int v = g.daughters[L.dbegin + k];
+          if (v == excl_pv) continue;
+          const V3<double> q = to_local_point(g.pvs[v].tr, p);
+          if (shape_query(g.shapes, g.pool, shape_of_pv(v), kQContains, q, q, 0.0) != 0.0) {
+            dp = q;
+            return k;
+          }
+        }
+      } else {
+        const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
+        if (box_contains(nbox + 6 * cl, p)) stack[sp++] = cl;
+        if (box_contains(nbox + 6 * cr, p)) stack[sp++] = cr;
+      }
+    } while (sp > 0);
+    return -1;
+  }
+
+  // LocateGlobalPoint(vol, point, path, top = false) from the node of vol
+  __host__ __device__ unsigned descend(unsigned node, V3<double> p, int excl_pv) const
+  {
+    for (;;) {
+      const int lv = g.pvs[g.nodes[node].pv].lv;
+      V3<double> dp;
+      const int k = level_locate(lv, excl_pv, p, dp);
+      if (k < 0) return node;
+      node = unsigned(g.children[g.nodes[node].child0 + k]);
+      p = dp;
+    }
+  }
+
+  __host__ __device__ unsigned locate_global(V3<double> gp) const
+  {
+    if (shape_query(g.shapes, g.pool, shape_of_pv(g.world_pv), kQContains, gp, gp, 0.0) == 0.0) return 0;
+    return descend(1, gp, -1);
+  }
+
+  // ---- one step -------------------------------------------------------------------
+  __host__ __device__ double step(V3<double> gp, V3<double> gd, double step_limit, const NavState &in, NavState &out) const
+  {
+    const NavNode &N = g.nodes[in.ind];
+    const V3<double> lp = to_local_point(N.global, gp), ld = to_local_dir(N.global, gd);
+    const int pv = N.pv;
+    const int lv = g.pvs[pv].lv;
+    double st = shape_query(g.shapes, g.pool, g.lvs[lv].shape, kQDistOut, lp, ld, step_limit);
+    if (st < 0.) st = 0.;
+    int hit = -1;
+    const int last_pv = in.last ? g.nodes[in.last].pv : -1;
+    check_daughters(lv, lp, ld, st, last_pv, hit);
+    out = in;
+    if (st == kInfLength && step_limit > 0.) {
+      out.boundary = true;
+      out.ind = unsigned(N.mother);
+      return kTolerance;
+    }
+    if (st > step_limit) {
+      out.boundary = false;
+      return step_limit;
+    }
+    out.boundary = true;
+    out.last = out.ind;
+    if (hit >= 0) out.ind = unsigned(g.children[N.child0 + hit]);
+    if (st < 0.) st = 0.;
+    const double e = st + kPushTolerance;
+    V3<double> q{lp.x + e * ld.x, lp.y + e * ld.y, lp.z + e * ld.z};
+    if (out.ind == in.ind) {
+      // RelocatePointFromPathForceDifferent: climb while the point is outside
+      // (the entry volume is always left), then descend excluding it
+      unsigned cur = out.ind;
+      const unsigned entry = cur;
+      const int entry_pv = pv;
+      while (cur) {
+        const int cpv = g.nodes[cur].pv;
+        if (cur == entry ||
+            shape_query(g.shapes, g.pool, shape_of_pv(cpv), kQContains, q, q, 0.0) == 0.0) {
+          q = to_master_point(g.pvs[cpv].tr, q);
+          cur = unsigned(g.nodes[cur].mother);
+        } else {
+          break;
+        }
+      }
+      out.ind = cur ? descend(cur, q, entry_pv) : 0u;
+    } else {
+      const unsigned next = out.ind;
+      out.ind = descend(next, to_local_point(g.pvs[g.nodes[next].pv].tr, q), -1);
+    }
+    return st;
+  }
+
+  // ---- safety -----------------------------------------------------------------------
+  __host__ __device__ double daughter_safety(int lv, V3<double> p, double safety) const
+  {
+    const LogicalRec &L = g.lvs[lv];
+    const BvhRec &B = g.bvhs[L.bvh];
+    const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
+    const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
+    unsigned stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    do {
+      const unsigned id = stack[--sp];
+      const int nc = nchild[id];
+      if (nc >= 0) {
+        for (int i = 0; i < nc; ++i) {
+          const int k = prim[offset[id] + i];
+          if (box_safety(pbox + 6 * k, p) < safety) {
+            const int v = g.daughters[L.dbegin + k];
+            const double dist =
+                shape_query(g.shapes, g.pool, shape_of_pv(v), kQSafIn, to_local_point(g.pvs[v].tr, p), p, 0.0);
+            if (dist < safety) safety = dist;
+          }
+        }
+      } else {
+        const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
+        const double sL = box_safety(nbox + 6 * cl, p), sR = box_safety(nbox + 6 * cr, p);
+        const bool tl = sL < safety, tr = sR < safety;
+        if (sR < sL) {
+          if (tr) stack[sp++] = cr;
+          if (tl) stack[sp++] = cl;
+        } else {
+          if (tl) stack[sp++] = cl;
+          if (tr) stack[sp++] = cr;
+        }
+      }
+    } while (sp > 0);
+    return safety;
+  }
+
+  __host__ __device__ double safety(V3<double> gp, unsigned node) const
+  {
+    if (!node) return 0.0;
+    const NavNode &N = g.nodes[node];
+    const V3<double> lp = to_local_point(N.global, gp);
+    const int lv = g.pvs[N.pv].lv;
+    double s = shape_query(g.shapes, g.pool, g.lvs[lv].shape, kQSafOut, lp, lp, 0.0);
+    if (s > 0.0 && g.lvs[lv].bvh >= 0) s = daughter_safety(lv, lp, s);
+    return s;
+  }
+};
+
+}  // namespace vgm

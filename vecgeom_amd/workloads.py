@@ -170,3 +170,57 @@ def ellipsoid_hull(ntri: int, axes, seed: int):
     flip = np.einsum("ij,ij->i", n, tris.mean(axis=1)) < 0
     tris[flip] = tris[flip][:, [0, 2, 1]]
     return np.ascontiguousarray(tris)
+
+
+# ---- navigation workload: a barrel of tube layers holding box modules --------
+def barrel_layers(nlayers: int = 20):
+    """Layer table of the navigation benchmark geometry.
+
+    World box half-lengths (2000, 2000, 3000) mm.  Layer i is a full tube with
+    rmin = 100 + 95 i, thickness 80 and half-length 2800 mm, placed at the
+    origin.  It holds n_i = min(64 * 2**floor(3 i / 20), 512) box modules on a
+    ring of radius rmin + 40, module k rotated about z by 2 pi k / n_i.  The
+    modules are 60 mm thick radially, 2700 mm long, and 0.8 of the ring pitch
+    wide at their inner face, so neighbours never touch and all corners stay
+    inside the layer.
+    """
+    layers = []
+    for i in range(nlayers):
+        rmin = 100.0 + 95.0 * i
+        n = min(64 * 2 ** ((3 * i) // 20), 512)
+        rc, hr = rmin + 40.0, 30.0
+        dphi = 2.0 * np.pi / n
+        hw = 0.8 * (rc - hr) * np.sin(0.5 * dphi)
+        # corner inside rmax = rmin + 80
+        hw = min(hw, 0.9 * np.sqrt((rmin + 80.0) ** 2 - (rc + hr) ** 2))
+        layers.append({"rmin": rmin, "rmax": rmin + 80.0, "dz": 2800.0, "n": n, "rc": rc,
+                       "half": (hr, hw, 2700.0)})
+    return layers
+
+
+def rotz_transform(phi: float, t=(0.0, 0.0, 0.0)):
+    """Translation + rotation about z in the 12-double placement layout
+    (local = R^T (master - t), rotation stored row-major)."""
+    c, s = np.cos(phi), np.sin(phi)
+    return np.array([t[0], t[1], t[2], c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0], dtype=np.float64)
+
+
+def build_barrel(geom, nlayers: int = 20):
+    """Build the barrel on `geom` (library context or oracle); returns the world placed volume."""
+    world_lv = geom.logical_volume(geom.box(2000.0, 2000.0, 3000.0))
+    for L in barrel_layers(nlayers):
+        tube_lv = geom.logical_volume(geom.tube(L["rmin"], L["rmax"], L["dz"], 0.0, 2.0 * np.pi))
+        mod_lv = geom.logical_volume(geom.box(*L["half"]))
+        for k in range(L["n"]):
+            phi = 2.0 * np.pi * k / L["n"]
+            tr = rotz_transform(phi, (L["rc"] * np.cos(phi), L["rc"] * np.sin(phi), 0.0))
+            geom.add_daughter(tube_lv, geom.place(mod_lv, tr))
+        geom.add_daughter(world_lv, geom.place(tube_lv))
+    return geom.place(world_lv)
+
+
+def barrel_tracks(n: int, seed: int = 42):
+    """Points uniform in the world box, isotropic directions."""
+    pts = sampling.points_in_box(seed, 11, n, np.array([2000.0, 2000.0, 3000.0]), 1.0, np.zeros(3))
+    dirs = sampling.isotropic_directions(seed, 12, n)
+    return pts, dirs
