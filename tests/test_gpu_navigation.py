@@ -1,0 +1,76 @@
+"""GPU navigation vs the CPU oracle on the barrel geometry, bit-exact:
+located states, one step, 200-step transport and safeties."""
+import numpy as np
+import pytest
+
+import oracle
+import vecgeom_amd as vg
+from vecgeom_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+N = 50000
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+
+    ctx = vg.Context(0)
+    w = workloads.build_barrel(ctx)
+    nav = vg.Navigation(ctx, w)
+    geo = oracle.Geometry()
+    wo = workloads.build_barrel(geo)
+    assert geo.nav_close(wo, use_bvh=True) == nav.nnodes
+    pts, dirs = workloads.barrel_tracks(N, seed=9)
+    yield ctx, nav, geo, pts, dirs
+    ctx.close()
+
+
+def test_locate(setup):
+    print("[nav] locate", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = np.zeros(N, np.uint32)
+    nav.locate(*pts, st)
+    assert np.array_equal(st, geo.nav_locate(pts))
+
+
+def test_step_and_safety(setup):
+    print("[nav] step + safety", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    saf = np.zeros(N)
+    nav.safety(*pts, st, saf)
+    assert np.array_equal(saf, geo.nav_safety(pts, st))
+    s_g, l_g, b_g = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step = np.zeros(N)
+    nav.step(*pts, *dirs, s_g, l_g, b_g, step)
+    s_o, l_o, b_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    ref = geo.nav_step(pts, dirs, s_o, l_o, b_o)
+    assert np.array_equal(step, ref)
+    assert np.array_equal(s_g, s_o) and np.array_equal(l_g, l_o) and np.array_equal(b_g, b_o)
+
+
+@pytest.mark.parametrize("device_ptrs", [False, True], ids=["host", "device"])
+def test_transport_200_steps(setup, device_ptrs):
+    print(f"[nav] transport device_ptrs={device_ptrs}", flush=True)
+    import torch
+
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    ref = [a.copy() for a in pts]
+    s_o, l_o, b_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    n_o = np.zeros(N, np.int32)
+    geo.nav_transport(*ref, *dirs, s_o, l_o, b_o, 200, n_o)
+    arrs = [a.copy() for a in pts] + [a.copy() for a in dirs] + [st.copy(), np.zeros(N, np.uint32),
+                                                                    np.zeros(N, np.uint8), np.zeros(N, np.int32)]
+    if device_ptrs:
+        arrs = [torch.from_numpy(a).cuda() for a in arrs]
+        torch.cuda.synchronize()
+    nav.transport(*arrs[:6], arrs[6], arrs[7], arrs[8], 200, arrs[9])
+    got = [a.cpu().numpy() if device_ptrs else a for a in arrs]
+    for k, r in enumerate(ref):
+        assert np.array_equal(got[k], r), k
+    assert np.array_equal(got[6], s_o) and np.array_equal(got[7], l_o) and np.array_equal(got[8], b_o)
+    assert np.array_equal(got[9], n_o)
+    assert n_o.sum() > 20 * N

@@ -58,7 +58,7 @@ struct ShapeCall {
   }
 };
 
-__host__ __device__ __noinline__ double shape_query(const ShapeRec *shapes, const double *pool, int shape, int query,
+static __host__ __device__ __noinline__ double shape_query(const ShapeRec *shapes, const double *pool, int shape, int query,
                                            V3<double> p, V3<double> d, double step)
 {
   const ShapeRec &r = shapes[shape];
