@@ -44,6 +44,65 @@ def available_families():
     return tuple(fams)
 
 
+def run_navigation(args, ctx, dev, torch):
+    """Barrel navigation: straight-line transport of located tracks through
+    20 tube layers of box modules, nsteps of ComputeStepAndPropagatedState per
+    track (tracks that leave the world stop).  Value: track-steps/s."""
+    import vecgeom_amd as vg
+    from vecgeom_amd import workloads
+
+    t0 = time.perf_counter()
+    world = workloads.build_barrel(ctx)
+    nav = vg.Navigation(ctx, world)
+    pts, dirs = workloads.barrel_tracks(args.nav_tracks, seed=42)
+    n = args.nav_tracks
+    P = [torch.from_numpy(a).to(dev) for a in pts]
+    D = [torch.from_numpy(a).to(dev) for a in dirs]
+    st0 = torch.zeros(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    nav.locate(*P, st0)
+    log(f"[bench] navigation: {nav.nnodes} states, {n} tracks located ({time.perf_counter() - t0:.1f}s)")
+    stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+
+    def one(timed):
+        x, y, z = (a.clone() for a in P)
+        st = st0.clone()
+        last = torch.zeros(n, dtype=torch.int32, device=dev)
+        bnd = torch.zeros(n, dtype=torch.uint8, device=dev)
+        steps = torch.zeros(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        a.record(stream)
+        nav.transport(x, y, z, *D, st, last, bnd, args.nav_steps, steps, asynchronous=True)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, a.elapsed_time(b) * 1e-3, int(steps.sum().item())
+
+    one(False)
+    wall, kern, total = one(True)
+    out = {"workload": f"barrel: 20 tube layers, {sum(l['n'] for l in workloads.barrel_layers())} box modules, "
+                       f"{n} tracks x up to {args.nav_steps} steps", "value": total / kern, "unit": "track-steps/s",
+           "track_steps": total, "kernel_ms": kern * 1e3, "wall_ms": wall * 1e3}
+    log(f"[bench] navigation: {total / kern / 1e9:.3f} G track-steps/s ({total} steps, kernel {kern * 1e3:.1f} ms)")
+    if not args.no_cpu:
+        import oracle
+
+        geo = oracle.Geometry()
+        geo.nav_close(workloads.build_barrel(geo), use_bvh=True)
+        m = min(n, 100_000)
+        cp = [a[:m].copy() for a in pts]
+        st = geo.nav_locate(cp)
+        last, bnd, steps = np.zeros(m, np.uint32), np.zeros(m, np.uint8), np.zeros(m, np.int32)
+        nth = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        t = time.perf_counter()
+        geo.nav_transport(*cp, *[a[:m].copy() for a in dirs], st, last, bnd, args.nav_steps, steps, nthreads=nth)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": float(steps.sum()) / dt, "unit": "track-steps/s", "cores": nth,
+                               "kind": "port", "sample": f"{m} tracks x up to {args.nav_steps} steps, BVH oracle"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +113,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-tracks", type=int, default=400_000)
+    ap.add_argument("--nav-tracks", type=int, default=4_000_000, help="navigation benchmark tracks (0: skip)")
+    ap.add_argument("--nav-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; runs on several GPUs are not part of this build", file=sys.stderr)
@@ -153,6 +214,10 @@ def main():
         cpu = {"value": cn * S / dt, "unit": "track-steps/s", "cores": nth, "kind": "port",
                "sample": f"{cn} tracks x {S} track-steps, same solids and generator, FP64 oracle, OpenMP"}
 
+    navigation = None
+    if args.nav_tracks > 0:
+        navigation = run_navigation(args, ctx, dev, torch)
+
     line = {
         "metric": "track-steps/sec on 1 MI355X (mixed+learned vs plain FP64), 0 safety violations",
         "value": ratemx,
@@ -176,6 +241,7 @@ def main():
         "launch": {"fp64": launch64, "mixed": launchmx},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "navigation": navigation,
     }
     print(json.dumps(line))
 

@@ -9,6 +9,8 @@ namespace vgm {
 struct BoxShape {
   using Data = BoxData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
 
   // max(|p| - h) (BoxImplementation.h:76-90)
   template <typename R>

@@ -11,6 +11,8 @@ namespace vgm {
 struct PolyconeShape {
   using Data = PolyconeData;
   static constexpr bool kStage = true;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
 
   VGM_DEV static int section_index(const Data &d, double zp)
   {

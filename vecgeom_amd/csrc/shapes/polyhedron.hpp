@@ -17,6 +17,8 @@ namespace vgm {
 struct PolyhedronShape {
   using Data = PolyhedronData;
   static constexpr bool kStage = true;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
   using Quad = PolyhedronQuad;
 
   template <typename R>

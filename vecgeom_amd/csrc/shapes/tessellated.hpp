@@ -18,6 +18,8 @@ namespace vgm {
 struct TessellatedShape {
   using Data = TessData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = true;
   static constexpr int kStack = 64;      // tree depth is below 32 (median splits)
 
   template <typename R>

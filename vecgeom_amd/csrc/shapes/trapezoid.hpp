@@ -10,6 +10,8 @@ namespace vgm {
 struct TrapezoidShape {
   using Data = TrapezoidData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
 
   template <typename R>
   VGM_DEV static R pdist(const Data &t, int i, V3<R> p)

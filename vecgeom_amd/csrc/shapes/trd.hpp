@@ -9,6 +9,8 @@ namespace vgm {
 struct TrdShape {
   using Data = TrdData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

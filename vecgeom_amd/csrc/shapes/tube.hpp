@@ -12,6 +12,8 @@ namespace vgm {
 struct TubeShape {
   using Data = TubeData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
+  static constexpr bool kSafetyIsDistance = false;
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>

@@ -31,20 +31,48 @@ struct TrackState {
   V3<double> p, d;
   bool in;
   int32_t crossings;
+  double carry;  // lower bound on the safety at p (0: none), see step_fp64
 };
+
+// Slack of the carried safety bound: rounding of the moves and the
+// tolerance-level slack of the facet tests.
+VGM_DEV double carry_margin(const V3<double> &p)
+{
+  return 1e-7 + 1e-12 * (vabs(p.x) + vabs(p.y) + vabs(p.z));
+}
 
 // One track-step in FP64 (see stepping.hpp for the definition).  Returns
 // true when the step ended on a boundary crossing.
+//
+// Families whose safety is a distance to their surface (kSafetyIsDistance:
+// the tessellated solid's minimum point-to-facet distance) can skip it: a
+// safety s at p bounds the safety at any p' by s - |p' - p|, so after a free
+// move of l the track carries s - l (less a margin) and the next step moves
+// without any query while its length stays below that bound.  The decision
+// is the one the exact safety would take, so trajectories are unchanged; a
+// crossing or a wrap of the periodic world clears the bound.
 template <class S>
 VGM_DEV bool step_fp64(const typename S::Data &s, TrackState &t, double l)
 {
+  if constexpr (S::kSafetyIsDistance) {
+    if (l <= t.carry) {
+      const double m = carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      t.carry = t.carry - l - m;
+      return false;
+    }
+  }
   const double saf = t.in ? S::template safety_to_out<double>(s, t.p) : S::template safety_to_in<double>(s, t.p);
   if (l <= saf) {
+    if constexpr (S::kSafetyIsDistance) t.carry = saf - l - carry_margin(t.p);
     t.p.x = t.p.x + l * t.d.x;
     t.p.y = t.p.y + l * t.d.y;
     t.p.z = t.p.z + l * t.d.z;
     return false;
   }
+  t.carry = 0.0;
   double db = t.in ? S::template distance_to_out<double>(s, t.p, t.d, kInfLength)
                    : S::template distance_to_in<double>(s, t.p, t.d, kInfLength);
   if (db < 0.0) db = 0.0;
@@ -73,6 +101,9 @@ VGM_DEV float guard_band(const V3<float> &p, float scale)
 template <class S>
 VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, float scale, uint32_t &fp64_steps)
 {
+  if constexpr (S::kSafetyIsDistance) {
+    if (l <= t.carry) return step_fp64<S>(s, t, l);  // bound carried from an FP64 safety
+  }
   const V3<float> pf{float(t.p.x), float(t.p.y), float(t.p.z)};
   const V3<float> df{float(t.d.x), float(t.d.y), float(t.d.z)};
   const float lf = float(l);
@@ -88,6 +119,7 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
     t.p.x = t.p.x + l * t.d.x;
     t.p.y = t.p.y + l * t.d.y;
     t.p.z = t.p.z + l * t.d.z;
+    t.carry = 0.0;
     return false;
   }
   fp64_steps++;
@@ -105,17 +137,45 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
     t.d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
     t.in = a.side[i] != 0;
     t.crossings = 0;
-    for (int k = 0; k < a.nsteps; ++k) {
-      const uint32_t step = uint32_t(a.step0 + k);
-      const double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
-      const bool crossed =
-          (Mode == kTransportFP64) ? step_fp64<S>(s, t, l) : step_mixed<S>(s, t, l, scale, fp64_steps);
+    t.carry = 0.0;
+    // after a step: a new direction unless the step crossed, and the
+    // periodic world for tracks outside (a wrap invalidates the carried bound)
+    auto finish = [&](uint32_t step, bool crossed) {
       if (!crossed) t.d = new_direction(a.seed, uint64_t(i), step, t.d);
       if (!t.in) {
+        const V3<double> q = t.p;
         t.p.x = wrap(t.p.x, sr.wc[0], sr.wh[0]);
         t.p.y = wrap(t.p.y, sr.wc[1], sr.wh[1]);
         t.p.z = wrap(t.p.z, sr.wc[2], sr.wh[2]);
+        if (t.p.x != q.x || t.p.y != q.y || t.p.z != q.z) t.carry = 0.0;
       }
+    };
+    int k = 0;
+    while (k < a.nsteps) {
+      uint32_t step = uint32_t(a.step0 + k);
+      double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
+      if constexpr (S::kSafetyIsDistance) {
+        // Steps covered by the carried bound need no geometry query.  Each
+        // lane runs through its own run of them before the next query, so
+        // the lanes of a wave meet at the expensive step instead of idling
+        // through each other's cheap ones.
+        while (l <= t.carry) {
+          const double m = carry_margin(t.p);
+          t.p.x = t.p.x + l * t.d.x;
+          t.p.y = t.p.y + l * t.d.y;
+          t.p.z = t.p.z + l * t.d.z;
+          t.carry = t.carry - l - m;
+          finish(step, false);
+          if (++k >= a.nsteps) break;
+          step = uint32_t(a.step0 + k);
+          l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
+        }
+        if (k >= a.nsteps) break;
+      }
+      const bool crossed =
+          (Mode == kTransportFP64) ? step_fp64<S>(s, t, l) : step_mixed<S>(s, t, l, scale, fp64_steps);
+      finish(step, crossed);
+      ++k;
     }
     a.x[i] = t.p.x;
     a.y[i] = t.p.y;
