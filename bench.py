@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "track_step_flops.json")
+TRAFFIC_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "hbm_traffic.json")
 
 
 def log(msg):
@@ -42,6 +43,43 @@ def available_families():
         if hasattr(L, f"vgm_{f}_create"):
             fams.append(f)
     return tuple(fams)
+
+
+def copy_probe(torch):
+    """A plain copy of known size: 2^27 doubles (1 GiB) read once and written
+    once by an elementwise kernel, five times.  Under rocprofv3 --pmc its
+    FETCH_SIZE and WRITE_SIZE per dispatch give the counters' scale factor."""
+    n = 1 << 27
+    x = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+
+
+def family_roofline(fams_ms, counts_by_family, S):
+    """Roofline of the dominant (longest) family kernel of a timed transport:
+    algorithmic FP64 flops per launch (flop fixture x the family's tracks x
+    track-steps) over its average launch time."""
+    if not os.path.exists(FLOPS_FIXTURE) or not fams_ms:
+        return None
+    fx = json.load(open(FLOPS_FIXTURE))
+    fam = max(fams_ms, key=fams_ms.get)
+    per_step = float(fx["per_family"][fam])
+    flops = per_step * counts_by_family[fam] * S
+    t = fams_ms[fam] * 1e-3
+    achieved = flops / t / 1e12
+    out = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+           "kernel": f"transport_family_kernel<{fam}, FP64>", "kernel_ms": fams_ms[fam],
+           "flops_per_track_step": per_step, "tracks": counts_by_family[fam], "track_steps": S,
+           "note": "FP64 vector peak; flops counted on the FP64 track-step path (+ - * / sqrt)"}
+    if os.path.exists(TRAFFIC_FIXTURE):
+        tr = json.load(open(TRAFFIC_FIXTURE))
+        if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
+            out["traffic"] = tr["bytes_per_launch"]
+    return out
 
 
 def run_navigation(args, ctx, dev, torch):
@@ -115,12 +153,19 @@ def main():
     ap.add_argument("--cpu-tracks", type=int, default=400_000)
     ap.add_argument("--nav-tracks", type=int, default=4_000_000, help="navigation benchmark tracks (0: skip)")
     ap.add_argument("--nav-steps", type=int, default=200)
+    ap.add_argument("--launch", default=None, help="block,grid,tile launch shape instead of autotuning")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain copy kernel of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; runs on several GPUs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        copy_probe(torch)
+        return
 
     import vecgeom_amd as vg
     from vecgeom_amd import workloads
@@ -148,7 +193,9 @@ def main():
         side = side0.clone()
         cr = torch.zeros(n, dtype=torch.int32, device=dev)
         torch.cuda.synchronize()
-        if not args.no_autotune:
+        if args.launch:
+            batch.set_launch(*[int(v) for v in args.launch.split(",")])
+        elif not args.no_autotune:
             log(f"[bench] autotune {'mixed' if mixed else 'fp64'} ...")
             t = time.perf_counter()
             shape = batch.autotune(*state, side, nsteps=2, mixed=mixed)
@@ -159,18 +206,21 @@ def main():
             batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True)
             step0 += S
         torch.cuda.synchronize()
-        evs = []
+        evs, fms = [], []
         t = time.perf_counter()
         for _ in range(args.steps):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True)
+            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True,
+                            timing=True)
             b.record(stream)
             evs.append((a, b))
+            fms.append(batch.family_ms())  # waits for this launch's family kernels
             step0 += S
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         kern = float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
+        fam_ms = {f: float(np.mean([m[f] for m in fms])) for f in fms[0]}
         nfp64 = 0
         if mixed:
             st2 = [x.clone() for x in init]
@@ -178,24 +228,22 @@ def main():
             torch.cuda.synchronize()
             nfp64 = batch.transport(*st2, s2, c2, S,
                                     step0=0, seed=7, mixed=True, count_fp64=True)
-        return wall, kern, nfp64, batch.launch()
+        return wall, kern, nfp64, batch.launch(), fam_ms
 
-    wall64, kern64, _, launch64 = run(False)
+    counts_by_family = {}
+    for (kind, _), c in zip(specs, counts):
+        counts_by_family[kind] = counts_by_family.get(kind, 0) + int(c)
+    wall64, kern64, _, launch64, fam64 = run(False)
     rate64 = args.steps * n * S / wall64
     log(f"[bench] fp64: {rate64 / 1e9:.2f} G track-steps/s (kernel {kern64 * 1e3:.2f} ms/launch)")
-    wallmx, kernmx, nfp64, launchmx = run(True)
+    log("[bench] fp64 family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam64.items()))
+    wallmx, kernmx, nfp64, launchmx, fammx = run(True)
     ratemx = args.steps * n * S / wallmx
     log(f"[bench] mixed: {ratemx / 1e9:.2f} G track-steps/s (kernel {kernmx * 1e3:.2f} ms/launch), "
         f"fp64 redo fraction {nfp64 / (n * S):.4f}")
 
-    roofline = None
-    if os.path.exists(FLOPS_FIXTURE):
-        flops = json.load(open(FLOPS_FIXTURE))
-        per_step = float(flops["fp64_flops_per_track_step"])
-        achieved = per_step * n * S / kern64 / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "transport_kernel<FP64>", "flops_per_track_step": per_step}
+    log("[bench] mixed family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fammx.items()))
+    roofline = family_roofline(fam64, counts_by_family, S)
 
     cpu = None
     if not args.no_cpu:
@@ -238,6 +286,7 @@ def main():
         "mixed_over_fp64": ratemx / rate64,
         "fp64_redo_fraction": nfp64 / (n * S),
         "kernel_ms": {"fp64": kern64 * 1e3, "mixed": kernmx * 1e3},
+        "family_kernel_ms": {"fp64": fam64, "mixed": fammx},
         "launch": {"fp64": launch64, "mixed": launchmx},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -27,6 +27,7 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_DEVICE_PTRS 1u /* inputs/outputs are device memory on the context's device */
 #define VGM_ASYNC 2u       /* with VGM_DEVICE_PTRS: return after enqueueing on the context stream */
 #define VGM_MIXED 4u       /* FP32 bulk arithmetic, FP64 refinement and tolerance decisions */
+#define VGM_TIMING 8u      /* vgm_batch_transport: time each shape family's kernel (vgm_batch_family_ms) */
 
 /* Status codes. */
 #define VGM_OK 0
@@ -210,6 +211,11 @@ int vgm_batch_classify(vgm_batch *batch, const double *x, const double *y, const
 int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, double *dx, double *dy, double *dz,
                         int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
                         unsigned flags, unsigned long long *fp64_steps);
+/* Kernel time in ms of each shape family (index = shape type: box, tube, cone,
+ * trd, trapezoid, polycone, polyhedron, tessellated; 8 entries) in the last
+ * vgm_batch_transport called with VGM_TIMING, measured with events on the
+ * stream that family ran on; -1 for families not run.  Waits for them. */
+int vgm_batch_family_ms(vgm_batch *batch, float *ms8);
 int vgm_batch_set_launch(vgm_batch *batch, int block, int grid, int tile_tracks);
 int vgm_batch_get_launch(vgm_batch *batch, int *shape3);
 int vgm_batch_autotune(vgm_batch *batch, const double *x, const double *y, const double *z, const double *dx,

@@ -23,6 +23,8 @@ HOST_PTRS = 0
 DEVICE_PTRS = 1
 ASYNC = 2
 MIXED = 4
+TIMING = 8
+SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated")
 
 # Inside_t (VecGeom/base/Global.h:71-88)
 kInside, kSurface, kOutside = 1, 2, 3
@@ -116,6 +118,7 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_ulonglong)],
         "vgm_batch_set_launch": [_P, _I, _I, _I],
         "vgm_batch_get_launch": [_P, _P],
+        "vgm_batch_family_ms": [_P, _P],
         "vgm_batch_autotune": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _U, _P],
     }
     for name, args in bsig.items():
@@ -372,8 +375,8 @@ class Batch:
         check(self._L.vgm_batch_classify(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(side)))
 
     def transport(self, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0, seed=1, mixed=False,
-                  asynchronous=False, count_fp64=False):
-        flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0)
+                  asynchronous=False, count_fp64=False, timing=False):
+        flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0) | (TIMING if timing else 0)
         cnt = ctypes.c_ulonglong(0)
         check(self._L.vgm_batch_transport(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                           _ptr(side), _ptr(crossings), nsteps, step0, seed, flags,
@@ -382,6 +385,12 @@ class Batch:
 
     def set_launch(self, block, grid, tile_tracks):
         check(self._L.vgm_batch_set_launch(self.handle, block, grid, tile_tracks))
+
+    def family_ms(self):
+        """{family: kernel ms} of the last transport run with timing=True."""
+        out = np.zeros(8, dtype=np.float32)
+        check(self._L.vgm_batch_family_ms(self.handle, out.ctypes.data))
+        return {SHAPE_TYPES[t]: float(v) for t, v in enumerate(out) if v >= 0}
 
     def launch(self):
         out = np.zeros(3, dtype=np.int32)

@@ -30,6 +30,9 @@ struct vgm_batch {
   unsigned long long *d_fp64 = nullptr;
   hipStream_t fam_stream[kFamilyStreams] = {};
   hipEvent_t fork = nullptr, join[kFamilyStreams] = {};
+  // VGM_TIMING: events around each family's kernel, on that family's stream
+  hipEvent_t fam_t0[kNumShapeTypes] = {}, fam_t1[kNumShapeTypes] = {};
+  bool fam_timed[kNumShapeTypes] = {};
 };
 
 namespace {
@@ -68,6 +71,10 @@ int ensure_streams(vgm_batch *b)
   for (int i = 0; i < kFamilyStreams; ++i) {
     VGM_HIP(hipStreamCreateWithFlags(&b->fam_stream[i], hipStreamNonBlocking));
     VGM_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+  }
+  for (int t = 0; t < kNumShapeTypes; ++t) {
+    VGM_HIP(hipEventCreate(&b->fam_t0[t]));
+    VGM_HIP(hipEventCreate(&b->fam_t1[t]));
   }
   VGM_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   return VGM_OK;
@@ -136,9 +143,21 @@ LaunchShape resolved(vgm_batch *b)
 // then each family's kernel on one of the family streams (heaviest families
 // first, each to the least loaded stream), joined back into the context
 // stream.  A single family runs on the context stream directly.
-hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const LaunchShape &ls)
+hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const LaunchShape &ls, bool timing = false)
 {
   hipStream_t main = b->ctx->stream;
+  for (bool &t : b->fam_timed) t = false;
+  // one family's kernel on `st`, bracketed by timing events when asked
+  auto one = [&](hipStream_t st, const FamilyLaunch &f) -> hipError_t {
+    hipError_t r;
+    if (timing && (r = hipEventRecord(b->fam_t0[f.type], st)) != hipSuccess) return r;
+    if ((r = launch_family(st, mode, a, f, ls.block)) != hipSuccess) return r;
+    if (timing) {
+      if ((r = hipEventRecord(b->fam_t1[f.type], st)) != hipSuccess) return r;
+      b->fam_timed[f.type] = true;
+    }
+    return hipSuccess;
+  };
   hipError_t e = launch_reset(main, b->d_counters, kNumShapeTypes + 1, a.fp64_steps);
   if (e != hipSuccess) return e;
   double wsum = 0;
@@ -155,7 +174,7 @@ hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const
       f.counter = nullptr;
     }
   }
-  if (fl.size() == 1) return launch_family(main, mode, a, fl[0], ls.block);
+  if (fl.size() == 1) return one(main, fl[0]);
   std::sort(fl.begin(), fl.end(),
             [&](const FamilyLaunch &x, const FamilyLaunch &y) { return b->weight[x.type] > b->weight[y.type]; });
   double load[kFamilyStreams] = {};
@@ -170,7 +189,7 @@ hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const
       used[s] = true;
     }
     load[s] += b->weight[f.type];
-    if ((e = launch_family(b->fam_stream[s], mode, a, f, ls.block)) != hipSuccess) return e;
+    if ((e = one(b->fam_stream[s], f)) != hipSuccess) return e;
   }
   for (int s = 0; s < kFamilyStreams; ++s) {
     if (!used[s]) continue;
@@ -230,7 +249,23 @@ int vgm_batch_destroy(vgm_batch *b)
     if (b->join[i]) (void)hipEventDestroy(b->join[i]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
+  for (int t = 0; t < kNumShapeTypes; ++t) {
+    if (b->fam_t0[t]) (void)hipEventDestroy(b->fam_t0[t]);
+    if (b->fam_t1[t]) (void)hipEventDestroy(b->fam_t1[t]);
+  }
   delete b;
+  return VGM_OK;
+}
+
+int vgm_batch_family_ms(vgm_batch *b, float *ms)
+{
+  if (!b || !ms) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t) {
+    ms[t] = -1.0f;
+    if (!b->fam_timed[t]) continue;
+    VGM_HIP(hipEventSynchronize(b->fam_t1[t]));
+    VGM_HIP(hipEventElapsedTime(&ms[t], b->fam_t0[t], b->fam_t1[t]));
+  }
   return VGM_OK;
 }
 
@@ -280,7 +315,7 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
   TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed);
   const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
   if ((rc = ensure_streams(b))) return rc;
-  VGM_HIP(launch_families(b, mode, a, resolved(b)));
+  VGM_HIP(launch_families(b, mode, a, resolved(b), (flags & VGM_TIMING) != 0));
   if (fp64_steps) {
     VGM_HIP(hipMemcpyAsync(fp64_steps, b->d_fp64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->ctx->stream));

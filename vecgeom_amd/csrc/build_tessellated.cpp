@@ -11,6 +11,7 @@
 #include <vector>
 #include "geometry.h"
 #include "shapes/tessellated_data.h"
+#include "shapes/tessellated.hpp"
 
 namespace vgm {
 
@@ -77,8 +78,11 @@ struct Builder {
 
   static double centroid(const TessFacet &f, int c) { return f.v[0][c] + f.v[1][c] + f.v[2][c]; }
 
-  int build(int b, int e)
+  int max_depth = 0;
+
+  int build(int b, int e, int depth = 0)
   {
+    max_depth = std::max(max_depth, depth);
     const int id = int(nodes.size());
     nodes.push_back(TessNode{});
     double lo[3] = {1e308, 1e308, 1e308}, hi[3] = {-1e308, -1e308, -1e308};
@@ -111,8 +115,8 @@ struct Builder {
                        const double cx = centroid(x, axis), cy = centroid(y, axis);
                        return cx < cy || (cx == cy && x.orig < y.orig);
                      });
-    const int l = build(b, mid);
-    const int r = build(mid, e);
+    const int l = build(b, mid, depth + 1);
+    const int r = build(mid, e, depth + 1);
     nodes[id].left = l;
     nodes[id].right = r;
     nodes[id].first = nodes[id].count = 0;
@@ -155,6 +159,12 @@ int build_tessellated(HostGeometry &g, int nfacets, const double *tri, const dou
   B.nodes.reserve(2 * n);
   B.build(0, n);
   const int nn = int(B.nodes.size());
+  // the device walks keep 16-bit node indices, at most one deferred sibling
+  // per level, on a stack of kTessWalkDepth entries
+  if (nn > 65535 || B.max_depth > kTessWalkDepth - 2) {
+    err = "tessellated solid too large for the facet tree walk (" + std::to_string(n) + " facets)";
+    return -1;
+  }
   std::vector<double> block(TessData::bytes(n, nn) / sizeof(double), 0.0);
   TessData *h = reinterpret_cast<TessData *>(block.data());
   h->nfacets = n;

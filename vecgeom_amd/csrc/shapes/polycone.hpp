@@ -14,20 +14,15 @@ struct PolyconeShape {
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
 
-  VGM_DEV static int section_index(const Data &d, double zp)
+  // GetSectionIndex: linear search over the z-planes, compared in the
+  // arithmetic type of the query
+  template <typename R>
+  VGM_DEV static int section_index(const Data &d, R zp)
   {
     const double *zs = d.zs();
-    if (zp < zs[0]) return -1;
+    if (zp < R(zs[0])) return -1;
     for (int i = 0; i < d.nsec; ++i)
-      if (zp >= zs[i] && zp <= zs[i + 1]) return i;
-    return -2;
-  }
-  VGM_DEV static int section_index(const Data &d, float zp)
-  {
-    const double *zs = d.zs();
-    if (zp < float(zs[0])) return -1;
-    for (int i = 0; i < d.nsec; ++i)
-      if (zp >= float(zs[i]) && zp <= float(zs[i + 1])) return i;
+      if (zp >= R(zs[i]) && zp <= R(zs[i + 1])) return i;
     return -2;
   }
 

@@ -15,12 +15,35 @@
 
 namespace vgm {
 
+// Tree-walk stack.  On the device it lives in LDS, one column of 16-bit node
+// indices per thread of the workgroup (a private array would be spilled to
+// scratch memory, whose traffic dominated the walks); on the host it is an
+// ordinary local array.  The tree builder keeps node indices below 65536 and
+// the depth below kTessWalkDepth - 2, so a nearest-first walk (one deferred
+// sibling per level) never overflows.
+constexpr int kTessWalkDepth = 32;
+constexpr int kTessWalkThreads = 256;  // largest workgroup of any kernel that walks
+#if defined(__HIP_DEVICE_COMPILE__)
+static __shared__ uint16_t tess_walk_lds[kTessWalkDepth * kTessWalkThreads];
+struct TessWalkStack {
+  uint16_t *col;
+  __device__ TessWalkStack() : col(tess_walk_lds + threadIdx.x) {}
+  __device__ void set(int i, int v) { col[i * kTessWalkThreads] = uint16_t(v); }
+  __device__ int get(int i) const { return col[i * kTessWalkThreads]; }
+};
+#else
+struct TessWalkStack {
+  int a[kTessWalkDepth];
+  void set(int i, int v) { a[i] = v; }
+  int get(int i) const { return a[i]; }
+};
+#endif
+
 struct TessellatedShape {
   using Data = TessData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = true;
-  static constexpr int kStack = 64;      // tree depth is below 32 (median splits)
 
   template <typename R>
   VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)
@@ -120,11 +143,11 @@ struct TessellatedShape {
     const V3<R> inv{R(1) / nonzero(d.x), R(1) / nonzero(d.y), R(1) / nonzero(d.z)};
     // nearest-first walk: the nearer child is pushed last; every node is
     // re-tested against the (shrinking) bound when it is popped
-    int stack[kStack];
+    TessWalkStack stack;
     int sp = 0;
-    stack[sp++] = 0;
+    stack.set(sp++, 0);
     while (sp) {
-      const TessNode &nd = nodes[stack[--sp]];
+      const TessNode &nd = nodes[stack.get(--sp)];
       const R bound = to_in ? (h.iIn >= 0 ? h.dIn : inf_length<R>()) : (h.iOut >= 0 ? h.dOut : inf_length<R>());
       const R te = box_entry<R>(nd, p, inv);
       if (te == inf_length<R>() || te > bound) continue;
@@ -133,12 +156,12 @@ struct TessellatedShape {
         const bool l_ok = !(tl == inf_length<R>() || tl > bound), r_ok = !(tr == inf_length<R>() || tr > bound);
         if (l_ok && r_ok) {
           const bool left_first = tl <= tr;
-          stack[sp++] = left_first ? nd.right : nd.left;
-          stack[sp++] = left_first ? nd.left : nd.right;
+          stack.set(sp++, left_first ? nd.right : nd.left);
+          stack.set(sp++, left_first ? nd.left : nd.right);
         } else if (l_ok) {
-          stack[sp++] = nd.left;
+          stack.set(sp++, nd.left);
         } else if (r_ok) {
-          stack[sp++] = nd.right;
+          stack.set(sp++, nd.right);
         }
         continue;
       }
@@ -172,23 +195,23 @@ struct TessellatedShape {
     const TessFacet *fa = t.facets();
     const TessNode *nodes = t.nodes();
     R best = inf_length<R>();
-    int stack[kStack];
+    TessWalkStack stack;
     int sp = 0;
-    stack[sp++] = 0;
+    stack.set(sp++, 0);
     while (sp) {
-      const TessNode &nd = nodes[stack[--sp]];
+      const TessNode &nd = nodes[stack.get(--sp)];
       if (box_dist2<R>(nd, p) > best) continue;
       if (nd.left >= 0) {  // nearer child last, so it is visited first
         const R dl = box_dist2<R>(nodes[nd.left], p), dr = box_dist2<R>(nodes[nd.right], p);
         const bool l_ok = !(dl > best), r_ok = !(dr > best);
         if (l_ok && r_ok) {
           const bool left_first = dl <= dr;
-          stack[sp++] = left_first ? nd.right : nd.left;
-          stack[sp++] = left_first ? nd.left : nd.right;
+          stack.set(sp++, left_first ? nd.right : nd.left);
+          stack.set(sp++, left_first ? nd.left : nd.right);
         } else if (l_ok) {
-          stack[sp++] = nd.left;
+          stack.set(sp++, nd.left);
         } else if (r_ok) {
-          stack[sp++] = nd.right;
+          stack.set(sp++, nd.right);
         }
         continue;
       }
@@ -318,6 +341,15 @@ struct TessellatedShape {
     Hits<R> h;
     int isurf;
     return distance_to_solid<R, false>(t, p, d, step_max, h, isurf);
+  }
+
+  // Distance from p to the surface, without the side check of
+  // SafetyToIn/Out: SafetyToOut(p) = inside ? surface_distance(p, true) : -1,
+  // SafetyToIn(p) = inside ? -1 : surface_distance(p, false).
+  template <typename R>
+  VGM_DEV static R surface_distance(const Data &t, V3<R> p, bool from_inside)
+  {
+    return from_inside ? vsqrt(safety2<R, false>(t, p)) : vsqrt(safety2<R, true>(t, p));
   }
 
   template <typename R>

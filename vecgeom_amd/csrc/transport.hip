@@ -64,7 +64,16 @@ VGM_DEV bool step_fp64(const typename S::Data &s, TrackState &t, double l)
       return false;
     }
   }
-  const double saf = t.in ? S::template safety_to_out<double>(s, t.p) : S::template safety_to_in<double>(s, t.p);
+  double saf;
+  if constexpr (S::kSafetyIsDistance) {
+    // SafetyToOut/In is the surface distance when the point is on the
+    // track's side and -1 otherwise; the side check (a ray walk) only matters
+    // when the distance would let the step move freely.
+    saf = S::template surface_distance<double>(s, t.p, t.in);
+    if (l <= saf && S::template contains<double>(s, t.p) != t.in) saf = -1.0;
+  } else {
+    saf = t.in ? S::template safety_to_out<double>(s, t.p) : S::template safety_to_in<double>(s, t.p);
+  }
   if (l <= saf) {
     if constexpr (S::kSafetyIsDistance) t.carry = saf - l - carry_margin(t.p);
     t.p.x = t.p.x + l * t.d.x;
@@ -108,18 +117,29 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
   const V3<float> df{float(t.d.x), float(t.d.y), float(t.d.z)};
   const float lf = float(l);
   const float g = guard_band(pf, scale);
-  const float saf = t.in ? S::template safety_to_out<float>(s, pf) : S::template safety_to_in<float>(s, pf);
+  float saf;
+  if constexpr (S::kSafetyIsDistance) {
+    saf = S::template surface_distance<float>(s, pf, t.in);
+    if (lf + g <= saf && S::template contains<float>(s, pf) != t.in) saf = -1.0f;
+  } else {
+    saf = t.in ? S::template safety_to_out<float>(s, pf) : S::template safety_to_in<float>(s, pf);
+  }
   bool move_free = lf + g <= saf;
+  const bool by_safety = move_free;
   if (!move_free) {
     const float db = t.in ? S::template distance_to_out<float>(s, pf, df, FLT_MAX)
                           : S::template distance_to_in<float>(s, pf, df, FLT_MAX);
     move_free = (db < FLT_MAX) ? (lf + g + 1.0e-3f * db < db) : (!t.in);
   }
   if (move_free) {
+    // a guarded float safety bounds the safety as the FP64 one does, so a
+    // surface-distance family carries it like an FP64 safety
+    t.carry = 0.0;
+    if constexpr (S::kSafetyIsDistance)
+      if (by_safety) t.carry = (double(saf) - double(g)) - l - carry_margin(t.p);
     t.p.x = t.p.x + l * t.d.x;
     t.p.y = t.p.y + l * t.d.y;
     t.p.z = t.p.z + l * t.d.z;
-    t.carry = 0.0;
     return false;
   }
   fp64_steps++;

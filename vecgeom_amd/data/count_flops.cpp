@@ -1,0 +1,272 @@
+// Algorithmic FLOP count of the FP64 track-step, per solid.
+//
+// The shape kernels are templates on their scalar type and compile for the
+// host as well as for gfx950.  Instantiated here with `Counted`, a double
+// that counts every +, -, *, / and sqrt it performs (comparisons, abs,
+// min/max and copysign are not counted), they run the exact FP64 track-step
+// of the batched transport (safety, physics step, boundary distance, carried
+// safety bound) on a sample of tracks; the count divided by the track-steps
+// is the algorithmic work per track-step of that solid.  Random-number and
+// direction sampling are bookkeeping and are not counted.  The same sample is
+// also stepped with plain double and must end on bit-identical positions, so
+// the counted run is the FP64 path itself.
+//
+// Built as a host-only shared library (Makefile in this folder) and driven
+// from make_flops.py, which writes track_step_flops.json for the benchmark's
+// roofline.
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../csrc/geometry.h"
+#include "../csrc/vgm_math.hpp"
+
+namespace vgm {
+
+inline uint64_t g_flops = 0;
+
+struct Counted {
+  double v;
+  Counted() = default;
+  constexpr Counted(double x) : v(x) {}
+  explicit operator double() const { return v; }
+  explicit operator float() const { return float(v); }
+  explicit operator int() const { return int(v); }
+  Counted &operator+=(Counted o) { ++g_flops; v = v + o.v; return *this; }
+  Counted &operator-=(Counted o) { ++g_flops; v = v - o.v; return *this; }
+  Counted &operator*=(Counted o) { ++g_flops; v = v * o.v; return *this; }
+  Counted &operator/=(Counted o) { ++g_flops; v = v / o.v; return *this; }
+};
+inline Counted operator+(Counted a, Counted b) { ++g_flops; return Counted(a.v + b.v); }
+inline Counted operator-(Counted a, Counted b) { ++g_flops; return Counted(a.v - b.v); }
+inline Counted operator*(Counted a, Counted b) { ++g_flops; return Counted(a.v * b.v); }
+inline Counted operator/(Counted a, Counted b) { ++g_flops; return Counted(a.v / b.v); }
+inline Counted operator-(Counted a) { return Counted(-a.v); }
+inline bool operator<(Counted a, Counted b) { return a.v < b.v; }
+inline bool operator>(Counted a, Counted b) { return a.v > b.v; }
+inline bool operator<=(Counted a, Counted b) { return a.v <= b.v; }
+inline bool operator>=(Counted a, Counted b) { return a.v >= b.v; }
+inline bool operator==(Counted a, Counted b) { return a.v == b.v; }
+inline bool operator!=(Counted a, Counted b) { return a.v != b.v; }
+inline Counted vabs(Counted a) { return Counted(__builtin_fabs(a.v)); }
+inline Counted vsqrt(Counted a) { ++g_flops; return Counted(__builtin_sqrt(a.v)); }
+inline Counted vsign(Counted a) { return Counted(__builtin_copysign(1.0, a.v)); }
+inline Counted vcopysign(Counted a, Counted b) { return Counted(__builtin_copysign(a.v, b.v)); }
+inline Counted nonzero(Counted x) { ++g_flops; return Counted(x.v + __builtin_copysign(1.0e-30, x.v)); }
+inline Counted nonzero_abs(Counted x) { ++g_flops; return Counted(__builtin_fabs(x.v) + 1.0e-30); }
+template <>
+VGM_DEV Counted inf_length<Counted>() { return Counted(DBL_MAX); }
+
+}  // namespace vgm
+
+#include "../csrc/shapes/all_shapes.hpp"
+#include "../csrc/stepping.hpp"
+
+namespace vgm {
+namespace {
+
+template <typename R>
+struct Track {
+  V3<R> p, d;
+  bool in;
+  int32_t crossings;
+  R carry;
+};
+
+template <typename R>
+R carry_margin(const V3<R> &p)
+{
+  return R(1e-7) + R(1e-12) * (vabs(p.x) + vabs(p.y) + vabs(p.z));
+}
+
+// The FP64 track-step of the batched transport kernel, on scalar type R.
+template <class S, typename R>
+bool step(const typename S::Data &s, Track<R> &t, R l)
+{
+  if constexpr (S::kSafetyIsDistance) {
+    if (l <= t.carry) {
+      const R m = carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      t.carry = t.carry - l - m;
+      return false;
+    }
+  }
+  R saf;
+  if constexpr (S::kSafetyIsDistance) {
+    saf = S::template surface_distance<R>(s, t.p, t.in);
+    if (l <= saf && S::template contains<R>(s, t.p) != t.in) saf = R(-1.0);
+  } else {
+    saf = t.in ? S::template safety_to_out<R>(s, t.p) : S::template safety_to_in<R>(s, t.p);
+  }
+  if (l <= saf) {
+    if constexpr (S::kSafetyIsDistance) t.carry = saf - l - carry_margin(t.p);
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  t.carry = R(0.0);
+  R db = t.in ? S::template distance_to_out<R>(s, t.p, t.d, inf_length<R>())
+              : S::template distance_to_in<R>(s, t.p, t.d, inf_length<R>());
+  if (db < R(0.0)) db = R(0.0);
+  if (l < db) {
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  const R m = db + R(kBoundaryPush);
+  t.p.x = t.p.x + m * t.d.x;
+  t.p.y = t.p.y + m * t.d.y;
+  t.p.z = t.p.z + m * t.d.z;
+  t.in = !t.in;
+  t.crossings++;
+  return true;
+}
+
+template <typename R>
+V3<double> dbl(const V3<R> &a)
+{
+  return V3<double>{double(a.x), double(a.y), double(a.z)};
+}
+
+// Steps track i of the sample through nsteps track-steps on scalar type R;
+// returns the final position.
+template <class S, typename R>
+V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lambda, uint64_t seed, uint64_t i,
+                     const double *p0, const double *d0, int nsteps, bool &in_out)
+{
+  double wc[3], wh[3];
+  for (int k = 0; k < 3; ++k) {
+    wc[k] = 0.5 * (rec.bmin[k] + rec.bmax[k]);
+    wh[k] = 1.3 * 0.5 * (rec.bmax[k] - rec.bmin[k]);
+  }
+  Track<R> t;
+  t.p = V3<R>{R(p0[0]), R(p0[1]), R(p0[2])};
+  t.d = V3<R>{R(d0[0]), R(d0[1]), R(d0[2])};
+  t.in = S::template contains<double>(s, V3<double>{p0[0], p0[1], p0[2]});
+  t.crossings = 0;
+  t.carry = R(0.0);
+  for (int k = 0; k < nsteps; ++k) {
+    const uint32_t st = uint32_t(k);
+    const double l = physics_step(lambda, seed, i, st);
+    const bool crossed = step<S, R>(s, t, R(l));
+    if (!crossed) {
+      const V3<double> d = new_direction(seed, i, st, dbl(t.d));
+      t.d = V3<R>{R(d.x), R(d.y), R(d.z)};
+    }
+    if (!t.in) {
+      const V3<double> q = dbl(t.p);
+      const V3<double> w{wrap(q.x, wc[0], wh[0]), wrap(q.y, wc[1], wh[1]), wrap(q.z, wc[2], wh[2])};
+      if (w.x != q.x || w.y != q.y || w.z != q.z) {
+        t.p = V3<R>{R(w.x), R(w.y), R(w.z)};
+        t.carry = R(0.0);
+      }
+    }
+  }
+  in_out = t.in;
+  return dbl(t.p);
+}
+
+struct Counter {
+  const HostGeometry &g;
+  const ShapeRec &rec;
+  double lambda;
+  uint64_t seed;
+  uint64_t first;
+  size_t n;
+  const double *const *soa;  // x, y, z, dx, dy, dz
+  int nsteps;
+  uint64_t flops = 0;
+  size_t mismatches = 0;
+
+  template <class S>
+  void operator()()
+  {
+    const auto &s = *reinterpret_cast<const typename S::Data *>(g.pool.data() + rec.off);
+    for (size_t i = 0; i < n; ++i) {
+      const double p0[3] = {soa[0][i], soa[1][i], soa[2][i]};
+      const double d0[3] = {soa[3][i], soa[4][i], soa[5][i]};
+      bool in_d = false, in_c = false;
+      const V3<double> a = run_track<S, double>(s, rec, lambda, seed, first + i, p0, d0, nsteps, in_d);
+      const uint64_t f0 = g_flops;
+      const V3<double> b = run_track<S, Counted>(s, rec, lambda, seed, first + i, p0, d0, nsteps, in_c);
+      flops += g_flops - f0;
+      if (std::memcmp(&a, &b, sizeof a) != 0 || in_d != in_c) ++mismatches;
+    }
+  }
+};
+
+}  // namespace
+}  // namespace vgm
+
+using namespace vgm;
+
+struct cf_geom {
+  HostGeometry g;
+  std::string err;
+};
+
+extern "C" {
+
+cf_geom *cf_create() { return new cf_geom(); }
+void cf_destroy(cf_geom *h) { delete h; }
+const char *cf_error(cf_geom *h) { return h->err.c_str(); }
+
+int cf_box(cf_geom *h, double dx, double dy, double dz) { return build_box(h->g, dx, dy, dz, h->err); }
+int cf_tube(cf_geom *h, double rmin, double rmax, double dz, double sphi, double dphi)
+{
+  return build_tube(h->g, rmin, rmax, dz, sphi, dphi, h->err);
+}
+int cf_cone(cf_geom *h, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi)
+{
+  return build_cone(h->g, rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi, h->err);
+}
+int cf_trd(cf_geom *h, double x1, double x2, double y1, double y2, double z)
+{
+  return build_trd(h->g, x1, x2, y1, y2, z, h->err);
+}
+int cf_trapezoid(cf_geom *h, const double *p11) { return build_trapezoid(h->g, p11, h->err); }
+int cf_polycone(cf_geom *h, double sphi, double dphi, int nz, const double *z, const double *rmin, const double *rmax)
+{
+  return build_polycone(h->g, sphi, dphi, nz, z, rmin, rmax, h->err);
+}
+int cf_polyhedron(cf_geom *h, double sphi, double dphi, int nsides, int nz, const double *z, const double *rmin,
+                  const double *rmax)
+{
+  return build_polyhedron(h->g, sphi, dphi, nsides, nz, z, rmin, rmax, h->err);
+}
+int cf_tessellated(cf_geom *h, int nfacets, const double *triangles)
+{
+  return build_tessellated(h->g, nfacets, triangles, nullptr, h->err);
+}
+int cf_extent(cf_geom *h, int shape, double *mn, double *mx)
+{
+  if (shape < 0 || size_t(shape) >= h->g.shapes.size()) return -1;
+  const ShapeRec &r = h->g.shapes[size_t(shape)];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = r.bmin[k];
+    mx[k] = r.bmax[k];
+  }
+  return 0;
+}
+
+// FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape`; sample
+// track i draws its random numbers as track first + i of the transport.
+// Returns 0, or -1 for a bad shape, or -2 when the counted run left the FP64
+// trajectory of some track.
+int cf_count(cf_geom *h, int shape, double lambda, uint64_t seed, uint64_t first, size_t n, const double *x, const double *y,
+             const double *z, const double *dx, const double *dy, const double *dz, int nsteps, uint64_t *flops)
+{
+  if (shape < 0 || size_t(shape) >= h->g.shapes.size()) return -1;
+  const double *soa[6] = {x, y, z, dx, dy, dz};
+  const ShapeRec &rec = h->g.shapes[size_t(shape)];
+  Counter c{h->g, rec, lambda, seed, first, n, soa, nsteps};
+  if (!dispatch_shape_type(rec.type, c)) return -1;
+  *flops = c.flops;
+  return c.mismatches ? -2 : 0;
+}
+
+}  // extern "C"
