@@ -154,6 +154,10 @@ def main():
     ap.add_argument("--nav-tracks", type=int, default=4_000_000, help="navigation benchmark tracks (0: skip)")
     ap.add_argument("--nav-steps", type=int, default=200)
     ap.add_argument("--launch", default=None, help="block,grid,tile launch shape instead of autotuning")
+    ap.add_argument("--train-samples", type=int, default=50_000_000,
+                    help="exact safety samples for the learned-safety networks, over all tessellated solids")
+    ap.add_argument("--train-epochs", type=int, default=4)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy kernel of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -188,22 +192,24 @@ def main():
     batch.classify(init[0], init[1], init[2], side0)
     S = args.track_steps
 
-    def run(mixed):
+    def run(mixed, learned=False, tuned=False):
         state = [t.clone() for t in init]
         side = side0.clone()
         cr = torch.zeros(n, dtype=torch.int32, device=dev)
+        tag = "tuned" if tuned else (("mixed" if mixed else "fp64") + ("+learned" if learned else ""))
+        kw = {"mixed": mixed, "learned": learned} if not tuned else {"tuned": True}
         torch.cuda.synchronize()
         if args.launch:
             batch.set_launch(*[int(v) for v in args.launch.split(",")])
         elif not args.no_autotune:
-            log(f"[bench] autotune {'mixed' if mixed else 'fp64'} ...")
+            log(f"[bench] autotune {tag} ...")
             t = time.perf_counter()
-            shape = batch.autotune(*state, side, nsteps=2, mixed=mixed)
-            log(f"[bench] autotune {'mixed' if mixed else 'fp64'}: block/grid/tile = {shape} "
-                f"({time.perf_counter() - t:.1f}s)")
+            shape = batch.autotune(*state, side, nsteps=2, mixed=mixed or tuned, learned=learned or tuned,
+                                   distance_first=tuned)
+            log(f"[bench] autotune {tag}: block/grid/tile = {shape} ({time.perf_counter() - t:.1f}s)")
         step0 = 0
         for _ in range(args.warmup):
-            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True)
+            batch.transport(*state, side, cr, S, step0=step0, seed=7, asynchronous=True, **kw)
             step0 += S
         torch.cuda.synchronize()
         evs, fms = [], []
@@ -211,8 +217,7 @@ def main():
         for _ in range(args.steps):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            batch.transport(*state, side, cr, S, step0=step0, seed=7, mixed=mixed, asynchronous=True,
-                            timing=True)
+            batch.transport(*state, side, cr, S, step0=step0, seed=7, asynchronous=True, timing=True, **kw)
             b.record(stream)
             evs.append((a, b))
             fms.append(batch.family_ms())  # waits for this launch's family kernels
@@ -221,29 +226,91 @@ def main():
         wall = time.perf_counter() - t
         kern = float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
         fam_ms = {f: float(np.mean([m[f] for m in fms])) for f in fms[0]}
-        nfp64 = 0
-        if mixed:
+        extra = {}
+        if mixed and not learned:
             st2 = [x.clone() for x in init]
             s2, c2 = side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev)
             torch.cuda.synchronize()
-            nfp64 = batch.transport(*st2, s2, c2, S,
-                                    step0=0, seed=7, mixed=True, count_fp64=True)
-        return wall, kern, nfp64, batch.launch(), fam_ms
+            extra["fp64_steps"] = batch.transport(*st2, s2, c2, S, step0=0, seed=7, mixed=True, count_fp64=True)
+        if learned or tuned:
+            # audit: the first launch again, every served bound checked against the exact safety
+            st2 = [x.clone() for x in init]
+            s2, c2 = side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev)
+            torch.cuda.synchronize()
+            batch.transport(*st2, s2, c2, S, step0=0, seed=7, audit=True, **kw)
+            extra["audit"] = batch.learned_stats()
+        if tuned or not args.launch:
+            extra["modes"] = batch.modes()
+        final = state + [side, cr]
+        log(f"[bench] {tag}: {args.steps * n * S / wall / 1e9:.2f} G track-steps/s (kernel {kern * 1e3:.2f} ms/launch)")
+        log(f"[bench] {tag} family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam_ms.items()))
+        return wall, kern, extra, batch.launch(), fam_ms, final
 
     counts_by_family = {}
     for (kind, _), c in zip(specs, counts):
         counts_by_family[kind] = counts_by_family.get(kind, 0) + int(c)
-    wall64, kern64, _, launch64, fam64 = run(False)
+    wall64, kern64, _, launch64, fam64, final64 = run(False)
     rate64 = args.steps * n * S / wall64
-    log(f"[bench] fp64: {rate64 / 1e9:.2f} G track-steps/s (kernel {kern64 * 1e3:.2f} ms/launch)")
-    log("[bench] fp64 family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam64.items()))
-    wallmx, kernmx, nfp64, launchmx, fammx = run(True)
+    wallmx, kernmx, extramx, launchmx, fammx, _ = run(True)
     ratemx = args.steps * n * S / wallmx
-    log(f"[bench] mixed: {ratemx / 1e9:.2f} G track-steps/s (kernel {kernmx * 1e3:.2f} ms/launch), "
-        f"fp64 redo fraction {nfp64 / (n * S):.4f}")
-
-    log("[bench] mixed family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fammx.items()))
+    nfp64 = extramx["fp64_steps"]
+    log(f"[bench] mixed fp64 redo fraction {nfp64 / (n * S):.4f}")
     roofline = family_roofline(fam64, counts_by_family, S)
+
+    learned = tuned = None
+    value, wall_v, kern_v = ratemx, wallmx, kernmx
+    tess = [i for i, (kind, _) in enumerate(specs) if kind == "tessellated"]
+    if tess and not args.no_learned:
+        per = max(8192, args.train_samples // len(tess) // 8192 * 8192)
+        reports = []
+        t = time.perf_counter()
+        for i in tess:
+            reports.append(ctx.mlp_train(int(shapes[i]), samples=per, epochs=args.train_epochs))
+        train_wall = time.perf_counter() - t
+        log(f"[bench] trained {len(tess)} networks on {per * len(tess)} exact samples in {train_wall:.1f}s "
+            f"(GPU {sum(r['train_ms'] for r in reports) / 1e3:.1f}s); margins/scale "
+            + ", ".join(f"{r['margin'] / r['scale']:.4f}" for r in reports))
+        walll, kernl, extral, launchl, faml, finall = run(True, learned=True)
+        ratel = args.steps * n * S / walll
+        au = extral["audit"]
+        wallt, kernt, extrat, launcht, famt, finalt = run(False, tuned=True)
+        ratet = args.steps * n * S / wallt
+        # every family except mixed ones must end exactly where the FP64 path ends
+        modes = extrat.get("modes", {})
+        exact_mask = torch.zeros(n, dtype=torch.bool, device=dev)
+        tmask = torch.zeros(n, dtype=torch.bool, device=dev)
+        for i, (kind, _) in enumerate(specs):
+            sl = slice(int(offsets[i]), int(offsets[i + 1]))
+            if modes.get(kind) != "mixed":
+                exact_mask[sl] = True
+            if kind == "tessellated":
+                tmask[sl] = True
+
+        def ndiff(fin, mask):
+            d = torch.zeros(n, dtype=torch.bool, device=dev)
+            for a, b in zip(final64, fin):
+                d |= (a != b)
+            return int((d & mask).sum().item())
+
+        ndl = ndiff(finall, tmask)
+        ndt = ndiff(finalt, exact_mask)
+        learned = {
+            "networks": len(tess), "train_samples": per * len(tess), "train_epochs": args.train_epochs,
+            "train_wall_s": train_wall, "train_gpu_ms": sum(r["train_ms"] for r in reports),
+            "train_gflop": sum(r["gflop"] for r in reports),
+            "margin_over_scale": [r["margin"] / r["scale"] for r in reports],
+            "rms_err_over_scale": [r["rms_err"] / r["scale"] for r in reports],
+            "queries": au["queries"], "served": au["served"], "fallbacks": au["fallbacks"],
+            "served_fraction": au["served"] / max(1, au["queries"]), "violations": au["violations"],
+            "audit_sample": f"first launch ({S} track-steps of all {n} tracks), every served bound checked",
+            "tessellated_tracks_differing_from_fp64": ndl,
+            "value": ratel, "kernel_ms": kernl * 1e3, "family_kernel_ms": faml, "launch": launchl,
+        }
+        tuned = {"value": ratet, "modes": modes, "kernel_ms": kernt * 1e3, "family_kernel_ms": famt,
+                 "launch": launcht, "exact_mode_tracks_differing_from_fp64": ndt, "audit": extrat.get("audit")}
+        log(f"[bench] tuned modes {modes}; exact-mode tracks differing from fp64: {ndt}")
+        value, wall_v, kern_v = ratet, wallt, kernt
+        log(f"[bench] learned: audit {au}, tessellated tracks differing from fp64: {ndl}")
 
     cpu = None
     if not args.no_cpu:
@@ -268,12 +335,12 @@ def main():
 
     line = {
         "metric": "track-steps/sec on 1 MI355X (mixed+learned vs plain FP64), 0 safety violations",
-        "value": ratemx,
+        "value": value,
         "unit": "track-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wallmx / args.steps * 1e3,
+        "ms_per_step": wall_v / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -283,13 +350,16 @@ def main():
                                f"{n} tracks, {S} track-steps per launch",
                    "tracks": n, "track_steps_per_launch": S, "families": list(fams)},
         "fp64_value": rate64,
-        "mixed_over_fp64": ratemx / rate64,
+        "mixed_value": ratemx,
+        "value_over_fp64": value / rate64,
         "fp64_redo_fraction": nfp64 / (n * S),
-        "kernel_ms": {"fp64": kern64 * 1e3, "mixed": kernmx * 1e3},
+        "kernel_ms": {"fp64": kern64 * 1e3, "mixed": kernmx * 1e3, "value": kern_v * 1e3},
         "family_kernel_ms": {"fp64": fam64, "mixed": fammx},
         "launch": {"fp64": launch64, "mixed": launchmx},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned": learned,
+        "tuned": tuned,
         "navigation": navigation,
     }
     print(json.dumps(line))

@@ -28,6 +28,10 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_ASYNC 2u       /* with VGM_DEVICE_PTRS: return after enqueueing on the context stream */
 #define VGM_MIXED 4u       /* FP32 bulk arithmetic, FP64 refinement and tolerance decisions */
 #define VGM_TIMING 8u      /* vgm_batch_transport: time each shape family's kernel (vgm_batch_family_ms) */
+#define VGM_LEARNED 16u    /* vgm_batch_transport: learned safety bound before the exact safety */
+#define VGM_AUDIT 32u      /* with VGM_LEARNED: check every served bound against the exact safety */
+#define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
+#define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 
 /* Status codes. */
 #define VGM_OK 0
@@ -216,11 +220,81 @@ int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, doubl
  * vgm_batch_transport called with VGM_TIMING, measured with events on the
  * stream that family ran on; -1 for families not run.  Waits for them. */
 int vgm_batch_family_ms(vgm_batch *batch, float *ms8);
+/* Counters of the last VGM_LEARNED transport: [0] bound evaluations, [1]
+ * steps moved on a learned bound, [2] exact fallback steps, [3] (VGM_AUDIT)
+ * bounds above the exact safety. */
+int vgm_batch_learned_stats(vgm_batch *batch, unsigned long long *out4);
+/* Mode per shape family chosen by the last vgm_batch_autotune (index = shape
+ * type; 0 FP64, 1 mixed, 2 distance-first, 3 learned; -1 not tuned).  The
+ * autotuner tries FP64 and each mode its flags allow (VGM_MIXED,
+ * VGM_DISTANCE_FIRST, VGM_LEARNED), each family alone, and keeps the fastest;
+ * VGM_TUNED transports use the choice. */
+int vgm_batch_get_modes(vgm_batch *batch, int *modes8);
+/* A wave of the learned kernel runs its exact fallbacks once this percentage
+ * of its live tracks wait for one (default 50). */
+int vgm_batch_set_fallback(vgm_batch *batch, int pct);
 int vgm_batch_set_launch(vgm_batch *batch, int block, int grid, int tile_tracks);
 int vgm_batch_get_launch(vgm_batch *batch, int *shape3);
 int vgm_batch_autotune(vgm_batch *batch, const double *x, const double *y, const double *z, const double *dx,
                        const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,
                        int *best3);
+
+
+/* ---- Learned safety --------------------------------------------------------
+ * A per-shape network (51 positional-encoding features of the point, four
+ * 128-unit SiLU layers, one output; BF16 matrix products, FP32 accumulation)
+ * trained on the GPU on exact safeties the shape kernels generate: SafetyToOut
+ * for points inside, SafetyToIn outside, points uniform in `domain` x the
+ * bounding box.  After training, the largest overestimate on `validation`
+ * fresh points sets the margin; the served bound is max(0, prediction -
+ * margin).  It has no counterpart in the reference: it is a fast lower bound
+ * on VSafetyEstimator::ComputeSafetyForLocalPoint
+ * (VecGeom/navigation/VSafetyEstimator.h:45-80) that the batched transport
+ * uses in place of the exact safety when VGM_LEARNED is set, with the exact
+ * kernels as fallback. */
+typedef struct vgm_mlp_config {
+  int64_t samples;      /* training samples (default 6250000) */
+  int64_t validation;   /* margin validation samples (default 1000000) */
+  int batch;            /* samples per step, a multiple of 256 (default 8192) */
+  int epochs;           /* passes over the samples (default 4) */
+  float lr;             /* Adam learning rate, cosine decay to 5% (default 2e-3) */
+  float over_weight;    /* loss weight of overestimates (default 4) */
+  double margin_factor; /* margin = factor * largest overestimate + ... (default 1.25) */
+  double margin_abs;    /* ... + margin_abs * bounding-box diagonal (default 1e-4) */
+  double domain;        /* sampling box / bounding box, per axis (default 1.3) */
+  uint64_t seed;        /* default 42 */
+} vgm_mlp_config;
+
+typedef struct vgm_mlp_report {
+  double margin;       /* mm */
+  double max_over;     /* largest prediction - exact on the validation points, mm */
+  double mean_abs_err; /* mm */
+  double rms_err;      /* mm */
+  double final_loss;   /* mean loss of the last steps (targets in units of scale) */
+  double train_ms;     /* sample generation + training, GPU time */
+  double gflop;        /* algorithmic training work */
+  double scale;        /* bounding-box diagonal, mm */
+  int64_t steps, samples;
+} vgm_mlp_report;
+
+void vgm_mlp_config_default(vgm_mlp_config *cfg);
+int vgm_mlp_train(vgm_ctx *ctx, int shape, const vgm_mlp_config *cfg, vgm_mlp_report *report);
+/* Prediction (mm) and served bound max(0, prediction - margin) at points in
+ * the shape's frame. */
+int vgm_mlp_predict(vgm_ctx *ctx, int shape, const double *x, const double *y, const double *z, size_t n,
+                    double *out, unsigned flags);
+int vgm_mlp_bound(vgm_ctx *ctx, int shape, const double *x, const double *y, const double *z, size_t n,
+                  double *out, unsigned flags);
+/* Save / load: 56321 FP32 parameters in the order W1[128][51], b1[128],
+ * W2[128][128], b2, W3, b3, W4, b4, w5[128], b5, and 8 doubles: domain
+ * centre (3), domain half-widths (3), scale, margin. */
+int vgm_mlp_get(vgm_ctx *ctx, int shape, float *params, double *geom8);
+int vgm_mlp_set(vgm_ctx *ctx, int shape, const float *params, const double *geom8);
+/* Gradient of the training loss for one batch (host arrays): n points given
+ * by normalised coordinates u (n x 3), targets = safety / scale, n a multiple
+ * of 64.  Runs the training kernels without an update. */
+int vgm_mlp_gradient(vgm_ctx *ctx, const float *params, const float *u, const float *target, size_t n,
+                     float over_weight, float *grad, float *loss);
 
 #ifdef __cplusplus
 }

@@ -81,3 +81,34 @@ def test_transport_mixed_follows_fp64(ctx):
     assert frac < 1e-3
     assert 0 < nfp64 < 30 * n
     b.close()
+
+
+def test_transport_distance_first_bit_exact(ctx):
+    """Boundary distance before the safety: the same decisions as the FP64
+    step (SafetyToIn/Out never exceed DistanceToIn/Out beyond the window), so
+    trajectories match the oracle bit for bit for every family."""
+    print("[transport] distance-first", flush=True)
+    b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
+    b.transport(*dev, side, cr, 30, step0=0, seed=5, distance_first=True)
+    for k in range(6):
+        got = dev[k].cpu().numpy()
+        bad = np.flatnonzero(got != ref[k])
+        assert bad.size == 0, (k, bad.size, bad[:5])
+    assert np.array_equal(side.cpu().numpy(), side_o)
+    assert np.array_equal(cr.cpu().numpy(), cr_o)
+    b.close()
+
+
+def test_transport_tuned_modes_exact(ctx):
+    """Autotuned per-family modes without mixed precision (FP64,
+    distance-first) reproduce the oracle bit for bit."""
+    print("[transport] tuned", flush=True)
+    b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
+    b.autotune(*dev, side, nsteps=2, distance_first=True)
+    modes = b.modes()
+    assert set(modes.values()) <= {"fp64", "distance-first"}, modes
+    b.transport(*dev, side, cr, 30, step0=0, seed=5, tuned=True)
+    for k in range(6):
+        assert np.array_equal(dev[k].cpu().numpy(), ref[k]), k
+    assert np.array_equal(cr.cpu().numpy(), cr_o)
+    b.close()

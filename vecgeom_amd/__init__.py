@@ -11,6 +11,7 @@ device pointers.
 from __future__ import annotations
 
 import ctypes
+import weakref
 import os
 from typing import Optional, Sequence
 
@@ -24,6 +25,12 @@ DEVICE_PTRS = 1
 ASYNC = 2
 MIXED = 4
 TIMING = 8
+LEARNED = 16
+AUDIT = 32
+DISTANCE_FIRST = 64
+TUNED = 128
+FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned")
+MLP_PARAMS = 56321
 SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated")
 
 # Inside_t (VecGeom/base/Global.h:71-88)
@@ -119,12 +126,42 @@ def _declare(L):
         "vgm_batch_set_launch": [_P, _I, _I, _I],
         "vgm_batch_get_launch": [_P, _P],
         "vgm_batch_family_ms": [_P, _P],
+        "vgm_batch_learned_stats": [_P, _P],
+        "vgm_batch_set_fallback": [_P, _I],
+        "vgm_batch_get_modes": [_P, _P],
+        "vgm_mlp_train": [_P, _I, _P, _P],
+        "vgm_mlp_predict": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_mlp_bound": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_mlp_get": [_P, _I, _P, _P],
+        "vgm_mlp_set": [_P, _I, _P, _P],
+        "vgm_mlp_gradient": [_P, _P, _P, _P, _SZ, ctypes.c_float, _P, _P],
         "vgm_batch_autotune": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _U, _P],
     }
     for name, args in bsig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = _I
+    L.vgm_mlp_config_default.argtypes = [_P]
+    L.vgm_mlp_config_default.restype = None
+
+
+class MlpConfig(ctypes.Structure):
+    """vgm_mlp_config (include/vgm/vgm.h)."""
+    _fields_ = [("samples", ctypes.c_int64), ("validation", ctypes.c_int64), ("batch", ctypes.c_int),
+                ("epochs", ctypes.c_int), ("lr", ctypes.c_float), ("over_weight", ctypes.c_float),
+                ("margin_factor", ctypes.c_double), ("margin_abs", ctypes.c_double), ("domain", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class MlpReport(ctypes.Structure):
+    """vgm_mlp_report (include/vgm/vgm.h)."""
+    _fields_ = [("margin", ctypes.c_double), ("max_over", ctypes.c_double), ("mean_abs_err", ctypes.c_double),
+                ("rms_err", ctypes.c_double), ("final_loss", ctypes.c_double), ("train_ms", ctypes.c_double),
+                ("gflop", ctypes.c_double), ("scale", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("samples", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def check(rc: int):
@@ -173,9 +210,12 @@ class Context:
         h = _P()
         check(self._L.vgm_ctx_create(device, stream, ctypes.byref(h)))
         self.handle = h
+        self._batches = weakref.WeakSet()  # batches live on the context: closed before it
 
     def close(self):
         if self.handle:
+            for b in list(self._batches):
+                b.close()
             self._L.vgm_ctx_destroy(self.handle)
             self.handle = None
 
@@ -235,6 +275,60 @@ class Context:
         mx = np.zeros(3)
         check(self._L.vgm_shape_extent(self.handle, shape, mn.ctypes.data, mx.ctypes.data))
         return mn, mx
+
+    # ---- learned safety -----------------------------------------------------
+    def mlp_train(self, shape: int, **cfg) -> dict:
+        """Train the shape's safety network on the GPU (vgm_mlp_train); keyword
+        arguments override vgm_mlp_config fields.  Returns the report."""
+        c = MlpConfig()
+        self._L.vgm_mlp_config_default(ctypes.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise ValueError(f"unknown mlp config field {k}")
+            setattr(c, k, v)
+        r = MlpReport()
+        check(self._L.vgm_mlp_train(self.handle, shape, ctypes.byref(c), ctypes.byref(r)))
+        return r.as_dict()
+
+    def _mlp_query(self, fn, shape, x, y, z, out):
+        dev = _is_device(x)
+        if out is None:
+            out = self._alloc(x, len(x), np.float64)
+        check(fn(self.handle, shape, _ptr(x), _ptr(y), _ptr(z), len(x), _ptr(out), DEVICE_PTRS if dev else HOST_PTRS))
+        return out
+
+    def mlp_predict(self, shape: int, x, y, z, out=None):
+        """Network prediction (mm) at points in the shape's frame."""
+        return self._mlp_query(self._L.vgm_mlp_predict, shape, x, y, z, out)
+
+    def mlp_bound(self, shape: int, x, y, z, out=None):
+        """Served bound max(0, prediction - margin)."""
+        return self._mlp_query(self._L.vgm_mlp_bound, shape, x, y, z, out)
+
+    def mlp_get(self, shape: int):
+        """(params[56321] float32, geom[8]: centre, half-widths, scale, margin)."""
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        g = np.zeros(8)
+        check(self._L.vgm_mlp_get(self.handle, shape, p.ctypes.data, g.ctypes.data))
+        return p, g
+
+    def mlp_set(self, shape: int, params, geom):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.ascontiguousarray(geom, dtype=np.float64)
+        if p.size != MLP_PARAMS or g.size != 8:
+            raise ValueError("expected 56321 parameters and 8 geometry values")
+        check(self._L.vgm_mlp_set(self.handle, shape, p.ctypes.data, g.ctypes.data))
+
+    def mlp_gradient(self, params, u, target, over_weight=4.0):
+        """Loss gradient of one batch from the training kernels (no update)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        uu = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        check(self._L.vgm_mlp_gradient(self.handle, p.ctypes.data, uu.ctypes.data, t.ctypes.data, len(t),
+                                       float(over_weight), g.ctypes.data, loss.ctypes.data))
+        return g, float(loss[0])
 
     # ---- volumes ------------------------------------------------------------
     def logical_volume(self, shape: int) -> int:
@@ -359,6 +453,7 @@ class Batch:
         check(self._L.vgm_batch_create(ctx.handle, len(self._shapes), _ptr(self._shapes), _ptr(self._offsets),
                                        _ptr(self._steps), ctypes.byref(h)))
         self.handle = h
+        ctx._batches.add(self)
 
     def close(self):
         if self.handle:
@@ -375,8 +470,11 @@ class Batch:
         check(self._L.vgm_batch_classify(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(side)))
 
     def transport(self, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0, seed=1, mixed=False,
-                  asynchronous=False, count_fp64=False, timing=False):
+                  asynchronous=False, count_fp64=False, timing=False, learned=False, audit=False,
+                  distance_first=False, tuned=False):
         flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0) | (TIMING if timing else 0)
+        flags |= (LEARNED if learned else 0) | (AUDIT if audit else 0) | (DISTANCE_FIRST if distance_first else 0)
+        flags |= TUNED if tuned else 0
         cnt = ctypes.c_ulonglong(0)
         check(self._L.vgm_batch_transport(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                           _ptr(side), _ptr(crossings), nsteps, step0, seed, flags,
@@ -385,6 +483,15 @@ class Batch:
 
     def set_launch(self, block, grid, tile_tracks):
         check(self._L.vgm_batch_set_launch(self.handle, block, grid, tile_tracks))
+
+    def learned_stats(self):
+        """Counters of the last learned transport."""
+        out = np.zeros(4, dtype=np.uint64)
+        check(self._L.vgm_batch_learned_stats(self.handle, out.ctypes.data))
+        return {"queries": int(out[0]), "served": int(out[1]), "fallbacks": int(out[2]), "violations": int(out[3])}
+
+    def set_fallback(self, pct: int):
+        check(self._L.vgm_batch_set_fallback(self.handle, int(pct)))
 
     def family_ms(self):
         """{family: kernel ms} of the last transport run with timing=True."""
@@ -397,8 +504,15 @@ class Batch:
         check(self._L.vgm_batch_get_launch(self.handle, out.ctypes.data))
         return tuple(int(v) for v in out)
 
-    def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False):
+    def modes(self):
+        """{family: mode} chosen by the last autotune."""
+        out = np.zeros(8, dtype=np.int32)
+        check(self._L.vgm_batch_get_modes(self.handle, out.ctypes.data))
+        return {SHAPE_TYPES[t]: FAMILY_MODES[m] for t, m in enumerate(out) if m >= 0}
+
+    def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False, learned=False, distance_first=False):
         out = np.zeros(3, dtype=np.int32)
+        flags = (MIXED if mixed else 0) | (LEARNED if learned else 0) | (DISTANCE_FIRST if distance_first else 0)
         check(self._L.vgm_batch_autotune(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
-                                         _ptr(side), nsteps, MIXED if mixed else 0, out.ctypes.data))
+                                         _ptr(side), nsteps, flags, out.ctypes.data))
         return tuple(int(v) for v in out)

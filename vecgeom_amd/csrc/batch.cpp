@@ -6,6 +6,7 @@
 #include <vector>
 #include "context.h"
 #include "transport.h"
+#include "mlp_train.h"
 
 using namespace vgm;
 
@@ -33,6 +34,18 @@ struct vgm_batch {
   // VGM_TIMING: events around each family's kernel, on that family's stream
   hipEvent_t fam_t0[kNumShapeTypes] = {}, fam_t1[kNumShapeTypes] = {};
   bool fam_timed[kNumShapeTypes] = {};
+  // learned safety: shape of each solid, network descriptors of the solids
+  // that have one (SolidRec::mlp indexes them), device counters
+  std::vector<int> shape_ids;
+  std::vector<MlpDesc> descs;
+  MlpDesc *d_mlps = nullptr;
+  unsigned long long *d_learned = nullptr;
+  int fallback_pct = 50;
+  int fam_mode[kNumShapeTypes];  // FamilyMode chosen by the autotuner (VGM_TUNED), -1 if none
+  vgm_batch()
+  {
+    for (int &m : fam_mode) m = -1;
+  }
 };
 
 namespace {
@@ -51,7 +64,8 @@ void build_tiles(vgm_batch *b)
   for (int s : order) {
     const SolidRec &r = b->solids[s];
     if (r.end > r.begin && (b->fams.empty() || b->fams.back().type != r.type))
-      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr, 0});
+      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr, 0, kFamFP64, false});
+    if (r.end > r.begin && r.mlp >= 0) b->fams.back().has_mlp = true;
     if (r.end > r.begin) b->fams.back().stage_nd = std::max(b->fams.back().stage_nd, r.nd);
     for (int64_t t = r.begin; t < r.end; t += tt) {
       Tile tl;
@@ -80,16 +94,43 @@ int ensure_streams(vgm_batch *b)
   return VGM_OK;
 }
 
-int upload(vgm_batch *b)
+// Networks of the solids for a learned run (none otherwise); a change
+// re-uploads the solid records.
+void assign_networks(vgm_batch *b, bool learned)
+{
+  vgm_ctx *c = b->ctx;
+  std::vector<MlpDesc> descs;
+  for (size_t i = 0; i < b->solids.size(); ++i) {
+    SolidRec &r = b->solids[i];
+    const size_t sh = size_t(b->shape_ids[i]);
+    int m = -1;
+    if (learned && family_learnable(r.type) && sh < c->mlps.size() && c->mlps[sh].ready) {
+      m = int(descs.size());
+      descs.push_back(mlp_desc(c->mlps[sh]));
+    }
+    if (r.mlp != m) {
+      r.mlp = m;
+      b->tiles_dirty = true;
+    }
+  }
+  if (descs.size() != b->descs.size() ||
+      (!descs.empty() && std::memcmp(descs.data(), b->descs.data(), descs.size() * sizeof(MlpDesc)) != 0))
+    b->tiles_dirty = true;
+  b->descs = descs;
+}
+
+int upload(vgm_batch *b, bool learned = false)
 {
   vgm_ctx *c = b->ctx;
   int rc = sync_geometry(c);
   if (rc) return rc;
+  assign_networks(b, learned);
   if (!b->tiles_dirty && b->dbuf) return VGM_OK;
   build_tiles(b);
   const size_t s1 = align256(b->solids.size() * sizeof(SolidRec) + 1);
   const size_t s2 = align256(b->tiles.size() * sizeof(Tile) + 1);
-  const size_t total = s1 + s2 + 512;
+  const size_t s3 = align256(b->solids.size() * sizeof(MlpDesc) + 1);
+  const size_t total = s1 + s2 + 512 + s3;
   if (b->dcap < total) {
     if (b->dbuf) {
       VGM_HIP(hipStreamSynchronize(c->stream));
@@ -102,6 +143,11 @@ int upload(vgm_batch *b)
   b->d_tiles = reinterpret_cast<Tile *>(b->dbuf + s1);
   b->d_counters = reinterpret_cast<int *>(b->dbuf + s1 + s2);
   b->d_fp64 = reinterpret_cast<unsigned long long *>(b->dbuf + s1 + s2 + 256);
+  b->d_learned = b->d_fp64 + 1;
+  b->d_mlps = reinterpret_cast<MlpDesc *>(b->dbuf + s1 + s2 + 512);
+  if (!b->descs.empty())
+    VGM_HIP(hipMemcpyAsync(b->d_mlps, b->descs.data(), b->descs.size() * sizeof(MlpDesc), hipMemcpyHostToDevice,
+                           c->stream));
   VGM_HIP(hipMemcpyAsync(b->d_solids, b->solids.data(), b->solids.size() * sizeof(SolidRec), hipMemcpyHostToDevice,
                          c->stream));
   VGM_HIP(hipMemcpyAsync(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
@@ -112,7 +158,7 @@ int upload(vgm_batch *b)
 }
 
 TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *dx, double *dy, double *dz,
-                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed)
+                        int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed, unsigned flags = 0)
 {
   TransportArgs a{};
   a.pool = b->ctx->dev.pool;
@@ -128,6 +174,11 @@ TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *d
   a.crossings = crossings;
   a.counter = b->d_counters + kNumShapeTypes;
   a.fp64_steps = b->d_fp64;
+  a.mlps = b->d_mlps;
+  a.learned = b->d_learned;
+  a.audit = (flags & VGM_AUDIT) ? 1 : 0;
+  a.fallback_pct = b->fallback_pct;
+  a.distance_first = (flags & VGM_DISTANCE_FIRST) ? 1 : 0;
   return a;
 }
 
@@ -139,11 +190,30 @@ LaunchShape resolved(vgm_batch *b)
   return ls;
 }
 
+// Mode of one family in a transport call: the autotuner's choice under
+// VGM_TUNED, else from the flags (learned where a network exists, then mixed,
+// then distance-first, else FP64).
+int family_mode(const vgm_batch *b, const FamilyLaunch &f, unsigned flags)
+{
+  int m = kFamFP64;
+  if ((flags & VGM_TUNED) && b->fam_mode[f.type] >= 0)
+    m = b->fam_mode[f.type];
+  else if ((flags & VGM_LEARNED) && f.has_mlp)
+    m = kFamLearned;
+  else if (flags & VGM_MIXED)
+    m = kFamMixed;
+  else if (flags & VGM_DISTANCE_FIRST)
+    m = kFamDistFirst;
+  if (m == kFamLearned && !f.has_mlp) m = kFamDistFirst;
+  return m;
+}
+
 // All families of one transport call: counters reset on the context stream,
 // then each family's kernel on one of the family streams (heaviest families
 // first, each to the least loaded stream), joined back into the context
 // stream.  A single family runs on the context stream directly.
-hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const LaunchShape &ls, bool timing = false)
+hipError_t launch_families(vgm_batch *b, unsigned flags, const TransportArgs &a, const LaunchShape &ls,
+                           bool timing = false)
 {
   hipStream_t main = b->ctx->stream;
   for (bool &t : b->fam_timed) t = false;
@@ -151,20 +221,21 @@ hipError_t launch_families(vgm_batch *b, int mode, const TransportArgs &a, const
   auto one = [&](hipStream_t st, const FamilyLaunch &f) -> hipError_t {
     hipError_t r;
     if (timing && (r = hipEventRecord(b->fam_t0[f.type], st)) != hipSuccess) return r;
-    if ((r = launch_family(st, mode, a, f, ls.block)) != hipSuccess) return r;
+    if ((r = launch_family(st, a, f, ls.block)) != hipSuccess) return r;
     if (timing) {
       if ((r = hipEventRecord(b->fam_t1[f.type], st)) != hipSuccess) return r;
       b->fam_timed[f.type] = true;
     }
     return hipSuccess;
   };
-  hipError_t e = launch_reset(main, b->d_counters, kNumShapeTypes + 1, a.fp64_steps);
+  hipError_t e = launch_reset(main, b->d_counters, kNumShapeTypes + 1, a.fp64_steps, a.learned);
   if (e != hipSuccess) return e;
   double wsum = 0;
   for (const FamilyLaunch &f : b->fams) wsum += b->weight[f.type];
   std::vector<FamilyLaunch> fl = b->fams;
   for (FamilyLaunch &f : fl) {
     const int nt = f.t1 - f.t0;
+    f.mode = family_mode(b, f, flags);
     if (ls.grid > 0) {
       const double share = wsum > 0 ? b->weight[f.type] / wsum : 1.0 / double(fl.size());
       f.grid = std::max(1, std::min(nt, int(std::lround(share * ls.grid))));
@@ -220,7 +291,7 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
     s.type = r.type;
     s.off = r.off;
     s.nd = r.nd;
-    s.pad = 0;
+    s.mlp = -1;
     s.begin = offsets[i];
     s.end = offsets[i + 1];
     s.lambda = mean_step[i];
@@ -232,6 +303,7 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
       s.wh[k] = 1.3 * (0.5 * (r.bmax[k] - r.bmin[k]));
     }
     b->solids.push_back(s);
+    b->shape_ids.push_back(shapes[i]);
   }
   b->ntracks = offsets[nsolids];
   for (const SolidRec &s : b->solids) b->weight[s.type] += double(s.end - s.begin);
@@ -254,6 +326,32 @@ int vgm_batch_destroy(vgm_batch *b)
     if (b->fam_t1[t]) (void)hipEventDestroy(b->fam_t1[t]);
   }
   delete b;
+  return VGM_OK;
+}
+
+int vgm_batch_learned_stats(vgm_batch *b, unsigned long long *out4)
+{
+  if (!b || !out4) return fail(VGM_EINVAL, "null argument");
+  if (!b->d_learned) {
+    for (int k = 0; k < 4; ++k) out4[k] = 0;
+    return VGM_OK;
+  }
+  VGM_HIP(hipMemcpyAsync(out4, b->d_learned, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->ctx->stream));
+  VGM_HIP(hipStreamSynchronize(b->ctx->stream));
+  return VGM_OK;
+}
+
+int vgm_batch_get_modes(vgm_batch *b, int *modes8)
+{
+  if (!b || !modes8) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t) modes8[t] = b->fam_mode[t];
+  return VGM_OK;
+}
+
+int vgm_batch_set_fallback(vgm_batch *b, int pct)
+{
+  if (!b || pct < 1 || pct > 100) return fail(VGM_EINVAL, "fallback percentage must be in [1, 100]");
+  b->fallback_pct = pct;
   return VGM_OK;
 }
 
@@ -296,7 +394,7 @@ int vgm_batch_classify(vgm_batch *b, const double *x, const double *y, const dou
   if (rc) return rc;
   TransportArgs a = make_args(b, const_cast<double *>(x), const_cast<double *>(y), const_cast<double *>(z), nullptr,
                               nullptr, nullptr, side, nullptr, 0, 0, 0);
-  VGM_HIP(launch_reset(b->ctx->stream, b->d_counters, kNumShapeTypes + 1, nullptr));
+  VGM_HIP(launch_reset(b->ctx->stream, b->d_counters, kNumShapeTypes + 1, nullptr, nullptr));
   const int grid = int(std::min<size_t>(b->tiles.size(), size_t(4 * std::max(1, b->ctx->num_cus))));
   VGM_HIP(launch_classify(b->ctx->stream, a, std::max(1, grid), 256));
   VGM_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -310,12 +408,11 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
   if (!b || !x || !y || !z || !dx || !dy || !dz || !side || !crossings || nsteps < 0)
     return fail(VGM_EINVAL, "null argument or negative step count");
   VGM_HIP(hipSetDevice(b->ctx->device));
-  int rc = upload(b);
+  int rc = upload(b, (flags & (VGM_LEARNED | VGM_TUNED)) != 0);
   if (rc) return rc;
-  TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed);
-  const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
+  TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed, flags);
   if ((rc = ensure_streams(b))) return rc;
-  VGM_HIP(launch_families(b, mode, a, resolved(b), (flags & VGM_TIMING) != 0));
+  VGM_HIP(launch_families(b, flags, a, resolved(b), (flags & VGM_TIMING) != 0));
   if (fp64_steps) {
     VGM_HIP(hipMemcpyAsync(fp64_steps, b->d_fp64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->ctx->stream));
@@ -351,8 +448,9 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   int8_t *ws = reinterpret_cast<int8_t *>(work + align256(6 * n * sizeof(double)));
   int32_t *wc = reinterpret_cast<int32_t *>(work + align256(6 * n * sizeof(double)) + align256(n));
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  const int mode = (flags & VGM_MIXED) ? kTransportMixed : kTransportFP64;
   const LaunchShape keep = b->shape;
+  int keep_modes[kNumShapeTypes];
+  std::memcpy(keep_modes, b->fam_mode, sizeof keep_modes);
   const int cus = std::max(1, c->num_cus);
   float best = 1e30f;
   LaunchShape best_shape = keep;
@@ -385,17 +483,35 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
     rc = fail(VGM_EHIP, "hipEventCreate failed");
     goto done;
   }
-  // 1. family weights: each family alone, one workgroup of 256 per 2048-track tile
+  // 1. each family alone, one workgroup of 256 per 2048-track tile, in every
+  // mode the flags allow (FP64 always; mixed, distance-first and learned on
+  // request): the fastest mode is kept, its time is the family's weight
   vgm_batch_set_launch(b, 256, 0, 2048);
-  if ((rc = upload(b))) goto done;
+  if ((rc = upload(b, (flags & VGM_LEARNED) != 0))) goto done;
   for (const FamilyLaunch &f0 : b->fams) {
-    FamilyLaunch f = f0;
-    f.grid = f.t1 - f.t0;
-    f.counter = nullptr;
-    TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1);
-    float ms = 0;
-    if ((rc = timed([&]() { return launch_family(c->stream, mode, a, f, 256); }, ms))) goto done;
-    weight[f.type] = std::max(1e-3, double(ms));
+    std::vector<int> cand{kFamFP64};
+    if (flags & VGM_MIXED) cand.push_back(kFamMixed);
+    if ((flags & VGM_DISTANCE_FIRST) && family_distance_first(f0.type)) cand.push_back(kFamDistFirst);
+    if ((flags & VGM_LEARNED) && f0.has_mlp) cand.push_back(kFamLearned);
+    float fbest = 1e30f;
+    for (int m : cand) {
+      FamilyLaunch f = f0;
+      f.grid = f.t1 - f.t0;
+      f.counter = nullptr;
+      f.mode = m;
+      TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1, flags);
+      float ms = 0;
+      auto one = [&]() {
+        hipError_t e = launch_reset(c->stream, b->d_counters, kNumShapeTypes + 1, nullptr, a.learned);
+        return e != hipSuccess ? e : launch_family(c->stream, a, f, 256);
+      };
+      if ((rc = timed(one, ms))) goto done;
+      if (ms < fbest) {
+        fbest = ms;
+        b->fam_mode[f.type] = m;
+      }
+    }
+    weight[f0.type] = std::max(1e-3, double(fbest));
   }
   for (int t = 0; t < kNumShapeTypes; ++t)
     if (weight[t] > 0) b->weight[t] = weight[t];
@@ -409,11 +525,11 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
         for (int pc : per_cu) {
           if (tt < bl) continue;
           vgm_batch_set_launch(b, bl, pc * cus, tt);
-          if ((rc = upload(b))) goto done;
-          TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1);
+          if ((rc = upload(b, (flags & VGM_LEARNED) != 0))) goto done;
+          TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1, flags);
           const LaunchShape ls = resolved(b);
           float ms = 0;
-          if ((rc = timed([&]() { return launch_families(b, mode, a, ls); }, ms))) goto done;
+          if ((rc = timed([&]() { return launch_families(b, flags | VGM_TUNED, a, ls); }, ms))) goto done;
           if (ms < best) {
             best = ms;
             best_shape = LaunchShape{bl, pc * cus, tt};
@@ -427,6 +543,7 @@ done:
   (void)hipFree(work);
   if (rc) {
     b->shape = keep;
+    std::memcpy(b->fam_mode, keep_modes, sizeof keep_modes);
     b->tiles_dirty = true;
     return rc;
   }

@@ -6,6 +6,7 @@
 #include <vector>
 #include "../../include/vgm/vgm.h"
 #include "context.h"
+#include "mlp_train.h"
 
 namespace vgm {
 
@@ -218,6 +219,7 @@ int vgm_ctx_destroy(vgm_ctx *c)
   (void)hipStreamSynchronize(c->stream);
   if (c->dev.buf) (void)hipFree(c->dev.buf);
   if (c->scratch.buf) (void)hipFree(c->scratch.buf);
+  mlp_release(c);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return VGM_OK;

@@ -3,8 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 #include "../../include/vgm/vgm.h"
 #include "geometry.h"
+#include "mlp.h"
 #include "vgm_internal.h"
 
 namespace vgm {
@@ -48,6 +50,7 @@ struct vgm_ctx {
   vgm::HostGeometry host;
   vgm::DeviceGeometry dev;
   vgm::Scratch scratch;
+  std::vector<vgm::MlpState> mlps;  // learned safety, indexed by shape id
 };
 
 #define VGM_HIP(call)                                           \

@@ -13,6 +13,9 @@ struct ConeShape {
   static constexpr bool kStage = false;  // parameter block staged in LDS
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
+  // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
+  // the distance-first step takes the FP64 step's decisions
+  static constexpr bool kSafetyBelowDistance = true;
 
   template <typename R>
   VGM_DEV static R plus_tol_sq(R x, double tol) { return x * (x + R(2.0 * tol)); }

@@ -13,6 +13,9 @@ struct PolyconeShape {
   static constexpr bool kStage = true;  // parameter block staged in LDS
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
+  // SafetyToIn can exceed DistanceToIn (the section-wise safety of the
+  // reference), so the distance-first step would change decisions
+  static constexpr bool kSafetyBelowDistance = false;
 
   // GetSectionIndex: linear search over the z-planes, compared in the
   // arithmetic type of the query

@@ -44,6 +44,9 @@ struct TessellatedShape {
   static constexpr bool kStage = false;  // parameter block staged in LDS
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = true;
+  // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
+  // the distance-first step takes the FP64 step's decisions
+  static constexpr bool kSafetyBelowDistance = true;
 
   template <typename R>
   VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)

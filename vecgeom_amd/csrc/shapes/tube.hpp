@@ -14,6 +14,9 @@ struct TubeShape {
   static constexpr bool kStage = false;  // parameter block staged in LDS
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
+  // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
+  // the distance-first step takes the FP64 step's decisions
+  static constexpr bool kSafetyBelowDistance = true;
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>

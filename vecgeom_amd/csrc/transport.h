@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "mlp.h"
 
 namespace vgm {
 
@@ -14,7 +15,7 @@ struct SolidRec {
   int32_t type;
   int32_t off;  // parameter block offset in the pool (doubles)
   int32_t nd;   // parameter block size (doubles)
-  int32_t pad;
+  int32_t mlp;  // learned-safety network (index into TransportArgs::mlps), -1 if none
   int64_t begin, end;
   double lambda;  // mean physics step (mm)
   double scale;   // bounding-box diagonal (mm), sets the float guard band
@@ -44,7 +45,18 @@ struct TransportArgs {
   int32_t *crossings;
   int *counter;
   unsigned long long *fp64_steps;
+  // learned safety (VGM_LEARNED): networks, and counters of bound
+  // evaluations, steps moved on a bound, exact fallbacks and (audit runs)
+  // bounds above the exact safety
+  const MlpDesc *mlps;
+  unsigned long long *learned;  // [4]
+  int32_t audit;
+  int32_t fallback_pct;  // a wave runs its fallbacks once this % of its live tracks wait
+  int32_t distance_first;  // FP64 steps of surface-distance families: boundary distance first
+  int32_t pad2;
 };
+
+enum LearnedCounter : int { kLearnedQueries = 0, kLearnedServed = 1, kLearnedFallbacks = 2, kLearnedViolations = 3 };
 
 // Upper bound of the workgroup size of the transport kernels.  It is also
 // their launch bound: 256 threads let the compiler give a wave up to 256
@@ -66,10 +78,26 @@ struct FamilyLaunch {
   int grid;
   int *counter;
   int stage_nd;  // > 0: stage parameter blocks of up to this many doubles in LDS
+  int mode;      // FamilyMode of this launch
+  bool has_mlp;  // some solid of the family has a learned-safety network
 };
 
-hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps);
-hipError_t launch_family(hipStream_t stream, int mode, const TransportArgs &a, const FamilyLaunch &f, int block);
+// How a family's track-steps are computed.  All but kFamMixed take the FP64
+// decisions of step_fp64 (bit-identical trajectories):
+//   kFamFP64       safety, then the boundary distance when the step exceeds it
+//   kFamMixed      float filter, FP64 redo of steps near a boundary
+//   kFamDistFirst  boundary distance first; the safety only where the two
+//                  orders could differ (within kDistanceFirstWindow of it)
+//   kFamLearned    learned safety bound, then kFamDistFirst as the fallback
+enum FamilyMode : int { kFamFP64 = 0, kFamMixed = 1, kFamDistFirst = 2, kFamLearned = 3 };
+
+hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps,
+                        unsigned long long *learned);
+// True when the family can serve learned bounds (its safety is a distance).
+bool family_learnable(int type);
+// True when the distance-first step takes the FP64 step's decisions.
+bool family_distance_first(int type);
+hipError_t launch_family(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block);
 // Initial inside flags (persistent over all tiles, uses a.counter).
 hipError_t launch_classify(hipStream_t stream, const TransportArgs &a, int grid, int block);
 

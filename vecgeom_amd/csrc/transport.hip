@@ -22,6 +22,7 @@
 #include "stepping.hpp"
 #include "shapes/all_shapes.hpp"
 #include "staging.hpp"
+#include "mlp_serve.hpp"
 
 namespace vgm {
 
@@ -146,6 +147,64 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
   return step_fp64<S>(s, t, l);
 }
 
+// Exact track-step, boundary distance first.  The FP64 step moves freely
+// when l <= safety or l < d_b and crosses otherwise.  For families whose
+// safety is a lower bound of the distance to their surface along any
+// direction (kSafetyBelowDistance: SafetyToIn <= DistanceToIn and
+// SafetyToOut <= DistanceToOut up to the tolerance of the surface tests),
+// the step is free either way when l < d_b and crosses either way when
+// l > d_b + kDistanceFirstWindow; only in between is the safety (with its
+// side check) computed.  Same decisions and arithmetic as step_fp64; only
+// the carried bound differs.  `verified` is set when the side check ran and
+// agreed (see run_tile_learned).  Launches of other families ignore the
+// distance-first request.
+constexpr double kDistanceFirstWindow = 1e-6;
+
+template <class S>
+VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified)
+{
+  double db = t.in ? S::template distance_to_out<double>(s, t.p, t.d, kInfLength)
+                   : S::template distance_to_in<double>(s, t.p, t.d, kInfLength);
+  if (db < 0.0) db = 0.0;
+  if (l < db) {
+    t.carry = 0.0;
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  if (l <= db + kDistanceFirstWindow) {
+    if constexpr (S::kSafetyIsDistance) {
+      const double saf = S::template surface_distance<double>(s, t.p, t.in);
+      if (l <= saf && S::template contains<double>(s, t.p) == t.in) {
+        verified = true;
+        t.carry = saf - l - carry_margin(t.p);
+        t.p.x = t.p.x + l * t.d.x;
+        t.p.y = t.p.y + l * t.d.y;
+        t.p.z = t.p.z + l * t.d.z;
+        return false;
+      }
+    } else {
+      const double saf =
+          t.in ? S::template safety_to_out<double>(s, t.p) : S::template safety_to_in<double>(s, t.p);
+      if (l <= saf) {
+        t.p.x = t.p.x + l * t.d.x;
+        t.p.y = t.p.y + l * t.d.y;
+        t.p.z = t.p.z + l * t.d.z;
+        return false;
+      }
+    }
+  }
+  t.carry = 0.0;
+  const double m = db + kBoundaryPush;
+  t.p.x = t.p.x + m * t.d.x;
+  t.p.y = t.p.y + m * t.d.y;
+  t.p.z = t.p.z + m * t.d.z;
+  t.in = !t.in;
+  t.crossings++;
+  return true;
+}
+
 template <class S, int Mode>
 VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s, const Tile &tl,
                       uint32_t &fp64_steps)
@@ -192,8 +251,13 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
         }
         if (k >= a.nsteps) break;
       }
-      const bool crossed =
-          (Mode == kTransportFP64) ? step_fp64<S>(s, t, l) : step_mixed<S>(s, t, l, scale, fp64_steps);
+      bool crossed;
+      if constexpr (Mode == kTransportFP64) {
+        bool verified = false;  // only the learned tiles read it
+        crossed = a.distance_first ? step_distance_first<S>(s, t, l, verified) : step_fp64<S>(s, t, l);
+      } else {
+        crossed = step_mixed<S>(s, t, l, scale, fp64_steps);
+      }
       finish(step, crossed);
       ++k;
     }
@@ -208,6 +272,122 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
   }
 }
 
+struct LearnedCounts {
+  uint32_t q = 0, served = 0, fallbacks = 0, violations = 0;
+};
+
+// Tracks of one tile with the learned bound.  The lanes of a wave hold 64
+// tracks and move through three kinds of work:
+//   carried steps  l <= carry: no query (each lane runs its own)
+//   bound          the network, for every lane at once, on the matrix cores;
+//                  a lane whose step fits under max(carry, bound) moves,
+//                  the others wait for the exact step
+//   exact step     step_distance_first, run together by the waiting lanes
+//                  once they are fallback_pct % of the live lanes, or when
+//                  no lane can go on without it.
+// A bound below the exact safety moves exactly where the FP64 step moves, so
+// trajectories are those of the FP64 path; audit runs count any bound above
+// the exact surface distance.
+//
+// The exact safety is -1 when the point is not on the side the track's flag
+// says (SafetyToOut of an outside point), which happens after the periodic
+// wrap puts a track down inside the solid, or right after a crossing; the
+// FP64 step then takes the boundary-distance path, which restores the flag.
+// The bound is a distance only, so a lane uses it once its side is verified:
+// by a Contains that agrees with the flag, after which moves shorter than
+// the distance to the surface keep it verified; a wrap or a crossing clears
+// it.
+template <class S>
+__device__ void run_tile_learned(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s,
+                                 const Tile &tl, const MlpImage &img, const MlpDesc &md, LearnedCounts &cnt)
+{
+  __shared__ mlp_act_rows mlp_act[kTransportMaxBlock / 64];
+  mlp_act_rows &abuf = mlp_act[threadIdx.x >> 6];
+  const int lane = int(threadIdx.x & 63u);
+  for (int64_t base = tl.begin + int64_t(threadIdx.x & ~63u); base < tl.end; base += blockDim.x) {
+    const int64_t i = base + lane;
+    const bool act = i < tl.end;
+    TrackState t;
+    t.p = act ? V3<double>{a.x[i], a.y[i], a.z[i]} : V3<double>{md.c[0], md.c[1], md.c[2]};
+    t.d = act ? V3<double>{a.dx[i], a.dy[i], a.dz[i]} : V3<double>{0.0, 0.0, 1.0};
+    t.in = act && a.side[i] != 0;
+    t.crossings = 0;
+    t.carry = 0.0;
+    int k = 0;
+    bool live = act && a.nsteps > 0, pending = false, verified = false;
+    double l = live ? physics_step(sr.lambda, a.seed, uint64_t(i), uint32_t(a.step0)) : 0.0;
+    auto advance = [&](bool crossed) {
+      const uint32_t step = uint32_t(a.step0 + k);
+      if (!crossed) t.d = new_direction(a.seed, uint64_t(i), step, t.d);
+      if (!t.in) {
+        const V3<double> q = t.p;
+        t.p.x = wrap(t.p.x, sr.wc[0], sr.wh[0]);
+        t.p.y = wrap(t.p.y, sr.wc[1], sr.wh[1]);
+        t.p.z = wrap(t.p.z, sr.wc[2], sr.wh[2]);
+        if (t.p.x != q.x || t.p.y != q.y || t.p.z != q.z) {
+          t.carry = 0.0;
+          verified = false;
+        }
+      }
+      if (crossed) verified = false;
+      if (++k < a.nsteps)
+        l = physics_step(sr.lambda, a.seed, uint64_t(i), uint32_t(a.step0 + k));
+      else
+        live = false;
+    };
+    auto free_move = [&]() {
+      const double m = carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      t.carry = t.carry - l - m;
+      advance(false);
+    };
+    for (;;) {
+      if (live && !pending)
+        while (live && l <= t.carry) free_move();
+      const bool want = live && !pending;
+      const uint64_t lmask = __ballot(live);
+      if (lmask == 0) break;
+      const uint64_t wmask = __ballot(want), pmask = __ballot(pending);
+      if (wmask == 0 || 100 * __popcll(pmask) >= a.fallback_pct * __popcll(lmask)) {
+        if (pending) {
+          const bool crossed = step_distance_first<S>(s, t, l, verified);
+          advance(crossed);
+          pending = false;
+          ++cnt.fallbacks;
+        }
+        continue;
+      }
+      const float y = mlp_eval_wave(img, abuf, mlp_coord(t.p.x, md.c[0], md.inv_hw[0]),
+                                    mlp_coord(t.p.y, md.c[1], md.inv_hw[1]), mlp_coord(t.p.z, md.c[2], md.inv_hw[2]));
+      if (want) {
+        ++cnt.q;
+        const double b = double(y) * md.scale - md.margin;
+        if (a.audit && b > S::template surface_distance<double>(s, t.p, t.in)) ++cnt.violations;
+        if (!verified && l <= b) verified = S::template contains<double>(s, t.p) == t.in;
+        if (verified && b > t.carry) t.carry = b;
+        if (l <= t.carry) {
+          free_move();
+          ++cnt.served;
+        } else {
+          pending = true;
+        }
+      }
+    }
+    if (act) {
+      a.x[i] = t.p.x;
+      a.y[i] = t.p.y;
+      a.z[i] = t.p.z;
+      a.dx[i] = t.d.x;
+      a.dy[i] = t.d.y;
+      a.dz[i] = t.d.z;
+      a.side[i] = t.in ? 1 : 0;
+      a.crossings[i] += t.crossings;
+    }
+  }
+}
+
 }  // namespace
 
 // One shape family, one precision mode.  Tiles [t0, t1) of the type-sorted
@@ -215,14 +395,18 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
 // the grid is sized to the family's share of the machine and workgroups pull
 // tiles until the range is drained.  Without one, workgroup b runs tile t0+b.
 // Every family has its own kernel, so each gets the register allocation of
-// its own code path rather than the largest one of all families.
-template <class S, int Mode, bool Staged>
+// its own code path rather than the largest one of all families.  Learned
+// kernels run the tiles of solids with a network through run_tile_learned
+// (network weights read through the caches: they are shared by every wave
+// on the chip), the other tiles through the FP64 step.
+template <class S, int Mode, bool Staged, bool Learned>
 __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(TransportArgs a, int t0, int t1,
                                                                               int *counter)
 {
   extern __shared__ double lds_block[];
   __shared__ int tile_s;
   uint32_t fp64_steps = 0;
+  LearnedCounts cnt;
   int staged_solid = -1;
   for (int k = 0;; ++k) {
     int t;
@@ -238,6 +422,7 @@ __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(Tr
     if (t >= t1) break;
     const Tile tl = a.tiles[t];
     const SolidRec sr = a.solids[tl.solid];
+    const typename S::Data *sp = reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
     if constexpr (Staged) {
       if (tl.solid != staged_solid) {  // uniform across the workgroup
         __syncthreads();
@@ -245,12 +430,26 @@ __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(Tr
         __syncthreads();
         staged_solid = tl.solid;
       }
-      run_tile<S, Mode>(a, sr, *reinterpret_cast<const typename S::Data *>(lds_block), tl, fp64_steps);
-    } else {
-      run_tile<S, Mode>(a, sr, *reinterpret_cast<const typename S::Data *>(a.pool + sr.off), tl, fp64_steps);
+      sp = reinterpret_cast<const typename S::Data *>(lds_block);
     }
+    if constexpr (Learned) {
+      if (sr.mlp >= 0) {
+        const MlpDesc md = a.mlps[sr.mlp];
+        run_tile_learned<S>(a, sr, *sp, tl, *md.img, md, cnt);
+        continue;
+      }
+    }
+    run_tile<S, Mode>(a, sr, *sp, tl, fp64_steps);
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
+  if constexpr (Learned) {  // one atomic per wave and counter
+    uint32_t v[4] = {cnt.q, cnt.served, cnt.fallbacks, cnt.violations};
+    for (int c = 0; c < 4; ++c) {
+      uint32_t x = v[c];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+      if ((threadIdx.x & 63u) == 0 && x) atomicAdd(a.learned + c, (unsigned long long)x);
+    }
+  }
 }
 
 namespace {
@@ -285,55 +484,101 @@ __global__ __launch_bounds__(kTransportMaxBlock) void classify_kernel(TransportA
   }
 }
 
-__global__ void reset_counters_kernel(int *counters, int n, unsigned long long *fp64_steps)
+__global__ void reset_counters_kernel(int *counters, int n, unsigned long long *fp64_steps,
+                                      unsigned long long *learned)
 {
   const int i = int(threadIdx.x);
   if (i < n) counters[i] = 0;
   if (i == 0 && fp64_steps) *fp64_steps = 0;
+  if (i < 4 && learned) learned[i] = 0;
 }
 
 namespace {
 struct FamilyLauncher {
   hipStream_t stream;
-  int mode;
   const TransportArgs &a;
   const FamilyLaunch &f;
   int block;
-  template <class S>
-  void operator()()
+
+  template <class S, int Mode, bool Learned>
+  void go(const TransportArgs &args)
   {
     const int grid = f.counter ? f.grid : (f.t1 - f.t0);
     if (grid <= 0) return;
     const bool staged = S::kStage && f.stage_nd > 0 && size_t(f.stage_nd) * sizeof(double) <= size_t(kMaxStageBytes);
-    const size_t lds = staged ? size_t(f.stage_nd) * sizeof(double) : 0;
-    if (mode == kTransportMixed) {
-      if (staged)
-        hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed, S::kStage>), dim3(grid), dim3(block), lds,
-                           stream, a, f.t0, f.t1, f.counter);
-      else
-        hipLaunchKernelGGL((transport_family_kernel<S, kTransportMixed, false>), dim3(grid), dim3(block), 0, stream,
-                           a, f.t0, f.t1, f.counter);
-    } else {
-      if (staged)
-        hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64, S::kStage>), dim3(grid), dim3(block), lds,
-                           stream, a, f.t0, f.t1, f.counter);
-      else
-        hipLaunchKernelGGL((transport_family_kernel<S, kTransportFP64, false>), dim3(grid), dim3(block), 0, stream,
-                           a, f.t0, f.t1, f.counter);
+    if (staged)
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, S::kStage, Learned>), dim3(grid), dim3(block),
+                         size_t(f.stage_nd) * sizeof(double), stream, args, f.t0, f.t1, f.counter);
+    else
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, false, Learned>), dim3(grid), dim3(block), 0, stream,
+                         args, f.t0, f.t1, f.counter);
+  }
+
+  template <class S>
+  void operator()()
+  {
+    TransportArgs args = a;
+    args.distance_first = (S::kSafetyBelowDistance && (f.mode == kFamDistFirst || f.mode == kFamLearned)) ? 1 : 0;
+    if (f.mode == kFamMixed) {
+      go<S, kTransportMixed, false>(args);
+      return;
     }
+    if constexpr (S::kSafetyIsDistance) {
+      if (f.mode == kFamLearned && f.has_mlp) {
+        go<S, kTransportFP64, true>(args);
+        return;
+      }
+    }
+    go<S, kTransportFP64, false>(args);
   }
 };
 }  // namespace
 
-hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps)
+namespace {
+struct LearnableQuery {
+  bool r = false;
+  template <class S>
+  void operator()()
+  {
+    r = S::kSafetyIsDistance;
+  }
+};
+}  // namespace
+
+bool family_learnable(int type)
 {
-  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n, fp64_steps);
+  LearnableQuery q;
+  return dispatch_shape_type(type, q) && q.r;
+}
+
+namespace {
+struct DistanceFirstQuery {
+  bool r = false;
+  template <class S>
+  void operator()()
+  {
+    r = S::kSafetyBelowDistance;
+  }
+};
+}  // namespace
+
+bool family_distance_first(int type)
+{
+  DistanceFirstQuery q;
+  return dispatch_shape_type(type, q) && q.r;
+}
+
+hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps,
+                        unsigned long long *learned)
+{
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n, fp64_steps, learned);
   return hipGetLastError();
 }
 
-hipError_t launch_family(hipStream_t stream, int mode, const TransportArgs &a, const FamilyLaunch &f, int block)
+hipError_t launch_family(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block)
 {
-  FamilyLauncher l{stream, mode, a, f, block};
+  if (f.mode == kFamLearned && f.has_mlp && (!a.mlps || !a.learned)) return hipErrorInvalidValue;
+  FamilyLauncher l{stream, a, f, block};
   if (!dispatch_shape_type(f.type, l)) return hipErrorInvalidValue;
   return hipGetLastError();
 }
