@@ -4,14 +4,22 @@ Workload (synthetic, built by vecgeom_amd.workloads): 8 solids per shape
 family with log-uniform parameters, per-solid track counts log-uniform in
 [1k, 4M] rescaled to --tracks (default 40M), points uniform in 1.3x each
 solid's bounding box, isotropic directions, seed 42.  One bench step = one
-persistent launch advancing every track by --track-steps (default 50)
-track-steps.  Inputs are resident in HBM when timing starts.
+launch advancing every track by --track-steps (default 50) track-steps.
+Inputs are resident in HBM when timing starts.
 
-Both precision paths run in the same process on identical initial states:
-plain FP64 (bit-identical to the CPU oracle) and mixed (float filter, FP64
-boundary steps).  `value` is the mixed path's whole-job track-steps/s; the
-FP64 figure and the ratio are reported beside it.  The CPU baseline is the
-FP64 oracle on the host cores over a bounded sample of the same workload.
+Runs, in one process on identical initial states (--only picks a subset):
+  fp64     plain FP64 step (safety, then boundary distance), bit-identical to
+           the CPU oracle
+  mixed    float filter with FP64 redo of steps near boundaries
+  learned  learned safety bound on the tessellated solids (networks trained on
+           the GPU first, on exact samples), FP64 fallback; mixed elsewhere
+  tuned    each family in the mode the autotuner measured fastest among FP64,
+           mixed, distance-first (same decisions as FP64) and learned
+`value` is the last run's whole-job track-steps/s (tuned by default); the
+others are reported beside it, with the number of tracks whose final state
+differs from the FP64 run and the audited safety violations of learned bounds.
+The CPU baseline is the FP64 oracle on the host cores over a bounded sample of
+the same workload.
 """
 import argparse
 import json
@@ -58,23 +66,34 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
-def family_roofline(fams_ms, counts_by_family, S):
-    """Roofline of the dominant (longest) family kernel of a timed transport:
-    algorithmic FP64 flops per launch (flop fixture x the family's tracks x
-    track-steps) over its average launch time."""
+def family_roofline(fams_ms, counts_by_family, S, modes=None, weights=None):
+    """Roofline of the dominant family kernel of a timed transport (the one
+    with the largest standalone time, `weights`, from the autotuner):
+    algorithmic FP64 flops per launch (flop fixture for the family's step
+    order x its tracks x track-steps) over its average launch time, from
+    events on its stream.  The heaviest family is launched first on its
+    stream, so its events bracket the kernel alone; a later kernel on a
+    stream that shares a hardware queue would include the wait."""
     if not os.path.exists(FLOPS_FIXTURE) or not fams_ms:
         return None
     fx = json.load(open(FLOPS_FIXTURE))
-    fam = max(fams_ms, key=fams_ms.get)
-    per_step = float(fx["per_family"][fam])
+    w = {k: v for k, v in (weights or {}).items() if k in fams_ms}
+    fam = max(w, key=w.get) if w else max(fams_ms, key=fams_ms.get)
+    mode = (modes or {}).get(fam, "fp64")
+    if mode not in ("fp64", "distance-first"):
+        return None  # float or matrix-core work: not priced against the FP64 peak
+    table = fx["per_family_distance_first"] if mode == "distance-first" else fx["per_family"]
+    per_step = float(table[fam])
     flops = per_step * counts_by_family[fam] * S
     t = fams_ms[fam] * 1e-3
     achieved = flops / t / 1e12
     out = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-           "kernel": f"transport_family_kernel<{fam}, FP64>", "kernel_ms": fams_ms[fam],
-           "flops_per_track_step": per_step, "tracks": counts_by_family[fam], "track_steps": S,
-           "note": "FP64 vector peak; flops counted on the FP64 track-step path (+ - * / sqrt)"}
+           "kernel": f"transport_family_kernel<{fam}, FP64{', distance-first' if mode == 'distance-first' else ''}>",
+           "kernel_ms": fams_ms[fam], "flops_per_track_step": per_step, "tracks": counts_by_family[fam],
+           "track_steps": S,
+           "note": "FP64 vector peak; flops counted on the step the kernel runs (+ - * / sqrt); kernel time from "
+                   "events on its stream while the other families run concurrently"}
     if os.path.exists(TRAFFIC_FIXTURE):
         tr = json.load(open(TRAFFIC_FIXTURE))
         if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
@@ -158,6 +177,11 @@ def main():
                     help="exact safety samples for the learned-safety networks, over all tessellated solids")
     ap.add_argument("--train-epochs", type=int, default=4)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--modes", default=None,
+                    help="tuned run without the autotuner: family=mode,... (modes fp64, mixed, distance-first, "
+                         "learned); with --launch")
+    ap.add_argument("--only", default=None,
+                    help="comma list of runs out of fp64,mixed,learned,tuned (default all; the headline is the last)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy kernel of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -201,6 +225,8 @@ def main():
         torch.cuda.synchronize()
         if args.launch:
             batch.set_launch(*[int(v) for v in args.launch.split(",")])
+            if tuned and args.modes:
+                batch.set_modes(dict(kv.split("=") for kv in args.modes.split(",")))
         elif not args.no_autotune:
             log(f"[bench] autotune {tag} ...")
             t = time.perf_counter()
@@ -241,6 +267,7 @@ def main():
             extra["audit"] = batch.learned_stats()
         if tuned or not args.launch:
             extra["modes"] = batch.modes()
+        extra["weights"] = batch.weights()
         final = state + [side, cr]
         log(f"[bench] {tag}: {args.steps * n * S / wall / 1e9:.2f} G track-steps/s (kernel {kern * 1e3:.2f} ms/launch)")
         log(f"[bench] {tag} family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam_ms.items()))
@@ -249,68 +276,90 @@ def main():
     counts_by_family = {}
     for (kind, _), c in zip(specs, counts):
         counts_by_family[kind] = counts_by_family.get(kind, 0) + int(c)
-    wall64, kern64, _, launch64, fam64, final64 = run(False)
-    rate64 = args.steps * n * S / wall64
-    wallmx, kernmx, extramx, launchmx, fammx, _ = run(True)
-    ratemx = args.steps * n * S / wallmx
-    nfp64 = extramx["fp64_steps"]
-    log(f"[bench] mixed fp64 redo fraction {nfp64 / (n * S):.4f}")
-    roofline = family_roofline(fam64, counts_by_family, S)
-
-    learned = tuned = None
-    value, wall_v, kern_v = ratemx, wallmx, kernmx
+    only = set(args.only.split(",")) if args.only else {"fp64", "mixed", "learned", "tuned"}
+    res = {}
+    if "fp64" in only:
+        res["fp64"] = run(False)
+    if "mixed" in only:
+        res["mixed"] = run(True)
+        log(f"[bench] mixed fp64 redo fraction {res['mixed'][2]['fp64_steps'] / (n * S):.4f}")
     tess = [i for i, (kind, _) in enumerate(specs) if kind == "tessellated"]
-    if tess and not args.no_learned:
+    train = None
+    if tess and not args.no_learned and only & {"learned", "tuned"}:
         per = max(8192, args.train_samples // len(tess) // 8192 * 8192)
         reports = []
         t = time.perf_counter()
         for i in tess:
             reports.append(ctx.mlp_train(int(shapes[i]), samples=per, epochs=args.train_epochs))
-        train_wall = time.perf_counter() - t
-        log(f"[bench] trained {len(tess)} networks on {per * len(tess)} exact samples in {train_wall:.1f}s "
-            f"(GPU {sum(r['train_ms'] for r in reports) / 1e3:.1f}s); margins/scale "
-            + ", ".join(f"{r['margin'] / r['scale']:.4f}" for r in reports))
-        walll, kernl, extral, launchl, faml, finall = run(True, learned=True)
-        ratel = args.steps * n * S / walll
-        au = extral["audit"]
-        wallt, kernt, extrat, launcht, famt, finalt = run(False, tuned=True)
-        ratet = args.steps * n * S / wallt
-        # every family except mixed ones must end exactly where the FP64 path ends
-        modes = extrat.get("modes", {})
-        exact_mask = torch.zeros(n, dtype=torch.bool, device=dev)
-        tmask = torch.zeros(n, dtype=torch.bool, device=dev)
+        train = {"networks": len(tess), "train_samples": per * len(tess), "train_epochs": args.train_epochs,
+                 "train_wall_s": time.perf_counter() - t, "train_gpu_ms": sum(r["train_ms"] for r in reports),
+                 "train_gflop": sum(r["gflop"] for r in reports),
+                 "margin_over_scale": [r["margin"] / r["scale"] for r in reports],
+                 "rms_err_over_scale": [r["rms_err"] / r["scale"] for r in reports]}
+        log(f"[bench] trained {len(tess)} networks on {per * len(tess)} exact samples in {train['train_wall_s']:.1f}s "
+            f"(GPU {train['train_gpu_ms'] / 1e3:.1f}s); margins/scale "
+            + ", ".join(f"{v:.4f}" for v in train["margin_over_scale"]))
+        if "learned" in only:
+            res["learned"] = run(True, learned=True)
+    if "tuned" in only:
+        res["tuned"] = run(False, tuned=True)
+
+    def rate(key):
+        return args.steps * n * S / res[key][0] if key in res else None
+
+    def ndiff(key, mask):
+        """Tracks of `mask` whose final state differs from the FP64 run's."""
+        if "fp64" not in res or key not in res:
+            return None
+        d = torch.zeros(n, dtype=torch.bool, device=dev)
+        for a, b in zip(res["fp64"][5], res[key][5]):
+            d |= (a != b)
+        return int((d & mask).sum().item())
+
+    def family_mask(pred):
+        m = torch.zeros(n, dtype=torch.bool, device=dev)
         for i, (kind, _) in enumerate(specs):
-            sl = slice(int(offsets[i]), int(offsets[i + 1]))
-            if modes.get(kind) != "mixed":
-                exact_mask[sl] = True
-            if kind == "tessellated":
-                tmask[sl] = True
+            if pred(kind):
+                m[int(offsets[i]):int(offsets[i + 1])] = True
+        return m
 
-        def ndiff(fin, mask):
-            d = torch.zeros(n, dtype=torch.bool, device=dev)
-            for a, b in zip(final64, fin):
-                d |= (a != b)
-            return int((d & mask).sum().item())
-
-        ndl = ndiff(finall, tmask)
-        ndt = ndiff(finalt, exact_mask)
-        learned = {
-            "networks": len(tess), "train_samples": per * len(tess), "train_epochs": args.train_epochs,
-            "train_wall_s": train_wall, "train_gpu_ms": sum(r["train_ms"] for r in reports),
-            "train_gflop": sum(r["gflop"] for r in reports),
-            "margin_over_scale": [r["margin"] / r["scale"] for r in reports],
-            "rms_err_over_scale": [r["rms_err"] / r["scale"] for r in reports],
+    learned = tuned = None
+    if "learned" in res:
+        au = res["learned"][2]["audit"]
+        learned = dict(train or {})
+        learned.update({
             "queries": au["queries"], "served": au["served"], "fallbacks": au["fallbacks"],
             "served_fraction": au["served"] / max(1, au["queries"]), "violations": au["violations"],
             "audit_sample": f"first launch ({S} track-steps of all {n} tracks), every served bound checked",
-            "tessellated_tracks_differing_from_fp64": ndl,
-            "value": ratel, "kernel_ms": kernl * 1e3, "family_kernel_ms": faml, "launch": launchl,
-        }
-        tuned = {"value": ratet, "modes": modes, "kernel_ms": kernt * 1e3, "family_kernel_ms": famt,
-                 "launch": launcht, "exact_mode_tracks_differing_from_fp64": ndt, "audit": extrat.get("audit")}
-        log(f"[bench] tuned modes {modes}; exact-mode tracks differing from fp64: {ndt}")
-        value, wall_v, kern_v = ratet, wallt, kernt
-        log(f"[bench] learned: audit {au}, tessellated tracks differing from fp64: {ndl}")
+            "tessellated_tracks_differing_from_fp64": ndiff("learned", family_mask(lambda k: k == "tessellated")),
+            "value": rate("learned"), "kernel_ms": res["learned"][1] * 1e3, "family_kernel_ms": res["learned"][4],
+            "launch": res["learned"][3]})
+        log(f"[bench] learned: audit {au}, tessellated tracks differing from fp64: "
+            f"{learned['tessellated_tracks_differing_from_fp64']}")
+    if "tuned" in res:
+        modes = res["tuned"][2].get("modes", {})
+        tuned = {"value": rate("tuned"), "modes": modes, "kernel_ms": res["tuned"][1] * 1e3,
+                 "family_kernel_ms": res["tuned"][4], "launch": res["tuned"][3],
+                 "exact_mode_tracks_differing_from_fp64": ndiff("tuned", family_mask(lambda k: modes.get(k) != "mixed")),
+                 "audit": res["tuned"][2].get("audit"), "learned_networks": train}
+        log(f"[bench] tuned modes {modes}; exact-mode tracks differing from fp64: "
+            f"{tuned['exact_mode_tracks_differing_from_fp64']}")
+    head = next(k for k in ("tuned", "learned", "mixed", "fp64") if k in res)
+    value, wall_v, kern_v = rate(head), res[head][0], res[head][1]
+    roofline = family_roofline(res[head][4], counts_by_family, S,
+                               res[head][2].get("modes") if head == "tuned" else None, res[head][2].get("weights"))
+    if roofline:
+        roofline["run"] = head
+    rate64, ratemx = rate("fp64"), rate("mixed")
+    used = set(res[head][2].get("modes", {}).values()) if head == "tuned" else {head}
+    dtype = "f64" + ("+f32" if used & {"mixed"} else "") + ("+bf16" if used & {"learned"} else "")
+    nfp64 = res["mixed"][2]["fp64_steps"] if "mixed" in res else None
+    kern64 = res["fp64"][1] if "fp64" in res else None
+    kernmx = res["mixed"][1] if "mixed" in res else None
+    launch64 = res["fp64"][3] if "fp64" in res else None
+    launchmx = res["mixed"][3] if "mixed" in res else None
+    fam64 = res["fp64"][4] if "fp64" in res else None
+    fammx = res["mixed"][4] if "mixed" in res else None
 
     cpu = None
     if not args.no_cpu:
@@ -344,16 +393,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": f"batched heterogeneous solids: {len(specs)} solids ({'/'.join(fams)}), "
                                f"{n} tracks, {S} track-steps per launch",
                    "tracks": n, "track_steps_per_launch": S, "families": list(fams)},
         "fp64_value": rate64,
         "mixed_value": ratemx,
-        "value_over_fp64": value / rate64,
-        "fp64_redo_fraction": nfp64 / (n * S),
-        "kernel_ms": {"fp64": kern64 * 1e3, "mixed": kernmx * 1e3, "value": kern_v * 1e3},
+        "value_over_fp64": value / rate64 if rate64 else None,
+        "fp64_redo_fraction": nfp64 / (n * S) if nfp64 is not None else None,
+        "kernel_ms": {"fp64": kern64 * 1e3 if kern64 else None, "mixed": kernmx * 1e3 if kernmx else None,
+                      "value": kern_v * 1e3},
         "family_kernel_ms": {"fp64": fam64, "mixed": fammx},
         "launch": {"fp64": launch64, "mixed": launchmx},
         "roofline": roofline,

@@ -230,6 +230,13 @@ int vgm_batch_learned_stats(vgm_batch *batch, unsigned long long *out4);
  * VGM_DISTANCE_FIRST, VGM_LEARNED), each family alone, and keeps the fastest;
  * VGM_TUNED transports use the choice. */
 int vgm_batch_get_modes(vgm_batch *batch, int *modes8);
+/* Relative cost of each family (index = shape type): after
+ * vgm_batch_autotune, the ms of that family's kernel alone in its mode;
+ * before, its track count.  It splits persistent grids and orders launches. */
+int vgm_batch_get_weights(vgm_batch *batch, double *ms8);
+/* Sets the modes VGM_TUNED transports use (distance-first falls back to FP64
+ * for families whose safety can exceed their distance). */
+int vgm_batch_set_modes(vgm_batch *batch, const int *modes8);
 /* A wave of the learned kernel runs its exact fallbacks once this percentage
  * of its live tracks wait for one (default 50). */
 int vgm_batch_set_fallback(vgm_batch *batch, int pct);

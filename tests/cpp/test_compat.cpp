@@ -1,0 +1,83 @@
+// The VecGeom-shaped adapter (include/vgm/vecgeom_compat.h) used the way the
+// reference's unit tests use UnplacedBox / VPlacedVolume and the navigator
+// (test/unit_tests/TestBox.cpp:22-120 style assertions), with closed-form
+// answers: a box 10 x 20 x 30 placed at the origin, and a world box holding
+// one daughter box.  Exit status 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+#include "vgm/vecgeom_compat.h"
+
+using namespace vgm::compat;
+
+static int failures = 0;
+#define EXPECT(c)                                                  \
+  do {                                                             \
+    if (!(c)) {                                                    \
+      std::printf("FAILED line %d: %s\n", __LINE__, #c);           \
+      ++failures;                                                  \
+    }                                                              \
+  } while (0)
+
+int main()
+{
+  GeoManager geo(0);
+  const int box = geo.Box(10., 20., 30.);
+  PlacedVolume pv = geo.Place(geo.NewLogicalVolume(box));
+
+  // points: origin, outside on +x, on the +y face, inside near -z
+  std::vector<double> x{0, 20, 0, 0}, y{0, 0, 20, 0}, z{0, 0, 0, -25};
+  std::vector<double> dx{1, -1, 0, 0}, dy{0, 0, -1, 0}, dz{0, 0, 0, -1};
+  SOA3D<> P(x.data(), y.data(), z.data(), 4), D(dx.data(), dy.data(), dz.data(), 4);
+  bool contains[4];
+  Inside_t inside[4];
+  double din[4], dout[4], sin_[4], sout[4];
+  pv.Contains(P, contains);
+  pv.Inside(P, inside);
+  pv.DistanceToIn(P, D, nullptr, din);
+  pv.DistanceToOut(P, D, nullptr, dout);
+  pv.SafetyToIn(P, sin_);
+  pv.SafetyToOut(P, sout);
+  EXPECT(contains[0] && !contains[1] && contains[3]);
+  EXPECT(inside[0] == kInside && inside[1] == kOutside && inside[2] == kSurface && inside[3] == kInside);
+  EXPECT(dout[0] == 10.0);   // origin along +x to the x = 10 face
+  EXPECT(din[1] == 10.0);    // (20,0,0) along -x to the x = 10 face
+  EXPECT(dout[3] == 5.0);    // (0,0,-25) along -z to z = -30
+  EXPECT(sout[0] == 10.0 && sin_[1] == 10.0 && sout[3] == 5.0);
+  EXPECT(sin_[0] <= 0.0 && sout[1] <= 0.0);  // wrong-side safeties are not positive
+
+  // navigation: world 100^3 holding a 10^3 box at (30, 0, 0)
+  const int wshape = geo.Box(100., 100., 100.), dshape = geo.Box(10., 10., 10.);
+  LogicalVolume wlv = geo.NewLogicalVolume(wshape), dlv = geo.NewLogicalVolume(dshape);
+  PlacedVolume d = geo.PlaceDaughter(wlv, dlv, Transformation3D(30., 0., 0.));
+  PlacedVolume world = geo.Place(wlv);
+  Navigator nav(geo, world);
+  EXPECT(nav.NumberOfStates() == 2);
+  std::vector<double> gx{0, 30}, gy{0, 0}, gz{0, 0}, ux{1, 1}, uy{0, 0}, uz{0, 0};
+  SOA3D<> G(gx.data(), gy.data(), gz.data(), 2), U(ux.data(), uy.data(), uz.data(), 2);
+  uint32_t state[2], last[2] = {0, 0};
+  uint8_t boundary[2] = {0, 0};
+  double step[2], safety[2];
+  nav.LocateGlobalPoints(G, state);
+  EXPECT(state[0] == 1 && state[1] == 2);
+  nav.ComputeSafeties(G, state, safety);
+  EXPECT(safety[0] == 20.0 && safety[1] == 10.0);
+  nav.ComputeStepsAndPropagatedStates(G, U, nullptr, state, last, boundary, step);
+  EXPECT(step[0] == 20.0 && state[0] == 2 && boundary[0] == 1);  // enters the daughter
+  EXPECT(step[1] == 10.0 && state[1] == 1 && boundary[1] == 1);  // leaves it into the world
+  int dpv, level;
+  uint32_t mother;
+  nav.Decode(2, &dpv, &level, &mother);
+  EXPECT(dpv == d.id() && level == 1 && mother == 1);
+
+  // errors surface as exceptions with the library's message
+  bool threw = false;
+  try {
+    geo.Box(-1., 1., 1.);
+  } catch (const Error &e) {
+    threw = e.code == VGM_EINVAL;
+  }
+  EXPECT(threw);
+  std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
+  return failures ? 1 : 0;
+}

@@ -129,6 +129,8 @@ def _declare(L):
         "vgm_batch_learned_stats": [_P, _P],
         "vgm_batch_set_fallback": [_P, _I],
         "vgm_batch_get_modes": [_P, _P],
+        "vgm_batch_set_modes": [_P, _P],
+        "vgm_batch_get_weights": [_P, _P],
         "vgm_mlp_train": [_P, _I, _P, _P],
         "vgm_mlp_predict": [_P, _I, _P, _P, _P, _SZ, _P, _U],
         "vgm_mlp_bound": [_P, _I, _P, _P, _P, _SZ, _P, _U],
@@ -509,6 +511,19 @@ class Batch:
         out = np.zeros(8, dtype=np.int32)
         check(self._L.vgm_batch_get_modes(self.handle, out.ctypes.data))
         return {SHAPE_TYPES[t]: FAMILY_MODES[m] for t, m in enumerate(out) if m >= 0}
+
+    def weights(self):
+        """{family: weight}: standalone kernel ms after an autotune."""
+        out = np.zeros(8)
+        check(self._L.vgm_batch_get_weights(self.handle, out.ctypes.data))
+        return {SHAPE_TYPES[t]: float(v) for t, v in enumerate(out) if v > 0}
+
+    def set_modes(self, modes):
+        """modes: {family: mode name}; families not given are left untuned."""
+        out = np.full(8, -1, dtype=np.int32)
+        for f, m in modes.items():
+            out[SHAPE_TYPES.index(f)] = FAMILY_MODES.index(m)
+        check(self._L.vgm_batch_set_modes(self.handle, out.ctypes.data))
 
     def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False, learned=False, distance_first=False):
         out = np.zeros(3, dtype=np.int32)

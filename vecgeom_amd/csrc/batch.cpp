@@ -341,6 +341,26 @@ int vgm_batch_learned_stats(vgm_batch *b, unsigned long long *out4)
   return VGM_OK;
 }
 
+int vgm_batch_get_weights(vgm_batch *b, double *ms8)
+{
+  if (!b || !ms8) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t) ms8[t] = b->weight[t];
+  return VGM_OK;
+}
+
+int vgm_batch_set_modes(vgm_batch *b, const int *modes8)
+{
+  if (!b || !modes8) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t)
+    if (modes8[t] < -1 || modes8[t] > kFamLearned) return fail(VGM_EINVAL, "invalid family mode");
+  for (int t = 0; t < kNumShapeTypes; ++t) {
+    int m = modes8[t];
+    if (m == kFamDistFirst && !family_distance_first(t)) m = kFamFP64;
+    b->fam_mode[t] = m;
+  }
+  return VGM_OK;
+}
+
 int vgm_batch_get_modes(vgm_batch *b, int *modes8)
 {
   if (!b || !modes8) return fail(VGM_EINVAL, "null argument");

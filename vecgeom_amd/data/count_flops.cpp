@@ -79,6 +79,51 @@ R carry_margin(const V3<R> &p)
   return R(1e-7) + R(1e-12) * (vabs(p.x) + vabs(p.y) + vabs(p.z));
 }
 
+// The distance-first track-step (same decisions as step(), for families with
+// kSafetyBelowDistance), on scalar type R.
+template <class S, typename R>
+bool step_distance_first(const typename S::Data &s, Track<R> &t, R l)
+{
+  R db = t.in ? S::template distance_to_out<R>(s, t.p, t.d, inf_length<R>())
+              : S::template distance_to_in<R>(s, t.p, t.d, inf_length<R>());
+  if (db < R(0.0)) db = R(0.0);
+  if (l < db) {
+    t.carry = R(0.0);
+    t.p.x = t.p.x + l * t.d.x;
+    t.p.y = t.p.y + l * t.d.y;
+    t.p.z = t.p.z + l * t.d.z;
+    return false;
+  }
+  if (l <= db + R(1e-6)) {
+    if constexpr (S::kSafetyIsDistance) {
+      const R saf = S::template surface_distance<R>(s, t.p, t.in);
+      if (l <= saf && S::template contains<R>(s, t.p) == t.in) {
+        t.carry = saf - l - carry_margin(t.p);
+        t.p.x = t.p.x + l * t.d.x;
+        t.p.y = t.p.y + l * t.d.y;
+        t.p.z = t.p.z + l * t.d.z;
+        return false;
+      }
+    } else {
+      const R saf = t.in ? S::template safety_to_out<R>(s, t.p) : S::template safety_to_in<R>(s, t.p);
+      if (l <= saf) {
+        t.p.x = t.p.x + l * t.d.x;
+        t.p.y = t.p.y + l * t.d.y;
+        t.p.z = t.p.z + l * t.d.z;
+        return false;
+      }
+    }
+  }
+  t.carry = R(0.0);
+  const R m = db + R(kBoundaryPush);
+  t.p.x = t.p.x + m * t.d.x;
+  t.p.y = t.p.y + m * t.d.y;
+  t.p.z = t.p.z + m * t.d.z;
+  t.in = !t.in;
+  t.crossings++;
+  return true;
+}
+
 // The FP64 track-step of the batched transport kernel, on scalar type R.
 template <class S, typename R>
 bool step(const typename S::Data &s, Track<R> &t, R l)
@@ -136,7 +181,7 @@ V3<double> dbl(const V3<R> &a)
 // returns the final position.
 template <class S, typename R>
 V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lambda, uint64_t seed, uint64_t i,
-                     const double *p0, const double *d0, int nsteps, bool &in_out)
+                     const double *p0, const double *d0, int nsteps, bool distance_first, bool &in_out)
 {
   double wc[3], wh[3];
   for (int k = 0; k < 3; ++k) {
@@ -152,7 +197,10 @@ V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lamb
   for (int k = 0; k < nsteps; ++k) {
     const uint32_t st = uint32_t(k);
     const double l = physics_step(lambda, seed, i, st);
-    const bool crossed = step<S, R>(s, t, R(l));
+    // a step under the carried bound is the same in both orders (step()
+    // takes it first)
+    const bool carried = S::kSafetyIsDistance && R(l) <= t.carry;
+    const bool crossed = (distance_first && !carried) ? step_distance_first<S, R>(s, t, R(l)) : step<S, R>(s, t, R(l));
     if (!crossed) {
       const V3<double> d = new_direction(seed, i, st, dbl(t.d));
       t.d = V3<R>{R(d.x), R(d.y), R(d.z)};
@@ -179,6 +227,7 @@ struct Counter {
   size_t n;
   const double *const *soa;  // x, y, z, dx, dy, dz
   int nsteps;
+  bool distance_first;
   uint64_t flops = 0;
   size_t mismatches = 0;
 
@@ -190,9 +239,10 @@ struct Counter {
       const double p0[3] = {soa[0][i], soa[1][i], soa[2][i]};
       const double d0[3] = {soa[3][i], soa[4][i], soa[5][i]};
       bool in_d = false, in_c = false;
-      const V3<double> a = run_track<S, double>(s, rec, lambda, seed, first + i, p0, d0, nsteps, in_d);
+      const bool df = distance_first && S::kSafetyBelowDistance;
+      const V3<double> a = run_track<S, double>(s, rec, lambda, seed, first + i, p0, d0, nsteps, false, in_d);
       const uint64_t f0 = g_flops;
-      const V3<double> b = run_track<S, Counted>(s, rec, lambda, seed, first + i, p0, d0, nsteps, in_c);
+      const V3<double> b = run_track<S, Counted>(s, rec, lambda, seed, first + i, p0, d0, nsteps, df, in_c);
       flops += g_flops - f0;
       if (std::memcmp(&a, &b, sizeof a) != 0 || in_d != in_c) ++mismatches;
     }
@@ -253,17 +303,19 @@ int cf_extent(cf_geom *h, int shape, double *mn, double *mx)
   return 0;
 }
 
-// FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape`; sample
-// track i draws its random numbers as track first + i of the transport.
-// Returns 0, or -1 for a bad shape, or -2 when the counted run left the FP64
-// trajectory of some track.
-int cf_count(cf_geom *h, int shape, double lambda, uint64_t seed, uint64_t first, size_t n, const double *x, const double *y,
-             const double *z, const double *dx, const double *dy, const double *dz, int nsteps, uint64_t *flops)
+// FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape` (the
+// distance-first step when asked and the family allows it); sample track i
+// draws its random numbers as track first + i of the transport.  Returns 0,
+// or -1 for a bad shape, or -2 when the counted run left the FP64 trajectory
+// of some track.
+int cf_count(cf_geom *h, int shape, double lambda, uint64_t seed, uint64_t first, size_t n, const double *x,
+             const double *y, const double *z, const double *dx, const double *dy, const double *dz, int nsteps,
+             int distance_first, uint64_t *flops)
 {
   if (shape < 0 || size_t(shape) >= h->g.shapes.size()) return -1;
   const double *soa[6] = {x, y, z, dx, dy, dz};
   const ShapeRec &rec = h->g.shapes[size_t(shape)];
-  Counter c{h->g, rec, lambda, seed, first, n, soa, nsteps};
+  Counter c{h->g, rec, lambda, seed, first, n, soa, nsteps, distance_first != 0};
   if (!dispatch_shape_type(rec.type, c)) return -1;
   *flops = c.flops;
   return c.mismatches ? -2 : 0;

@@ -50,7 +50,8 @@ class FlopGeometry:
         L.cf_tessellated.argtypes = [ctypes.c_void_p, ctypes.c_int, _P]
         L.cf_extent.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _P]
         L.cf_count.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
-                               ctypes.c_size_t] + [_P] * 6 + [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+                               ctypes.c_size_t] + [_P] * 6 + [ctypes.c_int, ctypes.c_int,
+                                                              ctypes.POINTER(ctypes.c_uint64)]
         self.L = L
         self.h = ctypes.c_void_p(L.cf_create())
 
@@ -89,11 +90,11 @@ class FlopGeometry:
         self._chk(self.L.cf_extent(self.h, s, _p(mn), _p(mx)))
         return mn, mx
 
-    def count(self, shape, lam, seed, first, soa, nsteps):
+    def count(self, shape, lam, seed, first, soa, nsteps, distance_first=False):
         out = ctypes.c_uint64(0)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in soa]
         rc = self.L.cf_count(self.h, shape, lam, seed, first, len(arrs[0]), *[a.ctypes.data_as(_P) for a in arrs],
-                             nsteps, ctypes.byref(out))
+                             nsteps, 1 if distance_first else 0, ctypes.byref(out))
         if rc == -2:
             raise RuntimeError("counted run left the FP64 trajectory")
         self._chk(rc)
@@ -110,7 +111,7 @@ def main():
     g = FlopGeometry()
     specs, counts = workloads.batched_solids(args.tracks, per_family=8, seed=42)
     offsets = np.concatenate([[0], np.cumsum(counts)])
-    per_solid = []
+    per_solid, per_solid_df = [], []
     t0 = time.time()
     for j, ((kind, a), c) in enumerate(zip(specs, counts)):
         s = workloads.make_shape(g, kind, a)
@@ -118,24 +119,35 @@ def main():
         pts, dirs = workloads.tracks_for(ext, int(c), 42, j)
         k = min(int(c), args.sample)
         soa = [pts[0][:k], pts[1][:k], pts[2][:k], dirs[0][:k], dirs[1][:k], dirs[2][:k]]
-        f = g.count(s, workloads.mean_step(ext), 7, int(offsets[j]), soa, args.track_steps)
+        lam = workloads.mean_step(ext)
+        f = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps)
+        fd = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps, distance_first=True)
         per_solid.append(f / (k * args.track_steps))
-        print(f"{j:2d} {kind:12s} {per_solid[-1]:10.1f} flops/track-step ({time.time() - t0:.0f}s)", flush=True)
-    per_family = {}
-    for (kind, _), c, f in zip(specs, counts, per_solid):
-        d = per_family.setdefault(kind, [0.0, 0])
-        d[0] += f * int(c)
-        d[1] += int(c)
+        per_solid_df.append(fd / (k * args.track_steps))
+        print(f"{j:2d} {kind:12s} {per_solid[-1]:10.1f} flops/track-step, distance-first {per_solid_df[-1]:10.1f} "
+              f"({time.time() - t0:.0f}s)", flush=True)
+    def by_family(vals):
+        acc = {}
+        for (kind, _), c, f in zip(specs, counts, vals):
+            d = acc.setdefault(kind, [0.0, 0])
+            d[0] += f * int(c)
+            d[1] += int(c)
+        return {k: v[0] / v[1] for k, v in acc.items()}
+
     total = sum(f * int(c) for f, c in zip(per_solid, counts)) / float(sum(counts))
     out = {
         "description": "algorithmic FP64 flops (+ - * / sqrt) per track-step of the batched transport, "
-                       "steps 0..track_steps-1 of the first `sample` tracks of each solid",
+                       "steps 0..track_steps-1 of the first `sample` tracks of each solid; *_distance_first: the "
+                       "distance-first step (same decisions; families without kSafetyBelowDistance count "
+                       "the FP64 step)",
         "workload": {"tracks": int(sum(counts)), "solids": len(specs), "seed": 42, "transport_seed": 7,
                      "track_steps": args.track_steps, "sample": args.sample},
         "families": [kind for kind, _ in specs],
         "counts": [int(c) for c in counts],
         "per_solid": per_solid,
-        "per_family": {k: v[0] / v[1] for k, v in per_family.items()},
+        "per_family": by_family(per_solid),
+        "per_solid_distance_first": per_solid_df,
+        "per_family_distance_first": by_family(per_solid_df),
         "fp64_flops_per_track_step": total,
     }
     with open(args.out, "w") as fh:
