@@ -232,6 +232,10 @@ int close_navigation(HostGeometry &g, int world_pv, std::string &err)
     std::iota(T.prim.begin(), T.prim.end(), 0);
     const int half = n / 2;
     const int depth = std::min(std::max(0, half > 0 ? int(std::log2(double(half))) : 0), 32);
+    if (depth > 14) {  // the device walk keeps 16-bit node indices (at most 2^15 - 1 nodes)
+      err = "too many daughters in one volume for the BVH walk";
+      return -1;
+    }
     const unsigned nn = (2u << depth) - 1;
     T.nchild.assign(nn, 0);
     T.offset.assign(nn, -1);
@@ -289,6 +293,9 @@ int close_navigation(HostGeometry &g, int world_pv, std::string &err)
     return -1;
   }
   g.world_pv = world_pv;
+  g.nav_shape_mask = 0;
+  for (size_t k = 1; k < g.nav_nodes.size(); ++k)
+    g.nav_shape_mask |= 1u << g.shapes[g.lvs[g.pvs[g.nav_nodes[k].pv].lv].shape].type;
   ++g.version;
   return int(g.nav_nodes.size()) - 1;
 }

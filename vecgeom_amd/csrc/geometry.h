@@ -24,6 +24,7 @@ struct HostGeometry {
   std::vector<BvhRec> bvhs;
   std::vector<int32_t> nav_ipool;
   std::vector<double> nav_bpool;
+  uint32_t nav_shape_mask = 0;  // shape families of the navigation tree (bit per ShapeType)
   uint64_t version = 1;  // bumped on every change; the device copy follows it
 
   // Appends a parameter block of `ndoubles` and registers a shape; returns its id.

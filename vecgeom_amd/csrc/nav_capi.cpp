@@ -31,6 +31,7 @@ int run_nav(vgm_ctx *c, int kind, NavArgs &a, std::initializer_list<Arr> arrs, u
   const DeviceGeometry &d = c->dev;
   a.g = GeomView{d.shapes, d.pool, d.lvs, d.daughters, d.pvs, d.nav_nodes, d.nav_children, d.bvhs,
                  d.nav_ipool, d.nav_bpool, c->host.world_pv};
+  a.shape_mask = c->host.nav_shape_mask;
   if (a.n == 0) return VGM_OK;
   if (flags & VGM_DEVICE_PTRS) {
     for (const Arr &r : arrs) *r.slot = const_cast<void *>(r.user);

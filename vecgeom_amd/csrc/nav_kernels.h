@@ -21,6 +21,7 @@ struct NavArgs {
   int32_t *steps;  // transport: steps taken (accumulated)
   size_t n;
   int32_t nsteps;
+  uint32_t shape_mask;  // families used by the navigation tree (bit per ShapeType)
 };
 
 hipError_t launch_nav(hipStream_t stream, int kind, const NavArgs &a);

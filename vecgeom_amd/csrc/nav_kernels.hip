@@ -13,17 +13,26 @@
 namespace vgm {
 
 constexpr int kNavBlock = 128;
+static_assert(kNavBlock <= kNavWalkThreads, "walk stack columns");
 
+// Kernel variants by the set of shape families in the geometry: the barrel
+// of tubes and boxes needs neither the polygon nor the facet code.
+constexpr unsigned kMaskBoxTube = kShapeBit(kBox) | kShapeBit(kTube);
+constexpr unsigned kMaskSimple =
+    kShapeBit(kBox) | kShapeBit(kTube) | kShapeBit(kCone) | kShapeBit(kTrd) | kShapeBit(kTrapezoid);
+
+template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_locate_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
-  Navigator nav{a.g};
+  Navigator<Mask> nav{a.g};
   a.state[i] = nav.locate_global(V3<double>{a.x[i], a.y[i], a.z[i]});
   if (a.last) a.last[i] = 0;
   if (a.boundary) a.boundary[i] = 0;
 }
 
+template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_step_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
@@ -32,7 +41,7 @@ __global__ __launch_bounds__(kNavBlock) void nav_step_kernel(NavArgs a)
     a.step[i] = 0.0;
     return;
   }
-  Navigator nav{a.g};
+  Navigator<Mask> nav{a.g};
   const NavState in{a.state[i], a.last[i], a.boundary[i] != 0};
   NavState out;
   a.step[i] = nav.step(V3<double>{a.x[i], a.y[i], a.z[i]}, V3<double>{a.dx[i], a.dy[i], a.dz[i]},
@@ -42,11 +51,12 @@ __global__ __launch_bounds__(kNavBlock) void nav_step_kernel(NavArgs a)
   a.boundary[i] = out.boundary ? 1 : 0;
 }
 
+template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_transport_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
-  Navigator nav{a.g};
+  Navigator<Mask> nav{a.g};
   NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
   V3<double> p{a.x[i], a.y[i], a.z[i]};
   const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
@@ -68,25 +78,44 @@ __global__ __launch_bounds__(kNavBlock) void nav_transport_kernel(NavArgs a)
   a.steps[i] += k;
 }
 
+template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_safety_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
-  Navigator nav{a.g};
+  Navigator<Mask> nav{a.g};
   a.step[i] = nav.safety(V3<double>{a.x[i], a.y[i], a.z[i]}, a.state[i]);
 }
+
+namespace {
+template <unsigned Mask>
+void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned blocks, bool &ok)
+{
+  ok = true;
+  switch (kind) {
+  case kNavLocate: hipLaunchKernelGGL(nav_locate_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavStep: hipLaunchKernelGGL(nav_step_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavTransport:
+    hipLaunchKernelGGL(nav_transport_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    break;
+  case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  default: ok = false;
+  }
+}
+}  // namespace
 
 hipError_t launch_nav(hipStream_t stream, int kind, const NavArgs &a)
 {
   if (a.n == 0) return hipSuccess;
   const unsigned blocks = unsigned((a.n + kNavBlock - 1) / kNavBlock);
-  switch (kind) {
-  case kNavLocate: hipLaunchKernelGGL(nav_locate_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
-  case kNavStep: hipLaunchKernelGGL(nav_step_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
-  case kNavTransport: hipLaunchKernelGGL(nav_transport_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
-  case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
-  default: return hipErrorInvalidValue;
-  }
+  bool ok = false;
+  if ((a.shape_mask & ~kMaskBoxTube) == 0)
+    launch_nav_mask<kMaskBoxTube>(stream, kind, a, blocks, ok);
+  else if ((a.shape_mask & ~kMaskSimple) == 0)
+    launch_nav_mask<kMaskSimple>(stream, kind, a, blocks, ok);
+  else
+    launch_nav_mask<kAllShapes>(stream, kind, a, blocks, ok);
+  if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

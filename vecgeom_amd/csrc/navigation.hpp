@@ -12,8 +12,9 @@
 //   Safety: BVHSafetyEstimator::ComputeSafetyForLocalPoint
 //     (VecGeom/navigation/BVHSafetyEstimator.h:39-47) with BVH::ComputeSafety
 //     (VecGeom/base/BVH.h:172-209).
-// Shapes are reached through one out-of-line dispatcher per query, so the
-// navigation code is compiled once rather than once per shape family.
+// Shapes are reached through a dispatcher over the families the geometry
+// uses (a template mask): inlined for small sets such as boxes and tubes,
+// one out-of-line function for the set of all families.
 #pragma once
 #include "vgm_math.hpp"
 #include "transform.hpp"
@@ -58,15 +59,57 @@ struct ShapeCall {
   }
 };
 
-static __host__ __device__ __noinline__ double shape_query(const ShapeRec *shapes, const double *pool, int shape, int query,
-                                           V3<double> p, V3<double> d, double step)
+template <unsigned Mask>
+__host__ __device__ __forceinline__ double shape_query_inline(const ShapeRec *shapes, const double *pool, int shape,
+                                                              int query, V3<double> p, V3<double> d, double step)
 {
   const ShapeRec &r = shapes[shape];
   ShapeCall c{pool + r.off, query, p, d, step, 0.0};
-  dispatch_shape_type(r.type, c);
+  dispatch_shape_mask<Mask>(r.type, c);
   return c.result;
 }
 
+static __host__ __device__ __noinline__ double shape_query_all(const ShapeRec *shapes, const double *pool, int shape,
+                                                               int query, V3<double> p, V3<double> d, double step)
+{
+  return shape_query_inline<kAllShapes>(shapes, pool, shape, query, p, d, step);
+}
+
+template <unsigned Mask>
+__host__ __device__ __forceinline__ double shape_query(const ShapeRec *shapes, const double *pool, int shape,
+                                                       int query, V3<double> p, V3<double> d, double step)
+{
+  if constexpr (Mask == kAllShapes)
+    return shape_query_all(shapes, pool, shape, query, p, d, step);
+  else
+    return shape_query_inline<Mask>(shapes, pool, shape, query, p, d, step);
+}
+
+// BVH walk stack.  On the device it lives in LDS, one column of 16-bit node
+// indices per thread of the workgroup (a private array is kept in registers
+// with indexed moves, 32 of them per thread); on the host it is a local
+// array.  BVHs have at most 2^15 - 1 nodes (checked when they are built) and
+// a walk holds at most depth + 1 < kNavWalkDepth pending nodes.  The walks
+// (daughter intersections, level locate, daughter safety) never nest.
+constexpr int kNavWalkDepth = 32;
+constexpr int kNavWalkThreads = 128;  // workgroup of the navigation kernels
+#if defined(__HIP_DEVICE_COMPILE__)
+static __shared__ uint16_t nav_walk_lds[kNavWalkDepth * kNavWalkThreads];
+struct NavWalkStack {
+  uint16_t *col;
+  __device__ NavWalkStack() : col(nav_walk_lds + threadIdx.x) {}
+  __device__ void set(int i, unsigned v) { col[i * kNavWalkThreads] = uint16_t(v); }
+  __device__ unsigned get(int i) const { return col[i * kNavWalkThreads]; }
+};
+#else
+struct NavWalkStack {
+  unsigned a[kNavWalkDepth];
+  void set(int i, unsigned v) { a[i] = v; }
+  unsigned get(int i) const { return a[i]; }
+};
+#endif
+
+template <unsigned Mask = kAllShapes>
 struct Navigator {
   const GeomView &g;
 
@@ -116,11 +159,11 @@ struct Navigator {
     const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
     const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
     const V3<double> inv{1.0 / nonzero(d.x), 1.0 / nonzero(d.y), 1.0 / nonzero(d.z)};
-    unsigned stack[32];
+    NavWalkStack stack;
     int sp = 0;
-    stack[sp++] = 0;
+    stack.set(sp++, 0);
     do {
-      const unsigned id = stack[--sp];
+      const unsigned id = stack.get(--sp);
       const int nc = nchild[id];
       if (nc >= 0) {
         for (int i = 0; i < nc; ++i) {
@@ -130,7 +173,7 @@ struct Navigator {
           if (tmin <= tmax && tmax >= 0.0 && tmin < step) {
             const int v = g.daughters[L.dbegin + k];
             const Transform &T = g.pvs[v].tr;
-            const double dist = shape_query(g.shapes, g.pool, shape_of_pv(v), kQDistIn, to_local_point(T, p),
+            const double dist = shape_query<Mask>(g.shapes, g.pool, shape_of_pv(v), kQDistIn, to_local_point(T, p),
                                             to_local_dir(T, d), step);
             if (dist < step && !(dist <= 0.0 && v == last_pv)) {
               step = dist;
@@ -146,11 +189,11 @@ struct Navigator {
         const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
         const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
         if (tminR < tminL) {
-          if (tr) stack[sp++] = cr;
-          if (tl) stack[sp++] = cl;
+          if (tr) stack.set(sp++, cr);
+          if (tl) stack.set(sp++, cl);
         } else {
-          if (tl) stack[sp++] = cl;
-          if (tr) stack[sp++] = cr;
+          if (tl) stack.set(sp++, cl);
+          if (tr) stack.set(sp++, cr);
         }
       }
     } while (sp > 0);
@@ -165,11 +208,11 @@ struct Navigator {
     const BvhRec &B = g.bvhs[L.bvh];
     const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
     const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
-    unsigned stack[32];
+    NavWalkStack stack;
     int sp = 0;
-    stack[sp++] = 0;
+    stack.set(sp++, 0);
     do {
-      const unsigned id = stack[--sp];
+      const unsigned id = stack.get(--sp);
       const int nc = nchild[id];
       if (nc >= 0) {
         for (int i = 0; i < nc; ++i) {
@@ -178,15 +221,15 @@ struct Navigator {
           const int v = g.daughters[L.dbegin + k];
           if (v == excl_pv) continue;
           const V3<double> q = to_local_point(g.pvs[v].tr, p);
-          if (shape_query(g.shapes, g.pool, shape_of_pv(v), kQContains, q, q, 0.0) != 0.0) {
+          if (shape_query<Mask>(g.shapes, g.pool, shape_of_pv(v), kQContains, q, q, 0.0) != 0.0) {
             dp = q;
             return k;
           }
         }
       } else {
         const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
-        if (box_contains(nbox + 6 * cl, p)) stack[sp++] = cl;
-        if (box_contains(nbox + 6 * cr, p)) stack[sp++] = cr;
+        if (box_contains(nbox + 6 * cl, p)) stack.set(sp++, cl);
+        if (box_contains(nbox + 6 * cr, p)) stack.set(sp++, cr);
       }
     } while (sp > 0);
     return -1;
@@ -207,7 +250,7 @@ struct Navigator {
 
   __host__ __device__ unsigned locate_global(V3<double> gp) const
   {
-    if (shape_query(g.shapes, g.pool, shape_of_pv(g.world_pv), kQContains, gp, gp, 0.0) == 0.0) return 0;
+    if (shape_query<Mask>(g.shapes, g.pool, shape_of_pv(g.world_pv), kQContains, gp, gp, 0.0) == 0.0) return 0;
     return descend(1, gp, -1);
   }
 
@@ -218,7 +261,7 @@ struct Navigator {
     const V3<double> lp = to_local_point(N.global, gp), ld = to_local_dir(N.global, gd);
     const int pv = N.pv;
     const int lv = g.pvs[pv].lv;
-    double st = shape_query(g.shapes, g.pool, g.lvs[lv].shape, kQDistOut, lp, ld, step_limit);
+    double st = shape_query<Mask>(g.shapes, g.pool, g.lvs[lv].shape, kQDistOut, lp, ld, step_limit);
     if (st < 0.) st = 0.;
     int hit = -1;
     const int last_pv = in.last ? g.nodes[in.last].pv : -1;
@@ -248,7 +291,7 @@ struct Navigator {
       while (cur) {
         const int cpv = g.nodes[cur].pv;
         if (cur == entry ||
-            shape_query(g.shapes, g.pool, shape_of_pv(cpv), kQContains, q, q, 0.0) == 0.0) {
+            shape_query<Mask>(g.shapes, g.pool, shape_of_pv(cpv), kQContains, q, q, 0.0) == 0.0) {
           q = to_master_point(g.pvs[cpv].tr, q);
           cur = unsigned(g.nodes[cur].mother);
         } else {
@@ -270,11 +313,11 @@ struct Navigator {
     const BvhRec &B = g.bvhs[L.bvh];
     const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
     const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
-    unsigned stack[32];
+    NavWalkStack stack;
     int sp = 0;
-    stack[sp++] = 0;
+    stack.set(sp++, 0);
     do {
-      const unsigned id = stack[--sp];
+      const unsigned id = stack.get(--sp);
       const int nc = nchild[id];
       if (nc >= 0) {
         for (int i = 0; i < nc; ++i) {
@@ -282,7 +325,7 @@ struct Navigator {
           if (box_safety(pbox + 6 * k, p) < safety) {
             const int v = g.daughters[L.dbegin + k];
             const double dist =
-                shape_query(g.shapes, g.pool, shape_of_pv(v), kQSafIn, to_local_point(g.pvs[v].tr, p), p, 0.0);
+                shape_query<Mask>(g.shapes, g.pool, shape_of_pv(v), kQSafIn, to_local_point(g.pvs[v].tr, p), p, 0.0);
             if (dist < safety) safety = dist;
           }
         }
@@ -291,11 +334,11 @@ struct Navigator {
         const double sL = box_safety(nbox + 6 * cl, p), sR = box_safety(nbox + 6 * cr, p);
         const bool tl = sL < safety, tr = sR < safety;
         if (sR < sL) {
-          if (tr) stack[sp++] = cr;
-          if (tl) stack[sp++] = cl;
+          if (tr) stack.set(sp++, cr);
+          if (tl) stack.set(sp++, cl);
         } else {
-          if (tl) stack[sp++] = cl;
-          if (tr) stack[sp++] = cr;
+          if (tl) stack.set(sp++, cl);
+          if (tr) stack.set(sp++, cr);
         }
       }
     } while (sp > 0);
@@ -308,7 +351,7 @@ struct Navigator {
     const NavNode &N = g.nodes[node];
     const V3<double> lp = to_local_point(N.global, gp);
     const int lv = g.pvs[N.pv].lv;
-    double s = shape_query(g.shapes, g.pool, g.lvs[lv].shape, kQSafOut, lp, lp, 0.0);
+    double s = shape_query<Mask>(g.shapes, g.pool, g.lvs[lv].shape, kQSafOut, lp, lp, 0.0);
     if (s > 0.0 && g.lvs[lv].bvh >= 0) s = daughter_safety(lv, lp, s);
     return s;
   }

@@ -47,4 +47,44 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
   }
 }
 
+// Shape-family sets, one bit per ShapeType.
+constexpr unsigned kShapeBit(int type) { return 1u << type; }
+constexpr unsigned kAllShapes = (1u << kNumShapeTypes) - 1u;
+
+// dispatch_shape_type restricted to the families of Mask: only their code is
+// instantiated; a type outside the set returns false without a call.
+template <unsigned Mask, class F>
+__host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
+{
+  if (type < 0 || type >= kNumShapeTypes || !(Mask & (1u << type))) return false;
+  switch (type) {
+  case kBox:
+    if constexpr ((Mask & (1u << kBox)) != 0) f.template operator()<BoxShape>();
+    return true;
+  case kTube:
+    if constexpr ((Mask & (1u << kTube)) != 0) f.template operator()<TubeShape>();
+    return true;
+  case kCone:
+    if constexpr ((Mask & (1u << kCone)) != 0) f.template operator()<ConeShape>();
+    return true;
+  case kTrd:
+    if constexpr ((Mask & (1u << kTrd)) != 0) f.template operator()<TrdShape>();
+    return true;
+  case kTrapezoid:
+    if constexpr ((Mask & (1u << kTrapezoid)) != 0) f.template operator()<TrapezoidShape>();
+    return true;
+  case kPolycone:
+    if constexpr ((Mask & (1u << kPolycone)) != 0) f.template operator()<PolyconeShape>();
+    return true;
+  case kPolyhedron:
+    if constexpr ((Mask & (1u << kPolyhedron)) != 0) f.template operator()<PolyhedronShape>();
+    return true;
+  case kTessellated:
+    if constexpr ((Mask & (1u << kTessellated)) != 0) f.template operator()<TessellatedShape>();
+    return true;
+  default:
+    return false;
+  }
+}
+
 }  // namespace vgm
