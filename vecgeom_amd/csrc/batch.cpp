@@ -13,6 +13,10 @@ using namespace vgm;
 // Families run concurrently on this many streams of their own, forked from
 // and joined back into the context stream.
 constexpr int kFamilyStreams = 4;
+// Device counters: one tile counter per family, one for classify, then two
+// fallback-queue counters per family.
+constexpr int kNumCounters = kNumShapeTypes + 1 + 2 * kNumShapeTypes;
+static_assert(kNumCounters * sizeof(int) <= 256, "counters fit their 256-byte slot");
 
 struct vgm_batch {
   vgm_ctx *ctx = nullptr;
@@ -41,6 +45,11 @@ struct vgm_batch {
   MlpDesc *d_mlps = nullptr;
   unsigned long long *d_learned = nullptr;
   int fallback_pct = 50;
+  // learned families: per-track carried bound and flags, fallback queue
+  char *lbuf = nullptr;
+  double *d_tcarry = nullptr;
+  uint8_t *d_tflags = nullptr;
+  uint64_t *d_queue = nullptr;
   int fam_mode[kNumShapeTypes];  // FamilyMode chosen by the autotuner (VGM_TUNED), -1 if none
   vgm_batch()
   {
@@ -64,7 +73,8 @@ void build_tiles(vgm_batch *b)
   for (int s : order) {
     const SolidRec &r = b->solids[s];
     if (r.end > r.begin && (b->fams.empty() || b->fams.back().type != r.type))
-      b->fams.push_back(FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr, 0, kFamFP64, false});
+      b->fams.push_back(
+          FamilyLaunch{r.type, int(b->tiles.size()), int(b->tiles.size()), 0, nullptr, 0, kFamFP64, false, 0});
     if (r.end > r.begin && r.mlp >= 0) b->fams.back().has_mlp = true;
     if (r.end > r.begin) b->fams.back().stage_nd = std::max(b->fams.back().stage_nd, r.nd);
     for (int64_t t = r.begin; t < r.end; t += tt) {
@@ -76,6 +86,11 @@ void build_tiles(vgm_batch *b)
       b->tiles.push_back(tl);
     }
     if (!b->fams.empty()) b->fams.back().t1 = int(b->tiles.size());
+  }
+  int64_t off = 0;  // queue regions: each family's track count, in family order
+  for (FamilyLaunch &f : b->fams) {
+    f.queue_off = off;
+    for (int t = f.t0; t < f.t1; ++t) off += b->tiles[size_t(t)].end - b->tiles[size_t(t)].begin;
   }
 }
 
@@ -125,6 +140,14 @@ int upload(vgm_batch *b, bool learned = false)
   int rc = sync_geometry(c);
   if (rc) return rc;
   assign_networks(b, learned);
+  if (learned && !b->lbuf && !b->descs.empty()) {
+    const size_t n = size_t(std::max<int64_t>(1, b->ntracks));
+    const size_t s1 = align256(n * sizeof(double)), s2 = align256(n), s3 = align256(n * sizeof(uint64_t));
+    VGM_HIP(hipMalloc(&b->lbuf, s1 + s2 + s3));
+    b->d_tcarry = reinterpret_cast<double *>(b->lbuf);
+    b->d_tflags = reinterpret_cast<uint8_t *>(b->lbuf + s1);
+    b->d_queue = reinterpret_cast<uint64_t *>(b->lbuf + s1 + s2);
+  }
   if (!b->tiles_dirty && b->dbuf) return VGM_OK;
   build_tiles(b);
   const size_t s1 = align256(b->solids.size() * sizeof(SolidRec) + 1);
@@ -179,6 +202,11 @@ TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *d
   a.audit = (flags & VGM_AUDIT) ? 1 : 0;
   a.fallback_pct = b->fallback_pct;
   a.distance_first = (flags & VGM_DISTANCE_FIRST) ? 1 : 0;
+  a.fallback_grid = 8 * std::max(1, b->ctx->num_cus);
+  a.tcarry = b->d_tcarry;
+  a.tflags = b->d_tflags;
+  a.queue = b->d_queue;
+  a.qcount = b->d_counters + kNumShapeTypes + 1;
   return a;
 }
 
@@ -228,7 +256,7 @@ hipError_t launch_families(vgm_batch *b, unsigned flags, const TransportArgs &a,
     }
     return hipSuccess;
   };
-  hipError_t e = launch_reset(main, b->d_counters, kNumShapeTypes + 1, a.fp64_steps, a.learned);
+  hipError_t e = launch_reset(main, b->d_counters, kNumCounters, a.fp64_steps, a.learned);
   if (e != hipSuccess) return e;
   double wsum = 0;
   for (const FamilyLaunch &f : b->fams) wsum += b->weight[f.type];
@@ -316,6 +344,7 @@ int vgm_batch_destroy(vgm_batch *b)
   if (!b) return VGM_OK;
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->dbuf) (void)hipFree(b->dbuf);
+  if (b->lbuf) (void)hipFree(b->lbuf);
   for (int i = 0; i < kFamilyStreams; ++i) {
     if (b->fam_stream[i]) (void)hipStreamDestroy(b->fam_stream[i]);
     if (b->join[i]) (void)hipEventDestroy(b->join[i]);
@@ -414,7 +443,7 @@ int vgm_batch_classify(vgm_batch *b, const double *x, const double *y, const dou
   if (rc) return rc;
   TransportArgs a = make_args(b, const_cast<double *>(x), const_cast<double *>(y), const_cast<double *>(z), nullptr,
                               nullptr, nullptr, side, nullptr, 0, 0, 0);
-  VGM_HIP(launch_reset(b->ctx->stream, b->d_counters, kNumShapeTypes + 1, nullptr, nullptr));
+  VGM_HIP(launch_reset(b->ctx->stream, b->d_counters, kNumCounters, nullptr, nullptr));
   const int grid = int(std::min<size_t>(b->tiles.size(), size_t(4 * std::max(1, b->ctx->num_cus))));
   VGM_HIP(launch_classify(b->ctx->stream, a, std::max(1, grid), 256));
   VGM_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -522,7 +551,7 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
       TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1, flags);
       float ms = 0;
       auto one = [&]() {
-        hipError_t e = launch_reset(c->stream, b->d_counters, kNumShapeTypes + 1, nullptr, a.learned);
+        hipError_t e = launch_reset(c->stream, b->d_counters, kNumCounters, nullptr, a.learned);
         return e != hipSuccess ? e : launch_family(c->stream, a, f, 256);
       };
       if ((rc = timed(one, ms))) goto done;

@@ -70,13 +70,14 @@ __device__ __forceinline__ void mlp_wave_sync()
 }
 
 // acc = bias + W[32mb.., :] * input, input blocks from `in` (registers, NK
-// blocks) or from the activation buffer (NK = 8).
+// blocks) or from the activation buffer (NK = 8).  One accumulation chain:
+// the waves of the workgroup overlap each other's matrix-instruction latency.
 template <int NK, int ROW, bool FromLds>
 __device__ __forceinline__ mlp_f32x16 mlp_block(const __bf16 (*w)[ROW], const float *b, const mlp_bf16x8 *in,
                                                 const mlp_act_rows &act, int mb, int lane)
 {
   const int r32 = lane & 31, h = lane >> 5;
-  mlp_f32x16 acc0 = mlp_bias_acc(b, mb, h), acc1 = {};
+  mlp_f32x16 acc = mlp_bias_acc(b, mb, h);
 #pragma unroll
   for (int kb = 0; kb < NK; kb += 2) {
     const mlp_bf16x8 a0 = *reinterpret_cast<const mlp_bf16x8 *>(&w[32 * mb + r32][16 * kb + 8 * h]);
@@ -89,11 +90,11 @@ __device__ __forceinline__ mlp_f32x16 mlp_block(const __bf16 (*w)[ROW], const fl
       b0 = in[kb];
       b1 = in[kb + 1];
     }
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc1, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);  // operands of one pair of blocks in flight at a time
   }
-  return acc0 + acc1;
+  return acc;
 }
 
 typedef __bf16 mlp_bf16x4 __attribute__((ext_vector_type(4)));

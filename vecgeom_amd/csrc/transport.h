@@ -53,7 +53,14 @@ struct TransportArgs {
   int32_t audit;
   int32_t fallback_pct;  // a wave runs its fallbacks once this % of its live tracks wait
   int32_t distance_first;  // FP64 steps of surface-distance families: boundary distance first
-  int32_t pad2;
+  int32_t fallback_grid;   // workgroups of the learned fallback kernel
+  // learned families, between their per-step kernels: carried bound and
+  // flags per track (indexed like x), and the queue of exact fallbacks
+  // (entries solid << 40 | track) with its two counters (by step parity)
+  double *tcarry;
+  uint8_t *tflags;
+  uint64_t *queue;
+  int *qcount;
 };
 
 enum LearnedCounter : int { kLearnedQueries = 0, kLearnedServed = 1, kLearnedFallbacks = 2, kLearnedViolations = 3 };
@@ -80,6 +87,7 @@ struct FamilyLaunch {
   int stage_nd;  // > 0: stage parameter blocks of up to this many doubles in LDS
   int mode;      // FamilyMode of this launch
   bool has_mlp;  // some solid of the family has a learned-safety network
+  int64_t queue_off;  // this family's region of the fallback queue (its track count long)
 };
 
 // How a family's track-steps are computed.  All but kFamMixed take the FP64

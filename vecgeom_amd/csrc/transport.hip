@@ -156,7 +156,7 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
 // l > d_b + kDistanceFirstWindow; only in between is the safety (with its
 // side check) computed.  Same decisions and arithmetic as step_fp64; only
 // the carried bound differs.  `verified` is set when the side check ran and
-// agreed (see run_tile_learned).  Launches of other families ignore the
+// agreed (see learned_bound_kernel).  Launches of other families ignore the
 // distance-first request.
 constexpr double kDistanceFirstWindow = 1e-6;
 
@@ -272,121 +272,6 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
   }
 }
 
-struct LearnedCounts {
-  uint32_t q = 0, served = 0, fallbacks = 0, violations = 0;
-};
-
-// Tracks of one tile with the learned bound.  The lanes of a wave hold 64
-// tracks and move through three kinds of work:
-//   carried steps  l <= carry: no query (each lane runs its own)
-//   bound          the network, for every lane at once, on the matrix cores;
-//                  a lane whose step fits under max(carry, bound) moves,
-//                  the others wait for the exact step
-//   exact step     step_distance_first, run together by the waiting lanes
-//                  once they are fallback_pct % of the live lanes, or when
-//                  no lane can go on without it.
-// A bound below the exact safety moves exactly where the FP64 step moves, so
-// trajectories are those of the FP64 path; audit runs count any bound above
-// the exact surface distance.
-//
-// The exact safety is -1 when the point is not on the side the track's flag
-// says (SafetyToOut of an outside point), which happens after the periodic
-// wrap puts a track down inside the solid, or right after a crossing; the
-// FP64 step then takes the boundary-distance path, which restores the flag.
-// The bound is a distance only, so a lane uses it once its side is verified:
-// by a Contains that agrees with the flag, after which moves shorter than
-// the distance to the surface keep it verified; a wrap or a crossing clears
-// it.
-template <class S>
-__device__ void run_tile_learned(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s,
-                                 const Tile &tl, const MlpImage &img, const MlpDesc &md, LearnedCounts &cnt)
-{
-  __shared__ mlp_act_rows mlp_act[kTransportMaxBlock / 64];
-  mlp_act_rows &abuf = mlp_act[threadIdx.x >> 6];
-  const int lane = int(threadIdx.x & 63u);
-  for (int64_t base = tl.begin + int64_t(threadIdx.x & ~63u); base < tl.end; base += blockDim.x) {
-    const int64_t i = base + lane;
-    const bool act = i < tl.end;
-    TrackState t;
-    t.p = act ? V3<double>{a.x[i], a.y[i], a.z[i]} : V3<double>{md.c[0], md.c[1], md.c[2]};
-    t.d = act ? V3<double>{a.dx[i], a.dy[i], a.dz[i]} : V3<double>{0.0, 0.0, 1.0};
-    t.in = act && a.side[i] != 0;
-    t.crossings = 0;
-    t.carry = 0.0;
-    int k = 0;
-    bool live = act && a.nsteps > 0, pending = false, verified = false;
-    double l = live ? physics_step(sr.lambda, a.seed, uint64_t(i), uint32_t(a.step0)) : 0.0;
-    auto advance = [&](bool crossed) {
-      const uint32_t step = uint32_t(a.step0 + k);
-      if (!crossed) t.d = new_direction(a.seed, uint64_t(i), step, t.d);
-      if (!t.in) {
-        const V3<double> q = t.p;
-        t.p.x = wrap(t.p.x, sr.wc[0], sr.wh[0]);
-        t.p.y = wrap(t.p.y, sr.wc[1], sr.wh[1]);
-        t.p.z = wrap(t.p.z, sr.wc[2], sr.wh[2]);
-        if (t.p.x != q.x || t.p.y != q.y || t.p.z != q.z) {
-          t.carry = 0.0;
-          verified = false;
-        }
-      }
-      if (crossed) verified = false;
-      if (++k < a.nsteps)
-        l = physics_step(sr.lambda, a.seed, uint64_t(i), uint32_t(a.step0 + k));
-      else
-        live = false;
-    };
-    auto free_move = [&]() {
-      const double m = carry_margin(t.p);
-      t.p.x = t.p.x + l * t.d.x;
-      t.p.y = t.p.y + l * t.d.y;
-      t.p.z = t.p.z + l * t.d.z;
-      t.carry = t.carry - l - m;
-      advance(false);
-    };
-    for (;;) {
-      if (live && !pending)
-        while (live && l <= t.carry) free_move();
-      const bool want = live && !pending;
-      const uint64_t lmask = __ballot(live);
-      if (lmask == 0) break;
-      const uint64_t wmask = __ballot(want), pmask = __ballot(pending);
-      if (wmask == 0 || 100 * __popcll(pmask) >= a.fallback_pct * __popcll(lmask)) {
-        if (pending) {
-          const bool crossed = step_distance_first<S>(s, t, l, verified);
-          advance(crossed);
-          pending = false;
-          ++cnt.fallbacks;
-        }
-        continue;
-      }
-      const float y = mlp_eval_wave(img, abuf, mlp_coord(t.p.x, md.c[0], md.inv_hw[0]),
-                                    mlp_coord(t.p.y, md.c[1], md.inv_hw[1]), mlp_coord(t.p.z, md.c[2], md.inv_hw[2]));
-      if (want) {
-        ++cnt.q;
-        const double b = double(y) * md.scale - md.margin;
-        if (a.audit && b > S::template surface_distance<double>(s, t.p, t.in)) ++cnt.violations;
-        if (!verified && l <= b) verified = S::template contains<double>(s, t.p) == t.in;
-        if (verified && b > t.carry) t.carry = b;
-        if (l <= t.carry) {
-          free_move();
-          ++cnt.served;
-        } else {
-          pending = true;
-        }
-      }
-    }
-    if (act) {
-      a.x[i] = t.p.x;
-      a.y[i] = t.p.y;
-      a.z[i] = t.p.z;
-      a.dx[i] = t.d.x;
-      a.dy[i] = t.d.y;
-      a.dz[i] = t.d.z;
-      a.side[i] = t.in ? 1 : 0;
-      a.crossings[i] += t.crossings;
-    }
-  }
-}
 
 }  // namespace
 
@@ -395,18 +280,14 @@ __device__ void run_tile_learned(const TransportArgs &a, const SolidRec &sr, con
 // the grid is sized to the family's share of the machine and workgroups pull
 // tiles until the range is drained.  Without one, workgroup b runs tile t0+b.
 // Every family has its own kernel, so each gets the register allocation of
-// its own code path rather than the largest one of all families.  Learned
-// kernels run the tiles of solids with a network through run_tile_learned
-// (network weights read through the caches: they are shared by every wave
-// on the chip), the other tiles through the FP64 step.
-template <class S, int Mode, bool Staged, bool Learned>
+// its own code path rather than the largest one of all families.
+template <class S, int Mode, bool Staged>
 __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(TransportArgs a, int t0, int t1,
                                                                               int *counter)
 {
   extern __shared__ double lds_block[];
   __shared__ int tile_s;
   uint32_t fp64_steps = 0;
-  LearnedCounts cnt;
   int staged_solid = -1;
   for (int k = 0;; ++k) {
     int t;
@@ -432,24 +313,212 @@ __global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(Tr
       }
       sp = reinterpret_cast<const typename S::Data *>(lds_block);
     }
-    if constexpr (Learned) {
-      if (sr.mlp >= 0) {
-        const MlpDesc md = a.mlps[sr.mlp];
-        run_tile_learned<S>(a, sr, *sp, tl, *md.img, md, cnt);
-        continue;
-      }
-    }
     run_tile<S, Mode>(a, sr, *sp, tl, fp64_steps);
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
-  if constexpr (Learned) {  // one atomic per wave and counter
-    uint32_t v[4] = {cnt.q, cnt.served, cnt.fallbacks, cnt.violations};
-    for (int c = 0; c < 4; ++c) {
-      uint32_t x = v[c];
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-      if ((threadIdx.x & 63u) == 0 && x) atomicAdd(a.learned + c, (unsigned long long)x);
+}
+
+// ---- learned safety: one track-step per launch pair ------------------------
+//
+// A learned family advances its tracks one track-step at a time with two
+// kernels per step:
+//   learned_bound_kernel     every track of the family: a step under the
+//                            carried bound moves at once; otherwise the
+//                            network evaluates a bound for the whole wave on
+//                            the matrix cores and a track whose step fits
+//                            under it moves; the others are appended to a
+//                            compacted queue
+//   learned_fallback_kernel  the queued tracks: the exact distance-first step
+//                            (or the Contains that verifies a bound's side)
+// so the latency-bound facet walks run on full waves at the occupancy of
+// their own kernel, and the network's kernel does no walks.  Between the two
+// (and between steps) a track's carried bound and flags live in HBM.
+//
+// A bound at or below the exact safety moves where the FP64 step moves, so
+// trajectories are those of the FP64 path.  The exact safety is -1 when the
+// point is not on the side of the track's flag (after the periodic wrap puts
+// a track down inside the solid, or right after a crossing); the FP64 step
+// then takes the boundary-distance path, which restores the flag.  A bound is
+// only a distance, so it moves a track only once its side is verified: by a
+// Contains that agrees with the flag (in the fallback kernel); moves shorter
+// than the distance to the surface keep it verified, a wrap or a crossing
+// clears it.  Audit runs count any bound above the exact surface distance.
+
+constexpr uint8_t kTrackVerified = 1;
+
+struct LearnedCounts {
+  uint32_t q = 0, served = 0, fallbacks = 0, violations = 0;
+};
+
+__device__ __forceinline__ void add_learned_counts(const TransportArgs &a, const LearnedCounts &cnt)
+{
+  uint32_t v[4] = {cnt.q, cnt.served, cnt.fallbacks, cnt.violations};
+  for (int c = 0; c < 4; ++c) {
+    uint32_t x = v[c];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63u) == 0 && x) atomicAdd(a.learned + c, (unsigned long long)x);
+  }
+}
+
+// New direction unless the step crossed; the periodic world for tracks
+// outside (a wrap clears the carried bound and the side verification).
+__device__ __forceinline__ void finish_step(const TransportArgs &a, const SolidRec &sr, uint64_t i, uint32_t step,
+                                            bool crossed, TrackState &t, bool &verified)
+{
+  if (!crossed) t.d = new_direction(a.seed, i, step, t.d);
+  if (crossed) verified = false;
+  if (!t.in) {
+    const V3<double> q = t.p;
+    t.p.x = wrap(t.p.x, sr.wc[0], sr.wh[0]);
+    t.p.y = wrap(t.p.y, sr.wc[1], sr.wh[1]);
+    t.p.z = wrap(t.p.z, sr.wc[2], sr.wh[2]);
+    if (t.p.x != q.x || t.p.y != q.y || t.p.z != q.z) {
+      t.carry = 0.0;
+      verified = false;
     }
   }
+}
+
+// Step k of the tiles [t0, t1) of one family, one workgroup per tile.  The
+// audit variant also computes the exact surface distance of every query.
+template <class S, bool Audit>
+__global__ __launch_bounds__(kTransportMaxBlock) void learned_bound_kernel(TransportArgs a, int t0, int t1, int k)
+{
+  __shared__ mlp_act_rows act_buf[kTransportMaxBlock / 64];
+  const int t = t0 + int(blockIdx.x);
+  if (t >= t1) return;
+  const Tile tl = a.tiles[t];
+  const SolidRec sr = a.solids[tl.solid];
+  const typename S::Data &s = *reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
+  const bool net = sr.mlp >= 0;
+  MlpDesc md{};
+  if (net) md = a.mlps[sr.mlp];
+  const uint32_t step = uint32_t(a.step0 + k);
+  const int lane = int(threadIdx.x & 63u);
+  LearnedCounts cnt;
+  for (int64_t base = tl.begin + int64_t(threadIdx.x & ~63u); base < tl.end; base += blockDim.x) {
+    const int64_t i = base + lane;
+    const bool act = i < tl.end;
+    TrackState t;
+    t.p = act ? V3<double>{a.x[i], a.y[i], a.z[i]} : V3<double>{sr.wc[0], sr.wc[1], sr.wc[2]};
+    t.d = act ? V3<double>{a.dx[i], a.dy[i], a.dz[i]} : V3<double>{0.0, 0.0, 1.0};
+    t.in = act && a.side[i] != 0;
+    t.crossings = 0;
+    t.carry = (act && k > 0) ? a.tcarry[i] : 0.0;
+    bool verified = act && k > 0 && (a.tflags[i] & kTrackVerified);
+    const double l = act ? physics_step(sr.lambda, a.seed, uint64_t(i), step) : 0.0;
+    const bool carried = act && l <= t.carry;
+    const bool want = act && !carried && net;
+    double b = -1.0;
+    if (__any(want)) {  // uniform: the whole wave evaluates the network
+      const float y = mlp_eval_wave(*md.img, act_buf[threadIdx.x >> 6], mlp_coord(t.p.x, md.c[0], md.inv_hw[0]),
+                                    mlp_coord(t.p.y, md.c[1], md.inv_hw[1]), mlp_coord(t.p.z, md.c[2], md.inv_hw[2]));
+      b = double(y) * md.scale - md.margin;
+    }
+    bool moved = false, queued = false;
+    if (act) {
+      if (carried) {
+        moved = true;
+      } else if (want) {
+        ++cnt.q;
+        if constexpr (Audit)
+          if (b > S::template surface_distance<double>(s, t.p, t.in)) ++cnt.violations;
+        if (verified) {
+          if (b > t.carry) t.carry = b;
+          moved = l <= t.carry;
+          cnt.served += moved ? 1u : 0u;
+        } else if (l <= b) {
+          t.carry = b;  // a bound for the fallback kernel to use once the side is verified
+        }
+        queued = !moved;
+      } else {
+        queued = true;
+      }
+    }
+    if (moved) {
+      const double m = carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      t.carry = t.carry - l - m;
+      finish_step(a, sr, uint64_t(i), step, false, t, verified);
+    }
+    // append the queued tracks, one atomic per wave
+    const uint64_t qm = __ballot(queued);
+    if (qm) {
+      int slot0 = 0;
+      if (lane == 0) slot0 = atomicAdd(a.qcount + (k & 1), __popcll(qm));
+      slot0 = __shfl(slot0, 0);
+      if (queued) {
+        const int slot = slot0 + __popcll(qm & ((1ull << lane) - 1ull));
+        a.queue[slot] = (uint64_t(uint32_t(tl.solid)) << 40) | uint64_t(i);
+      }
+    }
+    if (act) {
+      if (moved) {
+        a.x[i] = t.p.x;
+        a.y[i] = t.p.y;
+        a.z[i] = t.p.z;
+        a.dx[i] = t.d.x;
+        a.dy[i] = t.d.y;
+        a.dz[i] = t.d.z;
+      }
+      a.tcarry[i] = t.carry;
+      a.tflags[i] = verified ? kTrackVerified : 0;
+    }
+  }
+  add_learned_counts(a, cnt);
+}
+
+// Step k of the queued tracks (grid-stride over the queue).  Also clears the
+// queue counter of step k + 1, which the next bound kernel fills.
+template <class S>
+__global__ __launch_bounds__(kTransportMaxBlock) void learned_fallback_kernel(TransportArgs a, int k)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.qcount[(k + 1) & 1] = 0;
+  const int nq = a.qcount[k & 1];
+  const uint32_t step = uint32_t(a.step0 + k);
+  LearnedCounts cnt;
+  for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < nq; j += int(gridDim.x * blockDim.x)) {
+    const uint64_t e = a.queue[j];
+    const int64_t i = int64_t(e & ((uint64_t(1) << 40) - 1));
+    const SolidRec sr = a.solids[int(e >> 40)];
+    const typename S::Data &s = *reinterpret_cast<const typename S::Data *>(a.pool + sr.off);
+    TrackState t;
+    t.p = V3<double>{a.x[i], a.y[i], a.z[i]};
+    t.d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
+    t.in = a.side[i] != 0;
+    t.crossings = 0;
+    t.carry = a.tcarry[i];
+    bool verified = (a.tflags[i] & kTrackVerified) != 0;
+    const double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
+    bool crossed = false;
+    if (!verified && l <= t.carry && S::template contains<double>(s, t.p) == t.in) {
+      verified = true;  // the bound of the bound kernel now applies
+      const double m = carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      t.carry = t.carry - l - m;
+      ++cnt.served;
+    } else {
+      if (!verified) t.carry = 0.0;
+      crossed = step_distance_first<S>(s, t, l, verified);
+      ++cnt.fallbacks;
+    }
+    finish_step(a, sr, uint64_t(i), step, crossed, t, verified);
+    a.x[i] = t.p.x;
+    a.y[i] = t.p.y;
+    a.z[i] = t.p.z;
+    a.dx[i] = t.d.x;
+    a.dy[i] = t.d.y;
+    a.dz[i] = t.d.z;
+    a.side[i] = t.in ? 1 : 0;
+    if (crossed) a.crossings[i] += 1;
+    a.tcarry[i] = t.carry;
+    a.tflags[i] = verified ? kTrackVerified : 0;
+  }
+  add_learned_counts(a, cnt);
 }
 
 namespace {
@@ -499,19 +568,20 @@ struct FamilyLauncher {
   const TransportArgs &a;
   const FamilyLaunch &f;
   int block;
+  hipError_t err = hipSuccess;
 
-  template <class S, int Mode, bool Learned>
+  template <class S, int Mode>
   void go(const TransportArgs &args)
   {
     const int grid = f.counter ? f.grid : (f.t1 - f.t0);
     if (grid <= 0) return;
     const bool staged = S::kStage && f.stage_nd > 0 && size_t(f.stage_nd) * sizeof(double) <= size_t(kMaxStageBytes);
     if (staged)
-      hipLaunchKernelGGL((transport_family_kernel<S, Mode, S::kStage, Learned>), dim3(grid), dim3(block),
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, S::kStage>), dim3(grid), dim3(block),
                          size_t(f.stage_nd) * sizeof(double), stream, args, f.t0, f.t1, f.counter);
     else
-      hipLaunchKernelGGL((transport_family_kernel<S, Mode, false, Learned>), dim3(grid), dim3(block), 0, stream,
-                         args, f.t0, f.t1, f.counter);
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, false>), dim3(grid), dim3(block), 0, stream, args, f.t0,
+                         f.t1, f.counter);
   }
 
   template <class S>
@@ -520,16 +590,28 @@ struct FamilyLauncher {
     TransportArgs args = a;
     args.distance_first = (S::kSafetyBelowDistance && (f.mode == kFamDistFirst || f.mode == kFamLearned)) ? 1 : 0;
     if (f.mode == kFamMixed) {
-      go<S, kTransportMixed, false>(args);
+      go<S, kTransportMixed>(args);
       return;
     }
     if constexpr (S::kSafetyIsDistance) {
       if (f.mode == kFamLearned && f.has_mlp) {
-        go<S, kTransportFP64, true>(args);
+        args.queue = a.queue + f.queue_off;
+        args.qcount = a.qcount + 2 * f.type;
+        const unsigned tiles = unsigned(f.t1 - f.t0);
+        for (int k = 0; k < a.nsteps && tiles > 0; ++k) {
+          if (a.audit)
+            hipLaunchKernelGGL((learned_bound_kernel<S, true>), dim3(tiles), dim3(block), 0, stream, args, f.t0, f.t1,
+                               k);
+          else
+            hipLaunchKernelGGL((learned_bound_kernel<S, false>), dim3(tiles), dim3(block), 0, stream, args, f.t0,
+                               f.t1, k);
+          hipLaunchKernelGGL(learned_fallback_kernel<S>, dim3(unsigned(a.fallback_grid)), dim3(kTransportMaxBlock),
+                             0, stream, args, k);
+        }
         return;
       }
     }
-    go<S, kTransportFP64, false>(args);
+    go<S, kTransportFP64>(args);
   }
 };
 }  // namespace
@@ -577,7 +659,10 @@ hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long 
 
 hipError_t launch_family(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block)
 {
-  if (f.mode == kFamLearned && f.has_mlp && (!a.mlps || !a.learned)) return hipErrorInvalidValue;
+  if (f.mode == kFamLearned && f.has_mlp &&
+      (!a.mlps || !a.learned || !a.queue || !a.qcount || !a.tcarry || !a.tflags || a.fallback_grid <= 0 ||
+       block > kTransportMaxBlock || block % 64))
+    return hipErrorInvalidValue;
   FamilyLauncher l{stream, a, f, block};
   if (!dispatch_shape_type(f.type, l)) return hipErrorInvalidValue;
   return hipGetLastError();
