@@ -16,6 +16,8 @@ struct ConeShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
+  // mixed-precision DistanceToOut may be evaluated in float
+  static constexpr bool kFloatDistanceToOut = true;
 
   template <typename R>
   VGM_DEV static R plus_tol_sq(R x, double tol) { return x * (x + R(2.0 * tol)); }

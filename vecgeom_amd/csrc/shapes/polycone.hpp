@@ -16,6 +16,10 @@ struct PolyconeShape {
   // SafetyToIn can exceed DistanceToIn (the section-wise safety of the
   // reference), so the distance-first step would change decisions
   static constexpr bool kSafetyBelowDistance = false;
+  // the section walk of DistanceToOut decides at every z-plane whether the
+  // ray continues; in float it can end on a different surface than in
+  // FP64, so mixed precision keeps this query in FP64
+  static constexpr bool kFloatDistanceToOut = false;
 
   // GetSectionIndex: linear search over the z-planes, compared in the
   // arithmetic type of the query

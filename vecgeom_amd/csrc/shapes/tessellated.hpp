@@ -47,6 +47,8 @@ struct TessellatedShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
+  // mixed-precision DistanceToOut may be evaluated in float
+  static constexpr bool kFloatDistanceToOut = true;
 
   template <typename R>
   VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)

@@ -15,6 +15,8 @@ struct TrapezoidShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
+  // mixed-precision DistanceToOut may be evaluated in float
+  static constexpr bool kFloatDistanceToOut = true;
 
   template <typename R>
   VGM_DEV static R pdist(const Data &t, int i, V3<R> p)

@@ -14,6 +14,8 @@ struct TrdShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
+  // mixed-precision DistanceToOut may be evaluated in float
+  static constexpr bool kFloatDistanceToOut = true;
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

@@ -17,6 +17,8 @@ struct TubeShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
+  // mixed-precision DistanceToOut may be evaluated in float
+  static constexpr bool kFloatDistanceToOut = true;
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>
