@@ -101,6 +101,77 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, weights=None):
     return out
 
 
+def run_precision(args, ctx, dev, torch, stream):
+    """Tube and cone SOA3D queries, FP64 vs mixed precision: 8M points uniform
+    in 1.2x the bounding box and 8M within +-1e-6 mm of the surface (entry or
+    exit points of random rays from the FP64 kernels, jittered along the
+    ray), isotropic directions.  Throughput of each query in both precisions
+    and the mixed results' error against FP64."""
+    import vecgeom_amd as vg
+    from vecgeom_amd import sampling
+
+    shapes = {"tube(5,20,30,0,3pi/2)": lambda: ctx.tube(5.0, 20.0, 30.0, 0.0, 1.5 * np.pi),
+              "cone(0,10,5,25,40)": lambda: ctx.cone(0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * np.pi)}
+    m = args.c2_points // 2
+    out = {}
+    for name, make in shapes.items():
+        s = make()
+        pv = ctx.place(ctx.logical_volume(s))
+        mn, mx = ctx.extent(s)
+        half, cen = 0.5 * (mx - mn), 0.5 * (mx + mn)
+        pts = [torch.from_numpy(a).to(dev) for a in sampling.points_in_box(2, 1, m, half, 1.2, cen)]
+        dirs = [torch.from_numpy(a).to(dev) for a in sampling.isotropic_directions(2, 2, m)]
+        P, D = vg.SOA3D(*pts), vg.SOA3D(*dirs)
+        din, dout = ctx.distance_to_in(pv, P, D), ctx.distance_to_out(pv, P, D)
+        t = torch.where((din > 0) & (din < 1e30), din, torch.where((dout > 0) & (dout < 1e30), dout,
+                                                                   torch.zeros_like(din)))
+        jit = (2.0 * torch.from_numpy(sampling.uniform(2, 9, m)).to(dev) - 1.0) * 1e-6
+        band = [pts[k] + (t + jit) * dirs[k] for k in range(3)]
+        dirs2 = [torch.from_numpy(a).to(dev) for a in sampling.isotropic_directions(2, 3, m)]
+        P = vg.SOA3D(*[torch.cat([pts[k], band[k]]) for k in range(3)])
+        D = vg.SOA3D(*[torch.cat([dirs[k], dirs2[k]]) for k in range(3)])
+        n = 2 * m
+        scale = float(np.linalg.norm(mx - mn))
+        res = {}
+        for q in ("Inside", "Contains", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"):
+            fn = {"Inside": lambda mx_: ctx.inside(pv, P, mixed=mx_),
+                  "Contains": lambda mx_: ctx.contains(pv, P, mixed=mx_),
+                  "DistanceToIn": lambda mx_: ctx.distance_to_in(pv, P, D, mixed=mx_),
+                  "DistanceToOut": lambda mx_: ctx.distance_to_out(pv, P, D, mixed=mx_),
+                  "SafetyToIn": lambda mx_: ctx.safety_to_in(pv, P, mixed=mx_),
+                  "SafetyToOut": lambda mx_: ctx.safety_to_out(pv, P, mixed=mx_)}[q]
+            r = {}
+            outs = {}
+            for mx_ in (False, True):
+                fn(mx_)
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                for _ in range(3):
+                    o = fn(mx_)
+                b.record(stream)
+                torch.cuda.synchronize()
+                ms = a.elapsed_time(b) / 3
+                r["mixed_gq_s" if mx_ else "fp64_gq_s"] = n / ms / 1e6
+                outs[mx_] = o
+            ref, got = outs[False], outs[True]
+            if q in ("Inside", "Contains"):
+                r["classification_mismatches"] = int((ref != got).sum().item())
+            else:
+                big_r, big_g = ref >= 1e300, got >= 1e300
+                fin = ~big_r & ~big_g
+                err = (ref[fin] - got[fin]).abs()
+                r["inf_disagree"] = int((big_r != big_g).sum().item())
+                r["max_abs_err"] = float(err.max().item()) if err.numel() else 0.0
+                r["mean_abs_err"] = float(err.mean().item()) if err.numel() else 0.0
+                r["exact_fraction"] = float((err == 0).double().mean().item()) if err.numel() else 1.0
+            res[q] = r
+        out[name] = {"points": n, "scale_mm": scale, "queries": res}
+        log(f"[bench] precision {name}: " + ", ".join(
+            f"{q} {v['fp64_gq_s']:.1f}/{v['mixed_gq_s']:.1f} Gq/s" for q, v in res.items()))
+    return out
+
+
 def run_navigation(args, ctx, dev, torch):
     """Barrel navigation: straight-line transport of located tracks through
     20 tube layers of box modules, nsteps of ComputeStepAndPropagatedState per
@@ -172,6 +243,8 @@ def main():
     ap.add_argument("--cpu-tracks", type=int, default=400_000)
     ap.add_argument("--nav-tracks", type=int, default=4_000_000, help="navigation benchmark tracks (0: skip)")
     ap.add_argument("--nav-steps", type=int, default=200)
+    ap.add_argument("--c2-points", type=int, default=16_000_000,
+                    help="points per shape for the tube/cone precision comparison (0: skip)")
     ap.add_argument("--launch", default=None, help="block,grid,tile launch shape instead of autotuning")
     ap.add_argument("--train-samples", type=int, default=50_000_000,
                     help="exact safety samples for the learned-safety networks, over all tessellated solids")
@@ -378,6 +451,10 @@ def main():
         cpu = {"value": cn * S / dt, "unit": "track-steps/s", "cores": nth, "kind": "port",
                "sample": f"{cn} tracks x {S} track-steps, same solids and generator, FP64 oracle, OpenMP"}
 
+    precision = None
+    if args.c2_points > 0:
+        precision = run_precision(args, ctx, dev, torch, stream)
+
     navigation = None
     if args.nav_tracks > 0:
         navigation = run_navigation(args, ctx, dev, torch)
@@ -410,6 +487,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "tuned": tuned,
+        "precision": precision,
         "navigation": navigation,
     }
     print(json.dumps(line))
