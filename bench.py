@@ -31,6 +31,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The batched transport runs its shape families on four streams of their own
+# next to the context stream and torch's; with HIP's default of four hardware
+# queues per process two of them share one and their kernels serialise.
+# Set before HIP initialises; an explicit setting in the environment wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "track_step_flops.json")

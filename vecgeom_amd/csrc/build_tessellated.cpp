@@ -74,7 +74,7 @@ struct Builder {
   std::vector<TessFacet> &facets;  // reordered in place into leaf order
   std::vector<TessNode> nodes;
   double margin;
-  static constexpr int kLeaf = 4;
+  int leaf = 1;  // facets per leaf
 
   static double centroid(const TessFacet &f, int c) { return f.v[0][c] + f.v[1][c] + f.v[2][c]; }
 
@@ -100,7 +100,7 @@ struct Builder {
       nodes[id].lo[c] = lo[c] - margin;
       nodes[id].hi[c] = hi[c] + margin;
     }
-    if (e - b <= kLeaf) {
+    if (e - b <= leaf) {
       nodes[id].left = nodes[id].right = -1;
       nodes[id].first = b;
       nodes[id].count = e - b;
@@ -155,9 +155,18 @@ int build_tessellated(HostGeometry &g, int nfacets, const double *tri, const dou
   }
   double big = 0;
   for (int c = 0; c < 3; ++c) big = std::max(big, std::max(std::fabs(mn[c]), std::fabs(mx[c])));
+  // One facet per leaf walks fastest (box tests are cheaper than facet
+  // tests); larger solids use the smallest leaf size whose tree keeps node
+  // indices within 16 bits.
   Builder B{facets, {}, 1e-7 + 1e-12 * big};
-  B.nodes.reserve(2 * n);
-  B.build(0, n);
+  for (;;) {
+    B.nodes.clear();
+    B.max_depth = 0;
+    B.nodes.reserve(2 * size_t(n) / size_t(B.leaf) + 1);
+    B.build(0, n);
+    if (B.nodes.size() <= 65535 || B.leaf >= n) break;
+    B.leaf *= 2;
+  }
   const int nn = int(B.nodes.size());
   // the device walks keep 16-bit node indices, at most one deferred sibling
   // per level, on a stack of kTessWalkDepth entries
