@@ -85,16 +85,17 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, weights=None):
     w = {k: v for k, v in (weights or {}).items() if k in fams_ms}
     fam = max(w, key=w.get) if w else max(fams_ms, key=fams_ms.get)
     mode = (modes or {}).get(fam, "fp64")
-    if mode not in ("fp64", "distance-first"):
+    if mode not in ("fp64", "distance-first", "culled"):
         return None  # float or matrix-core work: not priced against the FP64 peak
-    table = fx["per_family_distance_first"] if mode == "distance-first" else fx["per_family"]
+    table = fx[{"fp64": "per_family", "distance-first": "per_family_distance_first",
+                "culled": "per_family_culled"}[mode]]
     per_step = float(table[fam])
     flops = per_step * counts_by_family[fam] * S
     t = fams_ms[fam] * 1e-3
     achieved = flops / t / 1e12
     out = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-           "kernel": f"transport_family_kernel<{fam}, FP64{', distance-first' if mode == 'distance-first' else ''}>",
+           "kernel": f"transport_family_kernel<{fam}, FP64{'' if mode == 'fp64' else ', ' + mode}>",
            "kernel_ms": fams_ms[fam], "flops_per_track_step": per_step, "tracks": counts_by_family[fam],
            "track_steps": S,
            "note": "FP64 vector peak; flops counted on the step the kernel runs (+ - * / sqrt); kernel time from "
@@ -257,7 +258,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--modes", default=None,
                     help="tuned run without the autotuner: family=mode,... (modes fp64, mixed, distance-first, "
-                         "learned); with --launch")
+                         "learned, culled); with --launch")
     ap.add_argument("--only", default=None,
                     help="comma list of runs out of fp64,mixed,learned,tuned (default all; the headline is the last)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -308,8 +309,8 @@ def main():
         elif not args.no_autotune:
             log(f"[bench] autotune {tag} ...")
             t = time.perf_counter()
-            shape = batch.autotune(*state, side, nsteps=2, mixed=mixed or tuned, learned=learned or tuned,
-                                   distance_first=tuned)
+            shape = batch.autotune(*state, side, nsteps=min(S, 10), mixed=mixed or tuned,
+                                   learned=learned or tuned, distance_first=tuned, cull=tuned)
             log(f"[bench] autotune {tag}: block/grid/tile = {shape} ({time.perf_counter() - t:.1f}s)")
         step0 = 0
         for _ in range(args.warmup):

@@ -32,6 +32,8 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_AUDIT 32u      /* with VGM_LEARNED: check every served bound against the exact safety */
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
+#define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
+                          the queries (same decisions as FP64) */
 
 /* Status codes. */
 #define VGM_OK 0
@@ -225,17 +227,18 @@ int vgm_batch_family_ms(vgm_batch *batch, float *ms8);
  * bounds above the exact safety. */
 int vgm_batch_learned_stats(vgm_batch *batch, unsigned long long *out4);
 /* Mode per shape family chosen by the last vgm_batch_autotune (index = shape
- * type; 0 FP64, 1 mixed, 2 distance-first, 3 learned; -1 not tuned).  The
- * autotuner tries FP64 and each mode its flags allow (VGM_MIXED,
- * VGM_DISTANCE_FIRST, VGM_LEARNED), each family alone, and keeps the fastest;
- * VGM_TUNED transports use the choice. */
+ * type; 0 FP64, 1 mixed, 2 distance-first, 3 learned, 4 culled; -1 not
+ * tuned).  The autotuner tries FP64 and each mode its flags allow (VGM_MIXED,
+ * VGM_DISTANCE_FIRST, VGM_LEARNED, VGM_CULL), each family alone, and keeps the
+ * fastest; VGM_TUNED transports use the choice. */
 int vgm_batch_get_modes(vgm_batch *batch, int *modes8);
 /* Relative cost of each family (index = shape type): after
  * vgm_batch_autotune, the ms of that family's kernel alone in its mode;
  * before, its track count.  It splits persistent grids and orders launches. */
 int vgm_batch_get_weights(vgm_batch *batch, double *ms8);
-/* Sets the modes VGM_TUNED transports use (distance-first falls back to FP64
- * for families whose safety can exceed their distance). */
+/* Sets the modes VGM_TUNED transports use (distance-first, and the distance
+ * order of culled, fall back to FP64 for families whose safety can exceed
+ * their distance). */
 int vgm_batch_set_modes(vgm_batch *batch, const int *modes8);
 /* A wave of the learned kernel runs its exact fallbacks once this percentage
  * of its live tracks wait for one (default 50). */

@@ -83,13 +83,16 @@ def test_transport_mixed_follows_fp64(ctx):
     b.close()
 
 
-def test_transport_distance_first_bit_exact(ctx):
+@pytest.mark.parametrize("mode", ["distance_first", "cull"])
+def test_transport_distance_first_bit_exact(ctx, mode):
     """Boundary distance before the safety: the same decisions as the FP64
     step (SafetyToIn/Out never exceed DistanceToIn/Out beyond the window), so
-    trajectories match the oracle bit for bit for every family."""
-    print("[transport] distance-first", flush=True)
+    trajectories match the oracle bit for bit for every family.  The culled
+    mode also moves outside tracks whose step stays clear of the bounding box
+    without any query: still bit for bit."""
+    print(f"[transport] {mode}", flush=True)
     b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
-    b.transport(*dev, side, cr, 30, step0=0, seed=5, distance_first=True)
+    b.transport(*dev, side, cr, 30, step0=0, seed=5, **{mode: True})
     for k in range(6):
         got = dev[k].cpu().numpy()
         bad = np.flatnonzero(got != ref[k])
@@ -101,12 +104,12 @@ def test_transport_distance_first_bit_exact(ctx):
 
 def test_transport_tuned_modes_exact(ctx):
     """Autotuned per-family modes without mixed precision (FP64,
-    distance-first) reproduce the oracle bit for bit."""
+    distance-first, culled) reproduce the oracle bit for bit."""
     print("[transport] tuned", flush=True)
     b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
-    b.autotune(*dev, side, nsteps=2, distance_first=True)
+    b.autotune(*dev, side, nsteps=2, distance_first=True, cull=True)
     modes = b.modes()
-    assert set(modes.values()) <= {"fp64", "distance-first"}, modes
+    assert set(modes.values()) <= {"fp64", "distance-first", "culled"}, modes
     b.transport(*dev, side, cr, 30, step0=0, seed=5, tuned=True)
     for k in range(6):
         assert np.array_equal(dev[k].cpu().numpy(), ref[k]), k

@@ -29,7 +29,8 @@ LEARNED = 16
 AUDIT = 32
 DISTANCE_FIRST = 64
 TUNED = 128
-FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned")
+CULL = 256
+FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
 SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated")
 
@@ -473,10 +474,10 @@ class Batch:
 
     def transport(self, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0, seed=1, mixed=False,
                   asynchronous=False, count_fp64=False, timing=False, learned=False, audit=False,
-                  distance_first=False, tuned=False):
+                  distance_first=False, tuned=False, cull=False):
         flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0) | (TIMING if timing else 0)
         flags |= (LEARNED if learned else 0) | (AUDIT if audit else 0) | (DISTANCE_FIRST if distance_first else 0)
-        flags |= TUNED if tuned else 0
+        flags |= (TUNED if tuned else 0) | (CULL if cull else 0)
         cnt = ctypes.c_ulonglong(0)
         check(self._L.vgm_batch_transport(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                           _ptr(side), _ptr(crossings), nsteps, step0, seed, flags,
@@ -525,9 +526,11 @@ class Batch:
             out[SHAPE_TYPES.index(f)] = FAMILY_MODES.index(m)
         check(self._L.vgm_batch_set_modes(self.handle, out.ctypes.data))
 
-    def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False, learned=False, distance_first=False):
+    def autotune(self, x, y, z, dx, dy, dz, side, nsteps, mixed=False, learned=False, distance_first=False,
+                 cull=False):
         out = np.zeros(3, dtype=np.int32)
         flags = (MIXED if mixed else 0) | (LEARNED if learned else 0) | (DISTANCE_FIRST if distance_first else 0)
+        flags |= CULL if cull else 0
         check(self._L.vgm_batch_autotune(self.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                          _ptr(side), nsteps, flags, out.ctypes.data))
         return tuple(int(v) for v in out)

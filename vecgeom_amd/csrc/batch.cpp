@@ -230,9 +230,11 @@ int family_mode(const vgm_batch *b, const FamilyLaunch &f, unsigned flags)
     m = kFamLearned;
   else if (flags & VGM_MIXED)
     m = kFamMixed;
+  else if (flags & VGM_CULL)
+    m = kFamCulled;
   else if (flags & VGM_DISTANCE_FIRST)
     m = kFamDistFirst;
-  if (m == kFamLearned && !f.has_mlp) m = kFamDistFirst;
+  if (m == kFamLearned && !f.has_mlp) m = kFamCulled;
   return m;
 }
 
@@ -329,6 +331,8 @@ int vgm_batch_create(vgm_ctx *c, int nsolids, const int *shapes, const int64_t *
     for (int k = 0; k < 3; ++k) {
       s.wc[k] = 0.5 * (r.bmin[k] + r.bmax[k]);
       s.wh[k] = 1.3 * (0.5 * (r.bmax[k] - r.bmin[k]));
+      s.bc[k] = 0.5 * (r.bmin[k] + r.bmax[k]);
+      s.bh[k] = 0.5 * (r.bmax[k] - r.bmin[k]);
     }
     b->solids.push_back(s);
     b->shape_ids.push_back(shapes[i]);
@@ -381,12 +385,8 @@ int vgm_batch_set_modes(vgm_batch *b, const int *modes8)
 {
   if (!b || !modes8) return fail(VGM_EINVAL, "null argument");
   for (int t = 0; t < kNumShapeTypes; ++t)
-    if (modes8[t] < -1 || modes8[t] > kFamLearned) return fail(VGM_EINVAL, "invalid family mode");
-  for (int t = 0; t < kNumShapeTypes; ++t) {
-    int m = modes8[t];
-    if (m == kFamDistFirst && !family_distance_first(t)) m = kFamFP64;
-    b->fam_mode[t] = m;
-  }
+    if (modes8[t] < -1 || modes8[t] > kFamCulled) return fail(VGM_EINVAL, "invalid family mode");
+  for (int t = 0; t < kNumShapeTypes; ++t) b->fam_mode[t] = modes8[t];
   return VGM_OK;
 }
 
@@ -540,7 +540,8 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   for (const FamilyLaunch &f0 : b->fams) {
     std::vector<int> cand{kFamFP64};
     if (flags & VGM_MIXED) cand.push_back(kFamMixed);
-    if ((flags & VGM_DISTANCE_FIRST) && family_distance_first(f0.type)) cand.push_back(kFamDistFirst);
+    if (flags & VGM_DISTANCE_FIRST) cand.push_back(kFamDistFirst);
+    if (flags & VGM_CULL) cand.push_back(kFamCulled);
     if ((flags & VGM_LEARNED) && f0.has_mlp) cand.push_back(kFamLearned);
     float fbest = 1e30f;
     for (int m : cand) {

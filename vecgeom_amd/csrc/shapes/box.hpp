@@ -9,6 +9,8 @@ namespace vgm {
 struct BoxShape {
   using Data = BoxData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // waves per SIMD the transport kernel asks the compiler for (1: no request)
+  static constexpr int kWavesPerEU = 1;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:

@@ -42,6 +42,8 @@ struct TessWalkStack {
 struct TessellatedShape {
   using Data = TessData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
+  // waves per SIMD the transport kernel asks the compiler for (1: no request)
+  static constexpr int kWavesPerEU = 1;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = true;
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:

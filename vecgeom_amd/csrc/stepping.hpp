@@ -83,4 +83,47 @@ VGM_DEV double wrap(double p, double c, double h)
   return p;
 }
 
+// Exact reorderings of the track-step (same decisions, same arithmetic on
+// the position):
+//
+// Carried bound.  For a family whose safety is the distance to its surface,
+// a safety s at p bounds the safety at p' by s - |p' - p|; a move of l keeps
+// s - l - carry_margin(p) as the bound at the new point, and a later step
+// of length up to that bound moves without any query.
+template <typename R>
+VGM_DEV R carry_margin(const V3<R> &p)
+{
+  return R(1e-7) + R(1e-12) * (vabs(p.x) + vabs(p.y) + vabs(p.z));
+}
+
+// Distance first.  For a family whose safety never exceeds its boundary
+// distance, a step moves freely when l < d_b and crosses when l exceeds d_b
+// by more than this window; only in between is the safety needed.
+constexpr double kDistanceFirstWindow = 1e-6;
+
+// Bounding-box cull.  A track outside its solid whose move [p, p + l d]
+// stays clear of the solid's bounding box (centre bc, half-widths bh) cannot
+// reach the surface: every point DistanceToIn can return lies on the surface
+// (within the tolerance of the surface tests) and the surface lies in the
+// box, so DistanceToIn > l and the step moves the track by l without a
+// crossing whichever order it takes.  Separating-axis test of the segment,
+// lengthened by twice the distance-first window, against the box grown by
+// the window and the carried-bound margin (which cover the surface tolerance
+// and the rounding of the test): the three box axes, then the cross products
+// of the segment with them.  Rounding only errs towards "no", after which
+// the ordinary step runs.
+template <typename R>
+VGM_DEV bool move_clears_box(const double *bc, const double *bh, const V3<R> &p, const V3<R> &d, R l)
+{
+  const R hl = R(0.5) * (l + R(2.0 * kDistanceFirstWindow));
+  const R m = R(kDistanceFirstWindow) + carry_margin(p);
+  const R wx = hl * d.x, wy = hl * d.y, wz = hl * d.z;  // half segment
+  const R cx = (p.x + wx) - R(bc[0]), cy = (p.y + wy) - R(bc[1]), cz = (p.z + wz) - R(bc[2]);
+  const R hx = R(bh[0]) + m, hy = R(bh[1]) + m, hz = R(bh[2]) + m;
+  const R ex = vabs(wx), ey = vabs(wy), ez = vabs(wz);
+  if (vabs(cx) > hx + ex || vabs(cy) > hy + ey || vabs(cz) > hz + ez) return true;
+  return vabs(cy * wz - cz * wy) > hy * ez + hz * ey || vabs(cz * wx - cx * wz) > hx * ez + hz * ex ||
+         vabs(cx * wy - cy * wx) > hx * ey + hy * ex;
+}
+
 }  // namespace vgm

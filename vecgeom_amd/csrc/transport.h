@@ -21,6 +21,8 @@ struct SolidRec {
   double scale;   // bounding-box diagonal (mm), sets the float guard band
   double wc[3];   // periodic world box: centre and half-widths (1.3 x bounding box)
   double wh[3];
+  double bc[3];   // bounding box of the solid (Extent): centre and half-widths
+  double bh[3];
 };
 
 // A tile of the work queue: consecutive tracks of one solid.
@@ -54,6 +56,8 @@ struct TransportArgs {
   int32_t fallback_pct;  // a wave runs its fallbacks once this % of its live tracks wait
   int32_t distance_first;  // FP64 steps of surface-distance families: boundary distance first
   int32_t fallback_grid;   // workgroups of the learned fallback kernel
+  int32_t cull;            // outside moves that clear the bounding box skip the geometry query
+  int32_t pad2;
   // learned families, between their per-step kernels: carried bound and
   // flags per track (indexed like x), and the queue of exact fallbacks
   // (entries solid << 40 | track) with its two counters (by step parity)
@@ -94,17 +98,20 @@ struct FamilyLaunch {
 // decisions of step_fp64 (bit-identical trajectories):
 //   kFamFP64       safety, then the boundary distance when the step exceeds it
 //   kFamMixed      float filter, FP64 redo of steps near a boundary
-//   kFamDistFirst  boundary distance first; the safety only where the two
-//                  orders could differ (within kDistanceFirstWindow of it)
-//   kFamLearned    learned safety bound, then kFamDistFirst as the fallback
-enum FamilyMode : int { kFamFP64 = 0, kFamMixed = 1, kFamDistFirst = 2, kFamLearned = 3 };
+//   kFamDistFirst  for families whose safety never exceeds the boundary
+//                  distance: the boundary distance first and the safety only
+//                  where the two orders could differ (within
+//                  kDistanceFirstWindow of it); other families take the
+//                  kFamFP64 step
+//   kFamLearned    learned safety bound, then kFamCulled as the fallback
+//   kFamCulled     kFamDistFirst, and outside moves clear of the solid's
+//                  bounding box skip the queries
+enum FamilyMode : int { kFamFP64 = 0, kFamMixed = 1, kFamDistFirst = 2, kFamLearned = 3, kFamCulled = 4 };
 
 hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps,
                         unsigned long long *learned);
 // True when the family can serve learned bounds (its safety is a distance).
 bool family_learnable(int type);
-// True when the distance-first step takes the FP64 step's decisions.
-bool family_distance_first(int type);
 hipError_t launch_family(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block);
 // Initial inside flags (persistent over all tiles, uses a.counter).
 hipError_t launch_classify(hipStream_t stream, const TransportArgs &a, int grid, int block);

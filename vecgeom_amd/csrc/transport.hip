@@ -35,13 +35,6 @@ struct TrackState {
   double carry;  // lower bound on the safety at p (0: none), see step_fp64
 };
 
-// Slack of the carried safety bound: rounding of the moves and the
-// tolerance-level slack of the facet tests.
-VGM_DEV double carry_margin(const V3<double> &p)
-{
-  return 1e-7 + 1e-12 * (vabs(p.x) + vabs(p.y) + vabs(p.z));
-}
-
 // One track-step in FP64 (see stepping.hpp for the definition).  Returns
 // true when the step ended on a boundary crossing.
 //
@@ -158,8 +151,6 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
 // the carried bound differs.  `verified` is set when the side check ran and
 // agreed (see learned_bound_kernel).  Launches of other families ignore the
 // distance-first request.
-constexpr double kDistanceFirstWindow = 1e-6;
-
 template <class S>
 VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified)
 {
@@ -205,6 +196,17 @@ VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, doubl
   return true;
 }
 
+// A step that moves by l without any shape query: under the carried bound of
+// a surface-distance family, or (culling launches) an outside move clear of
+// the bounding box.
+template <class S>
+VGM_DEV bool cheap_step(const TransportArgs &a, const SolidRec &sr, const TrackState &t, double l)
+{
+  if constexpr (S::kSafetyIsDistance)
+    if (l <= t.carry) return true;
+  return a.cull && !t.in && move_clears_box(sr.bc, sr.bh, t.p, t.d, l);
+}
+
 template <class S, int Mode>
 VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s, const Tile &tl,
                       uint32_t &fp64_steps)
@@ -233,24 +235,22 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
     while (k < a.nsteps) {
       uint32_t step = uint32_t(a.step0 + k);
       double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
-      if constexpr (S::kSafetyIsDistance) {
-        // Steps covered by the carried bound need no geometry query.  Each
-        // lane runs through its own run of them before the next query, so
-        // the lanes of a wave meet at the expensive step instead of idling
-        // through each other's cheap ones.
-        while (l <= t.carry) {
-          const double m = carry_margin(t.p);
-          t.p.x = t.p.x + l * t.d.x;
-          t.p.y = t.p.y + l * t.d.y;
-          t.p.z = t.p.z + l * t.d.z;
-          t.carry = t.carry - l - m;
-          finish(step, false);
-          if (++k >= a.nsteps) break;
-          step = uint32_t(a.step0 + k);
-          l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
-        }
-        if (k >= a.nsteps) break;
+      // Steps that need no geometry query (under the carried bound, or clear
+      // of the bounding box).  Each lane runs through its own run of them
+      // before the next query, so the lanes of a wave meet at the expensive
+      // step instead of idling through each other's cheap ones.
+      while (cheap_step<S>(a, sr, t, l)) {
+        const double m = carry_margin(t.p);
+        t.p.x = t.p.x + l * t.d.x;
+        t.p.y = t.p.y + l * t.d.y;
+        t.p.z = t.p.z + l * t.d.z;
+        if constexpr (S::kSafetyIsDistance) t.carry = t.carry - l - m;
+        finish(step, false);
+        if (++k >= a.nsteps) break;
+        step = uint32_t(a.step0 + k);
+        l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
       }
+      if (k >= a.nsteps) break;
       bool crossed;
       if constexpr (Mode == kTransportFP64) {
         bool verified = false;  // only the learned tiles read it
@@ -282,8 +282,8 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
 // Every family has its own kernel, so each gets the register allocation of
 // its own code path rather than the largest one of all families.
 template <class S, int Mode, bool Staged>
-__global__ __launch_bounds__(kTransportMaxBlock) void transport_family_kernel(TransportArgs a, int t0, int t1,
-                                                                              int *counter)
+__global__ __launch_bounds__(kTransportMaxBlock) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU))) void
+transport_family_kernel(TransportArgs a, int t0, int t1, int *counter)
 {
   extern __shared__ double lds_block[];
   __shared__ int tile_s;
@@ -407,8 +407,9 @@ __global__ __launch_bounds__(kTransportMaxBlock) void learned_bound_kernel(Trans
     t.carry = (act && k > 0) ? a.tcarry[i] : 0.0;
     bool verified = act && k > 0 && (a.tflags[i] & kTrackVerified);
     const double l = act ? physics_step(sr.lambda, a.seed, uint64_t(i), step) : 0.0;
-    const bool carried = act && l <= t.carry;
-    const bool want = act && !carried && net;
+    // no query under the carried bound or for an outside move clear of the box
+    const bool cheap = act && (l <= t.carry || (a.cull && !t.in && move_clears_box(sr.bc, sr.bh, t.p, t.d, l)));
+    const bool want = act && !cheap && net;
     double b = -1.0;
     if (__any(want)) {  // uniform: the whole wave evaluates the network
       const float y = mlp_eval_wave(*md.img, act_buf[threadIdx.x >> 6], mlp_coord(t.p.x, md.c[0], md.inv_hw[0]),
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(kTransportMaxBlock) void learned_bound_kernel(Trans
     }
     bool moved = false, queued = false;
     if (act) {
-      if (carried) {
+      if (cheap) {
         moved = true;
       } else if (want) {
         ++cnt.q;
@@ -588,7 +589,9 @@ struct FamilyLauncher {
   void operator()()
   {
     TransportArgs args = a;
-    args.distance_first = (S::kSafetyBelowDistance && (f.mode == kFamDistFirst || f.mode == kFamLearned)) ? 1 : 0;
+    const bool culled = f.mode == kFamCulled || f.mode == kFamLearned;
+    args.distance_first = (S::kSafetyBelowDistance && (f.mode == kFamDistFirst || culled)) ? 1 : 0;
+    args.cull = culled ? 1 : 0;
     if (f.mode == kFamMixed) {
       go<S, kTransportMixed>(args);
       return;
@@ -630,23 +633,6 @@ struct LearnableQuery {
 bool family_learnable(int type)
 {
   LearnableQuery q;
-  return dispatch_shape_type(type, q) && q.r;
-}
-
-namespace {
-struct DistanceFirstQuery {
-  bool r = false;
-  template <class S>
-  void operator()()
-  {
-    r = S::kSafetyBelowDistance;
-  }
-};
-}  // namespace
-
-bool family_distance_first(int type)
-{
-  DistanceFirstQuery q;
   return dispatch_shape_type(type, q) && q.r;
 }
 

@@ -73,12 +73,6 @@ struct Track {
   R carry;
 };
 
-template <typename R>
-R carry_margin(const V3<R> &p)
-{
-  return R(1e-7) + R(1e-12) * (vabs(p.x) + vabs(p.y) + vabs(p.z));
-}
-
 // The distance-first track-step (same decisions as step(), for families with
 // kSafetyBelowDistance), on scalar type R.
 template <class S, typename R>
@@ -94,7 +88,7 @@ bool step_distance_first(const typename S::Data &s, Track<R> &t, R l)
     t.p.z = t.p.z + l * t.d.z;
     return false;
   }
-  if (l <= db + R(1e-6)) {
+  if (l <= db + R(kDistanceFirstWindow)) {
     if constexpr (S::kSafetyIsDistance) {
       const R saf = S::template surface_distance<R>(s, t.p, t.in);
       if (l <= saf && S::template contains<R>(s, t.p) == t.in) {
@@ -178,16 +172,22 @@ V3<double> dbl(const V3<R> &a)
 }
 
 // Steps track i of the sample through nsteps track-steps on scalar type R;
-// returns the final position.
+// returns the final position.  `mode` bit 0: the distance-first step where
+// the family allows it (else the FP64 step); bit 1: outside moves clear of
+// the bounding box skip the queries (the culled mode of the kernels).
 template <class S, typename R>
 V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lambda, uint64_t seed, uint64_t i,
-                     const double *p0, const double *d0, int nsteps, bool distance_first, bool &in_out)
+                     const double *p0, const double *d0, int nsteps, int mode, bool &in_out)
 {
-  double wc[3], wh[3];
+  double wc[3], wh[3], bc[3], bh[3];
   for (int k = 0; k < 3; ++k) {
     wc[k] = 0.5 * (rec.bmin[k] + rec.bmax[k]);
     wh[k] = 1.3 * 0.5 * (rec.bmax[k] - rec.bmin[k]);
+    bc[k] = 0.5 * (rec.bmin[k] + rec.bmax[k]);
+    bh[k] = 0.5 * (rec.bmax[k] - rec.bmin[k]);
   }
+  const bool distance_first = (mode & 1) && S::kSafetyBelowDistance;
+  const bool cull = (mode & 2) != 0;
   Track<R> t;
   t.p = V3<R>{R(p0[0]), R(p0[1]), R(p0[2])};
   t.d = V3<R>{R(d0[0]), R(d0[1]), R(d0[2])};
@@ -200,7 +200,17 @@ V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lamb
     // a step under the carried bound is the same in both orders (step()
     // takes it first)
     const bool carried = S::kSafetyIsDistance && R(l) <= t.carry;
-    const bool crossed = (distance_first && !carried) ? step_distance_first<S, R>(s, t, R(l)) : step<S, R>(s, t, R(l));
+    bool crossed;
+    if (cull && !carried && !t.in && move_clears_box<R>(bc, bh, t.p, t.d, R(l))) {
+      const R m = carry_margin(t.p);
+      t.p.x = t.p.x + R(l) * t.d.x;
+      t.p.y = t.p.y + R(l) * t.d.y;
+      t.p.z = t.p.z + R(l) * t.d.z;
+      if constexpr (S::kSafetyIsDistance) t.carry = t.carry - R(l) - m;
+      crossed = false;
+    } else {
+      crossed = (distance_first && !carried) ? step_distance_first<S, R>(s, t, R(l)) : step<S, R>(s, t, R(l));
+    }
     if (!crossed) {
       const V3<double> d = new_direction(seed, i, st, dbl(t.d));
       t.d = V3<R>{R(d.x), R(d.y), R(d.z)};
@@ -227,7 +237,7 @@ struct Counter {
   size_t n;
   const double *const *soa;  // x, y, z, dx, dy, dz
   int nsteps;
-  bool distance_first;
+  int mode;
   uint64_t flops = 0;
   size_t mismatches = 0;
 
@@ -239,10 +249,9 @@ struct Counter {
       const double p0[3] = {soa[0][i], soa[1][i], soa[2][i]};
       const double d0[3] = {soa[3][i], soa[4][i], soa[5][i]};
       bool in_d = false, in_c = false;
-      const bool df = distance_first && S::kSafetyBelowDistance;
-      const V3<double> a = run_track<S, double>(s, rec, lambda, seed, first + i, p0, d0, nsteps, false, in_d);
+      const V3<double> a = run_track<S, double>(s, rec, lambda, seed, first + i, p0, d0, nsteps, 0, in_d);
       const uint64_t f0 = g_flops;
-      const V3<double> b = run_track<S, Counted>(s, rec, lambda, seed, first + i, p0, d0, nsteps, df, in_c);
+      const V3<double> b = run_track<S, Counted>(s, rec, lambda, seed, first + i, p0, d0, nsteps, mode, in_c);
       flops += g_flops - f0;
       if (std::memcmp(&a, &b, sizeof a) != 0 || in_d != in_c) ++mismatches;
     }
@@ -303,19 +312,19 @@ int cf_extent(cf_geom *h, int shape, double *mn, double *mx)
   return 0;
 }
 
-// FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape` (the
-// distance-first step when asked and the family allows it); sample track i
+// FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape` (`mode`
+// as run_track's); sample track i
 // draws its random numbers as track first + i of the transport.  Returns 0,
 // or -1 for a bad shape, or -2 when the counted run left the FP64 trajectory
 // of some track.
 int cf_count(cf_geom *h, int shape, double lambda, uint64_t seed, uint64_t first, size_t n, const double *x,
              const double *y, const double *z, const double *dx, const double *dy, const double *dz, int nsteps,
-             int distance_first, uint64_t *flops)
+             int mode, uint64_t *flops)
 {
   if (shape < 0 || size_t(shape) >= h->g.shapes.size()) return -1;
   const double *soa[6] = {x, y, z, dx, dy, dz};
   const ShapeRec &rec = h->g.shapes[size_t(shape)];
-  Counter c{h->g, rec, lambda, seed, first, n, soa, nsteps, distance_first != 0};
+  Counter c{h->g, rec, lambda, seed, first, n, soa, nsteps, mode};
   if (!dispatch_shape_type(rec.type, c)) return -1;
   *flops = c.flops;
   return c.mismatches ? -2 : 0;

@@ -90,11 +90,12 @@ class FlopGeometry:
         self._chk(self.L.cf_extent(self.h, s, _p(mn), _p(mx)))
         return mn, mx
 
-    def count(self, shape, lam, seed, first, soa, nsteps, distance_first=False):
+    def count(self, shape, lam, seed, first, soa, nsteps, mode=0):
+        """mode: 0 FP64 step, 1 distance-first, 3 culled (distance-first and bounding-box cull)."""
         out = ctypes.c_uint64(0)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in soa]
         rc = self.L.cf_count(self.h, shape, lam, seed, first, len(arrs[0]), *[a.ctypes.data_as(_P) for a in arrs],
-                             nsteps, 1 if distance_first else 0, ctypes.byref(out))
+                             nsteps, mode, ctypes.byref(out))
         if rc == -2:
             raise RuntimeError("counted run left the FP64 trajectory")
         self._chk(rc)
@@ -111,7 +112,7 @@ def main():
     g = FlopGeometry()
     specs, counts = workloads.batched_solids(args.tracks, per_family=8, seed=42)
     offsets = np.concatenate([[0], np.cumsum(counts)])
-    per_solid, per_solid_df = [], []
+    per_solid, per_solid_df, per_solid_cu = [], [], []
     t0 = time.time()
     for j, ((kind, a), c) in enumerate(zip(specs, counts)):
         s = workloads.make_shape(g, kind, a)
@@ -121,10 +122,13 @@ def main():
         soa = [pts[0][:k], pts[1][:k], pts[2][:k], dirs[0][:k], dirs[1][:k], dirs[2][:k]]
         lam = workloads.mean_step(ext)
         f = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps)
-        fd = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps, distance_first=True)
+        fd = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps, mode=1)
+        fc = g.count(s, lam, 7, int(offsets[j]), soa, args.track_steps, mode=3)
         per_solid.append(f / (k * args.track_steps))
         per_solid_df.append(fd / (k * args.track_steps))
+        per_solid_cu.append(fc / (k * args.track_steps))
         print(f"{j:2d} {kind:12s} {per_solid[-1]:10.1f} flops/track-step, distance-first {per_solid_df[-1]:10.1f} "
+              f"culled {per_solid_cu[-1]:10.1f} "
               f"({time.time() - t0:.0f}s)", flush=True)
     def by_family(vals):
         acc = {}
@@ -139,7 +143,8 @@ def main():
         "description": "algorithmic FP64 flops (+ - * / sqrt) per track-step of the batched transport, "
                        "steps 0..track_steps-1 of the first `sample` tracks of each solid; *_distance_first: the "
                        "distance-first step (same decisions; families without kSafetyBelowDistance count "
-                       "the FP64 step)",
+                       "the FP64 step); *_culled: the distance-first step, and outside moves clear of the "
+                       "solid's bounding box skip the queries (the bounding-box test is counted)",
         "workload": {"tracks": int(sum(counts)), "solids": len(specs), "seed": 42, "transport_seed": 7,
                      "track_steps": args.track_steps, "sample": args.sample},
         "families": [kind for kind, _ in specs],
@@ -148,6 +153,8 @@ def main():
         "per_family": by_family(per_solid),
         "per_solid_distance_first": per_solid_df,
         "per_family_distance_first": by_family(per_solid_df),
+        "per_solid_culled": per_solid_cu,
+        "per_family_culled": by_family(per_solid_cu),
         "fp64_flops_per_track_step": total,
     }
     with open(args.out, "w") as fh:
