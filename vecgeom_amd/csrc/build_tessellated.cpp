@@ -70,42 +70,108 @@ bool make_facet(TessFacet &f, const T3 v[3], int orig)
   return true;
 }
 
+// Facet tree.  Splits follow a binned surface-area heuristic over the facet
+// centroids (fewer boxes per ray than median splits); a tree deeper than the
+// device walk allows is rebuilt with median splits on the longest axis.
+// Node boxes are the facets' vertex boxes enlarged by `margin`.
 struct Builder {
+  // binary tree of the build: node boxes, children (-1: leaf), facet range
+  struct Node {
+    double lo[3], hi[3];
+    int left, right, first, count;
+  };
   std::vector<TessFacet> &facets;  // reordered in place into leaf order
-  std::vector<TessNode> nodes;
+  std::vector<Node> nodes;
   double margin;
   int leaf = 1;  // facets per leaf
+  bool sah = true;
+  int max_depth = 0;
+
+  static constexpr int kBins = 32;
 
   static double centroid(const TessFacet &f, int c) { return f.v[0][c] + f.v[1][c] + f.v[2][c]; }
 
-  int max_depth = 0;
-
-  int build(int b, int e, int depth = 0)
-  {
-    max_depth = std::max(max_depth, depth);
-    const int id = int(nodes.size());
-    nodes.push_back(TessNode{});
+  struct Box {
     double lo[3] = {1e308, 1e308, 1e308}, hi[3] = {-1e308, -1e308, -1e308};
-    double clo[3] = {1e308, 1e308, 1e308}, chi[3] = {-1e308, -1e308, -1e308};
-    for (int i = b; i < e; ++i)
-      for (int c = 0; c < 3; ++c) {
-        for (int k = 0; k < 3; ++k) {
-          lo[c] = std::min(lo[c], facets[i].v[k][c]);
-          hi[c] = std::max(hi[c], facets[i].v[k][c]);
+    void add(const TessFacet &f)
+    {
+      for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = std::min(lo[c], f.v[k][c]);
+          hi[c] = std::max(hi[c], f.v[k][c]);
         }
-        clo[c] = std::min(clo[c], centroid(facets[i], c));
-        chi[c] = std::max(chi[c], centroid(facets[i], c));
+    }
+    void add(const Box &o)
+    {
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], o.lo[c]);
+        hi[c] = std::max(hi[c], o.hi[c]);
       }
+    }
+    double half_area() const
+    {
+      const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+      return dx < 0 ? 0.0 : dx * dy + dy * dz + dz * dx;
+    }
+  };
+
+  static int bin(double x, double lo, double scale)
+  {
+    const int k = int((x - lo) * scale);
+    return k < 0 ? 0 : (k >= kBins ? kBins - 1 : k);
+  }
+
+  // SAH split of [b, e): returns the split position, or -1 when no bin
+  // boundary separates the centroids.
+  int sah_split(int b, int e, const double *clo, const double *chi)
+  {
+    double best = 1e308;
+    int best_axis = -1, best_k = -1;
     for (int c = 0; c < 3; ++c) {
-      nodes[id].lo[c] = lo[c] - margin;
-      nodes[id].hi[c] = hi[c] + margin;
+      if (!(chi[c] > clo[c])) continue;
+      const double scale = kBins / (chi[c] - clo[c]);
+      Box bins[kBins];
+      int cnt[kBins] = {0};
+      for (int i = b; i < e; ++i) {
+        const int k = bin(centroid(facets[i], c), clo[c], scale);
+        ++cnt[k];
+        bins[k].add(facets[i]);
+      }
+      Box right[kBins];
+      int rcnt[kBins];
+      Box acc;
+      int n = 0;
+      for (int k = kBins - 1; k >= 0; --k) {
+        acc.add(bins[k]);
+        n += cnt[k];
+        right[k] = acc;
+        rcnt[k] = n;
+      }
+      Box left;
+      int lc = 0;
+      for (int k = 0; k + 1 < kBins; ++k) {
+        left.add(bins[k]);
+        lc += cnt[k];
+        if (lc == 0 || rcnt[k + 1] == 0) continue;
+        const double cost = left.half_area() * lc + right[k + 1].half_area() * rcnt[k + 1];
+        if (cost < best) {
+          best = cost;
+          best_axis = c;
+          best_k = k;
+        }
+      }
     }
-    if (e - b <= leaf) {
-      nodes[id].left = nodes[id].right = -1;
-      nodes[id].first = b;
-      nodes[id].count = e - b;
-      return id;
-    }
+    if (best_axis < 0) return -1;
+    const double scale = kBins / (chi[best_axis] - clo[best_axis]);
+    const auto it = std::stable_partition(facets.begin() + b, facets.begin() + e, [&](const TessFacet &f) {
+      return bin(centroid(f, best_axis), clo[best_axis], scale) <= best_k;
+    });
+    const int mid = int(it - facets.begin());
+    return (mid == b || mid == e) ? -1 : mid;
+  }
+
+  int median_split(int b, int e, const double *clo, const double *chi)
+  {
     int axis = 0;
     for (int c = 1; c < 3; ++c)
       if (chi[c] - clo[c] > chi[axis] - clo[axis]) axis = c;
@@ -115,12 +181,81 @@ struct Builder {
                        const double cx = centroid(x, axis), cy = centroid(y, axis);
                        return cx < cy || (cx == cy && x.orig < y.orig);
                      });
+    return mid;
+  }
+
+  int build(int b, int e, int depth = 0)
+  {
+    max_depth = std::max(max_depth, depth);
+    const int id = int(nodes.size());
+    nodes.push_back(Node{});
+    Box box;
+    double clo[3] = {1e308, 1e308, 1e308}, chi[3] = {-1e308, -1e308, -1e308};
+    for (int i = b; i < e; ++i) {
+      box.add(facets[i]);
+      for (int c = 0; c < 3; ++c) {
+        clo[c] = std::min(clo[c], centroid(facets[i], c));
+        chi[c] = std::max(chi[c], centroid(facets[i], c));
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      nodes[id].lo[c] = box.lo[c] - margin;
+      nodes[id].hi[c] = box.hi[c] + margin;
+    }
+    if (e - b <= leaf) {
+      nodes[id].left = nodes[id].right = -1;
+      nodes[id].first = b;
+      nodes[id].count = e - b;
+      return id;
+    }
+    int mid = sah ? sah_split(b, e, clo, chi) : -1;
+    if (mid < 0) mid = median_split(b, e, clo, chi);
     const int l = build(b, mid, depth + 1);
     const int r = build(mid, e, depth + 1);
     nodes[id].left = l;
     nodes[id].right = r;
     nodes[id].first = nodes[id].count = 0;
     return id;
+  }
+
+  // The device layout: one record per internal node, preorder, holding the
+  // boxes of both children; leaves are folded into their parents.  A tree
+  // that is a single leaf gets a root whose second child is empty.
+  std::vector<TessNode> wide() const
+  {
+    std::vector<TessNode> out;
+    std::vector<int> map(nodes.size(), -1);
+    std::vector<int> todo{0};
+    while (!todo.empty()) {  // preorder numbering of the internal nodes
+      const int o = todo.back();
+      todo.pop_back();
+      map[size_t(o)] = int(out.size());
+      out.push_back(TessNode{});
+      const Node &nd = nodes[size_t(o)];
+      if (nd.left < 0) break;  // single-leaf tree
+      if (nodes[size_t(nd.right)].left >= 0) todo.push_back(nd.right);
+      if (nodes[size_t(nd.left)].left >= 0) todo.push_back(nd.left);
+    }
+    for (size_t o = 0; o < nodes.size(); ++o) {
+      if (map[o] < 0) continue;
+      TessNode &w = out[size_t(map[o])];
+      const Node &nd = nodes[o];
+      const int kids[2] = {nd.left, nd.right};
+      for (int i = 0; i < 2; ++i) {
+        const Node &c = kids[0] < 0 ? nd : nodes[size_t(kids[i])];
+        const bool empty = kids[0] < 0 && i == 1;
+        for (int k = 0; k < 3; ++k) {
+          w.lo[i][k] = c.lo[k];
+          w.hi[i][k] = c.hi[k];
+        }
+        const bool leaf = kids[0] < 0 || c.left < 0;
+        w.child[i] = leaf ? -1 : map[size_t(kids[i])];
+        w.first[i] = leaf ? c.first : 0;
+        w.count[i] = (leaf && !empty) ? c.count : 0;
+      }
+      w.pad[0] = w.pad[1] = 0;
+    }
+    return out;
   }
 };
 
@@ -157,17 +292,26 @@ int build_tessellated(HostGeometry &g, int nfacets, const double *tri, const dou
   for (int c = 0; c < 3; ++c) big = std::max(big, std::max(std::fabs(mn[c]), std::fabs(mx[c])));
   // One facet per leaf walks fastest (box tests are cheaper than facet
   // tests); larger solids use the smallest leaf size whose tree keeps node
-  // indices within 16 bits.
+  // indices within 16 bits.  The input facet order is kept so that every
+  // rebuild starts from the same sequence.
+  const std::vector<TessFacet> input = facets;
   Builder B{facets, {}, 1e-7 + 1e-12 * big};
   for (;;) {
+    facets = input;
     B.nodes.clear();
     B.max_depth = 0;
     B.nodes.reserve(2 * size_t(n) / size_t(B.leaf) + 1);
     B.build(0, n);
-    if (B.nodes.size() <= 65535 || B.leaf >= n) break;
+    if (B.max_depth > kTessWalkDepth - 2 && B.sah) {
+      B.sah = false;  // median splits: depth log2(n / leaf)
+      continue;
+    }
+    // internal nodes: half of the binary tree's nodes
+    if (B.nodes.size() / 2 <= 65535 || B.leaf >= n) break;
     B.leaf *= 2;
   }
-  const int nn = int(B.nodes.size());
+  const std::vector<TessNode> wide = B.wide();
+  const int nn = int(wide.size());
   // the device walks keep 16-bit node indices, at most one deferred sibling
   // per level, on a stack of kTessWalkDepth entries
   if (nn > 65535 || B.max_depth > kTessWalkDepth - 2) {
@@ -184,7 +328,7 @@ int build_tessellated(HostGeometry &g, int nfacets, const double *tri, const dou
     h->test_dir[c] = test_dir ? test_dir[c] : kTessTestDir[c];
   }
   std::copy(facets.begin(), facets.end(), const_cast<TessFacet *>(h->facets()));
-  std::copy(B.nodes.begin(), B.nodes.end(), const_cast<TessNode *>(h->nodes()));
+  std::copy(wide.begin(), wide.end(), const_cast<TessNode *>(h->nodes()));
   return g.add_shape(kTessellated, block.data(), block.size(), n, nn, mn, mx);
 }
 

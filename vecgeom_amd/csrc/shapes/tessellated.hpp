@@ -15,35 +15,61 @@
 
 namespace vgm {
 
-// Tree-walk stack.  On the device it lives in LDS, one column of 16-bit node
-// indices per thread of the workgroup (a private array would be spilled to
-// scratch memory, whose traffic dominated the walks); on the host it is an
-// ordinary local array.  The tree builder keeps node indices below 65536 and
-// the depth below kTessWalkDepth - 2, so a nearest-first walk (one deferred
-// sibling per level) never overflows.
+// Tree-walk stack.  An entry is a node index and a lower bound of the
+// node's key (the entry distance of its box along the ray, or its squared
+// distance from the point) in the upper half of a float, so a popped node
+// whose key exceeds the current bound is dropped without reading it.  On the
+// device the stack lives in LDS, one column per thread of the workgroup (a
+// private array would be spilled to scratch memory, whose traffic dominated
+// the walks); on the host it is an ordinary local array.  The tree builder
+// keeps node indices below 65536 and the depth below kTessWalkDepth - 2, so a
+// nearest-first walk (one deferred sibling per level) never overflows.
 constexpr int kTessWalkDepth = 32;
 constexpr int kTessWalkThreads = 256;  // largest workgroup of any kernel that walks
 #if defined(__HIP_DEVICE_COMPILE__)
-static __shared__ uint16_t tess_walk_lds[kTessWalkDepth * kTessWalkThreads];
+static __shared__ uint32_t tess_walk_lds[kTessWalkDepth * kTessWalkThreads];
 struct TessWalkStack {
-  uint16_t *col;
+  uint32_t *col;
   __device__ TessWalkStack() : col(tess_walk_lds + threadIdx.x) {}
-  __device__ void set(int i, int v) { col[i * kTessWalkThreads] = uint16_t(v); }
-  __device__ int get(int i) const { return col[i * kTessWalkThreads]; }
+  __device__ void set(int i, uint32_t v) { col[i * kTessWalkThreads] = v; }
+  __device__ uint32_t get(int i) const { return col[i * kTessWalkThreads]; }
 };
 #else
 struct TessWalkStack {
-  int a[kTessWalkDepth];
-  void set(int i, int v) { a[i] = v; }
-  int get(int i) const { return a[i]; }
+  uint32_t a[kTessWalkDepth];
+  void set(int i, uint32_t v) { a[i] = v; }
+  uint32_t get(int i) const { return a[i]; }
 };
 #endif
+
+// Stack entry of node `node` with key x: the 16 upper bits of a float at or
+// below x (a negative x, which never prunes, is stored as -infinity).
+template <typename R>
+VGM_DEV uint32_t tess_walk_entry(int node, R x)
+{
+  uint32_t key = 0xff80u;  // -inf
+  if (x >= R(0)) {
+    const float f = float(x);
+    uint32_t bits = __builtin_bit_cast(uint32_t, f);
+    if (R(f) > x) bits -= 1u;  // the next float below
+    key = bits >> 16;          // truncation rounds a positive float down
+  }
+  return (key << 16) | uint32_t(node);
+}
+template <typename R>
+VGM_DEV R tess_walk_key(uint32_t e)
+{
+  return R(__builtin_bit_cast(float, e & 0xffff0000u));
+}
 
 struct TessellatedShape {
   using Data = TessData;
   static constexpr bool kStage = false;  // parameter block staged in LDS
-  // waves per SIMD the transport kernel asks the compiler for (1: no request)
-  static constexpr int kWavesPerEU = 1;
+  // waves per SIMD the transport kernel asks the compiler for: the latency
+  // of the tree walks hides better at 4 waves than at the 2 its
+  // unconstrained register allocation allows, although some values then
+  // spill to scratch (measured: 214 -> 164 ms for 50 steps of 3M tracks)
+  static constexpr int kWavesPerEU = 4;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = true;
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
@@ -105,28 +131,33 @@ struct TessellatedShape {
     return e;
   }
 
+  // Entry distance of the ray into child box i (-: the box contains p),
+  // infinity when the ray misses it.
   template <typename R>
-  VGM_DEV static R box_entry(const TessNode &nd, V3<R> p, V3<R> inv)
+  VGM_DEV static R box_entry(const TessNode &nd, int i, V3<R> p, V3<R> inv)
   {
-    R a = (R(nd.lo[0]) - p.x) * inv.x, b = (R(nd.hi[0]) - p.x) * inv.x;
+    const double *lo = nd.lo[i], *hi = nd.hi[i];
+    R a = (R(lo[0]) - p.x) * inv.x, b = (R(hi[0]) - p.x) * inv.x;
     R t0 = vmin(a, b), t1 = vmax(a, b);
-    a = (R(nd.lo[1]) - p.y) * inv.y;
-    b = (R(nd.hi[1]) - p.y) * inv.y;
+    a = (R(lo[1]) - p.y) * inv.y;
+    b = (R(hi[1]) - p.y) * inv.y;
     t0 = vmax(t0, vmin(a, b));
     t1 = vmin(t1, vmax(a, b));
-    a = (R(nd.lo[2]) - p.z) * inv.z;
-    b = (R(nd.hi[2]) - p.z) * inv.z;
+    a = (R(lo[2]) - p.z) * inv.z;
+    b = (R(hi[2]) - p.z) * inv.z;
     t0 = vmax(t0, vmin(a, b));
     t1 = vmin(t1, vmax(a, b));
     return (t0 <= t1) ? t0 : inf_length<R>();
   }
 
+  // Squared distance from p to child box i.
   template <typename R>
-  VGM_DEV static R box_dist2(const TessNode &nd, V3<R> p)
+  VGM_DEV static R box_dist2(const TessNode &nd, int i, V3<R> p)
   {
-    const R ex = vmax(vmax(R(nd.lo[0]) - p.x, p.x - R(nd.hi[0])), R(0));
-    const R ey = vmax(vmax(R(nd.lo[1]) - p.y, p.y - R(nd.hi[1])), R(0));
-    const R ez = vmax(vmax(R(nd.lo[2]) - p.z, p.z - R(nd.hi[2])), R(0));
+    const double *lo = nd.lo[i], *hi = nd.hi[i];
+    const R ex = vmax(vmax(R(lo[0]) - p.x, p.x - R(hi[0])), R(0));
+    const R ey = vmax(vmax(R(lo[1]) - p.y, p.y - R(hi[1])), R(0));
+    const R ez = vmax(vmax(R(lo[2]) - p.z, p.z - R(hi[2])), R(0));
     return ex * ex + ey * ey + ez * ez;
   }
 
@@ -148,50 +179,55 @@ struct TessellatedShape {
     const TessFacet *fa = t.facets();
     const TessNode *nodes = t.nodes();
     const V3<R> inv{R(1) / nonzero(d.x), R(1) / nonzero(d.y), R(1) / nonzero(d.z)};
-    // nearest-first walk: the nearer child is pushed last; every node is
-    // re-tested against the (shrinking) bound when it is popped
+    // nearest-first walk: at each node the leaf children are tested (nearer
+    // first), then the internal ones pushed (nearer last, so it is popped
+    // first); a popped node whose key exceeds the shrinking bound is dropped
+    auto bound = [&]() {
+      return to_in ? (h.iIn >= 0 ? h.dIn : inf_length<R>()) : (h.iOut >= 0 ? h.dOut : inf_length<R>());
+    };
     TessWalkStack stack;
     int sp = 0;
-    stack.set(sp++, 0);
+    stack.set(sp++, tess_walk_entry<R>(0, R(-1)));
     while (sp) {
-      const TessNode &nd = nodes[stack.get(--sp)];
-      const R bound = to_in ? (h.iIn >= 0 ? h.dIn : inf_length<R>()) : (h.iOut >= 0 ? h.dOut : inf_length<R>());
-      const R te = box_entry<R>(nd, p, inv);
-      if (te == inf_length<R>() || te > bound) continue;
-      if (nd.left >= 0) {
-        const R tl = box_entry<R>(nodes[nd.left], p, inv), tr = box_entry<R>(nodes[nd.right], p, inv);
-        const bool l_ok = !(tl == inf_length<R>() || tl > bound), r_ok = !(tr == inf_length<R>() || tr > bound);
-        if (l_ok && r_ok) {
-          const bool left_first = tl <= tr;
-          stack.set(sp++, left_first ? nd.right : nd.left);
-          stack.set(sp++, left_first ? nd.left : nd.right);
-        } else if (l_ok) {
-          stack.set(sp++, nd.left);
-        } else if (r_ok) {
-          stack.set(sp++, nd.right);
-        }
-        continue;
-      }
-      for (int k = nd.first; k < nd.first + nd.count; ++k) {
-        const TessFacet &f = fa[k];
-        const R ndd = nonzero(d.x * R(f.n[0]) + d.y * R(f.n[1]) + d.z * R(f.n[2]));
-        const R saf = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
-        const bool vIn = ndd < R(0) && saf > R(-kTolerance);
-        const bool vOut = ndd > R(0) && saf < R(kTolerance);
-        if (!vIn && !vOut) continue;
-        const R dist = -saf / ndd;
-        if (!facet_contains<R>(f, p.x + d.x * dist, p.y + d.y * dist, p.z + d.z * dist)) continue;
-        if (vIn) {
-          if (dist < h.dIn || (dist == h.dIn && f.orig < h.oIn)) {
-            h.dIn = dist;
-            h.iIn = k;
-            h.oIn = f.orig;
+      const uint32_t e = stack.get(--sp);
+      if (tess_walk_key<R>(e) > bound()) continue;
+      const TessNode &nd = nodes[e & 0xffffu];
+      const R t0 = box_entry<R>(nd, 0, p, inv), t1 = box_entry<R>(nd, 1, p, inv);
+      const int near = (t1 < t0) ? 1 : 0;
+#pragma unroll 1
+      for (int o = 0; o < 2; ++o) {
+        const int i = near ^ o;
+        const R te = i ? t1 : t0;
+        if (nd.child[i] >= 0 || te == inf_length<R>() || te > bound()) continue;
+        const int f0 = nd.first[i], f1 = f0 + nd.count[i];
+        for (int k = f0; k < f1; ++k) {
+          const TessFacet &f = fa[k];
+          const R ndd = nonzero(d.x * R(f.n[0]) + d.y * R(f.n[1]) + d.z * R(f.n[2]));
+          const R saf = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
+          const bool vIn = ndd < R(0) && saf > R(-kTolerance);
+          const bool vOut = ndd > R(0) && saf < R(kTolerance);
+          if (!vIn && !vOut) continue;
+          const R dist = -saf / ndd;
+          if (!facet_contains<R>(f, p.x + d.x * dist, p.y + d.y * dist, p.z + d.z * dist)) continue;
+          if (vIn) {
+            if (dist < h.dIn || (dist == h.dIn && f.orig < h.oIn)) {
+              h.dIn = dist;
+              h.iIn = k;
+              h.oIn = f.orig;
+            }
+          } else if (dist < h.dOut || (dist == h.dOut && f.orig < h.oOut)) {
+            h.dOut = dist;
+            h.iOut = k;
+            h.oOut = f.orig;
           }
-        } else if (dist < h.dOut || (dist == h.dOut && f.orig < h.oOut)) {
-          h.dOut = dist;
-          h.iOut = k;
-          h.oOut = f.orig;
         }
+      }
+#pragma unroll 1
+      for (int o = 1; o >= 0; --o) {
+        const int i = near ^ o;
+        const R te = i ? t1 : t0;
+        if (nd.child[i] < 0 || te == inf_length<R>() || te > bound()) continue;
+        stack.set(sp++, tess_walk_entry<R>(nd.child[i], te));
       }
     }
   }
@@ -204,27 +240,29 @@ struct TessellatedShape {
     R best = inf_length<R>();
     TessWalkStack stack;
     int sp = 0;
-    stack.set(sp++, 0);
+    stack.set(sp++, tess_walk_entry<R>(0, R(-1)));
     while (sp) {
-      const TessNode &nd = nodes[stack.get(--sp)];
-      if (box_dist2<R>(nd, p) > best) continue;
-      if (nd.left >= 0) {  // nearer child last, so it is visited first
-        const R dl = box_dist2<R>(nodes[nd.left], p), dr = box_dist2<R>(nodes[nd.right], p);
-        const bool l_ok = !(dl > best), r_ok = !(dr > best);
-        if (l_ok && r_ok) {
-          const bool left_first = dl <= dr;
-          stack.set(sp++, left_first ? nd.right : nd.left);
-          stack.set(sp++, left_first ? nd.left : nd.right);
-        } else if (l_ok) {
-          stack.set(sp++, nd.left);
-        } else if (r_ok) {
-          stack.set(sp++, nd.right);
+      const uint32_t e = stack.get(--sp);
+      if (tess_walk_key<R>(e) > best) continue;
+      const TessNode &nd = nodes[e & 0xffffu];
+      const R d0 = box_dist2<R>(nd, 0, p), d1 = box_dist2<R>(nd, 1, p);
+      const int near = (d1 < d0) ? 1 : 0;
+#pragma unroll 1
+      for (int o = 0; o < 2; ++o) {
+        const int i = near ^ o;
+        if (nd.child[i] >= 0 || (i ? d1 : d0) > best) continue;
+        const int f0 = nd.first[i], f1 = f0 + nd.count[i];
+        for (int k = f0; k < f1; ++k) {
+          const R s = facet_safety2<R, ToIn>(fa[k], p);
+          if (s < best) best = s;
         }
-        continue;
       }
-      for (int k = nd.first; k < nd.first + nd.count; ++k) {
-        const R s = facet_safety2<R, ToIn>(fa[k], p);
-        if (s < best) best = s;
+#pragma unroll 1
+      for (int o = 1; o >= 0; --o) {
+        const int i = near ^ o;
+        const R di = i ? d1 : d0;
+        if (nd.child[i] < 0 || di > best) continue;
+        stack.set(sp++, tess_walk_entry<R>(nd.child[i], di));
       }
     }
     return best;

@@ -18,12 +18,16 @@ struct TessFacet {   // 24 doubles
 };
 static_assert(sizeof(TessFacet) == 24 * sizeof(double), "facet is 24 doubles");
 
-struct TessNode {    // 8 doubles
-  double lo[3], hi[3];  // box enlarged by a margin (conservative pruning)
-  int32_t left, right;  // children; left < 0 marks a leaf
-  int32_t first, count; // leaf: facet range
+// Internal node of the facet tree, holding the boxes of both children so a
+// walk reads one node per level.  A child is another node (child >= 0) or a
+// leaf: the facets [first, first + count) of the leaf order (child < 0).
+struct TessNode {      // 16 doubles
+  double lo[2][3], hi[2][3];  // child boxes, enlarged by a margin (conservative pruning)
+  int32_t child[2];
+  int32_t first[2], count[2];
+  int32_t pad[2];
 };
-static_assert(sizeof(TessNode) == 8 * sizeof(double), "node is 8 doubles");
+static_assert(sizeof(TessNode) == 16 * sizeof(double), "node is 16 doubles");
 
 struct TessData {
   int32_t nfacets, nnodes;

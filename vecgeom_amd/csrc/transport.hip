@@ -474,7 +474,8 @@ __global__ __launch_bounds__(kTransportMaxBlock) void learned_bound_kernel(Trans
 // Step k of the queued tracks (grid-stride over the queue).  Also clears the
 // queue counter of step k + 1, which the next bound kernel fills.
 template <class S>
-__global__ __launch_bounds__(kTransportMaxBlock) void learned_fallback_kernel(TransportArgs a, int k)
+__global__ __launch_bounds__(kTransportMaxBlock) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU))) void
+learned_fallback_kernel(TransportArgs a, int k)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) a.qcount[(k + 1) & 1] = 0;
   const int nq = a.qcount[k & 1];
