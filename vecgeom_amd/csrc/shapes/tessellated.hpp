@@ -109,14 +109,15 @@ struct TessellatedShape {
     return ex * ex + ey * ey + ez * ez;
   }
 
-  // TessellatedCluster::SafetySq for one facet
-  template <typename R, bool ToIn>
-  VGM_DEV static R facet_safety2(const TessFacet &f, V3<R> p)
+  // TessellatedCluster::SafetySq for one facet.  The side is a run-time
+  // argument so that lanes on both sides run one copy of the walk.
+  template <typename R>
+  VGM_DEV static R facet_safety2(const TessFacet &f, V3<R> p, bool to_in)
   {
     R s = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
     const R xx = p.x - R(f.n[0]) * s, xy = p.y - R(f.n[1]) * s, xz = p.z - R(f.n[2]) * s;
     bool within = facet_contains<R>(f, xx, xy, xz);
-    if (ToIn)
+    if (to_in)
       within &= s > R(-kTolerance);
     else
       within &= s < R(kTolerance);
@@ -232,8 +233,8 @@ struct TessellatedShape {
     }
   }
 
-  template <typename R, bool ToIn>
-  VGM_DEV static R safety2(const Data &t, V3<R> p)
+  template <typename R>
+  VGM_DEV static R safety2(const Data &t, V3<R> p, bool to_in)
   {
     const TessFacet *fa = t.facets();
     const TessNode *nodes = t.nodes();
@@ -253,7 +254,7 @@ struct TessellatedShape {
         if (nd.child[i] >= 0 || (i ? d1 : d0) > best) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
-          const R s = facet_safety2<R, ToIn>(fa[k], p);
+          const R s = facet_safety2<R>(fa[k], p, to_in);
           if (s < best) best = s;
         }
       }
@@ -293,9 +294,11 @@ struct TessellatedShape {
     return tmin;
   }
 
-  // DistanceToSolid<ToIn> (TessellatedImplementation.h:176-279)
-  template <typename R, bool ToIn>
-  VGM_DEV static R distance_to_solid(const Data &t, V3<R> p, V3<R> d, R step_max, Hits<R> &h, int &isurf)
+  // DistanceToSolid<ToIn> (TessellatedImplementation.h:176-279), the side a
+  // run-time argument
+  template <typename R>
+  VGM_DEV static R distance_to_solid(const Data &t, V3<R> p, V3<R> d, R step_max, bool ToIn, Hits<R> &h,
+                                     int &isurf)
   {
     isurf = -1;
     R distance;
@@ -349,7 +352,7 @@ struct TessellatedShape {
   {
     Hits<R> h;
     int isurf;
-    distance_to_solid<R, false>(t, p, test_dir<R>(t), inf_length<R>(), h, isurf);
+    distance_to_solid<R>(t, p, test_dir<R>(t), inf_length<R>(), false, h, isurf);
     return isurf >= 0;
   }
 
@@ -359,7 +362,7 @@ struct TessellatedShape {
     Hits<R> h;
     int iOut;
     const V3<R> td = test_dir<R>(t);
-    const R dOut = distance_to_solid<R, false>(t, p, td, inf_length<R>(), h, iOut);
+    const R dOut = distance_to_solid<R>(t, p, td, inf_length<R>(), false, h, iOut);
     if (iOut < 0) return kOutsideV;
     const TessFacet *fa = t.facets();
     if (dOut < R(0) || dOut * (td.x * R(fa[iOut].n[0]) + td.y * R(fa[iOut].n[1]) + td.z * R(fa[iOut].n[2])) <
@@ -377,7 +380,7 @@ struct TessellatedShape {
   {
     Hits<R> h;
     int isurf;
-    return distance_to_solid<R, true>(t, p, d, step_max, h, isurf);
+    return distance_to_solid<R>(t, p, d, step_max, true, h, isurf);
   }
 
   template <typename R>
@@ -385,7 +388,17 @@ struct TessellatedShape {
   {
     Hits<R> h;
     int isurf;
-    return distance_to_solid<R, false>(t, p, d, step_max, h, isurf);
+    return distance_to_solid<R>(t, p, d, step_max, false, h, isurf);
+  }
+
+  // DistanceToOut (inside) or DistanceToIn (outside) in one walk, so the
+  // lanes of a wave on both sides of the surface share it.
+  template <typename R>
+  VGM_DEV static R boundary_distance(const Data &t, V3<R> p, V3<R> d, bool inside)
+  {
+    Hits<R> h;
+    int isurf;
+    return distance_to_solid<R>(t, p, d, inf_length<R>(), !inside, h, isurf);
   }
 
   // Distance from p to the surface, without the side check of
@@ -394,21 +407,21 @@ struct TessellatedShape {
   template <typename R>
   VGM_DEV static R surface_distance(const Data &t, V3<R> p, bool from_inside)
   {
-    return from_inside ? vsqrt(safety2<R, false>(t, p)) : vsqrt(safety2<R, true>(t, p));
+    return vsqrt(safety2<R>(t, p, !from_inside));
   }
 
   template <typename R>
   VGM_DEV static R safety_to_in(const Data &t, V3<R> p)
   {
     if (contains<R>(t, p)) return R(-1);
-    return vsqrt(safety2<R, true>(t, p));
+    return vsqrt(safety2<R>(t, p, true));
   }
 
   template <typename R>
   VGM_DEV static R safety_to_out(const Data &t, V3<R> p)
   {
     if (!contains<R>(t, p)) return R(-1);
-    return vsqrt(safety2<R, false>(t, p));
+    return vsqrt(safety2<R>(t, p, false));
   }
 };
 

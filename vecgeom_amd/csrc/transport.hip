@@ -35,6 +35,24 @@ struct TrackState {
   double carry;  // lower bound on the safety at p (0: none), see step_fp64
 };
 
+// DistanceToOut from inside, DistanceToIn from outside.  A family that
+// provides boundary_distance runs both in one code path, so a wave whose
+// lanes sit on both sides walks once instead of once per side.
+template <class S, class = void>
+struct HasBoundaryDistance : std::false_type {};
+template <class S>
+struct HasBoundaryDistance<S, std::void_t<decltype(&S::template boundary_distance<double>)>> : std::true_type {};
+
+template <class S>
+VGM_DEV double boundary_distance(const typename S::Data &s, const V3<double> &p, const V3<double> &d, bool in)
+{
+  if constexpr (HasBoundaryDistance<S>::value)
+    return S::template boundary_distance<double>(s, p, d, in);
+  else
+    return in ? S::template distance_to_out<double>(s, p, d, kInfLength)
+              : S::template distance_to_in<double>(s, p, d, kInfLength);
+}
+
 // One track-step in FP64 (see stepping.hpp for the definition).  Returns
 // true when the step ended on a boundary crossing.
 //
@@ -76,8 +94,7 @@ VGM_DEV bool step_fp64(const typename S::Data &s, TrackState &t, double l)
     return false;
   }
   t.carry = 0.0;
-  double db = t.in ? S::template distance_to_out<double>(s, t.p, t.d, kInfLength)
-                   : S::template distance_to_in<double>(s, t.p, t.d, kInfLength);
+  double db = boundary_distance<S>(s, t.p, t.d, t.in);
   if (db < 0.0) db = 0.0;
   if (l < db) {
     t.p.x = t.p.x + l * t.d.x;
@@ -154,8 +171,7 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
 template <class S>
 VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified)
 {
-  double db = t.in ? S::template distance_to_out<double>(s, t.p, t.d, kInfLength)
-                   : S::template distance_to_in<double>(s, t.p, t.d, kInfLength);
+  double db = boundary_distance<S>(s, t.p, t.d, t.in);
   if (db < 0.0) db = 0.0;
   if (l < db) {
     t.carry = 0.0;
