@@ -82,16 +82,16 @@ struct PolyhedronShape {
   }
 
   // Quadrilaterals::DistanceToIn<behindPlanesT> (Quadrilaterals.h:320-350):
-  // the first quad hit from its front (or back) side wins.
-  template <typename R, bool Behind>
-  VGM_DEV static R shell_distance_to_in(const Quad *q, int n, V3<R> p, V3<R> d)
+  // the first quad hit from its front (or, behind, back) side wins.
+  template <typename R>
+  VGM_DEV static R shell_distance_to_in(const Quad *q, int n, V3<R> p, V3<R> d, bool behind)
   {
     R best = inf_length<R>();
     for (int i = 0; i < n; ++i) {
       R distance = plane<R>(q[i], p);
-      if (!((Behind ? -distance : distance) > R(-kTolerance))) continue;
+      if (!((behind ? -distance : distance) > R(-kTolerance))) continue;
       const R proj = pdot<R>(q[i].n, d);
-      if (!((Behind ? proj : -proj) > R(0))) continue;
+      if (!((behind ? proj : -proj) > R(0))) continue;
       distance /= -(proj + vcopysign(R(1e-20), proj));
       const V3<R> x{p.x + d.x * distance, p.y + d.y * distance, p.z + d.z * distance};
       bool valid = true;
@@ -212,42 +212,29 @@ struct PolyhedronShape {
     return TubeShape::contains<R>(h.btube, V3<R>{p.x, p.y, p.z - R(h.bt_offset)});
   }
 
-  // DistanceToInZSegment (:368-400)
+  // DistanceToInZSegment (:368-400) and DistanceToOutZSegment (:403-445) as
+  // one sequence with the side a run-time argument, so that the lanes of a
+  // wave on both sides share the first two shell scans:
+  //   outside: outer shell (front), phi shell (front), inner shell (behind)
+  //   inside:  inner shell (front), phi shell (behind), outer shell exit
+  // each returning the first shell's hit.  (The inside sequence keeps a phi
+  // hit only when no inner hit came first, and a shell hit is never negative,
+  // so it reduces to the same first-hit rule.)
   template <typename R>
-  VGM_DEV static R distance_to_in_zseg(const Data &h, int si, V3<R> p, V3<R> d)
+  VGM_DEV static R zseg_distance(const Data &h, int si, R zMin, R zMax, V3<R> p, V3<R> d, bool in)
   {
     const PolyhedronSeg &s = h.segs()[si];
     const Quad *q = h.quads();
-    R distance = shell_distance_to_in<R, false>(q + s.outer, s.n_outer, p, d);
-    if (distance < inf_length<R>()) return distance;
-    if (h.phi_cutout) distance = shell_distance_to_in<R, false>(q + s.phi, s.n_phi, p, d);
-    if (distance < inf_length<R>()) return distance;
-    if (s.n_inner > 0) distance = shell_distance_to_in<R, true>(q + s.inner, s.n_inner, p, d);
-    return distance;
-  }
-
-  // DistanceToOutZSegment (:403-445)
-  template <typename R>
-  VGM_DEV static R distance_to_out_zseg(const Data &h, int si, R zMin, R zMax, V3<R> p, V3<R> d)
-  {
-    const PolyhedronSeg &s = h.segs()[si];
-    const Quad *q = h.quads();
-    bool done = false;
-    R distance = inf_length<R>();
-    if (s.n_inner > 0) {
-      distance = shell_distance_to_in<R, false>(q + s.inner, s.n_inner, p, d);
-      done = distance < inf_length<R>();
-      if (done) return distance;
+    if (!in || s.n_inner > 0) {
+      const R a = shell_distance_to_in<R>(q + (in ? s.inner : s.outer), in ? s.n_inner : s.n_outer, p, d, false);
+      if (a < inf_length<R>()) return a;
     }
     if (h.phi_cutout) {
-      const R distphi = shell_distance_to_in<R, true>(q + s.phi, s.n_phi, p, d);
-      if (!done && distance > R(-kTolerance)) distance = distphi;
-      done = distance > R(-kTolerance) && distance < inf_length<R>();
-      if (done) return distance;
+      const R b = shell_distance_to_in<R>(q + s.phi, s.n_phi, p, d, in);
+      if (b < inf_length<R>()) return b;
     }
-    const R distout = shell_distance_to_out<R>(q + s.outer, s.n_outer, p, d, zMin, zMax);
-    if (!done) distance = distout;
-    return distance;
+    if (in) return shell_distance_to_out<R>(q + s.outer, s.n_outer, p, d, zMin, zMax);
+    return s.n_inner > 0 ? shell_distance_to_in<R>(q + s.inner, s.n_inner, p, d, true) : inf_length<R>();
   }
 
   // InPhiCutoutWedge (:659-671).  A zero-height end segment has no phi
@@ -494,33 +481,49 @@ struct PolyhedronShape {
     return kInsideV;
   }
 
-  // DistanceToIn (:895-960)
+  // DistanceToIn (:895-960) from outside, DistanceToOut (:1080-1140) from
+  // inside, as one function of the side: the Inside check, the z-segment
+  // walk and the endcaps are shared by the lanes of a wave on both sides.
+  // (The reference's step_max reaches only the bounding tube's DistanceToIn,
+  // which ignores it.)
   template <typename R>
-  VGM_DEV static R distance_to_in(const Data &h, V3<R> p, V3<R> d, R step_max)
+  VGM_DEV static R boundary_distance(const Data &h, V3<R> p, V3<R> d, bool in)
   {
     const double *zp = h.zplanes();
-    if ((p.z < R(zp[0]) + R(kTolerance)) && d.z <= R(0)) return inf_length<R>();
-    if ((p.z > R(zp[h.nseg]) - R(kTolerance)) && d.z >= R(0)) return inf_length<R>();
-    if (inside<R>(h, p) == kInsideV) return R(-1);
+    const int zMax = h.nseg;
+    if (in) {
+      if ((p.z < R(zp[0]) - R(kTolerance)) || (p.z > R(zp[zMax]) + R(kTolerance))) return R(-1);
+    } else {
+      if ((p.z < R(zp[0]) + R(kTolerance)) && d.z <= R(0)) return inf_length<R>();
+      if ((p.z > R(zp[zMax]) - R(kTolerance)) && d.z >= R(0)) return inf_length<R>();
+    }
+    if (inside<R>(h, p) == (in ? kOutsideV : kInsideV)) return R(-1);
     R tubeDistance = R(0);
-    {
+    if (!in) {
       const V3<R> bp{p.x, p.y, p.z - R(h.bt_offset)};
       if (!TubeShape::contains<R>(h.btube, bp)) {
-        tubeDistance = TubeShape::distance_to_in<R>(h.btube, bp, d, step_max);
+        tubeDistance = TubeShape::distance_to_in<R>(h.btube, bp, d, inf_length<R>());
         if (tubeDistance == inf_length<R>()) return inf_length<R>();
       }
     }
     int zIndex = find_z_segment<R>(h, p.z);
-    const int zMax = h.nseg;
     zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
     R distance = inf_length<R>();
-    const int step = (d.z >= R(0)) ? 1 : -1;  // walk segments along the ray's z
-    for (; zIndex >= 0 && zIndex < zMax; zIndex += step) {
-      distance = distance_to_in_zseg<R>(h, zIndex, p, d);
+    const bool up = d.z >= R(0);  // walk segments along the ray's z
+    for (; zIndex >= 0 && zIndex < zMax; zIndex += up ? 1 : -1) {
+      distance = zseg_distance<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d, in);
       if (distance >= R(0) && distance < inf_length<R>()) break;
+      if (in && (up ? R(zp[zIndex]) - p.z : p.z - R(zp[zIndex])) > distance) break;
     }
-    distance_to_endcaps<R>(h, false, p, d, distance);
+    distance_to_endcaps<R>(h, in, p, d, distance);
+    if (in) return (distance >= inf_length<R>()) ? R(0) : distance;
     return (distance >= tubeDistance - R(1E-6)) ? distance : inf_length<R>();
+  }
+
+  template <typename R>
+  VGM_DEV static R distance_to_in(const Data &h, V3<R> p, V3<R> d, R /*step_max*/)
+  {
+    return boundary_distance<R>(h, p, d, false);
   }
 
   // SafetyToIn / SafetyToOut (:965-1075): walk z segments outwards from the
@@ -577,26 +580,10 @@ struct PolyhedronShape {
     return safety<R>(h, p, true);
   }
 
-  // DistanceToOut (:1080-1140)
   template <typename R>
   VGM_DEV static R distance_to_out(const Data &h, V3<R> p, V3<R> d, R /*step_max*/)
   {
-    const int zMax = h.nseg;
-    const double *zp = h.zplanes();
-    if ((p.z < R(zp[0]) - R(kTolerance)) || (p.z > R(zp[zMax]) + R(kTolerance))) return R(-1);
-    if (inside<R>(h, p) == kOutsideV) return R(-1);
-    int zIndex = find_z_segment<R>(h, p.z);
-    zIndex = zIndex < 0 ? 0 : (zIndex >= zMax ? zMax - 1 : zIndex);
-    R distance = inf_length<R>();
-    const bool up = d.z >= R(0);
-    for (; zIndex >= 0 && zIndex < zMax; zIndex += up ? 1 : -1) {
-      distance = distance_to_out_zseg<R>(h, zIndex, R(zp[zIndex]), R(zp[zIndex + 1]), p, d);
-      if (distance >= R(0) && distance < inf_length<R>()) break;
-      if ((up ? R(zp[zIndex]) - p.z : p.z - R(zp[zIndex])) > distance) break;
-    }
-    distance_to_endcaps<R>(h, true, p, d, distance);
-    if (distance >= inf_length<R>()) distance = R(0);
-    return distance;
+    return boundary_distance<R>(h, p, d, true);
   }
 };
 
