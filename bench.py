@@ -71,19 +71,18 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
-def family_roofline(fams_ms, counts_by_family, S, modes=None, weights=None):
-    """Roofline of the dominant family kernel of a timed transport (the one
-    with the largest standalone time, `weights`, from the autotuner):
-    algorithmic FP64 flops per launch (flop fixture for the family's step
-    order x its tracks x track-steps) over its average launch time, from
-    events on its stream.  The heaviest family is launched first on its
-    stream, so its events bracket the kernel alone; a later kernel on a
-    stream that shares a hardware queue would include the wait."""
+def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1):
+    """Roofline of the dominant family kernel of a timed transport (the
+    longest average launch time): algorithmic FP64 flops per launch (flop
+    fixture for the family's step order x its tracks x track-steps) over its
+    average launch time, from events on its stream.  With the families back
+    to back on one stream (streams == 1) the events bracket each kernel
+    alone; run concurrently, a kernel's interval also holds the time it
+    shares the machine with the others."""
     if not os.path.exists(FLOPS_FIXTURE) or not fams_ms:
         return None
     fx = json.load(open(FLOPS_FIXTURE))
-    w = {k: v for k, v in (weights or {}).items() if k in fams_ms}
-    fam = max(w, key=w.get) if w else max(fams_ms, key=fams_ms.get)
+    fam = max(fams_ms, key=fams_ms.get)
     mode = (modes or {}).get(fam, "fp64")
     if mode not in ("fp64", "distance-first", "culled"):
         return None  # float or matrix-core work: not priced against the FP64 peak
@@ -99,7 +98,8 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, weights=None):
            "kernel_ms": fams_ms[fam], "flops_per_track_step": per_step, "tracks": counts_by_family[fam],
            "track_steps": S,
            "note": "FP64 vector peak; flops counted on the step the kernel runs (+ - * / sqrt); kernel time from "
-                   "events on its stream while the other families run concurrently"}
+                   "events on its stream" + (", the families back to back on one stream" if streams == 1 else
+                                             ", while the other families run concurrently")}
     if os.path.exists(TRAFFIC_FIXTURE):
         tr = json.load(open(TRAFFIC_FIXTURE))
         if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
@@ -251,7 +251,8 @@ def main():
     ap.add_argument("--nav-steps", type=int, default=200)
     ap.add_argument("--c2-points", type=int, default=16_000_000,
                     help="points per shape for the tube/cone precision comparison (0: skip)")
-    ap.add_argument("--launch", default=None, help="block,grid,tile launch shape instead of autotuning")
+    ap.add_argument("--launch", default=None,
+                    help="block,grid,tile[,streams] launch shape instead of autotuning (streams 1 or 4)")
     ap.add_argument("--train-samples", type=int, default=50_000_000,
                     help="exact safety samples for the learned-safety networks, over all tessellated solids")
     ap.add_argument("--train-epochs", type=int, default=4)
@@ -303,7 +304,10 @@ def main():
         kw = {"mixed": mixed, "learned": learned} if not tuned else {"tuned": True}
         torch.cuda.synchronize()
         if args.launch:
-            batch.set_launch(*[int(v) for v in args.launch.split(",")])
+            lv = [int(v) for v in args.launch.split(",")]
+            batch.set_launch(*lv[:3])
+            if len(lv) > 3:
+                batch.set_streams(lv[3])
             if tuned and args.modes:
                 batch.set_modes(dict(kv.split("=") for kv in args.modes.split(",")))
         elif not args.no_autotune:
@@ -350,7 +354,7 @@ def main():
         final = state + [side, cr]
         log(f"[bench] {tag}: {args.steps * n * S / wall / 1e9:.2f} G track-steps/s (kernel {kern * 1e3:.2f} ms/launch)")
         log(f"[bench] {tag} family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam_ms.items()))
-        return wall, kern, extra, batch.launch(), fam_ms, final
+        return wall, kern, extra, batch.launch() + (batch.streams(),), fam_ms, final
 
     counts_by_family = {}
     for (kind, _), c in zip(specs, counts):
@@ -426,7 +430,7 @@ def main():
     head = next(k for k in ("tuned", "learned", "mixed", "fp64") if k in res)
     value, wall_v, kern_v = rate(head), res[head][0], res[head][1]
     roofline = family_roofline(res[head][4], counts_by_family, S,
-                               res[head][2].get("modes") if head == "tuned" else None, res[head][2].get("weights"))
+                               res[head][2].get("modes") if head == "tuned" else None, res[head][3][3])
     if roofline:
         roofline["run"] = head
     rate64, ratemx = rate("fp64"), rate("mixed")

@@ -244,6 +244,11 @@ int vgm_batch_set_modes(vgm_batch *batch, const int *modes8);
  * of its live tracks wait for one (default 50). */
 int vgm_batch_set_fallback(vgm_batch *batch, int pct);
 int vgm_batch_set_launch(vgm_batch *batch, int block, int grid, int tile_tracks);
+/* The families of a transport run concurrently on the batch's 4 family
+ * streams (4, the default) or back to back on the context stream (1); the
+ * autotuner tries both. */
+int vgm_batch_set_streams(vgm_batch *batch, int streams);
+int vgm_batch_get_streams(vgm_batch *batch, int *streams);
 int vgm_batch_get_launch(vgm_batch *batch, int *shape3);
 int vgm_batch_autotune(vgm_batch *batch, const double *x, const double *y, const double *z, const double *dx,
                        const double *dy, const double *dz, const int8_t *side, int nsteps, unsigned flags,

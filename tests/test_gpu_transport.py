@@ -39,13 +39,16 @@ def _setup(ctx, total, nsteps):
     return b, dev, side, cr, ref, side_o, cr_o, arr
 
 
-@pytest.mark.parametrize("launch", [None, (128, 64, 512), (256, 1024, 8192), (64, 0, 2048)],
-                         ids=["default", "small", "large", "per-tile"])
+@pytest.mark.parametrize("launch", [None, (128, 64, 512), (256, 1024, 8192), (64, 0, 2048), (256, 0, 512, 1)],
+                         ids=["default", "small", "large", "per-tile", "one-stream"])
 def test_transport_fp64_bit_exact(ctx, launch):
     print(f"[transport] fp64 launch={launch}", flush=True)
     b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
     if launch:
-        b.set_launch(*launch)
+        b.set_launch(*launch[:3])
+        if len(launch) > 3:
+            b.set_streams(launch[3])
+            assert b.streams() == launch[3]
     b.transport(*dev, side, cr, 30, step0=0, seed=5)
     for k in range(6):
         got = dev[k].cpu().numpy()

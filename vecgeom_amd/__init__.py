@@ -126,6 +126,8 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_ulonglong)],
         "vgm_batch_set_launch": [_P, _I, _I, _I],
         "vgm_batch_get_launch": [_P, _P],
+        "vgm_batch_set_streams": [_P, _I],
+        "vgm_batch_get_streams": [_P, _P],
         "vgm_batch_family_ms": [_P, _P],
         "vgm_batch_learned_stats": [_P, _P],
         "vgm_batch_set_fallback": [_P, _I],
@@ -486,6 +488,15 @@ class Batch:
 
     def set_launch(self, block, grid, tile_tracks):
         check(self._L.vgm_batch_set_launch(self.handle, block, grid, tile_tracks))
+
+    def set_streams(self, streams: int):
+        """1: families back to back on the context stream; 4: concurrently on the family streams."""
+        check(self._L.vgm_batch_set_streams(self.handle, int(streams)))
+
+    def streams(self) -> int:
+        out = ctypes.c_int(0)
+        check(self._L.vgm_batch_get_streams(self.handle, ctypes.byref(out)))
+        return int(out.value)
 
     def learned_stats(self):
         """Counters of the last learned transport."""

@@ -26,6 +26,7 @@ struct vgm_batch {
   double weight[kNumShapeTypes] = {};  // relative cost of each family's tiles
   int64_t ntracks = 0;
   LaunchShape shape{256, 0, 2048};
+  int streams = kFamilyStreams;  // 1: the families back to back on the context stream
   bool tiles_dirty = true;
   char *dbuf = nullptr;  // solids | tiles | counters | fp64 step counter
   size_t dcap = 0;
@@ -275,9 +276,13 @@ hipError_t launch_families(vgm_batch *b, unsigned flags, const TransportArgs &a,
       f.counter = nullptr;
     }
   }
-  if (fl.size() == 1) return one(main, fl[0]);
   std::sort(fl.begin(), fl.end(),
             [&](const FamilyLaunch &x, const FamilyLaunch &y) { return b->weight[x.type] > b->weight[y.type]; });
+  if (fl.size() == 1 || b->streams == 1) {
+    for (const FamilyLaunch &f : fl)
+      if ((e = one(main, f)) != hipSuccess) return e;
+    return hipSuccess;
+  }
   double load[kFamilyStreams] = {};
   bool used[kFamilyStreams] = {};
   if ((e = hipEventRecord(b->fork, main)) != hipSuccess) return e;
@@ -425,6 +430,21 @@ int vgm_batch_set_launch(vgm_batch *b, int block, int grid, int tile_tracks)
   return VGM_OK;
 }
 
+int vgm_batch_set_streams(vgm_batch *b, int streams)
+{
+  if (!b || (streams != 1 && streams != kFamilyStreams))
+    return fail(VGM_EINVAL, "streams must be 1 or " + std::to_string(kFamilyStreams));
+  b->streams = streams;
+  return VGM_OK;
+}
+
+int vgm_batch_get_streams(vgm_batch *b, int *streams)
+{
+  if (!b || !streams) return fail(VGM_EINVAL, "null argument");
+  *streams = b->streams;
+  return VGM_OK;
+}
+
 int vgm_batch_get_launch(vgm_batch *b, int *shape3)
 {
   if (!b || !shape3) return fail(VGM_EINVAL, "null argument");
@@ -498,11 +518,13 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   int32_t *wc = reinterpret_cast<int32_t *>(work + align256(6 * n * sizeof(double)) + align256(n));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const LaunchShape keep = b->shape;
+  const int keep_streams = b->streams;
   int keep_modes[kNumShapeTypes];
   std::memcpy(keep_modes, b->fam_mode, sizeof keep_modes);
   const int cus = std::max(1, c->num_cus);
   float best = 1e30f;
   LaunchShape best_shape = keep;
+  int best_streams = keep_streams;
   double weight[kNumShapeTypes] = {};
 
   auto stage = [&]() -> bool {
@@ -565,26 +587,31 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   }
   for (int t = 0; t < kNumShapeTypes; ++t)
     if (weight[t] > 0) b->weight[t] = weight[t];
-  // 2. launch shapes on the whole batch
+  // 2. launch shapes on the whole batch, the families concurrently on the
+  // family streams or back to back on one
   {
     const int blocks[] = {64, 128, 256};
     const int tiles[] = {512, 2048};
     const int per_cu[] = {0, 4};
-    for (int bl : blocks)
-      for (int tt : tiles)
-        for (int pc : per_cu) {
-          if (tt < bl) continue;
-          vgm_batch_set_launch(b, bl, pc * cus, tt);
-          if ((rc = upload(b, (flags & VGM_LEARNED) != 0))) goto done;
-          TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1, flags);
-          const LaunchShape ls = resolved(b);
-          float ms = 0;
-          if ((rc = timed([&]() { return launch_families(b, flags | VGM_TUNED, a, ls); }, ms))) goto done;
-          if (ms < best) {
-            best = ms;
-            best_shape = LaunchShape{bl, pc * cus, tt};
+    const int nstreams[] = {kFamilyStreams, 1};
+    for (int ns : nstreams)
+      for (int bl : blocks)
+        for (int tt : tiles)
+          for (int pc : per_cu) {
+            if (tt < bl) continue;
+            vgm_batch_set_launch(b, bl, pc * cus, tt);
+            b->streams = ns;
+            if ((rc = upload(b, (flags & VGM_LEARNED) != 0))) goto done;
+            TransportArgs a = make_args(b, wx, wy, wz, wdx, wdy, wdz, ws, wc, nsteps, 0, 1, flags);
+            const LaunchShape ls = resolved(b);
+            float ms = 0;
+            if ((rc = timed([&]() { return launch_families(b, flags | VGM_TUNED, a, ls); }, ms))) goto done;
+            if (ms < best) {
+              best = ms;
+              best_shape = LaunchShape{bl, pc * cus, tt};
+              best_streams = ns;
+            }
           }
-        }
   }
 done:
   if (e0) (void)hipEventDestroy(e0);
@@ -593,11 +620,13 @@ done:
   (void)hipFree(work);
   if (rc) {
     b->shape = keep;
+    b->streams = keep_streams;
     std::memcpy(b->fam_mode, keep_modes, sizeof keep_modes);
     b->tiles_dirty = true;
     return rc;
   }
   vgm_batch_set_launch(b, best_shape.block, best_shape.grid, best_shape.tile_tracks);
+  b->streams = best_streams;
   if (best3) {
     best3[0] = best_shape.block;
     best3[1] = best_shape.grid;
