@@ -313,7 +313,9 @@ transport_family_kernel(TransportArgs a, int t0, int t1, int *counter)
     } else {
       if (threadIdx.x == 0) tile_s = t0 + atomicAdd(counter, 1);
       __syncthreads();
-      t = tile_s;
+      // the same value in every lane: scalar registers, so the tile and solid
+      // records below load into scalar registers too
+      t = __builtin_amdgcn_readfirstlane(tile_s);
       __syncthreads();
     }
     if (t >= t1) break;
