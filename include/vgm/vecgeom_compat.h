@@ -14,6 +14,10 @@
 //                                                            SafetyToOut/PlacedDistanceToOut
 //   VNavigator::ComputeStepsAndPropagatedStates
 //     (VecGeom/navigation/VNavigator.h:128-132)             Navigator::ComputeStepsAndPropagatedStates
+//   VNavigator::ComputeStepsAndSafetiesAndPropagatedStates
+//     (VecGeom/navigation/VNavigator.h:136-148)             Navigator::ComputeStepsAndSafetiesAndPropagatedStates
+//   NavIndexTable::ValidateState (NavIndexTable.cpp:86-140),
+//     NavStatePath::At                                      Navigator::PathToState / StateToPath
 //   GlobalLocator::LocateGlobalPoint (GlobalLocator.h:35)   Navigator::LocateGlobalPoints
 //   VSafetyEstimator::ComputeVectorSafety (:45-80)          Navigator::ComputeSafeties
 //
@@ -239,6 +243,37 @@ class Navigator {
   {
     check(vgm_nav_step(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), step_max, p.size(), state, last, boundary,
                        step, flags(p)));
+  }
+  // ...AndSafeties...: safety[i] of the start point where calcsafety[i]
+  // (calcsafety may be null: every track), else 0, then the step.
+  void ComputeStepsAndSafetiesAndPropagatedStates(SOA3D<> const &p, SOA3D<> const &d, Precision const *step_max,
+                                                  uint32_t *state, uint32_t *last, uint8_t *boundary,
+                                                  Precision *step, bool const *calcsafety, Precision *safety) const
+  {
+    std::vector<uint8_t> calc;
+    if (calcsafety) {
+      if (p.on_device) throw Error(VGM_EINVAL, "calcsafety as bool* takes host arrays");
+      calc.resize(p.size());
+      for (size_t i = 0; i < p.size(); ++i) calc[i] = calcsafety[i] ? 1 : 0;
+    }
+    check(vgm_nav_step_safety(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), step_max, p.size(), state, last,
+                              boundary, step, calcsafety ? calc.data() : nullptr, safety, flags(p)));
+  }
+  // NavStatePath (placed volumes from the world down) <-> state index
+  uint32_t PathToState(std::vector<int> const &path) const
+  {
+    uint32_t s = 0;
+    check(vgm_nav_path_to_state(ctx_, path.data(), int(path.size()), &s));
+    return s;
+  }
+  std::vector<int> StateToPath(uint32_t state) const
+  {
+    int pv = 0, level = 0, mother = 0;
+    check(vgm_nav_node(ctx_, state, &pv, &level, &mother));
+    std::vector<int> path(size_t(level) + 1);
+    int depth = 0;
+    check(vgm_nav_state_to_path(ctx_, state, path.data(), int(path.size()), &depth));
+    return path;
   }
   void ComputeSafeties(SOA3D<> const &p, uint32_t const *state, Precision *safety) const
   {

@@ -164,6 +164,15 @@ int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, co
  * cached global matrices); call it again after changing the geometry. */
 int vgm_nav_close(vgm_ctx *ctx, int world_pv, int *nnodes);
 int vgm_nav_node(vgm_ctx *ctx, uint32_t state, int *pv, int *level, int *mother);
+/* NavStatePath <-> state index (NavIndexTable::ValidateState,
+ * source/NavIndexTable.cpp:86-140, and NavStatePath::At).  A path is the
+ * placed-volume ids from the world (pvs[0]) down to the top volume
+ * (pvs[depth - 1]); each must be a daughter of the previous one's logical
+ * volume.  vgm_nav_path_to_state fails (VGM_EINVAL) on a path that is not in
+ * the placement tree, where ValidateState throws.  vgm_nav_state_to_path
+ * writes depth = level + 1 entries (at most cap). */
+int vgm_nav_path_to_state(vgm_ctx *ctx, const int *pvs, int depth, uint32_t *state);
+int vgm_nav_state_to_path(vgm_ctx *ctx, uint32_t state, int *pvs, int cap, int *depth);
 /* state[i] = located state of global point i (0 if outside the world). */
 int vgm_nav_locate(vgm_ctx *ctx, const double *x, const double *y, const double *z, size_t n, uint32_t *state,
                    unsigned flags);
@@ -173,6 +182,15 @@ int vgm_nav_locate(vgm_ctx *ctx, const double *x, const double *y, const double 
 int vgm_nav_step(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx, const double *dy,
                  const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
                  uint8_t *boundary, double *step, unsigned flags);
+/* ComputeStepsAndSafetiesAndPropagatedStates (VecGeom/navigation/
+ * VNavigator.h:136-148; per track ComputeStepAndSafetyAndPropagatedState,
+ * :436-492 with the safety first): as vgm_nav_step, and before the step
+ * safety[i] = the safety of the point in its state where calcsafety[i] != 0
+ * (calcsafety may be NULL: every track), else 0; 0 for state 0. */
+int vgm_nav_step_safety(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx,
+                        const double *dy, const double *dz, const double *step_limit, size_t n, uint32_t *state,
+                        uint32_t *last, uint8_t *boundary, double *step, const uint8_t *calcsafety,
+                        double *safety, unsigned flags);
 /* nsteps straight-line steps per track (no physics limit): each step moves
  * the global point by the geometric step and takes the propagated state; a
  * track that leaves the world stops.  steps[i] is incremented by the number

@@ -69,6 +69,17 @@ int main()
   uint32_t mother;
   nav.Decode(2, &dpv, &level, &mother);
   EXPECT(dpv == d.id() && level == 1 && mother == 1);
+  // NavStatePath <-> index, and steps with safeties (the safety of the start
+  // point first: the tracks are back at their start and located again)
+  EXPECT(nav.PathToState({world.id(), d.id()}) == 2);
+  EXPECT(nav.StateToPath(2) == std::vector<int>({world.id(), d.id()}));
+  nav.LocateGlobalPoints(G, state);
+  last[0] = last[1] = 0;
+  boundary[0] = boundary[1] = 0;
+  bool calc[2] = {true, false};
+  nav.ComputeStepsAndSafetiesAndPropagatedStates(G, U, nullptr, state, last, boundary, step, calc, safety);
+  EXPECT(safety[0] == 20.0 && safety[1] == 0.0);
+  EXPECT(step[0] == 20.0 && state[0] == 2 && step[1] == 10.0 && state[1] == 1);
 
   // errors surface as exceptions with the library's message
   bool threw = false;

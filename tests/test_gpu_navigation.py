@@ -51,6 +51,56 @@ def test_step_and_safety(setup):
     assert np.array_equal(s_g, s_o) and np.array_equal(l_g, l_o) and np.array_equal(b_g, b_o)
 
 
+@pytest.mark.parametrize("mask", ["all", "half"])
+def test_step_with_safeties(setup, mask):
+    """ComputeStepsAndSafetiesAndPropagatedStates: the safety of the start
+    point (where asked, else 0) and the step, both equal to the oracle's
+    separate safety and step."""
+    print(f"[nav] step with safeties mask={mask}", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    calc = None if mask == "all" else (np.arange(N) % 2 == 0).astype(np.uint8)
+    s_g, l_g, b_g = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step, saf = np.zeros(N), np.full(N, -7.0)
+    nav.step_safety(*pts, *dirs, s_g, l_g, b_g, step, saf, calcsafety=calc)
+    ref_saf = geo.nav_safety(pts, st)
+    if calc is not None:
+        ref_saf = np.where(calc != 0, ref_saf, 0.0)
+    assert np.array_equal(saf, ref_saf)
+    s_o, l_o, b_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    ref = geo.nav_step(pts, dirs, s_o, l_o, b_o)
+    assert np.array_equal(step, ref)
+    assert np.array_equal(s_g, s_o) and np.array_equal(l_g, l_o) and np.array_equal(b_g, b_o)
+
+
+def test_state_path_round_trip(setup):
+    """NavStatePath <-> state index for every node of the placement tree:
+    the path runs from the world down the mother chain, and converts back to
+    the same index; a path that is not in the tree is an error."""
+    print("[nav] state <-> path", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    world = nav.state_to_path(1)
+    assert len(world) == 1
+    for s in range(1, nav.nnodes + 1):
+        path = nav.state_to_path(s)
+        pv, level, mother = nav.node(s)
+        assert len(path) == level + 1 and path[-1] == pv and path[0] == world[0]
+        if level > 0:
+            assert nav.state_to_path(mother) == path[:-1]
+        assert nav.path_to_state(path) == s
+    # a module directly under the world skips its layer: not in the tree
+    layer_path = nav.state_to_path(nav.path_to_state(nav.state_to_path(2)))
+    deep = next(s for s in range(1, nav.nnodes + 1) if nav.node(s)[1] == 2)
+    with pytest.raises(vg.VgmError):
+        nav.path_to_state([world[0], nav.node(deep)[0]])
+    with pytest.raises(vg.VgmError):
+        nav.path_to_state([layer_path[-1]])
+    # located states: the path's top volume is the state's placed volume
+    st = geo.nav_locate([a[:200] for a in pts])
+    for s in st[st > 0]:
+        assert nav.path_to_state(nav.state_to_path(int(s))) == int(s)
+
+
 @pytest.mark.parametrize("device_ptrs", [False, True], ids=["host", "device"])
 def test_transport_200_steps(setup, device_ptrs):
     print(f"[nav] transport device_ptrs={device_ptrs}", flush=True)

@@ -111,6 +111,9 @@ def _declare(L):
         "vgm_nav_node": [_P, ctypes.c_uint32, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)],
         "vgm_nav_locate": [_P, _P, _P, _P, _SZ, _P, _U],
         "vgm_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
+        "vgm_nav_step_safety": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P, _U],
+        "vgm_nav_path_to_state": [_P, _P, _I, ctypes.POINTER(ctypes.c_uint32)],
+        "vgm_nav_state_to_path": [_P, ctypes.c_uint32, _P, _I, ctypes.POINTER(_I)],
         "vgm_nav_transport": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _I, _P, _U],
         "vgm_nav_safety": [_P, _P, _P, _P, _SZ, _P, _P, _U],
     }
@@ -430,6 +433,26 @@ class Navigation:
         check(self._L.vgm_nav_step(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                    _ptr(step_limit), len(x), _ptr(state), _ptr(last), _ptr(boundary), _ptr(step),
                                    self._fl(x)))
+
+    def step_safety(self, x, y, z, dx, dy, dz, state, last, boundary, step, safety, calcsafety=None, step_limit=None):
+        """ComputeStepsAndSafetiesAndPropagatedStates: safety[i] before the step where calcsafety[i] (default all)."""
+        check(self._L.vgm_nav_step_safety(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                          _ptr(step_limit), len(x), _ptr(state), _ptr(last), _ptr(boundary),
+                                          _ptr(step), _ptr(calcsafety), _ptr(safety), self._fl(x)))
+
+    def path_to_state(self, pvs) -> int:
+        """NavStatePath (placed volumes from the world down) -> state index."""
+        a = np.ascontiguousarray(pvs, dtype=np.int32)
+        out = ctypes.c_uint32(0)
+        check(self._L.vgm_nav_path_to_state(self.ctx.handle, _ptr(a), len(a), ctypes.byref(out)))
+        return int(out.value)
+
+    def state_to_path(self, state: int):
+        """State index -> placed volumes from the world down to the top volume."""
+        buf = np.zeros(64, dtype=np.int32)
+        d = _I()
+        check(self._L.vgm_nav_state_to_path(self.ctx.handle, int(state), _ptr(buf), len(buf), ctypes.byref(d)))
+        return [int(v) for v in buf[:d.value]]
 
     def transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, asynchronous=False):
         check(self._L.vgm_nav_transport(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),

@@ -93,6 +93,40 @@ int vgm_nav_node(vgm_ctx *c, uint32_t node, int *pv, int *level, int *mother)
   return VGM_OK;
 }
 
+int vgm_nav_path_to_state(vgm_ctx *c, const int *pvs, int depth, uint32_t *state)
+{
+  if (!c || !pvs || !state) return fail(VGM_EINVAL, "null argument");
+  const HostGeometry &g = c->host;
+  if (g.nav_nodes.size() < 2) return fail(VGM_ESTATE, "navigation tables not built: call vgm_nav_close");
+  if (depth < 1) return fail(VGM_EINVAL, "a path holds at least the world");
+  if (pvs[0] != g.world_pv) return fail(VGM_EINVAL, "path does not start at the world volume");
+  uint32_t node = 1;
+  for (int lev = 1; lev < depth; ++lev) {
+    const NavNode &N = g.nav_nodes[node];
+    const std::vector<int32_t> &dl = g.lv_daughters[size_t(g.pvs[size_t(N.pv)].lv)];
+    int k = 0;
+    while (k < int(dl.size()) && dl[size_t(k)] != pvs[lev]) ++k;  // GetChildId
+    if (k == int(dl.size()))
+      return fail(VGM_EINVAL, "placed volume " + std::to_string(pvs[lev]) + " at level " + std::to_string(lev) +
+                                  " is not a daughter of placed volume " + std::to_string(N.pv));
+    node = uint32_t(g.nav_children[size_t(N.child0 + k)]);
+  }
+  *state = node;
+  return VGM_OK;
+}
+
+int vgm_nav_state_to_path(vgm_ctx *c, uint32_t state, int *pvs, int cap, int *depth)
+{
+  if (!c || !depth || (cap > 0 && !pvs)) return fail(VGM_EINVAL, "null argument");
+  const HostGeometry &g = c->host;
+  if (state == 0 || state >= g.nav_nodes.size()) return fail(VGM_EINVAL, "invalid state index");
+  const int d = g.nav_nodes[state].level + 1;
+  if (d > cap) return fail(VGM_EINVAL, "path needs " + std::to_string(d) + " entries");
+  for (uint32_t s = state; s != 0; s = uint32_t(g.nav_nodes[s].mother)) pvs[g.nav_nodes[s].level] = g.nav_nodes[s].pv;
+  *depth = d;
+  return VGM_OK;
+}
+
 int vgm_nav_locate(vgm_ctx *c, const double *x, const double *y, const double *z, size_t n, uint32_t *state,
                    unsigned flags)
 {
@@ -119,6 +153,25 @@ int vgm_nav_step(vgm_ctx *c, const double *x, const double *y, const double *z, 
                   {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit), true},
                   {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
                   {boundary, n, true, true, slot(a.boundary)}, {step, b8, false, true, slot(a.step)}},
+                 flags);
+}
+
+int vgm_nav_step_safety(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx,
+                        const double *dy, const double *dz, const double *step_limit, size_t n, uint32_t *state,
+                        uint32_t *last, uint8_t *boundary, double *step, const uint8_t *calcsafety, double *safety,
+                        unsigned flags)
+{
+  if (n && (!state || !last || !boundary || !step || !safety)) return fail(VGM_EINVAL, "null state or output array");
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavStep, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit), true},
+                  {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
+                  {boundary, n, true, true, slot(a.boundary)}, {step, b8, false, true, slot(a.step)},
+                  {calcsafety, n, true, false, slot(a.calcsafety), true}, {safety, b8, false, true, slot(a.safety)}},
                  flags);
 }
 

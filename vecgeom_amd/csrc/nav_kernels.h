@@ -18,6 +18,8 @@ struct NavArgs {
   uint32_t *state, *last;
   uint8_t *boundary;
   double *step;    // step: output step; safety: output safety
+  const uint8_t *calcsafety;  // step with safeties: per track, may be null (every track)
+  double *safety;             // step with safeties: output safety (null: plain step)
   int32_t *steps;  // transport: steps taken (accumulated)
   size_t n;
   int32_t nsteps;

@@ -1,6 +1,7 @@
 // Navigation kernels: one lane per track.
 //   locate     GlobalLocator::LocateGlobalPoint from the world (top = true)
-//   step       one ComputeStepAndPropagatedState per track
+//   step       one ComputeStepAndPropagatedState per track (with a safety
+//              output: ComputeStepAndSafetyAndPropagatedState)
 //   transport  nsteps of it with an unlimited physics step, moving the global
 //              point by each step; a track that leaves the world stops
 //   safety     VSafetyEstimator::ComputeSafety (mother SafetyToOut, then the
@@ -39,13 +40,16 @@ __global__ __launch_bounds__(kNavBlock) void nav_step_kernel(NavArgs a)
   if (i >= a.n) return;
   if (!a.state[i]) {
     a.step[i] = 0.0;
+    if (a.safety) a.safety[i] = 0.0;
     return;
   }
   Navigator<Mask> nav{a.g};
+  const V3<double> p{a.x[i], a.y[i], a.z[i]};
+  // ComputeStepAndSafetyAndPropagatedState: the safety of the start point first
+  if (a.safety) a.safety[i] = (!a.calcsafety || a.calcsafety[i]) ? nav.safety(p, a.state[i]) : 0.0;
   const NavState in{a.state[i], a.last[i], a.boundary[i] != 0};
   NavState out;
-  a.step[i] = nav.step(V3<double>{a.x[i], a.y[i], a.z[i]}, V3<double>{a.dx[i], a.dy[i], a.dz[i]},
-                       a.step_limit ? a.step_limit[i] : kInfLength, in, out);
+  a.step[i] = nav.step(p, V3<double>{a.dx[i], a.dy[i], a.dz[i]}, a.step_limit ? a.step_limit[i] : kInfLength, in, out);
   a.state[i] = out.ind;
   a.last[i] = out.last;
   a.boundary[i] = out.boundary ? 1 : 0;
