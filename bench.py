@@ -31,11 +31,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The batched transport runs its shape families on four streams of their own
-# next to the context stream and torch's; with HIP's default of four hardware
-# queues per process two of them share one and their kernels serialise.
-# Set before HIP initialises; an explicit setting in the environment wins.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# The batched transport can run its shape families on four streams of their
+# own next to the context stream and torch's; with HIP's default of four
+# hardware queues per process two of them would share one and their kernels
+# serialise.  Set before HIP initialises.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "track_step_flops.json")
@@ -71,24 +71,29 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
-def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1):
-    """Roofline of the dominant family kernel of a timed transport (the
-    longest average launch time): algorithmic FP64 flops per launch (flop
-    fixture for the family's step order x its tracks x track-steps) over its
-    average launch time, from events on its stream.  With the families back
-    to back on one stream (streams == 1) the events bracket each kernel
-    alone; run concurrently, a kernel's interval also holds the time it
-    shares the machine with the others."""
+FLOP_TABLES = {"fp64": "per_family", "distance-first": "per_family_distance_first", "culled": "per_family_culled"}
+
+
+def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights=None, launch_ms=None):
+    """Roofline of the dominant family kernel of a timed transport: the family
+    with the largest standalone time (`weights`, from the autotuner; else the
+    longest launch).  Algorithmic FP64 flops per launch (flop fixture for
+    the family's step order x its tracks x track-steps) over its average
+    launch time, from events on its stream.  With the families back to back
+    on one stream (streams == 1) the events bracket each kernel alone; run
+    concurrently, a kernel's interval also holds the time it shares the
+    machine with the others, and `aggregate` prices all families' flops
+    against the launch time."""
     if not os.path.exists(FLOPS_FIXTURE) or not fams_ms:
         return None
     fx = json.load(open(FLOPS_FIXTURE))
-    fam = max(fams_ms, key=fams_ms.get)
-    mode = (modes or {}).get(fam, "fp64")
-    if mode not in ("fp64", "distance-first", "culled"):
+    w = {k: v for k, v in (weights or {}).items() if k in fams_ms}
+    fam = max(w, key=w.get) if w else max(fams_ms, key=fams_ms.get)
+    modes = modes or {}
+    mode = modes.get(fam, "fp64")
+    if mode not in FLOP_TABLES:
         return None  # float or matrix-core work: not priced against the FP64 peak
-    table = fx[{"fp64": "per_family", "distance-first": "per_family_distance_first",
-                "culled": "per_family_culled"}[mode]]
-    per_step = float(table[fam])
+    per_step = float(fx[FLOP_TABLES[mode]][fam])
     flops = per_step * counts_by_family[fam] * S
     t = fams_ms[fam] * 1e-3
     achieved = flops / t / 1e12
@@ -100,6 +105,11 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1):
            "note": "FP64 vector peak; flops counted on the step the kernel runs (+ - * / sqrt); kernel time from "
                    "events on its stream" + (", the families back to back on one stream" if streams == 1 else
                                              ", while the other families run concurrently")}
+    if launch_ms and all(modes.get(f, "fp64") in FLOP_TABLES for f in counts_by_family):
+        total = sum(float(fx[FLOP_TABLES[modes.get(f, "fp64")]][f]) * c * S for f, c in counts_by_family.items())
+        out["aggregate"] = {"achieved": total / (launch_ms * 1e-3) / 1e12, "unit": "TFLOP/s",
+                            "frac": total / (launch_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops": total,
+                            "launch_ms": launch_ms, "note": "all families' flops over the whole launch"}
     if os.path.exists(TRAFFIC_FIXTURE):
         tr = json.load(open(TRAFFIC_FIXTURE))
         if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
@@ -430,7 +440,8 @@ def main():
     head = next(k for k in ("tuned", "learned", "mixed", "fp64") if k in res)
     value, wall_v, kern_v = rate(head), res[head][0], res[head][1]
     roofline = family_roofline(res[head][4], counts_by_family, S,
-                               res[head][2].get("modes") if head == "tuned" else None, res[head][3][3])
+                               res[head][2].get("modes") if head == "tuned" else None, res[head][3][3],
+                               res[head][2].get("weights") if not args.launch else None, kern_v * 1e3)
     if roofline:
         roofline["run"] = head
     rate64, ratemx = rate("fp64"), rate("mixed")

@@ -202,7 +202,6 @@ TransportArgs make_args(vgm_batch *b, double *x, double *y, double *z, double *d
   a.learned = b->d_learned;
   a.audit = (flags & VGM_AUDIT) ? 1 : 0;
   a.fallback_pct = b->fallback_pct;
-  a.distance_first = (flags & VGM_DISTANCE_FIRST) ? 1 : 0;
   a.fallback_grid = 8 * std::max(1, b->ctx->num_cus);
   a.tcarry = b->d_tcarry;
   a.tflags = b->d_tflags;
@@ -591,7 +590,7 @@ int vgm_batch_autotune(vgm_batch *b, const double *x, const double *y, const dou
   // family streams or back to back on one
   {
     const int blocks[] = {64, 128, 256};
-    const int tiles[] = {512, 2048};
+    const int tiles[] = {256, 512, 2048};
     const int per_cu[] = {0, 4};
     const int nstreams[] = {kFamilyStreams, 1};
     for (int ns : nstreams)

@@ -54,9 +54,9 @@ struct TransportArgs {
   unsigned long long *learned;  // [4]
   int32_t audit;
   int32_t fallback_pct;  // a wave runs its fallbacks once this % of its live tracks wait
-  int32_t distance_first;  // FP64 steps of surface-distance families: boundary distance first
+  int32_t pad3;
   int32_t fallback_grid;   // workgroups of the learned fallback kernel
-  int32_t cull;            // outside moves that clear the bounding box skip the geometry query
+  int32_t cull;            // learned launches: outside moves clear of the bounding box need no bound
   int32_t pad2;
   // learned families, between their per-step kernels: carried bound and
   // flags per track (indexed like x), and the queue of exact fallbacks

@@ -212,18 +212,25 @@ VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, doubl
   return true;
 }
 
+// Step order of an FP64 family kernel, fixed at compile time so that each
+// kernel holds the registers of one order only: the FP64 step, distance
+// first (families with kSafetyBelowDistance; the FP64 step otherwise), or
+// distance first with the bounding-box cull.
+enum StepOrder : int { kOrderFP64 = 0, kOrderDistFirst = 1, kOrderCulled = 2 };
+
 // A step that moves by l without any shape query: under the carried bound of
-// a surface-distance family, or (culling launches) an outside move clear of
-// the bounding box.
-template <class S>
-VGM_DEV bool cheap_step(const TransportArgs &a, const SolidRec &sr, const TrackState &t, double l)
+// a surface-distance family, or (culled order) an outside move clear of the
+// bounding box.
+template <class S, int Order>
+VGM_DEV bool cheap_step(const SolidRec &sr, const TrackState &t, double l)
 {
   if constexpr (S::kSafetyIsDistance)
     if (l <= t.carry) return true;
-  return a.cull && !t.in && move_clears_box(sr.bc, sr.bh, t.p, t.d, l);
+  if constexpr (Order == kOrderCulled) return !t.in && move_clears_box(sr.bc, sr.bh, t.p, t.d, l);
+  return false;
 }
 
-template <class S, int Mode>
+template <class S, int Mode, int Order>
 VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s, const Tile &tl,
                       uint32_t &fp64_steps)
 {
@@ -255,7 +262,7 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
       // of the bounding box).  Each lane runs through its own run of them
       // before the next query, so the lanes of a wave meet at the expensive
       // step instead of idling through each other's cheap ones.
-      while (cheap_step<S>(a, sr, t, l)) {
+      while (cheap_step<S, Order>(sr, t, l)) {
         const double m = carry_margin(t.p);
         t.p.x = t.p.x + l * t.d.x;
         t.p.y = t.p.y + l * t.d.y;
@@ -270,7 +277,10 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
       bool crossed;
       if constexpr (Mode == kTransportFP64) {
         bool verified = false;  // only the learned tiles read it
-        crossed = a.distance_first ? step_distance_first<S>(s, t, l, verified) : step_fp64<S>(s, t, l);
+        if constexpr (Order != kOrderFP64 && S::kSafetyBelowDistance)
+          crossed = step_distance_first<S>(s, t, l, verified);
+        else
+          crossed = step_fp64<S>(s, t, l);
       } else {
         crossed = step_mixed<S>(s, t, l, scale, fp64_steps);
       }
@@ -297,7 +307,7 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
 // tiles until the range is drained.  Without one, workgroup b runs tile t0+b.
 // Every family has its own kernel, so each gets the register allocation of
 // its own code path rather than the largest one of all families.
-template <class S, int Mode, bool Staged>
+template <class S, int Mode, bool Staged, int Order>
 __global__ __launch_bounds__(kTransportMaxBlock) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU))) void
 transport_family_kernel(TransportArgs a, int t0, int t1, int *counter)
 {
@@ -331,7 +341,7 @@ transport_family_kernel(TransportArgs a, int t0, int t1, int *counter)
       }
       sp = reinterpret_cast<const typename S::Data *>(lds_block);
     }
-    run_tile<S, Mode>(a, sr, *sp, tl, fp64_steps);
+    run_tile<S, Mode, Order>(a, sr, *sp, tl, fp64_steps);
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
 }
@@ -590,18 +600,18 @@ struct FamilyLauncher {
   int block;
   hipError_t err = hipSuccess;
 
-  template <class S, int Mode>
+  template <class S, int Mode, int Order>
   void go(const TransportArgs &args)
   {
     const int grid = f.counter ? f.grid : (f.t1 - f.t0);
     if (grid <= 0) return;
     const bool staged = S::kStage && f.stage_nd > 0 && size_t(f.stage_nd) * sizeof(double) <= size_t(kMaxStageBytes);
     if (staged)
-      hipLaunchKernelGGL((transport_family_kernel<S, Mode, S::kStage>), dim3(grid), dim3(block),
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, S::kStage, Order>), dim3(grid), dim3(block),
                          size_t(f.stage_nd) * sizeof(double), stream, args, f.t0, f.t1, f.counter);
     else
-      hipLaunchKernelGGL((transport_family_kernel<S, Mode, false>), dim3(grid), dim3(block), 0, stream, args, f.t0,
-                         f.t1, f.counter);
+      hipLaunchKernelGGL((transport_family_kernel<S, Mode, false, Order>), dim3(grid), dim3(block), 0, stream, args,
+                         f.t0, f.t1, f.counter);
   }
 
   template <class S>
@@ -609,10 +619,9 @@ struct FamilyLauncher {
   {
     TransportArgs args = a;
     const bool culled = f.mode == kFamCulled || f.mode == kFamLearned;
-    args.distance_first = (S::kSafetyBelowDistance && (f.mode == kFamDistFirst || culled)) ? 1 : 0;
     args.cull = culled ? 1 : 0;
     if (f.mode == kFamMixed) {
-      go<S, kTransportMixed>(args);
+      go<S, kTransportMixed, kOrderFP64>(args);
       return;
     }
     if constexpr (S::kSafetyIsDistance) {
@@ -633,7 +642,14 @@ struct FamilyLauncher {
         return;
       }
     }
-    go<S, kTransportFP64>(args);
+    // distance first is the FP64 step for families without kSafetyBelowDistance
+    constexpr int kDistFirst = S::kSafetyBelowDistance ? kOrderDistFirst : kOrderFP64;
+    if (f.mode == kFamCulled)
+      go<S, kTransportFP64, kOrderCulled>(args);
+    else if (f.mode == kFamDistFirst)
+      go<S, kTransportFP64, kDistFirst>(args);
+    else
+      go<S, kTransportFP64, kOrderFP64>(args);
   }
 };
 }  // namespace
