@@ -31,6 +31,10 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_LEARNED 16u    /* vgm_batch_transport: learned safety bound before the exact safety */
 #define VGM_AUDIT 32u      /* with VGM_LEARNED: check every served bound against the exact safety */
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
+/* Shape types, as indices of the per-family arrays of the batch calls:
+ * 0 box, 1 tube, 2 cone, 3 trd, 4 trapezoid, 5 polycone, 6 polyhedron,
+ * 7 tessellated, 8 orb. */
+#define VGM_NUM_SHAPE_TYPES 9
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
@@ -62,6 +66,9 @@ void *vgm_ctx_stream(vgm_ctx *ctx);
 
 /* UnplacedBox(dx, dy, dz): half-lengths (VecGeom/volumes/UnplacedBox.h). */
 int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
+/* UnplacedOrb(r) (VecGeom/volumes/UnplacedOrb.h; kernels
+ * VecGeom/volumes/kernel/OrbImplementation.h:64-225): a full sphere. */
+int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
 
 /* UnplacedTube(rmin, rmax, dz, sphi, dphi): radii, half-length, start and
  * delta phi in radians (VecGeom/volumes/UnplacedTube.h:29-35). */
@@ -236,10 +243,11 @@ int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, doubl
                         int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
                         unsigned flags, unsigned long long *fp64_steps);
 /* Kernel time in ms of each shape family (index = shape type: box, tube, cone,
- * trd, trapezoid, polycone, polyhedron, tessellated; 8 entries) in the last
+ * trd, trapezoid, polycone, polyhedron, tessellated, orb; VGM_NUM_SHAPE_TYPES
+ * entries) in the last
  * vgm_batch_transport called with VGM_TIMING, measured with events on the
  * stream that family ran on; -1 for families not run.  Waits for them. */
-int vgm_batch_family_ms(vgm_batch *batch, float *ms8);
+int vgm_batch_family_ms(vgm_batch *batch, float *ms);
 /* Counters of the last VGM_LEARNED transport: [0] bound evaluations, [1]
  * steps moved on a learned bound, [2] exact fallback steps, [3] (VGM_AUDIT)
  * bounds above the exact safety. */
@@ -249,15 +257,15 @@ int vgm_batch_learned_stats(vgm_batch *batch, unsigned long long *out4);
  * tuned).  The autotuner tries FP64 and each mode its flags allow (VGM_MIXED,
  * VGM_DISTANCE_FIRST, VGM_LEARNED, VGM_CULL), each family alone, and keeps the
  * fastest; VGM_TUNED transports use the choice. */
-int vgm_batch_get_modes(vgm_batch *batch, int *modes8);
+int vgm_batch_get_modes(vgm_batch *batch, int *modes);
 /* Relative cost of each family (index = shape type): after
  * vgm_batch_autotune, the ms of that family's kernel alone in its mode;
  * before, its track count.  It splits persistent grids and orders launches. */
-int vgm_batch_get_weights(vgm_batch *batch, double *ms8);
+int vgm_batch_get_weights(vgm_batch *batch, double *ms);
 /* Sets the modes VGM_TUNED transports use (distance-first, and the distance
  * order of culled, fall back to FP64 for families whose safety can exceed
  * their distance). */
-int vgm_batch_set_modes(vgm_batch *batch, const int *modes8);
+int vgm_batch_set_modes(vgm_batch *batch, const int *modes);
 /* A wave of the learned kernel runs its exact fallbacks once this percentage
  * of its live tracks wait for one (default 50). */
 int vgm_batch_set_fallback(vgm_batch *batch, int pct);

@@ -49,6 +49,7 @@ def lib():
         L.orc_geom_destroy.restype = None
         sig = {
             "orc_box_create": [_P, _D, _D, _D],
+            "orc_orb_create": [_P, _D],
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
@@ -107,6 +108,9 @@ class Geometry:
 
     def box(self, dx, dy, dz):
         return self._chk(self._L.orc_box_create(self.h, dx, dy, dz))
+
+    def orb(self, r):
+        return self._chk(self._L.orc_orb_create(self.h, r))
 
     def tube(self, rmin, rmax, dz, sphi, dphi):
         return self._chk(self._L.orc_tube_create(self.h, rmin, rmax, dz, sphi, dphi))

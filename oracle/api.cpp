@@ -23,6 +23,7 @@ static int add_shape(orc_geom *g, Shape *s)
 }
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz) { return add_shape(g, make_box(dx, dy, dz)); }
+int orc_orb_create(orc_geom *g, double r) { return r > 0 ? add_shape(g, make_orb(r)) : -1; }
 
 int orc_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi)
 {

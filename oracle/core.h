@@ -139,6 +139,7 @@ struct Shape {
 };
 
 Shape *make_box(double dx, double dy, double dz);
+Shape *make_orb(double r);
 Shape *make_cone(double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi);
 Shape *make_trd(double x1, double x2, double y1, double y2, double z);
 Shape *make_trapezoid(double dz, double theta, double phi, double dy1, double dx1, double dx2, double alpha1,

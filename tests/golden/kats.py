@@ -671,3 +671,56 @@ TESS_KATS = [
 ]
 # :232-244 Extent
 TESS_EXTENT = [(TSL1, (-20.0, -30.0, -40.0), (20.0, 30.0, 40.0)), (TSL2, (-30.0, -40.0, -40.0), (30.0, 40.0, 40.0))]
+
+
+# ---- orb: test/unit_tests/TestOrb.cpp -------------------------------------------
+ORB_B1, ORB_B2, ORB_B3 = (9.0,), (6.0,), (3.0,)
+_ORB_PO = (-8.363470934547895, 2.754420966126675, -2.665617952433236)
+_ORB_DO = tuple(c / 9.2 for c in _ORB_PO)
+_ORB_DI = tuple(-c / 9.2 for c in _ORB_PO)
+_ORB_POT = (8.884242447222299, 0.134875592787852, -1.432495973274375)
+_ORB_PI = (-3.618498437781364, 2.401810108299175, -6.718465394675017)
+
+ORB_KATS = (
+    # :110-133 DistanceToOut from the origin along the axes
+    [(ORB_B1, "DistanceToOut", O, v, ("eq", 9.0)) for v in (VX, VMX, VY, VMY, VZ, VMZ)]
+    # :134-181 from the surface points, outwards and across
+    + [(ORB_B1, "DistanceToOut", p, v, ("eq", w)) for p, v, w in (
+        ((9.0, 0, 0), VX, 0.0), ((9.0, 0, 0), VMX, 18.0), ((-9.0, 0, 0), VX, 18.0), ((-9.0, 0, 0), VMX, 0.0),
+        ((0, 9.0, 0), VY, 0.0), ((0, 9.0, 0), VMY, 18.0), ((0, -9.0, 0), VY, 18.0), ((0, -9.0, 0), VMY, 0.0),
+        ((0, 0, 9.0), VZ, 0.0), ((0, 0, 9.0), VMZ, 18.0), ((0, 0, -9.0), VZ, 18.0), ((0, 0, -9.0), VMZ, 0.0))]
+    # :184-199 Inside
+    + [(ORB_B1, "Inside", p, None, ("inside", c)) for p, c in (
+        (O, 1), ((100.0, 0, 0), 3), ((0, 100.0, 0), 3), ((0, 0, 100.0), 3),
+        ((9.0, 0, 0), 2), ((0, 9.0, 0), 2), ((0, 0, 9.0), 2))]
+    + [(ORB_B2, "Inside", p, None, ("inside", c)) for p, c in (
+        (O, 1), ((9.0, 0, 0), 3), ((0, 9.0, 0), 3), ((0, 0, 9.0), 3),
+        ((6.0, 0, 0), 2), ((0, 6.0, 0), 2), ((0, 0, 6.0), 2))]
+    # :201-223 safeties
+    + [(ORB_B1, "SafetyToOut", p, None, ("eq", w)) for p, w in ((O, 9.0), (VX, 8.0), (VY, 8.0), (VZ, 8.0))]
+    + [(ORB_B1, "SafetyToIn", p, None, ("eq", 91.0)) for p in (
+        (100.0, 0, 0), (-100.0, 0, 0), (0, 100.0, 0), (0, -100.0, 0), (0, 0, 100.0), (0, 0, -100.0))]
+    # :225-276 DistanceToIn
+    + [(ORB_B1, "DistanceToIn", p, v, ("eq", 91.0)) for p, v in (
+        ((100.0, 0, 0), VMX), ((-100.0, 0, 0), VX), ((0, 100.0, 0), VMY), ((0, -100.0, 0), VY),
+        ((0, 0, 100.0), VMZ), ((0, 0, -100.0), VZ))]
+    + [(ORB_B1, "DistanceToIn", (100.0, 0, 0), VXY, ("inf",)), (ORB_B1, "DistanceToIn", (-100.0, 0, 0), VXY, ("inf",)),
+       (ORB_B2, "DistanceToIn", (9.0, 0, 12.0), VXMZ, ("inf",)), (ORB_B2, "DistanceToIn", (12.0, 9.0, 0), VMXY, ("inf",)),
+       (ORB_B2, "DistanceToIn", (12.0, 9.0, 0), VMX, ("inf",)), (ORB_B2, "DistanceToIn", (8.0, 0, 0), VMX, ("eq", 2.0)),
+       (ORB_B2, "DistanceToIn", (7.0, 5.0, 0), VMX, ("eq", 3.6833752)),
+       (ORB_B1, "DistanceToIn", (-25.0, -35.0, 0), VX, ("inf",)),
+       (ORB_B1, "DistanceToIn", (-25.0, -35.0, 0), VY, ("inf",)),
+       # :278-330 points near and inside the surface
+       (ORB_B1, "DistanceToIn", _ORB_PO, _ORB_DO, ("inf",)),
+       (ORB_B1, "DistanceToOut", _ORB_PO, _ORB_DO, ("le", 0.0)),
+       (ORB_B1, "DistanceToIn", _ORB_PO, _ORB_DI, ("eq", 0.2)),
+       (ORB_B1, "DistanceToIn", _ORB_POT, tuple(c / 9 for c in _ORB_POT), ("inf",)),
+       (ORB_B1, "DistanceToIn", _ORB_POT, tuple(-c / 9 for c in _ORB_POT), ("eq", 0.0)),
+       (ORB_B1, "DistanceToIn", _ORB_PI, tuple(c / 8 for c in _ORB_PI), ("le", -1e-300)),
+       (ORB_B1, "DistanceToOut", _ORB_PI, tuple(c / 8 for c in _ORB_PI), ("eq", 1.0)),
+       (ORB_B1, "DistanceToIn", _ORB_PI, tuple(-c / 8 for c in _ORB_PI), ("le", -1e-300)),
+       (ORB_B1, "DistanceToOut", _ORB_PI, tuple(-c / 8 for c in _ORB_PI), ("eq", 17.0)),
+       # :337-341
+       (ORB_B3, "DistanceToIn", (-1.704652541027918744, 0.92532982039202238411, -2.2886512267834184797),
+        (0.55992127966252225324, -0.73207465430510332283, 0.38801399601708530529), ("eq", 0.0))]
+)

@@ -49,6 +49,13 @@ SHAPES = {
 }
 
 SHAPES.update({
+    # TestOrb.cpp radii
+    "orb_9": ("orb", (9.0,)),
+    "orb_6": ("orb", (6.0,)),
+    "orb_big": ("orb", (3000.0,)),
+})
+
+SHAPES.update({
     "trd_box": ("trd", (20.0, 20.0, 30.0, 30.0, 40.0)),
     "trd_2": ("trd", (10.0, 30.0, 20.0, 40.0, 40.0)),
     "trd_1": ("trd", (10.0, 30.0, 20.0, 20.0, 40.0)),

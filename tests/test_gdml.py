@@ -29,7 +29,7 @@ class Recorder:
         return len(self.calls) - 1
 
     def __getattr__(self, name):
-        if name in ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
+        if name in ("box", "orb", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
                     "logical_volume", "place", "add_daughter"):
             return lambda *a: self._rec(name, *a)
         raise AttributeError(name)
@@ -123,10 +123,20 @@ def test_numbers_read_like_stod():
     assert r.of("box")[0][1] == 0.0
 
 
-@pytest.mark.parametrize("name, what", [("oneOrb", "orb"), ("aConstant", "orb"), ("scaledPhysvol", "scale")])
+@pytest.mark.parametrize("name, what", [("scaledPhysvol", "scale")])
 def test_unsupported_is_an_error(name, what):
     with pytest.raises(gdml.GdmlError, match=what):
         read(name)
+    text = open(path("oneOrb")).read().replace("<orb ", "<sphere ")
+    with pytest.raises(gdml.GdmlError, match="sphere"):
+        gdml.load(text, Recorder())
+
+
+def test_orb_radius_and_constant():
+    # the radius is not halved; aConstant names it through a <constant>
+    for name in ("oneOrb", "aConstant"):
+        r, *_ = read(name)
+        assert r.of("orb") == [("orb", 100.0)]
 
 
 def test_two_boxes_structure():
@@ -138,6 +148,7 @@ def test_two_boxes_structure():
 
 @pytest.mark.parametrize("name, build", [
     ("oneBox", lambda g: g.box(50.0, 50.0, 50.0)),
+    ("oneOrb", lambda g: g.orb(100.0)),
     ("oneConeSegment", lambda g: g.cone(15.0, 30.0, 70.0, 100.0, 25.0, 1.0, 4.0)),
     ("oneTrapezoid", lambda g: g.trd(45.0, 40.0, 30.0, 25.0, 50.0)),
     ("onePolyhedron", None),

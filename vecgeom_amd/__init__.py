@@ -32,7 +32,7 @@ TUNED = 128
 CULL = 256
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
-SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated")
+SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb")
 
 # Inside_t (VecGeom/base/Global.h:71-88)
 kInside, kSurface, kOutside = 1, 2, 3
@@ -81,6 +81,7 @@ def _declare(L):
         "vgm_ctx_destroy": [_P],
         "vgm_ctx_synchronize": [_P],
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_orb_create": [_P, _D, ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
@@ -248,6 +249,9 @@ class Context:
 
     def box(self, dx, dy, dz) -> int:
         return self._new(self._L.vgm_box_create, dx, dy, dz)
+
+    def orb(self, r) -> int:
+        return self._new(self._L.vgm_orb_create, r)
 
     def tube(self, rmin, rmax, dz, sphi, dphi) -> int:
         return self._new(self._L.vgm_tube_create, rmin, rmax, dz, sphi, dphi)
@@ -532,7 +536,7 @@ class Batch:
 
     def family_ms(self):
         """{family: kernel ms} of the last transport run with timing=True."""
-        out = np.zeros(8, dtype=np.float32)
+        out = np.zeros(len(SHAPE_TYPES), dtype=np.float32)
         check(self._L.vgm_batch_family_ms(self.handle, out.ctypes.data))
         return {SHAPE_TYPES[t]: float(v) for t, v in enumerate(out) if v >= 0}
 
@@ -543,19 +547,19 @@ class Batch:
 
     def modes(self):
         """{family: mode} chosen by the last autotune."""
-        out = np.zeros(8, dtype=np.int32)
+        out = np.zeros(len(SHAPE_TYPES), dtype=np.int32)
         check(self._L.vgm_batch_get_modes(self.handle, out.ctypes.data))
         return {SHAPE_TYPES[t]: FAMILY_MODES[m] for t, m in enumerate(out) if m >= 0}
 
     def weights(self):
         """{family: weight}: standalone kernel ms after an autotune."""
-        out = np.zeros(8)
+        out = np.zeros(len(SHAPE_TYPES))
         check(self._L.vgm_batch_get_weights(self.handle, out.ctypes.data))
         return {SHAPE_TYPES[t]: float(v) for t, v in enumerate(out) if v > 0}
 
     def set_modes(self, modes):
         """modes: {family: mode name}; families not given are left untuned."""
-        out = np.full(8, -1, dtype=np.int32)
+        out = np.full(len(SHAPE_TYPES), -1, dtype=np.int32)
         for f, m in modes.items():
             out[SHAPE_TYPES.index(f)] = FAMILY_MODES.index(m)
         check(self._L.vgm_batch_set_modes(self.handle, out.ctypes.data))

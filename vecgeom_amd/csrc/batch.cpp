@@ -378,26 +378,26 @@ int vgm_batch_learned_stats(vgm_batch *b, unsigned long long *out4)
   return VGM_OK;
 }
 
-int vgm_batch_get_weights(vgm_batch *b, double *ms8)
+int vgm_batch_get_weights(vgm_batch *b, double *ms)
 {
-  if (!b || !ms8) return fail(VGM_EINVAL, "null argument");
-  for (int t = 0; t < kNumShapeTypes; ++t) ms8[t] = b->weight[t];
+  if (!b || !ms) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t) ms[t] = b->weight[t];
   return VGM_OK;
 }
 
-int vgm_batch_set_modes(vgm_batch *b, const int *modes8)
+int vgm_batch_set_modes(vgm_batch *b, const int *modes)
 {
-  if (!b || !modes8) return fail(VGM_EINVAL, "null argument");
+  if (!b || !modes) return fail(VGM_EINVAL, "null argument");
   for (int t = 0; t < kNumShapeTypes; ++t)
-    if (modes8[t] < -1 || modes8[t] > kFamCulled) return fail(VGM_EINVAL, "invalid family mode");
-  for (int t = 0; t < kNumShapeTypes; ++t) b->fam_mode[t] = modes8[t];
+    if (modes[t] < -1 || modes[t] > kFamCulled) return fail(VGM_EINVAL, "invalid family mode");
+  for (int t = 0; t < kNumShapeTypes; ++t) b->fam_mode[t] = modes[t];
   return VGM_OK;
 }
 
-int vgm_batch_get_modes(vgm_batch *b, int *modes8)
+int vgm_batch_get_modes(vgm_batch *b, int *modes)
 {
-  if (!b || !modes8) return fail(VGM_EINVAL, "null argument");
-  for (int t = 0; t < kNumShapeTypes; ++t) modes8[t] = b->fam_mode[t];
+  if (!b || !modes) return fail(VGM_EINVAL, "null argument");
+  for (int t = 0; t < kNumShapeTypes; ++t) modes[t] = b->fam_mode[t];
   return VGM_OK;
 }
 

@@ -244,6 +244,16 @@ int vgm_box_create(vgm_ctx *c, double dx, double dy, double dz, int *shape)
   return VGM_OK;
 }
 
+int vgm_orb_create(vgm_ctx *c, double r, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_orb(c->host, r, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_tube_create(vgm_ctx *c, double rmin, double rmax, double dz, double sphi, double dphi, int *shape)
 {
   if (!c || !shape) return fail(VGM_EINVAL, "null argument");

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <cmath>
 #include "shapes/box_data.h"
+#include "shapes/orb_data.h"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
 #include "shapes/trd_data.h"
@@ -81,6 +82,18 @@ void make_tube_data(TubeData &t, double rmin, double rmax, double z, double sphi
   t.le_pi = dphi <= M_PI;
   t.lt_pi = dphi < kPi;
   t.gt_pi = dphi > M_PI;
+}
+
+// UnplacedOrb(r): extent +-r (OrbStruct.h)
+int build_orb(HostGeometry &g, double r, std::string &err)
+{
+  if (!(r > 0)) {
+    err = "orb radius must be positive";
+    return -1;
+  }
+  OrbData o{r};
+  const double lo[3] = {-r, -r, -r}, hi[3] = {r, r, r};
+  return g.add_shape(kOrb, &o, sizeof(o) / sizeof(double), 0, 0, lo, hi);
 }
 
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)

@@ -14,18 +14,22 @@
 namespace vgm {
 
 constexpr int kNavBlock = 128;
-// Waves per SIMD the step kernels ask the compiler for: the BVH and shape
-// walks hide their latency better at 5 waves than at the 3 the unconstrained
-// register allocation gives (barrel transport 0.48 -> 0.62 G steps/s), even
-// with the spills that brings.
-constexpr int kNavWavesPerEU = 5;
+
 static_assert(kNavBlock <= kNavWalkThreads, "walk stack columns");
 
 // Kernel variants by the set of shape families in the geometry: the barrel
 // of tubes and boxes needs neither the polygon nor the facet code.
 constexpr unsigned kMaskBoxTube = kShapeBit(kBox) | kShapeBit(kTube);
-constexpr unsigned kMaskSimple =
-    kShapeBit(kBox) | kShapeBit(kTube) | kShapeBit(kCone) | kShapeBit(kTrd) | kShapeBit(kTrapezoid);
+constexpr unsigned kMaskSimple = kShapeBit(kBox) | kShapeBit(kTube) | kShapeBit(kCone) | kShapeBit(kTrd) |
+                                 kShapeBit(kTrapezoid) | kShapeBit(kOrb);
+
+// Waves per SIMD the step kernels of the box-and-tube variant ask the
+// compiler for: the BVH and shape walks hide their latency better at 5 waves
+// than at the 3 the unconstrained register allocation gives (barrel transport
+// 0.48 -> 0.62 G steps/s), even with the spills that brings.  The variants
+// with more families call out-of-line shape code and are left unconstrained.
+template <unsigned Mask>
+constexpr int kNavWavesPerEU = Mask == kMaskBoxTube ? 5 : 1;
 
 template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_locate_kernel(NavArgs a)
@@ -39,7 +43,7 @@ __global__ __launch_bounds__(kNavBlock) void nav_locate_kernel(NavArgs a)
 }
 
 template <unsigned Mask>
-__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU))) void nav_step_kernel(NavArgs a)
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_step_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -61,7 +65,7 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
 }
 
 template <unsigned Mask>
-__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU))) void nav_transport_kernel(NavArgs a)
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_transport_kernel(NavArgs a)
 {
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;

@@ -9,6 +9,7 @@
 #include "polycone.hpp"
 #include "polyhedron.hpp"
 #include "tessellated.hpp"
+#include "orb.hpp"
 
 namespace vgm {
 
@@ -41,6 +42,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kTessellated:
     f.template operator()<TessellatedShape>();
+    return true;
+  case kOrb:
+    f.template operator()<OrbShape>();
     return true;
   default:
     return false;
@@ -81,6 +85,9 @@ __host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
     return true;
   case kTessellated:
     if constexpr ((Mask & (1u << kTessellated)) != 0) f.template operator()<TessellatedShape>();
+    return true;
+  case kOrb:
+    if constexpr ((Mask & (1u << kOrb)) != 0) f.template operator()<OrbShape>();
     return true;
   default:
     return false;

@@ -17,7 +17,8 @@ enum ShapeType : int32_t {
   kPolycone = 5,
   kPolyhedron = 6,
   kTessellated = 7,
-  kNumShapeTypes = 8
+  kOrb = 8,
+  kNumShapeTypes = 9
 };
 
 enum Method : int32_t {

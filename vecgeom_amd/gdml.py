@@ -43,12 +43,12 @@ from typing import Dict, List, Tuple
 
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
-SUPPORTED_SOLIDS = ("box", "tube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated")
+SUPPORTED_SOLIDS = ("box", "tube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
 _REFERENCE_SOLIDS = SUPPORTED_SOLIDS + (
-    "orb", "sphere", "torus", "cutTube", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
+    "sphere", "torus", "cutTube", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
     "genericPolycone", "genericPolyhedra", "hype", "paraboloid", "tet", "twistedbox", "twistedtrap",
     "twistedtrd", "twistedtubs", "scaledSolid", "union", "subtraction", "intersection", "multiUnion")
 
@@ -196,6 +196,10 @@ class _Reader:
             z, rmin1, rmin2, rmax1, rmax2 = self.lengths(el, "z", "rmin1", "rmin2", "rmax1", "rmax2")
             sphi, dphi = self.angles(el, "startphi", "deltaphi")
             s = g.cone(rmin1, rmax1, rmin2, rmax2, z / 2.0, sphi, dphi)
+        elif tag == "orb":
+            # the radius as written, not halved (Middleware.cpp:650-662)
+            r, = self.lengths(el, "r")
+            s = g.orb(r)
         elif tag == "trd":
             x1, x2, y1, y2, z = self.lengths(el, "x1", "x2", "y1", "y2", "z")
             s = g.trd(x1 / 2.0, x2 / 2.0, y1 / 2.0, y2 / 2.0, z / 2.0)
