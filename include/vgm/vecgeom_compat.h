@@ -7,7 +7,7 @@
 //   SOA3D<Precision> (VecGeom/base/SOA3D.h:36-41,178-183)   vgm::compat::SOA3D
 //   Transformation3D (VecGeom/base/Transformation3D.h)      vgm::compat::Transformation3D
 //   UnplacedBox/Tube/Cone/Trd/Trapezoid/Polycone/
-//     Polyhedron/Tessellated/Orb, LogicalVolume, PlaceDaughter  GeoManager::Box(...), ..., Place
+//     Polyhedron/Tessellated/Orb/CutTube, LogicalVolume, PlaceDaughter  GeoManager::Box(...), ..., Place
 //   VPlacedVolume SOA3D virtuals
 //     (VecGeom/volumes/PlacedVolume.h:273,299,332,372,405,421)  PlacedVolume::Contains/Inside/
 //                                                            DistanceToIn/DistanceToOut/SafetyToIn/
@@ -157,6 +157,11 @@ class GeoManager {
   // unplaced shapes (constructor arguments of the reference's classes)
   int Box(double dx, double dy, double dz) { return make(vgm_box_create(ctx_, dx, dy, dz, &s_)); }
   int Orb(double r) { return make(vgm_orb_create(ctx_, r, &s_)); }
+  int CutTube(double rmin, double rmax, double z, double sphi, double dphi, double bx, double by, double bz, double tx,
+              double ty, double tz)
+  {
+    return make(vgm_cut_tube_create(ctx_, rmin, rmax, z, sphi, dphi, bx, by, bz, tx, ty, tz, &s_));
+  }
   int Tube(double rmin, double rmax, double dz, double sphi, double dphi)
   {
     return make(vgm_tube_create(ctx_, rmin, rmax, dz, sphi, dphi, &s_));

@@ -33,8 +33,8 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
 /* Shape types, as indices of the per-family arrays of the batch calls:
  * 0 box, 1 tube, 2 cone, 3 trd, 4 trapezoid, 5 polycone, 6 polyhedron,
- * 7 tessellated, 8 orb. */
-#define VGM_NUM_SHAPE_TYPES 9
+ * 7 tessellated, 8 orb, 9 cut tube. */
+#define VGM_NUM_SHAPE_TYPES 10
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
@@ -69,6 +69,13 @@ int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
 /* UnplacedOrb(r) (VecGeom/volumes/UnplacedOrb.h; kernels
  * VecGeom/volumes/kernel/OrbImplementation.h:64-225): a full sphere. */
 int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
+/* UnplacedCutTube(rmin, rmax, z, sphi, dphi, bx, by, bz, tx, ty, tz)
+ * (VecGeom/volumes/UnplacedCutTube.h:54-70; kernels
+ * VecGeom/volumes/kernel/CutTubeImplementation.h:106-321): a tube of
+ * half-length z cut by planes through (0,0,-z) and (0,0,z) with outward
+ * normals (bx,by,bz), bz < 0, and (tx,ty,tz), tz > 0. */
+int vgm_cut_tube_create(vgm_ctx *ctx, double rmin, double rmax, double z, double sphi, double dphi, double bx,
+                        double by, double bz, double tx, double ty, double tz, int *shape);
 
 /* UnplacedTube(rmin, rmax, dz, sphi, dphi): radii, half-length, start and
  * delta phi in radians (VecGeom/volumes/UnplacedTube.h:29-35). */

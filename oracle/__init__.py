@@ -50,6 +50,7 @@ def lib():
         sig = {
             "orc_box_create": [_P, _D, _D, _D],
             "orc_orb_create": [_P, _D],
+            "orc_cut_tube_create": [_P] + [_D] * 11,
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
@@ -111,6 +112,9 @@ class Geometry:
 
     def orb(self, r):
         return self._chk(self._L.orc_orb_create(self.h, r))
+
+    def cut_tube(self, rmin, rmax, z, sphi, dphi, bottom, top):
+        return self._chk(self._L.orc_cut_tube_create(self.h, rmin, rmax, z, sphi, dphi, *bottom, *top))
 
     def tube(self, rmin, rmax, dz, sphi, dphi):
         return self._chk(self._L.orc_tube_create(self.h, rmin, rmax, dz, sphi, dphi))

@@ -24,6 +24,13 @@ static int add_shape(orc_geom *g, Shape *s)
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz) { return add_shape(g, make_box(dx, dy, dz)); }
 int orc_orb_create(orc_geom *g, double r) { return r > 0 ? add_shape(g, make_orb(r)) : -1; }
+int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
+                        double bz, double tx, double ty, double tz)
+{
+  if (!(rmin >= 0 && rmin < rmax && z > 0 && dphi > 0) || bz >= 0 || tz <= 0) return -1;
+  const double bot[3] = {bx, by, bz}, top[3] = {tx, ty, tz};
+  return add_shape(g, make_cut_tube(rmin, rmax, z, sphi, dphi, bot, top));
+}
 
 int orc_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi)
 {

@@ -140,6 +140,8 @@ struct Shape {
 
 Shape *make_box(double dx, double dy, double dz);
 Shape *make_orb(double r);
+Shape *make_cut_tube(double rmin, double rmax, double z, double sphi, double dphi, const double bot[3],
+                     const double top[3]);
 Shape *make_cone(double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi);
 Shape *make_trd(double x1, double x2, double y1, double y2, double z);
 Shape *make_trapezoid(double dz, double theta, double phi, double dy1, double dx1, double dx2, double alpha1,

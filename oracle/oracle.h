@@ -27,6 +27,8 @@ void orc_geom_destroy(orc_geom *g);
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz);
 int orc_orb_create(orc_geom *g, double r);
+int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
+                        double bz, double tx, double ty, double tz);
 int orc_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi);
 int orc_cone_create(orc_geom *g, double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi,
                     double dphi);

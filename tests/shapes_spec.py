@@ -55,6 +55,25 @@ SHAPES.update({
     "orb_big": ("orb", (3000.0,)),
 })
 
+def cut_normals(thb, phib, tht, phit):
+    """Bottom and top cut normals from polar and azimuthal angles, as
+    test/shape_tester/shape_testCutTube.cpp:50-51 builds them."""
+    b = (math.sin(thb) * math.cos(phib), math.sin(thb) * math.sin(phib), math.cos(thb))
+    t = (math.sin(tht) * math.cos(phit), math.sin(tht) * math.sin(phit), math.cos(tht))
+    return b, t
+
+
+_CT = cut_normals(3 * PI / 4, PI / 3, PI / 4, 2 * PI / 3)  # shape_testCutTube.cpp:23-31
+SHAPES.update({
+    # the four shape_testCutTube.cpp cases (:33-45)
+    "cut_tube_t0": ("cut_tube", (0.0, 5.0, 10.0, 0.0, 2 * PI) + _CT),
+    "cut_tube_t1": ("cut_tube", (3.0, 5.0, 10.0, 0.0, 2 * PI) + _CT),
+    "cut_tube_t2": ("cut_tube", (0.0, 5.0, 10.0, 0.0, 2 * PI / 3) + _CT),
+    "cut_tube_t3": ("cut_tube", (3.0, 5.0, 10.0, 0.0, 2 * PI / 3) + _CT),
+    # shallow cuts on a wide sector with a bore
+    "cut_tube_wide": ("cut_tube", (20.0, 60.0, 40.0, -0.5, 4.0) + cut_normals(PI - 0.2, 0.3, 0.15, 2.0)),
+})
+
 SHAPES.update({
     "trd_box": ("trd", (20.0, 20.0, 30.0, 30.0, 40.0)),
     "trd_2": ("trd", (10.0, 30.0, 20.0, 40.0, 40.0)),

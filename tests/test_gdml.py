@@ -29,7 +29,7 @@ class Recorder:
         return len(self.calls) - 1
 
     def __getattr__(self, name):
-        if name in ("box", "orb", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
+        if name in ("box", "orb", "tube", "cut_tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
                     "logical_volume", "place", "add_daughter"):
             return lambda *a: self._rec(name, *a)
         raise AttributeError(name)
@@ -130,6 +130,13 @@ def test_unsupported_is_an_error(name, what):
     text = open(path("oneOrb")).read().replace("<orb ", "<sphere ")
     with pytest.raises(gdml.GdmlError, match="sphere"):
         gdml.load(text, Recorder())
+
+
+def test_cut_tube_reads_normals_as_lengths():
+    r, *_ = read("oneCutTube")  # lunit cm: 1 cm = 1.0
+    (_, rmin, rmax, dz, sphi, dphi, low, high), = r.of("cut_tube")
+    assert (rmin, rmax, dz, sphi, dphi) == (1.0, 5.0, 10.0, 1.0, 4.0)
+    assert low == (-15.0, -15.0, -15.0) and high == (10.0, 10.0, 10.0)
 
 
 def test_orb_radius_and_constant():

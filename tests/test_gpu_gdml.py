@@ -15,7 +15,7 @@ from tests.test_gpu_shapes import compare, gpu_query, make_inputs
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gdml")
-FILES = ["aConstant", "aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "oneOrb", "oneGeneralTrapezoid", "onePolycone",
+FILES = ["aConstant", "aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "oneCutTube", "oneOrb", "oneGeneralTrapezoid", "onePolycone",
          "onePolyhedron", "oneTesselatedSolid", "oneTrapezoid", "oneTube", "twoBoxes"]
 QUERIES = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 

@@ -32,7 +32,7 @@ TUNED = 128
 CULL = 256
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
-SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb")
+SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube")
 
 # Inside_t (VecGeom/base/Global.h:71-88)
 kInside, kSurface, kOutside = 1, 2, 3
@@ -82,6 +82,7 @@ def _declare(L):
         "vgm_ctx_synchronize": [_P],
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_orb_create": [_P, _D, ctypes.POINTER(_I)],
+        "vgm_cut_tube_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
@@ -252,6 +253,10 @@ class Context:
 
     def orb(self, r) -> int:
         return self._new(self._L.vgm_orb_create, r)
+
+    def cut_tube(self, rmin, rmax, z, sphi, dphi, bottom, top) -> int:
+        """UnplacedCutTube(rmin, rmax, z, sphi, dphi, bottomNormal, topNormal)."""
+        return self._new(self._L.vgm_cut_tube_create, rmin, rmax, z, sphi, dphi, *bottom, *top)
 
     def tube(self, rmin, rmax, dz, sphi, dphi) -> int:
         return self._new(self._L.vgm_tube_create, rmin, rmax, dz, sphi, dphi)

@@ -244,6 +244,18 @@ int vgm_box_create(vgm_ctx *c, double dx, double dy, double dz, int *shape)
   return VGM_OK;
 }
 
+int vgm_cut_tube_create(vgm_ctx *c, double rmin, double rmax, double z, double sphi, double dphi, double bx,
+                        double by, double bz, double tx, double ty, double tz, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  const double bot[3] = {bx, by, bz}, top[3] = {tx, ty, tz};
+  int id = build_cut_tube(c->host, rmin, rmax, z, sphi, dphi, bot, top, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_orb_create(vgm_ctx *c, double r, int *shape)
 {
   if (!c || !shape) return fail(VGM_EINVAL, "null argument");

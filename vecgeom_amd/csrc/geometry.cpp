@@ -2,8 +2,10 @@
 #include "geometry.h"
 #include <cstring>
 #include <cmath>
+#include <cfloat>
 #include "shapes/box_data.h"
 #include "shapes/orb_data.h"
+#include "shapes/cut_tube_data.h"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
 #include "shapes/trd_data.h"
@@ -94,6 +96,87 @@ int build_orb(HostGeometry &g, double r, std::string &err)
   OrbData o{r};
   const double lo[3] = {-r, -r, -r}, hi[3] = {r, r, r};
   return g.add_shape(kOrb, &o, sizeof(o) / sizeof(double), 0, 0, lo, hi);
+}
+
+// Plane::Set(normal, origin) (VecGeom/volumes/Plane.h:30-37) of the unit
+// normal: components below kTolerance are zeroed, the normal renormalised and
+// the distance taken from the origin point (0, 0, oz)
+static void set_cut_plane(double n[3], double &dd, const double in[3], double oz)
+{
+  const double kTolerance = 1e-9;
+  // Vector3D::Unit (VecGeom/base/Vector3D.h:284-290): divided by sqrt(mag2 + kMinimum)
+  const double mag2 = in[0] * in[0] + in[1] * in[1] + in[2] * in[2];
+  const double s = std::sqrt(mag2 + DBL_MIN);
+  double f[3] = {in[0] / s, in[1] / s, in[2] / s};
+  for (double &c : f)
+    if (std::fabs(c) < kTolerance) c = 0.0;  // FixZeroes
+  const double inverseLength = 1. / std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  for (int i = 0; i < 3; ++i) n[i] = inverseLength * f[i];
+  dd = inverseLength * -(f[0] * 0.0 + f[1] * 0.0 + f[2] * oz);
+}
+
+// Wedge::Contains (VecGeom/volumes/Wedge_Evolution.h:296-338)
+static bool wedge_contains(const TubeData &t, double x, double y)
+{
+  const double startCheck = (-x * t.along1y + y * t.along1x);
+  const double endCheck = (-t.along2x * y + t.along2y * x);
+  bool out = startCheck < 0.;
+  if (t.lt_pi)
+    out = out || endCheck < 0.;
+  else
+    out = out && endCheck < 0.;
+  return !out;
+}
+
+// UnplacedCutTube(rmin, rmax, z, sphi, dphi, bx, by, bz, tx, ty, tz)
+// (VecGeom/volumes/UnplacedCutTube.h:54-70, CutTubeStruct.h:38-49) and its
+// Extent (source/UnplacedCutTube.cpp:128-172)
+int build_cut_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, const double bot[3],
+                   const double top[3], std::string &err)
+{
+  if (!(rmin >= 0 && rmin < rmax && z > 0 && dphi > 0)) {
+    err = "cut tube needs 0 <= rmin < rmax, z > 0, dphi > 0";
+    return -1;
+  }
+  if (bot[2] >= 0 || top[2] <= 0) {
+    err = "illegal normal direction for cut planes (bottom z < 0 and top z > 0 required)";
+    return -1;
+  }
+  CutTubeData c{};
+  const double kPi = 3.14159265358979323846;
+  make_tube_data(c.tube, rmin, rmax, DBL_MAX, sphi, dphi);  // kInfLength
+  double n0[3], n1[3];
+  set_cut_plane(n0, c.d0, bot, -z);
+  set_cut_plane(n1, c.d1, top, z);
+  c.n0x = n0[0], c.n0y = n0[1], c.n0z = n0[2];
+  c.n1x = n1[0], c.n1y = n1[1], c.n1z = n1[2];
+  c.dz = z;
+  c.maxval = rmax > z ? rmax : z;
+  const double rmn = c.tube.rmin, rmx = rmax;
+  const double dztop = rmx * std::sqrt(1. - n1[2] * n1[2]) / n1[2];
+  const double dzbottom = -rmx * std::sqrt(1. - n0[2] * n0[2]) / n0[2];
+  double lo[3] = {-rmx, -rmx, -z - dzbottom}, hi[3] = {rmx, rmx, z + dztop};
+  if (!(dphi == 2. * kPi)) {
+    auto zt = [&](double r, double phi) { return z - (r / n1[2]) * (n1[0] * std::cos(phi) + n1[1] * std::sin(phi)); };
+    auto zb = [&](double r, double phi) { return -z - (r / n0[2]) * (n0[0] * std::cos(phi) + n0[1] * std::sin(phi)); };
+    auto mx = [](double a, double b) { return a > b ? a : b; };
+    auto mn = [](double a, double b) { return a < b ? a : b; };
+    const double s1 = sphi, s2 = sphi + dphi;
+    const double c1 = std::cos(sphi), si1 = std::sin(sphi), c2 = std::cos(sphi + dphi), si2 = std::sin(sphi + dphi);
+    if (!wedge_contains(c.tube, -n1[0], -n1[1])) {
+      hi[2] = mx(zt(rmx, s1), zt(rmx, s2));
+      hi[2] = mx(hi[2], mx(zt(rmn, s1), zt(rmn, s2)));
+    }
+    if (!wedge_contains(c.tube, -n0[0], -n0[1])) {
+      lo[2] = mn(zb(rmx, s1), zb(rmx, s2));
+      lo[2] = mn(lo[2], mn(zb(rmn, s1), zb(rmn, s2)));
+    }
+    if (!wedge_contains(c.tube, 1., 0.)) hi[0] = mx(mx(rmx * c1, rmx * c2), mx(rmn * c1, rmn * c2));
+    if (!wedge_contains(c.tube, -1., 0.)) lo[0] = mn(mn(rmx * c1, rmx * c2), mn(rmn * c1, rmn * c2));
+    if (!wedge_contains(c.tube, 0., 1.)) hi[1] = mx(mx(rmx * si1, rmx * si2), mx(rmn * si1, rmn * si2));
+    if (!wedge_contains(c.tube, 0., -1.)) lo[1] = mn(mn(rmx * si1, rmx * si2), mn(rmn * si1, rmn * si2));
+  }
+  return g.add_shape(kCutTube, &c, sizeof(c) / sizeof(double), 0, 0, lo, hi);
 }
 
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)

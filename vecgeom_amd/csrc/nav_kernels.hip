@@ -21,7 +21,8 @@ static_assert(kNavBlock <= kNavWalkThreads, "walk stack columns");
 // of tubes and boxes needs neither the polygon nor the facet code.
 constexpr unsigned kMaskBoxTube = kShapeBit(kBox) | kShapeBit(kTube);
 constexpr unsigned kMaskSimple = kShapeBit(kBox) | kShapeBit(kTube) | kShapeBit(kCone) | kShapeBit(kTrd) |
-                                 kShapeBit(kTrapezoid) | kShapeBit(kOrb);
+                                 kShapeBit(kTrapezoid) | kShapeBit(kOrb) |
+                                 kShapeBit(kCutTube);
 
 // Waves per SIMD the step kernels of the box-and-tube variant ask the
 // compiler for: the BVH and shape walks hide their latency better at 5 waves

@@ -10,6 +10,7 @@
 #include "polyhedron.hpp"
 #include "tessellated.hpp"
 #include "orb.hpp"
+#include "cut_tube.hpp"
 
 namespace vgm {
 
@@ -45,6 +46,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kOrb:
     f.template operator()<OrbShape>();
+    return true;
+  case kCutTube:
+    f.template operator()<CutTubeShape>();
     return true;
   default:
     return false;
@@ -88,6 +92,9 @@ __host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
     return true;
   case kOrb:
     if constexpr ((Mask & (1u << kOrb)) != 0) f.template operator()<OrbShape>();
+    return true;
+  case kCutTube:
+    if constexpr ((Mask & (1u << kCutTube)) != 0) f.template operator()<CutTubeShape>();
     return true;
   default:
     return false;

@@ -18,6 +18,8 @@ struct BoxShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   // max(|p| - h) (BoxImplementation.h:76-90)
   template <typename R>

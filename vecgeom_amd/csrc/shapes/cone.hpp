@@ -20,6 +20,8 @@ struct ConeShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   template <typename R>
   VGM_DEV static R plus_tol_sq(R x, double tol) { return x * (x + R(2.0 * tol)); }

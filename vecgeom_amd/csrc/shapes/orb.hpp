@@ -18,6 +18,8 @@ struct OrbShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   template <typename R>
   VGM_DEV static R mag2(V3<R> p)

@@ -25,6 +25,8 @@ struct PolyconeShape {
   // ray continues; in float it can end on a different surface than in
   // FP64, so mixed precision keeps this query in FP64
   static constexpr bool kFloatDistanceToOut = false;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   // GetSectionIndex: linear search over the z-planes, compared in the
   // arithmetic type of the query

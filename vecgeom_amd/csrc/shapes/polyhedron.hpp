@@ -29,6 +29,8 @@ struct PolyhedronShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
   using Quad = PolyhedronQuad;
 
   template <typename R>

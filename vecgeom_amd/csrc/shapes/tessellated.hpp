@@ -77,6 +77,8 @@ struct TessellatedShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   template <typename R>
   VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)

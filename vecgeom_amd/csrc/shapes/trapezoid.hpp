@@ -19,6 +19,8 @@ struct TrapezoidShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   template <typename R>
   VGM_DEV static R pdist(const Data &t, int i, V3<R> p)

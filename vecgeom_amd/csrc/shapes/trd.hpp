@@ -18,6 +18,8 @@ struct TrdShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

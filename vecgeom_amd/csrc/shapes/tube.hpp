@@ -21,6 +21,8 @@ struct TubeShape {
   static constexpr bool kSafetyBelowDistance = true;
   // mixed-precision DistanceToOut may be evaluated in float
   static constexpr bool kFloatDistanceToOut = true;
+  // mixed-precision DistanceToIn may be evaluated in float
+  static constexpr bool kFloatDistanceToIn = true;
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>

@@ -65,6 +65,8 @@ __global__ __launch_bounds__(kSoaBlock) void soa_fp64_kernel(const typename S::D
 //                     not on the surface (float safety there above 16 guards);
 //                     FP64 throughout for families whose float DistanceToOut
 //                     can end on another surface (kFloatDistanceToOut false)
+//                     or whose float DistanceToIn can miss where FP64 hits
+//                     (kFloatDistanceToIn false)
 //   SafetyToIn/Out    float; FP64 when the float safety is below the guard band
 // The guard band is a few float ulps of the largest coordinate in play.  The
 // error of the float results against FP64 is measured by the tests and the
@@ -99,8 +101,10 @@ __global__ __launch_bounds__(kSoaBlock) void soa_mixed_kernel(const typename S::
   } else {
     V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
     if (use_tr) d = to_local_dir(tr, d);
-    if constexpr (M == kDistanceToOut && !S::kFloatDistanceToOut) {
-      static_cast<double *>(a.out)[i] = S::template distance_to_out<double>(s, p, d, a.step ? a.step[i] : kInfLength);
+    if constexpr ((M == kDistanceToOut && !S::kFloatDistanceToOut) || (M == kDistanceToIn && !S::kFloatDistanceToIn)) {
+      const double step = a.step ? a.step[i] : kInfLength;
+      static_cast<double *>(a.out)[i] = (M == kDistanceToOut) ? S::template distance_to_out<double>(s, p, d, step)
+                                                              : S::template distance_to_in<double>(s, p, d, step);
       return;
     }
     const V3<float> df{float(d.x), float(d.y), float(d.z)};

@@ -137,6 +137,13 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
   }
   bool move_free = lf + g <= saf;
   const bool by_safety = move_free;
+  if constexpr (!S::kFloatDistanceToIn) {
+    // a float miss from outside is not trusted for such a family
+    if (!move_free && !t.in) {
+      fp64_steps++;
+      return step_fp64<S>(s, t, l);
+    }
+  }
   if (!move_free) {
     const float db = t.in ? S::template distance_to_out<float>(s, pf, df, FLT_MAX)
                           : S::template distance_to_in<float>(s, pf, df, FLT_MAX);

@@ -18,7 +18,8 @@ enum ShapeType : int32_t {
   kPolyhedron = 6,
   kTessellated = 7,
   kOrb = 8,
-  kNumShapeTypes = 9
+  kCutTube = 9,
+  kNumShapeTypes = 10
 };
 
 enum Method : int32_t {

@@ -43,12 +43,12 @@ from typing import Dict, List, Tuple
 
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
-SUPPORTED_SOLIDS = ("box", "tube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb")
+SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
 _REFERENCE_SOLIDS = SUPPORTED_SOLIDS + (
-    "sphere", "torus", "cutTube", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
+    "sphere", "torus", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
     "genericPolycone", "genericPolyhedra", "hype", "paraboloid", "tet", "twistedbox", "twistedtrap",
     "twistedtrd", "twistedtubs", "scaledSolid", "union", "subtraction", "intersection", "multiUnion")
 
@@ -192,6 +192,12 @@ class _Reader:
             z, rmin, rmax = self.lengths(el, "z", "rmin", "rmax")
             sphi, dphi = self.angles(el, "startphi", "deltaphi")
             s = g.tube(rmin, rmax, z / 2.0, sphi, dphi)
+        elif tag == "cutTube":
+            # the normals read as lengths; z halved (Middleware.cpp:738-768)
+            z, rmin, rmax, lx, ly, lz, hx, hy, hz = self.lengths(el, "z", "rmin", "rmax", "lowX", "lowY", "lowZ",
+                                                                  "highX", "highY", "highZ")
+            sphi, dphi = self.angles(el, "startphi", "deltaphi")
+            s = g.cut_tube(rmin, rmax, z / 2.0, sphi, dphi, (lx, ly, lz), (hx, hy, hz))
         elif tag == "cone":
             z, rmin1, rmin2, rmax1, rmax2 = self.lengths(el, "z", "rmin1", "rmin2", "rmax1", "rmax2")
             sphi, dphi = self.angles(el, "startphi", "deltaphi")
