@@ -281,6 +281,13 @@ class Navigator {
     check(vgm_nav_state_to_path(ctx_, state, path.data(), int(path.size()), &depth));
     return path;
   }
+  // BVH::ApproachNextDaughter (source/BVH.cpp:395-449) per track: step (in and
+  // out) shortened to the nearest daughter bounding box of the track's volume
+  void ApproachNextDaughters(SOA3D<> const &p, SOA3D<> const &d, uint32_t const *state, uint32_t const *last,
+                             Precision *step) const
+  {
+    check(vgm_nav_approach(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), p.size(), state, last, step, flags(p)));
+  }
   void ComputeSafeties(SOA3D<> const &p, uint32_t const *state, Precision *safety) const
   {
     check(vgm_nav_safety(ctx_, p.x(), p.y(), p.z(), p.size(), state, safety, flags(p)));

@@ -212,6 +212,26 @@ int vgm_nav_step_safety(vgm_ctx *ctx, const double *x, const double *y, const do
 int vgm_nav_transport(vgm_ctx *ctx, double *x, double *y, double *z, const double *dx, const double *dy,
                       const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
                       int32_t *steps, unsigned flags);
+/* nsteps steps per track under a physics step limit step_limit[i] > 0: each
+ * step is ComputeStepAndSafetyAndPropagatedState with calcsafety
+ * (VecGeom/navigation/VNavigator.h:543-592): the safety first, and when it
+ * covers the limit the track moves by it with no boundary distance computed;
+ * otherwise the geometric step limited to it.  With reuse_safety the safety
+ * left after such a move is carried to the next step.  steps[i] and
+ * geo_steps[i] (steps that computed a boundary distance) are incremented. */
+int vgm_nav_transport_limited(vgm_ctx *ctx, double *x, double *y, double *z, const double *dx, const double *dy,
+                              const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                              uint8_t *boundary, int nsteps, int reuse_safety, int32_t *steps, int32_t *geo_steps,
+                              unsigned flags);
+/* BVH::ApproachNextDaughter (source/BVH.cpp:395-449, VecGeom/base/BVH.h:158)
+ * per track: step[i] (in: the current step, out: the approached step) is
+ * shortened to the distance along the ray to the nearest daughter bounding
+ * box of the volume of state[i] (negative when the point is inside one),
+ * skipping the volume last[i] from its surface.  Tracks outside the world
+ * keep their step. */
+int vgm_nav_approach(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx,
+                     const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *last,
+                     double *step, unsigned flags);
 /* out[i] = safety of global point i in its state (mother SafetyToOut, then
  * the daughters through the BVH); 0 for state 0. */
 int vgm_nav_safety(vgm_ctx *ctx, const double *x, const double *y, const double *z, size_t n,

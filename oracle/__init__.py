@@ -62,6 +62,8 @@ def lib():
             "orc_nav_locate": [_P, _P, _P, _P, ctypes.c_size_t, _P, _I],
             "orc_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _P, _I],
             "orc_nav_transport": [_P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _I, _P, _I],
+            "orc_nav_transport_limited": [_P] * 8 + [ctypes.c_size_t, _P, _P, _P, _I, _I, _P, _P, _I],
+            "orc_nav_approach": [_P] * 7 + [ctypes.c_size_t, _P, _P, _P, _I],
             "orc_nav_safety": [_P, _P, _P, _P, ctypes.c_size_t, _P, _P, _I],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
@@ -174,6 +176,20 @@ class Geometry:
     def nav_transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, nthreads=0):
         self._L.orc_nav_transport(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), len(x), _p(state), _p(last),
                                   _p(boundary), nsteps, _p(steps), nthreads)
+
+    def nav_transport_limited(self, x, y, z, dx, dy, dz, step_limit, state, last, boundary, nsteps, steps, geo_steps,
+                              reuse_safety=False, nthreads=0):
+        sl = np.ascontiguousarray(step_limit, dtype=np.float64)
+        self._L.orc_nav_transport_limited(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), _p(sl), len(x),
+                                          _p(state), _p(last), _p(boundary), nsteps, 1 if reuse_safety else 0,
+                                          _p(steps), _p(geo_steps), nthreads)
+
+    def nav_approach(self, pts, dirs, state, last, step, nthreads=0):
+        """BVH::ApproachNextDaughter per track; step is updated in place."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        dx, dy, dz = (np.ascontiguousarray(a, dtype=np.float64) for a in dirs)
+        self._L.orc_nav_approach(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), len(x), _p(state), _p(last),
+                                 _p(step), nthreads)
 
     def nav_safety(self, pts, state, nthreads=0):
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)

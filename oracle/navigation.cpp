@@ -418,6 +418,92 @@ struct Navigator {
     } while (ptr > stack);
   }
 
+  // UnplacedVolume::ApproachSolid (VecGeom/volumes/UnplacedVolume.h:272-281)
+  // = BoxImplementation::IntersectCachedKernel2 (BoxImplementation.h:326-368)
+  // on the extent, t0 = 0, t1 = kInfLength
+  static double ApproachSolid(const Vec3 corners[2], const Vec3 &point, const Vec3 &inverseray)
+  {
+    const int signx = inverseray.x < 0, signy = inverseray.y < 0, signz = inverseray.z < 0;
+    double tmin = (corners[signx].x - point.x) * inverseray.x;
+    double tymax = (corners[1 - signy].y - point.y) * inverseray.y;
+    if (tmin > tymax) return kInfLength;
+    double tmax = (corners[1 - signx].x - point.x) * inverseray.x;
+    double tymin = (corners[signy].y - point.y) * inverseray.y;
+    if (tymin > tmax) return kInfLength;
+    tmin = Max(tmin, tymin);
+    tmax = Min(tmax, tymax);
+    double tzmin = (corners[signz].z - point.z) * inverseray.z;
+    double tzmax = (corners[1 - signz].z - point.z) * inverseray.z;
+    if ((tmin > tzmax) || (tzmin > tmax)) return kInfLength;
+    tmin = Max(tmin, tzmin);
+    tmax = Min(tmax, tzmax);
+    if (!((tmin <= kInfLength + kTolerance) && (tmax > 0.0 - kTolerance))) return kInfLength;
+    return tmin;
+  }
+
+  // BVH::ApproachNextDaughter (source/BVH.cpp:395-449)
+  void ApproachNextDaughter(const LogicalVolume &lv, int lvid, const Vec3 &point, const Vec3 &dir, double &step,
+                            int last) const
+  {
+    if (lv.daughters.empty()) return;
+    if (!use_bvh) {  // linear scan over the daughters, no bounding-box pruning
+      for (int vol : lv.daughters) {
+        const Vec3 localdir = G.pvs[vol].tr.TransformDirection(dir);
+        const Vec3 invlocaldir(1.0 / NonZero(localdir[0]), 1.0 / NonZero(localdir[1]), 1.0 / NonZero(localdir[2]));
+        Vec3 bbox[2];
+        ShapeOf(vol).Extent(bbox[0], bbox[1]);
+        const double dist = ApproachSolid(bbox, G.pvs[vol].tr.Transform_(point), invlocaldir);
+        if (dist < step && !(dist <= 0.0 && vol == last)) step = dist;
+      }
+      return;
+    }
+    const Bvh &B = bvh[lvid];
+    const Vec3 invdir(1.0 / NonZero(dir[0]), 1.0 / NonZero(dir[1]), 1.0 / NonZero(dir[2]));
+    unsigned stack[40] = {0}, *ptr = &stack[1];
+    do {
+      const unsigned id = *--ptr;
+      if (B.nchild[id] >= 0) {
+        for (int i = 0; i < B.nchild[id]; ++i) {
+          const int prim = B.prim[B.offset[id] + i];
+          if (B.aabbs[prim].HitWithin(point, invdir, step)) {
+            const int vol = lv.daughters[prim];
+            const Vec3 localpoint = G.pvs[vol].tr.Transform_(point);
+            const Vec3 localdir = G.pvs[vol].tr.TransformDirection(dir);
+            const Vec3 invlocaldir(1.0 / NonZero(localdir[0]), 1.0 / NonZero(localdir[1]),
+                                   1.0 / NonZero(localdir[2]));
+            Vec3 bbox[2];
+            ShapeOf(vol).Extent(bbox[0], bbox[1]);
+            const double dist = ApproachSolid(bbox, localpoint, invlocaldir);
+            if (dist < step && !(dist <= 0.0 && vol == last)) step = dist;
+          }
+        }
+      } else {
+        const unsigned childL = 2 * id + 1, childR = 2 * id + 2;
+        double tminL = kInfLength, tmaxL = -kInfLength, tminR = kInfLength, tmaxR = -kInfLength;
+        B.nodes[childL].IntersectInvDir(point, invdir, tminL, tmaxL);
+        B.nodes[childR].IntersectInvDir(point, invdir, tminR, tmaxR);
+        const bool traverseL = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
+        const bool traverseR = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
+        if (tminR < tminL) {
+          if (traverseR) *ptr++ = childR;
+          if (traverseL) *ptr++ = childL;
+        } else {
+          if (traverseL) *ptr++ = childL;
+          if (traverseR) *ptr++ = childR;
+        }
+      }
+    } while (ptr > stack);
+  }
+
+  double Approach(const Vec3 &gp, const Vec3 &gd, double step, const State &in) const
+  {
+    const Transform &T = nodes[in.ind].global;
+    const int pv = nodes[in.ind].pv;
+    const int last = in.last ? nodes[in.last].pv : -1;
+    ApproachNextDaughter(LvOf(pv), G.pvs[pv].lv, T.Transform_(gp), T.TransformDirection(gd), step, last);
+    return step;
+  }
+
   // BVH::ComputeSafety (VecGeom/base/BVH.h:172-209)
   double DaughterSafety(const LogicalVolume &lv, int lvid, const Vec3 &p, double safety) const
   {
@@ -617,6 +703,72 @@ int orc_nav_transport(orc_geom *g, double *x, double *y, double *z, const double
     last[i] = s.last;
     boundary[i] = s.boundary;
     steps[i] += k;
+  }
+  return 0;
+}
+
+// Transport under a physics step limit: nsteps of
+// ComputeStepAndSafetyAndPropagatedState with calcsafety
+// (VecGeom/navigation/VNavigator.h:543-592): when the safety covers the
+// limit the track moves by it with no boundary distance (safetydone);
+// otherwise the step limited to it.  reuse_safety carries safety - limit
+// after such a move; a geometry step forces a fresh safety.
+int orc_nav_transport_limited(orc_geom *g, double *x, double *y, double *z, const double *dx, const double *dy,
+                              const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                              uint8_t *boundary, int nsteps, int reuse_safety, int32_t *steps, int32_t *geo_steps,
+                              int nthreads)
+{
+  Navigator *N = nav_of(g);
+  if (!N) return -1;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long i = 0; i < long(n); ++i) {
+    State s{state[i], last[i], boundary[i] != 0};
+    Vec3 p(x[i], y[i], z[i]);
+    const Vec3 d(dx[i], dy[i], dz[i]);
+    const double L = step_limit[i];
+    double safety = -1.0;
+    int k = 0, geo = 0;
+    for (; k < nsteps && s.ind; ++k) {
+      const bool carried = reuse_safety && safety >= L;
+      if (!carried) safety = N->Safety(p, s.ind);
+      double st;
+      if (safety >= L) {  // safetydone
+        st = L;
+        s.boundary = false;
+        safety = safety - L;
+      } else {
+        State out;
+        st = N->Step(p, d, L, s, out);
+        s = out;
+        safety = -1.0;
+        ++geo;
+      }
+      p = Vec3(p.x + st * d.x, p.y + st * d.y, p.z + st * d.z);
+    }
+    x[i] = p.x;
+    y[i] = p.y;
+    z[i] = p.z;
+    state[i] = s.ind;
+    last[i] = s.last;
+    boundary[i] = s.boundary;
+    steps[i] += k;
+    geo_steps[i] += geo;
+  }
+  return 0;
+}
+
+int orc_nav_approach(orc_geom *g, const double *x, const double *y, const double *z, const double *dx,
+                     const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *last,
+                     double *step, int nthreads)
+{
+  Navigator *N = nav_of(g);
+  if (!N) return -1;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (long i = 0; i < long(n); ++i) {
+    if (!state[i]) continue;
+    step[i] = N->Approach(Vec3(x[i], y[i], z[i]), Vec3(dx[i], dy[i], dz[i]), step[i], State{state[i], last[i], false});
   }
   return 0;
 }

@@ -124,3 +124,40 @@ def test_transport_200_steps(setup, device_ptrs):
     assert np.array_equal(got[6], s_o) and np.array_equal(got[7], l_o) and np.array_equal(got[8], b_o)
     assert np.array_equal(got[9], n_o)
     assert n_o.sum() > 20 * N
+
+
+def test_approach_next_daughter(setup):
+    print("[nav] approach", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    big = np.finfo(np.float64).max
+    # from the located points, then from the boundary after one step (last set)
+    s_o, l_o, b_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    stp = geo.nav_step(pts, dirs, s_o, l_o, b_o)
+    moved = tuple(np.ascontiguousarray(pts[k] + stp * dirs[k]) for k in range(3))
+    for p, s, last in ((pts, st, np.zeros(N, np.uint32)), (moved, s_o, l_o)):
+        ref = np.full(N, big)
+        geo.nav_approach(p, dirs, s, last, ref)
+        got = np.full(N, big)
+        nav.approach(*p, *dirs, s.copy(), last.copy(), got)
+        assert np.array_equal(got, ref)
+        assert np.mean(ref < big) > 0.2
+
+
+@pytest.mark.parametrize("reuse", [False, True], ids=["fresh", "carried"])
+@pytest.mark.parametrize("L", [2.0, 40.0])
+def test_transport_limited(setup, L, reuse):
+    print(f"[nav] limited transport L={L} reuse={reuse}", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    lim = np.full(N, L)
+    outs = []
+    for run in (geo.nav_transport_limited, nav.transport_limited):
+        x, y, z = (a.copy() for a in pts)
+        s, last, b = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+        steps, geo_steps = np.zeros(N, np.int32), np.zeros(N, np.int32)
+        run(x, y, z, *dirs, lim, s, last, b, 100, steps, geo_steps, reuse_safety=reuse)
+        outs.append((x, y, z, s, last, b, steps, geo_steps))
+    for k, (r, g) in enumerate(zip(*outs)):
+        assert np.array_equal(g, r), k
+    assert outs[0][7].sum() < outs[0][6].sum()

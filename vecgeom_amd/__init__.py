@@ -117,6 +117,8 @@ def _declare(L):
         "vgm_nav_path_to_state": [_P, _P, _I, ctypes.POINTER(ctypes.c_uint32)],
         "vgm_nav_state_to_path": [_P, ctypes.c_uint32, _P, _I, ctypes.POINTER(_I)],
         "vgm_nav_transport": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _I, _P, _U],
+        "vgm_nav_transport_limited": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _I, _I, _P, _P, _U],
+        "vgm_nav_approach": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _U],
         "vgm_nav_safety": [_P, _P, _P, _P, _SZ, _P, _P, _U],
     }
     for name, args in nsig.items():
@@ -467,6 +469,19 @@ class Navigation:
         check(self._L.vgm_nav_transport(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                         len(x), _ptr(state), _ptr(last), _ptr(boundary), nsteps, _ptr(steps),
                                         self._fl(x, asynchronous)))
+
+    def transport_limited(self, x, y, z, dx, dy, dz, step_limit, state, last, boundary, nsteps, steps, geo_steps,
+                          reuse_safety=False):
+        """nsteps of ComputeStepAndSafetyAndPropagatedState under the physics step step_limit[i]."""
+        check(self._L.vgm_nav_transport_limited(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy),
+                                                _ptr(dz), _ptr(step_limit), len(x), _ptr(state), _ptr(last),
+                                                _ptr(boundary), nsteps, 1 if reuse_safety else 0, _ptr(steps),
+                                                _ptr(geo_steps), self._fl(x)))
+
+    def approach(self, x, y, z, dx, dy, dz, state, last, step):
+        """BVH::ApproachNextDaughter: step shortened to the nearest daughter bounding box."""
+        check(self._L.vgm_nav_approach(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
+                                       len(x), _ptr(state), _ptr(last), _ptr(step), self._fl(x)))
 
     def safety(self, x, y, z, state, out):
         check(self._L.vgm_nav_safety(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), len(x), _ptr(state), _ptr(out),

@@ -194,6 +194,45 @@ int vgm_nav_transport(vgm_ctx *c, double *x, double *y, double *z, const double 
                  flags);
 }
 
+int vgm_nav_transport_limited(vgm_ctx *c, double *x, double *y, double *z, const double *dx, const double *dy,
+                              const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                              uint8_t *boundary, int nsteps, int reuse_safety, int32_t *steps, int32_t *geo_steps,
+                              unsigned flags)
+{
+  if (n && (!state || !last || !boundary || !steps || !geo_steps)) return fail(VGM_EINVAL, "null state or counter array");
+  if (n && !step_limit) return fail(VGM_EINVAL, "null physics step array");
+  if (nsteps < 0) return fail(VGM_EINVAL, "negative step count");
+  NavArgs a{};
+  a.n = n;
+  a.nsteps = nsteps;
+  a.reuse_safety = reuse_safety ? 1 : 0;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavTransportLimited, a,
+                 {{x, b8, true, true, slot(a.x)}, {y, b8, true, true, slot(a.y)}, {z, b8, true, true, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit)},
+                  {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
+                  {boundary, n, true, true, slot(a.boundary)}, {steps, b4, true, true, slot(a.steps)},
+                  {geo_steps, b4, true, true, slot(a.geo_steps)}},
+                 flags);
+}
+
+int vgm_nav_approach(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx,
+                     const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *last,
+                     double *step, unsigned flags)
+{
+  if (n && (!state || !last || !step)) return fail(VGM_EINVAL, "null state or step array");
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavApproach, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {state, b4, true, false, slot(a.state)},
+                  {last, b4, true, false, slot(a.last)}, {step, b8, true, true, slot(a.step)}},
+                 flags);
+}
+
 int vgm_nav_safety(vgm_ctx *c, const double *x, const double *y, const double *z, size_t n, const uint32_t *state,
                    double *out, unsigned flags)
 {

@@ -8,7 +8,14 @@
 
 namespace vgm {
 
-enum NavKind : int32_t { kNavLocate = 0, kNavStep = 1, kNavTransport = 2, kNavSafety = 3 };
+enum NavKind : int32_t {
+  kNavLocate = 0,
+  kNavStep = 1,
+  kNavTransport = 2,
+  kNavSafety = 3,
+  kNavApproach = 4,           // ApproachNextDaughter per track
+  kNavTransportLimited = 5,   // transport under a physics step, safety first
+};
 
 struct NavArgs {
   GeomView g;
@@ -21,6 +28,8 @@ struct NavArgs {
   const uint8_t *calcsafety;  // step with safeties: per track, may be null (every track)
   double *safety;             // step with safeties: output safety (null: plain step)
   int32_t *steps;  // transport: steps taken (accumulated)
+  int32_t *geo_steps;  // limited transport: steps that needed the boundary distance (accumulated)
+  int32_t reuse_safety;  // limited transport: carry the safety left after a safety-only move
   size_t n;
   int32_t nsteps;
   uint32_t shape_mask;  // families used by the navigation tree (bit per ShapeType)

@@ -92,6 +92,64 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
   a.steps[i] += k;
 }
 
+// Transport under a physics step limit L per track: each step is
+// ComputeStepAndSafetyAndPropagatedState with calcsafety
+// (VecGeom/navigation/VNavigator.h:543-592): the safety first, and when it
+// covers L the track moves L with no boundary distance computed
+// (`safetydone`); otherwise the step limited to L.  With reuse_safety the
+// safety left after a safety-only move (safety - L, still a lower bound in
+// the same volume) is carried, and the safety is recomputed only when that
+// no longer covers L or after a geometry step.
+template <unsigned Mask>
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_transport_limited_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  Navigator<Mask> nav{a.g};
+  NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
+  V3<double> p{a.x[i], a.y[i], a.z[i]};
+  const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
+  const double L = a.step_limit[i];
+  double saf = -1.0;
+  int k = 0, geo = 0;
+  for (; k < a.nsteps && s.ind; ++k) {
+    if (!(a.reuse_safety && saf >= L)) saf = nav.safety(p, s.ind);
+    double st;
+    if (saf >= L) {
+      st = L;
+      s.boundary = false;
+      saf = saf - L;
+    } else {
+      NavState out;
+      st = nav.step(p, d, L, s, out);
+      s = out;
+      saf = -1.0;
+      ++geo;
+    }
+    p.x = p.x + st * d.x;
+    p.y = p.y + st * d.y;
+    p.z = p.z + st * d.z;
+  }
+  a.x[i] = p.x;
+  a.y[i] = p.y;
+  a.z[i] = p.z;
+  a.state[i] = s.ind;
+  a.last[i] = s.last;
+  a.boundary[i] = s.boundary ? 1 : 0;
+  a.steps[i] += k;
+  a.geo_steps[i] += geo;
+}
+
+template <unsigned Mask>
+__global__ __launch_bounds__(kNavBlock) void nav_approach_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n || !a.state[i]) return;
+  Navigator<Mask> nav{a.g};
+  const NavState s{a.state[i], a.last[i], false};
+  a.step[i] = nav.approach(V3<double>{a.x[i], a.y[i], a.z[i]}, V3<double>{a.dx[i], a.dy[i], a.dz[i]}, a.step[i], s);
+}
+
 template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) void nav_safety_kernel(NavArgs a)
 {
@@ -113,6 +171,12 @@ void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned bl
     hipLaunchKernelGGL(nav_transport_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
     break;
   case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
+  case kNavApproach:
+    hipLaunchKernelGGL(nav_approach_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    break;
+  case kNavTransportLimited:
+    hipLaunchKernelGGL(nav_transport_limited_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    break;
   default: ok = false;
   }
 }

@@ -199,6 +199,90 @@ struct Navigator {
     } while (sp > 0);
   }
 
+  // UnplacedVolume::ApproachSolid (VecGeom/volumes/UnplacedVolume.h:272-281):
+  // BoxImplementation::IntersectCachedKernel2 (BoxImplementation.h:326-368) on
+  // the extent with t0 = 0, t1 = kInfLength; negative inside the box
+  __host__ __device__ static double approach_solid(const double *mn, const double *mx, V3<double> p, V3<double> inv)
+  {
+    const bool sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
+    double tmin = ((sx ? mx[0] : mn[0]) - p.x) * inv.x;
+    const double tymax = ((sy ? mn[1] : mx[1]) - p.y) * inv.y;
+    if (tmin > tymax) return kInfLength;
+    double tmax = ((sx ? mn[0] : mx[0]) - p.x) * inv.x;
+    const double tymin = ((sy ? mx[1] : mn[1]) - p.y) * inv.y;
+    if (tymin > tmax) return kInfLength;
+    tmin = vmax(tmin, tymin);
+    tmax = vmin(tmax, tymax);
+    const double tzmin = ((sz ? mx[2] : mn[2]) - p.z) * inv.z;
+    const double tzmax = ((sz ? mn[2] : mx[2]) - p.z) * inv.z;
+    if (tmin > tzmax || tzmin > tmax) return kInfLength;
+    tmin = vmax(tmin, tzmin);
+    tmax = vmin(tmax, tzmax);
+    if (!((tmin <= kInfLength + kTolerance) && (tmax > 0.0 - kTolerance))) return kInfLength;
+    return tmin;
+  }
+
+  // BVH::ApproachNextDaughter (source/BVH.cpp:395-449): the daughter traversal
+  // of check_daughters with the daughter's bounding box (ApproachSolid) in
+  // place of its shape
+  __host__ __device__ void approach_next_daughter(int lv, V3<double> p, V3<double> d, double &step, int last_pv) const
+  {
+    const LogicalRec &L = g.lvs[lv];
+    if (L.bvh < 0) return;
+    const BvhRec &B = g.bvhs[L.bvh];
+    const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
+    const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
+    const V3<double> inv{1.0 / nonzero(d.x), 1.0 / nonzero(d.y), 1.0 / nonzero(d.z)};
+    NavWalkStack stack;
+    int sp = 0;
+    stack.set(sp++, 0);
+    do {
+      const unsigned id = stack.get(--sp);
+      const int nc = nchild[id];
+      if (nc >= 0) {
+        for (int i = 0; i < nc; ++i) {
+          const int k = prim[offset[id] + i];
+          double tmin, tmax;
+          box_intersect(pbox + 6 * k, p, inv, tmin, tmax);
+          if (tmin <= tmax && tmax >= 0.0 && tmin < step) {
+            const int v = g.daughters[L.dbegin + k];
+            const Transform &T = g.pvs[v].tr;
+            const V3<double> lp = to_local_point(T, p), ld = to_local_dir(T, d);
+            const V3<double> li{1.0 / nonzero(ld.x), 1.0 / nonzero(ld.y), 1.0 / nonzero(ld.z)};
+            const ShapeRec &S = g.shapes[shape_of_pv(v)];
+            const double dist = approach_solid(S.bmin, S.bmax, lp, li);
+            if (dist < step && !(dist <= 0.0 && v == last_pv)) step = dist;
+          }
+        }
+      } else {
+        const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
+        double tminL, tmaxL, tminR, tmaxR;
+        box_intersect(nbox + 6 * cl, p, inv, tminL, tmaxL);
+        box_intersect(nbox + 6 * cr, p, inv, tminR, tmaxR);
+        const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
+        const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
+        if (tminR < tminL) {
+          if (tr) stack.set(sp++, cr);
+          if (tl) stack.set(sp++, cl);
+        } else {
+          if (tl) stack.set(sp++, cl);
+          if (tr) stack.set(sp++, cr);
+        }
+      }
+    } while (sp > 0);
+  }
+
+  // ApproachNextDaughter from a global point in the volume of `in`; the
+  // volume last exited (in.last) is not approached again from its surface
+  __host__ __device__ double approach(V3<double> gp, V3<double> gd, double step, const NavState &in) const
+  {
+    const NavNode &N = g.nodes[in.ind];
+    const V3<double> lp = to_local_point(N.global, gp), ld = to_local_dir(N.global, gd);
+    const int last_pv = in.last ? g.nodes[in.last].pv : -1;
+    approach_next_daughter(g.pvs[N.pv].lv, lp, ld, step, last_pv);
+    return step;
+  }
+
   // BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint): first
   // daughter in tree order (skipping excl_pv) that contains the point
   __host__ __device__ int level_locate(int lv, int excl_pv, V3<double> p, V3<double> &dp) const
