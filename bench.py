@@ -229,6 +229,35 @@ def run_navigation(args, ctx, dev, torch):
                        f"{n} tracks x up to {args.nav_steps} steps", "value": total / kern, "unit": "track-steps/s",
            "track_steps": total, "kernel_ms": kern * 1e3, "wall_ms": wall * 1e3}
     log(f"[bench] navigation: {total / kern / 1e9:.3f} G track-steps/s ({total} steps, kernel {kern * 1e3:.1f} ms)")
+
+    # the same tracks under a 10 mm physics step: the safety first, a boundary
+    # distance only when the safety does not cover the step
+    lim = torch.full((n,), 10.0, dtype=torch.float64, device=dev)
+
+    def limited(reuse):
+        x, y, z = (a.clone() for a in P)
+        st = st0.clone()
+        last = torch.zeros(n, dtype=torch.int32, device=dev)
+        bnd = torch.zeros(n, dtype=torch.uint8, device=dev)
+        steps = torch.zeros(n, dtype=torch.int32, device=dev)
+        geo_steps = torch.zeros(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        nav.transport_limited(x, y, z, *D, lim, st, last, bnd, args.nav_steps, steps, geo_steps, reuse_safety=reuse)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) * 1e-3, int(steps.sum().item()), int(geo_steps.sum().item())
+
+    out["limited_10mm"] = {}
+    for reuse in (False, True):
+        limited(reuse)
+        k2, s2, g2 = limited(reuse)
+        key = "carried_safety" if reuse else "fresh_safety"
+        out["limited_10mm"][key] = {"value": s2 / k2, "unit": "track-steps/s", "track_steps": s2,
+                                    "geometry_steps": g2, "kernel_ms": k2 * 1e3}
+        log(f"[bench] navigation, 10 mm physics step, {key}: {s2 / k2 / 1e9:.3f} G track-steps/s "
+            f"({g2 / max(s2, 1):.2f} of the steps computed a boundary distance)")
     if not args.no_cpu:
         import oracle
 
