@@ -33,8 +33,8 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
 /* Shape types, as indices of the per-family arrays of the batch calls:
  * 0 box, 1 tube, 2 cone, 3 trd, 4 trapezoid, 5 polycone, 6 polyhedron,
- * 7 tessellated, 8 orb, 9 cut tube. */
-#define VGM_NUM_SHAPE_TYPES 10
+ * 7 tessellated, 8 orb, 9 cut tube, 10 boolean. */
+#define VGM_NUM_SHAPE_TYPES 11
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
@@ -69,6 +69,16 @@ int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
 /* UnplacedOrb(r) (VecGeom/volumes/UnplacedOrb.h; kernels
  * VecGeom/volumes/kernel/OrbImplementation.h:64-225): a full sphere. */
 int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
+/* UnplacedBooleanVolume<op>(left placed, right placed)
+ * (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:8-31; kernels
+ * VecGeom/volumes/kernel/BooleanImplementation.h:39-290 and
+ * BooleanIntersectionImplementation.h): op 1 = kIntersection, 2 =
+ * kSubtraction (left minus right); union (0) is rejected.  The operands are
+ * existing primitive shapes (not booleans or tessellated solids) placed by
+ * left_tr / right_tr (12 doubles: translation, row-major rotation; NULL =
+ * identity).  The operand parameters are copied at creation. */
+int vgm_boolean_create(vgm_ctx *ctx, int op, int left, const double *left_tr, int right, const double *right_tr,
+                       int *shape);
 /* UnplacedCutTube(rmin, rmax, z, sphi, dphi, bx, by, bz, tx, ty, tz)
  * (VecGeom/volumes/UnplacedCutTube.h:54-70; kernels
  * VecGeom/volumes/kernel/CutTubeImplementation.h:106-321): a tube of

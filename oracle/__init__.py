@@ -51,6 +51,7 @@ def lib():
             "orc_box_create": [_P, _D, _D, _D],
             "orc_orb_create": [_P, _D],
             "orc_cut_tube_create": [_P] + [_D] * 11,
+            "orc_boolean_create": [_P, _I, _I, _P, _I, _P],
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
             "orc_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D],
             "orc_trd_create": [_P, _D, _D, _D, _D, _D],
@@ -114,6 +115,13 @@ class Geometry:
 
     def orb(self, r):
         return self._chk(self._L.orc_orb_create(self.h, r))
+
+    def boolean(self, op, left, right, left_tr=None, right_tr=None):
+        ops = {"intersection": 1, "subtraction": 2}
+        ids = [getattr(self, s[0])(*s[1]) if isinstance(s, tuple) else s for s in (left, right)]
+        trs = [None if t is None else np.ascontiguousarray(t, dtype=np.float64) for t in (left_tr, right_tr)]
+        return self._chk(self._L.orc_boolean_create(self.h, ops[op], ids[0], None if trs[0] is None else _p(trs[0]),
+                                                    ids[1], None if trs[1] is None else _p(trs[1])))
 
     def cut_tube(self, rmin, rmax, z, sphi, dphi, bottom, top):
         return self._chk(self._L.orc_cut_tube_create(self.h, rmin, rmax, z, sphi, dphi, *bottom, *top))

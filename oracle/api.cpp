@@ -23,6 +23,13 @@ static int add_shape(orc_geom *g, Shape *s)
 }
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz) { return add_shape(g, make_box(dx, dy, dz)); }
+int orc_boolean_create(orc_geom *g, int op, int left, const double *ltr, int right, const double *rtr)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  const int n = int(G->shapes.size());
+  if ((op != 1 && op != 2) || left < 0 || left >= n || right < 0 || right >= n) return -1;
+  return add_shape(g, make_boolean(op, G->shapes[left].get(), ltr, G->shapes[right].get(), rtr));
+}
 int orc_orb_create(orc_geom *g, double r) { return r > 0 ? add_shape(g, make_orb(r)) : -1; }
 int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
                         double bz, double tx, double ty, double tz)

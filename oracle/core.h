@@ -140,6 +140,8 @@ struct Shape {
 
 Shape *make_box(double dx, double dy, double dz);
 Shape *make_orb(double r);
+// op 1 intersection, 2 subtraction; ltr/rtr: 12 doubles (translation, rotation) or null
+Shape *make_boolean(int op, const Shape *left, const double *ltr, const Shape *right, const double *rtr);
 Shape *make_cut_tube(double rmin, double rmax, double z, double sphi, double dphi, const double bot[3],
                      const double top[3]);
 Shape *make_cone(double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi);

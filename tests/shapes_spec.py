@@ -75,6 +75,18 @@ SHAPES.update({
 })
 
 SHAPES.update({
+    # booleans: (operation, left (kind, args), right (kind, args), left placement, right placement)
+    "bool_box_minus_tube": ("boolean", ("subtraction", ("box", (20.0, 20.0, 20.0)),
+                                        ("tube", (0.0, 8.0, 30.0, 0.0, 2 * PI)), None, None)),
+    "bool_box_minus_rot_box": ("boolean", ("subtraction", ("box", (20.0, 15.0, 10.0)), ("box", (8.0, 8.0, 8.0)),
+                                           None, transform12((10.0, 5.0, 0.0), angle=0.4))),
+    "bool_tube_and_box": ("boolean", ("intersection", ("tube", (5.0, 25.0, 30.0, 0.0, 2 * PI)),
+                                      ("box", (20.0, 20.0, 40.0)), None, transform12((3.0, 0.0, 0.0)))),
+    "bool_cone_and_orb": ("boolean", ("intersection", ("cone", (0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * PI)),
+                                      ("orb", (30.0,)), None, transform12((0.0, 0.0, 15.0), angle=0.0))),
+})
+
+SHAPES.update({
     "trd_box": ("trd", (20.0, 20.0, 30.0, 30.0, 40.0)),
     "trd_2": ("trd", (10.0, 30.0, 20.0, 40.0, 40.0)),
     "trd_1": ("trd", (10.0, 30.0, 20.0, 20.0, 40.0)),

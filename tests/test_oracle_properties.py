@@ -99,7 +99,10 @@ def test_safety_bounds_and_surface_hits(name):
     hit = inn & (dout < 1e30)
     q = tuple(pts[k][hit] + dout[hit] * dirs[k][hit] for k in range(3))
     assert np.all(geo.query(pv, "Inside", q) == 2)
-    # entry points lie on the surface
+    # entry points lie on the surface (a boolean subtraction pushes every entry
+    # kPushTolerance = 1e-6 past it: BooleanImplementation.h:176-183)
+    if kind == "boolean" and args[0] == "subtraction":
+        din = np.where(din < 1e30, din - 1e-6, din)
     hit = out & (din < 1e30)
     q = tuple(pts[k][hit] + din[hit] * dirs[k][hit] for k in range(3))
     res = geo.query(pv, "Inside", q)

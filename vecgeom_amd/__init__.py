@@ -32,7 +32,7 @@ TUNED = 128
 CULL = 256
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
-SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube")
+SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean")
 
 # Inside_t (VecGeom/base/Global.h:71-88)
 kInside, kSurface, kOutside = 1, 2, 3
@@ -83,6 +83,7 @@ def _declare(L):
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_orb_create": [_P, _D, ctypes.POINTER(_I)],
         "vgm_cut_tube_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
+        "vgm_boolean_create": [_P, _I, _I, _P, _I, _P, ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cone_create": [_P, _D, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_trd_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
@@ -255,6 +256,14 @@ class Context:
 
     def orb(self, r) -> int:
         return self._new(self._L.vgm_orb_create, r)
+
+    def boolean(self, op, left, right, left_tr=None, right_tr=None) -> int:
+        """UnplacedBooleanVolume: op "intersection" or "subtraction"; left and
+        right are shape ids or (kind, args) pairs built here first."""
+        ops = {"intersection": 1, "subtraction": 2}
+        ids = [getattr(self, s[0])(*s[1]) if isinstance(s, tuple) else s for s in (left, right)]
+        trs = [None if t is None else np.ascontiguousarray(t, dtype=np.float64) for t in (left_tr, right_tr)]
+        return self._new(self._L.vgm_boolean_create, ops[op], ids[0], _ptr(trs[0]), ids[1], _ptr(trs[1]))
 
     def cut_tube(self, rmin, rmax, z, sphi, dphi, bottom, top) -> int:
         """UnplacedCutTube(rmin, rmax, z, sphi, dphi, bottomNormal, topNormal)."""

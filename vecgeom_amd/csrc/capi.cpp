@@ -244,6 +244,17 @@ int vgm_box_create(vgm_ctx *c, double dx, double dy, double dz, int *shape)
   return VGM_OK;
 }
 
+int vgm_boolean_create(vgm_ctx *c, int op, int left, const double *left_tr, int right, const double *right_tr,
+                       int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_boolean(c->host, op, left, left_tr, right, right_tr, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_cut_tube_create(vgm_ctx *c, double rmin, double rmax, double z, double sphi, double dphi, double bx,
                         double by, double bz, double tx, double ty, double tz, int *shape)
 {

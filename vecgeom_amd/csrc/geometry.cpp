@@ -6,6 +6,8 @@
 #include "shapes/box_data.h"
 #include "shapes/orb_data.h"
 #include "shapes/cut_tube_data.h"
+#include "shapes/boolean_data.h"
+#include "transform.hpp"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
 #include "shapes/trd_data.h"
@@ -177,6 +179,79 @@ int build_cut_tube(HostGeometry &g, double rmin, double rmax, double z, double s
     if (!wedge_contains(c.tube, 0., -1.)) lo[1] = mn(mn(rmx * si1, rmx * si2), mn(rmn * si1, rmn * si2));
   }
   return g.add_shape(kCutTube, &c, sizeof(c) / sizeof(double), 0, 0, lo, hi);
+}
+
+// TransformedExtent (source/UnplacedBooleanVolume.cpp:403-442): the box of
+// the eight corners of an operand's extent taken to the boolean's frame
+static void transformed_extent(const ShapeRec &s, const Transform &T, double mn[3], double mx[3])
+{
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) mn[k] = DBL_MAX, mx[k] = -DBL_MAX, lo[k] = s.bmin[k], hi[k] = s.bmax[k];
+  const double delta[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  for (int x = 0; x <= 1; ++x)
+    for (int y = 0; y <= 1; ++y)
+      for (int z = 0; z <= 1; ++z) {
+        const V3<double> c{lo[0] + x * delta[0], lo[1] + y * delta[1], lo[2] + z * delta[2]};
+        const V3<double> m = to_master_point<double>(T, c);
+        const double v[3] = {m.x, m.y, m.z};
+        for (int k = 0; k < 3; ++k) {
+          mn[k] = v[k] < mn[k] ? v[k] : mn[k];
+          mx[k] = mx[k] < v[k] ? v[k] : mx[k];
+        }
+      }
+}
+
+// UnplacedBooleanVolume<kIntersection / kSubtraction>(left, right)
+// (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:19-31) and its
+// Extent (source/UnplacedBooleanVolume.cpp:444-482): the operand blocks are
+// copied after the header so that the boolean's block is self-contained
+int build_boolean(HostGeometry &g, int op, int left, const double *ltr, int right, const double *rtr,
+                  std::string &err)
+{
+  if (op != kBoolIntersection && op != kBoolSubtraction) {
+    err = "boolean operation must be intersection (1) or subtraction (2); union is not implemented";
+    return -1;
+  }
+  const int n = int(g.shapes.size());
+  if (left < 0 || left >= n || right < 0 || right >= n) {
+    err = "boolean operand is not a shape of this context";
+    return -1;
+  }
+  const ShapeRec L = g.shapes[left], R = g.shapes[right];
+  for (const ShapeRec *s : {&L, &R})
+    if (s->type == kBoolean || s->type == kTessellated) {
+      err = "boolean operands must be primitive solids (no booleans, no tessellated solids)";
+      return -1;
+    }
+  BooleanData b{};
+  b.op = op;
+  b.ltype = L.type;
+  b.rtype = R.type;
+  auto set_tr = [](Transform &T, const double *tr) {
+    for (int k = 0; k < 3; ++k) T.t[k] = tr ? tr[k] : 0.0;
+    for (int k = 0; k < 9; ++k) T.r[k] = tr ? tr[3 + k] : ((k % 4 == 0) ? 1.0 : 0.0);
+  };
+  set_tr(b.ltr, ltr);
+  set_tr(b.rtr, rtr);
+  const int hdr = int((sizeof(BooleanData) + sizeof(double) - 1) / sizeof(double));
+  b.loff = hdr;
+  b.roff = hdr + L.nd;
+  std::vector<double> block(size_t(hdr + L.nd + R.nd), 0.0);
+  std::memcpy(block.data(), &b, sizeof(b));
+  std::memcpy(block.data() + b.loff, g.pool.data() + L.off, size_t(L.nd) * sizeof(double));
+  std::memcpy(block.data() + b.roff, g.pool.data() + R.off, size_t(R.nd) * sizeof(double));
+  double mnL[3], mxL[3], mnR[3], mxR[3], lo[3], hi[3];
+  transformed_extent(L, b.ltr, mnL, mxL);
+  transformed_extent(R, b.rtr, mnR, mxR);
+  for (int k = 0; k < 3; ++k) {
+    if (op == kBoolSubtraction) {
+      lo[k] = mnL[k], hi[k] = mxL[k];
+    } else {
+      lo[k] = mnL[k] > mnR[k] ? mnL[k] : mnR[k];
+      hi[k] = mxL[k] < mxR[k] ? mxL[k] : mxR[k];
+    }
+  }
+  return g.add_shape(kBoolean, block.data(), block.size(), 0, 0, lo, hi);
 }
 
 int build_tube(HostGeometry &g, double rmin, double rmax, double z, double sphi, double dphi, std::string &err)

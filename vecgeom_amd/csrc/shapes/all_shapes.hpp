@@ -11,6 +11,7 @@
 #include "tessellated.hpp"
 #include "orb.hpp"
 #include "cut_tube.hpp"
+#include "boolean.hpp"
 
 namespace vgm {
 
@@ -49,6 +50,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kCutTube:
     f.template operator()<CutTubeShape>();
+    return true;
+  case kBoolean:
+    f.template operator()<BooleanShape>();
     return true;
   default:
     return false;
@@ -95,6 +99,9 @@ __host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
     return true;
   case kCutTube:
     if constexpr ((Mask & (1u << kCutTube)) != 0) f.template operator()<CutTubeShape>();
+    return true;
+  case kBoolean:
+    if constexpr ((Mask & (1u << kBoolean)) != 0) f.template operator()<BooleanShape>();
     return true;
   default:
     return false;

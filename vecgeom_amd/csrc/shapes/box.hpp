@@ -20,6 +20,8 @@ struct BoxShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   // max(|p| - h) (BoxImplementation.h:76-90)
   template <typename R>

@@ -24,6 +24,8 @@ struct CutTubeShape {
   // mixed-precision DistanceToIn may be evaluated in float (no: the cut planes test
   // their hit point against kTolerance, below float resolution)
   static constexpr bool kFloatDistanceToIn = false;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   // Plane::DistPlane (Plane.h:85-90): point . normal + distance
   template <typename R>

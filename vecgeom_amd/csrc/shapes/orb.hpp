@@ -20,6 +20,8 @@ struct OrbShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   template <typename R>
   VGM_DEV static R mag2(V3<R> p)

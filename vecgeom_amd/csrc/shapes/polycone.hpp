@@ -27,6 +27,8 @@ struct PolyconeShape {
   static constexpr bool kFloatDistanceToOut = false;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   // GetSectionIndex: linear search over the z-planes, compared in the
   // arithmetic type of the query

@@ -31,6 +31,8 @@ struct PolyhedronShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
   using Quad = PolyhedronQuad;
 
   template <typename R>

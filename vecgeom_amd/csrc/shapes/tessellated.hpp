@@ -79,6 +79,8 @@ struct TessellatedShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   template <typename R>
   VGM_DEV static bool facet_contains(const TessFacet &f, R px, R py, R pz)

@@ -21,6 +21,8 @@ struct TrapezoidShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   template <typename R>
   VGM_DEV static R pdist(const Data &t, int i, V3<R> p)

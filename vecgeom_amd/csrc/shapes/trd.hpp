@@ -20,6 +20,8 @@ struct TrdShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

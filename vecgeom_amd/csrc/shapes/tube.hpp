@@ -23,6 +23,8 @@ struct TubeShape {
   static constexpr bool kFloatDistanceToOut = true;
   // mixed-precision DistanceToIn may be evaluated in float
   static constexpr bool kFloatDistanceToIn = true;
+  // mixed precision may evaluate the queries in float at all
+  static constexpr bool kFloatQueries = true;
 
   // PointInCyclicalSector (TubeImplementation.h:62-101)
   template <typename R>

@@ -80,6 +80,21 @@ __global__ __launch_bounds__(kSoaBlock) void soa_mixed_kernel(const typename S::
   const typename S::Data &s = *sd;
   V3<double> p{a.px[i], a.py[i], a.pz[i]};
   if (use_tr) p = to_local_point(tr, p);
+  if constexpr (!S::kFloatQueries) {  // every query in FP64
+    V3<double> d{0.0, 0.0, 0.0};
+    if constexpr (M == kDistanceToIn || M == kDistanceToOut) {
+      d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
+      if (use_tr) d = to_local_dir(tr, d);
+    }
+    const double step = (M == kDistanceToIn || M == kDistanceToOut) && a.step ? a.step[i] : kInfLength;
+    if constexpr (M == kContains) static_cast<uint8_t *>(a.out)[i] = S::template contains<double>(s, p) ? 1 : 0;
+    else if constexpr (M == kInside) static_cast<int32_t *>(a.out)[i] = S::template inside<double>(s, p);
+    else if constexpr (M == kSafetyToIn) static_cast<double *>(a.out)[i] = S::template safety_to_in<double>(s, p);
+    else if constexpr (M == kSafetyToOut) static_cast<double *>(a.out)[i] = S::template safety_to_out<double>(s, p);
+    else if constexpr (M == kDistanceToIn) static_cast<double *>(a.out)[i] = S::template distance_to_in<double>(s, p, d, step);
+    else static_cast<double *>(a.out)[i] = S::template distance_to_out<double>(s, p, d, step);
+    return;
+  }
   const V3<float> pf{float(p.x), float(p.y), float(p.z)};
   const float g = 4.0e-6f * (fabsf(pf.x) + fabsf(pf.y) + fabsf(pf.z) + scale);
   if constexpr (M == kContains || M == kInside) {

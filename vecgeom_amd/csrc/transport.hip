@@ -137,6 +137,10 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
   }
   bool move_free = lf + g <= saf;
   const bool by_safety = move_free;
+  if constexpr (!S::kFloatQueries) {
+    fp64_steps++;
+    return step_fp64<S>(s, t, l);
+  }
   if constexpr (!S::kFloatDistanceToIn) {
     // a float miss from outside is not trusted for such a family
     if (!move_free && !t.in) {

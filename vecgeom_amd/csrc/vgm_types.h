@@ -19,7 +19,8 @@ enum ShapeType : int32_t {
   kTessellated = 7,
   kOrb = 8,
   kCutTube = 9,
-  kNumShapeTypes = 10
+  kBoolean = 10,
+  kNumShapeTypes = 11
 };
 
 enum Method : int32_t {
