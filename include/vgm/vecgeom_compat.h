@@ -157,6 +157,12 @@ class GeoManager {
   // unplaced shapes (constructor arguments of the reference's classes)
   int Box(double dx, double dy, double dz) { return make(vgm_box_create(ctx_, dx, dy, dz, &s_)); }
   int Orb(double r) { return make(vgm_orb_create(ctx_, r, &s_)); }
+  // UnplacedBooleanVolume<op>: op 1 intersection, 2 subtraction; placements
+  // of 12 doubles (translation, rotation) or null for the identity
+  int Boolean(int op, int left, const double *left_tr, int right, const double *right_tr)
+  {
+    return make(vgm_boolean_create(ctx_, op, left, left_tr, right, right_tr, &s_));
+  }
   int CutTube(double rmin, double rmax, double z, double sphi, double dphi, double bx, double by, double bz, double tx,
               double ty, double tz)
   {

@@ -9,7 +9,8 @@ SRC = "/root/reference/persistency/gdml/gdmls"
 FILES = ["aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "oneGeneralTrapezoid", "onePolycone",
          "onePolyhedron", "oneTesselatedSolid", "oneTrapezoid", "oneTube", "twoBoxes",
          # refused: a solid outside the implemented families, a scaled physvol
-         "oneOrb", "scaledPhysvol", "aConstant", "oneCutTube"]
+         "oneOrb", "scaledPhysvol", "aConstant", "oneCutTube", "solidSubtraction",
+         "solidIntersection", "solidUnion"]
 
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))

@@ -29,7 +29,7 @@ class Recorder:
         return len(self.calls) - 1
 
     def __getattr__(self, name):
-        if name in ("box", "orb", "tube", "cut_tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
+        if name in ("box", "orb", "tube", "cut_tube", "boolean", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
                     "logical_volume", "place", "add_daughter"):
             return lambda *a: self._rec(name, *a)
         raise AttributeError(name)
@@ -123,7 +123,7 @@ def test_numbers_read_like_stod():
     assert r.of("box")[0][1] == 0.0
 
 
-@pytest.mark.parametrize("name, what", [("scaledPhysvol", "scale")])
+@pytest.mark.parametrize("name, what", [("scaledPhysvol", "scale"), ("solidUnion", "union")])
 def test_unsupported_is_an_error(name, what):
     with pytest.raises(gdml.GdmlError, match=what):
         read(name)
@@ -137,6 +137,16 @@ def test_cut_tube_reads_normals_as_lengths():
     (_, rmin, rmax, dz, sphi, dphi, low, high), = r.of("cut_tube")
     assert (rmin, rmax, dz, sphi, dphi) == (1.0, 5.0, 10.0, 1.0, 4.0)
     assert low == (-15.0, -15.0, -15.0) and high == (10.0, 10.0, 10.0)
+
+
+def test_boolean_operands_and_placement():
+    r, world, solids, _ = read("solidSubtraction")
+    (_, op, first, second, ltr, rtr), = r.of("boolean")
+    assert (op, first, second, ltr) == ("subtraction", solids["minuend"], solids["subtrahend"], None)
+    assert rtr[:3] == [40.0, 35.0, 30.0]  # positionref in cm
+    R = np.array(rtr[3:]).reshape(3, 3)
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-15) and not np.allclose(R, np.eye(3))
+    assert r.of("box")[0] == ("box", 50.0, 50.0, 50.0)
 
 
 def test_orb_radius_and_constant():

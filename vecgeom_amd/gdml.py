@@ -43,14 +43,15 @@ from typing import Dict, List, Tuple
 
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
-SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb")
+SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb",
+                    "subtraction", "intersection")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
 _REFERENCE_SOLIDS = SUPPORTED_SOLIDS + (
     "sphere", "torus", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
     "genericPolycone", "genericPolyhedra", "hype", "paraboloid", "tet", "twistedbox", "twistedtrap",
-    "twistedtrd", "twistedtubs", "scaledSolid", "union", "subtraction", "intersection", "multiUnion")
+    "twistedtrd", "twistedtubs", "scaledSolid", "union", "multiUnion")
 
 _STOD = re.compile(r"\s*[+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan)", re.IGNORECASE)
 _KTOL = 1e-9  # kTolerance, the facet validity threshold
@@ -192,6 +193,31 @@ class _Reader:
             z, rmin, rmax = self.lengths(el, "z", "rmin", "rmax")
             sphi, dphi = self.angles(el, "startphi", "deltaphi")
             s = g.tube(rmin, rmax, z / 2.0, sphi, dphi)
+        elif tag in ("subtraction", "intersection"):
+            # Middleware.cpp:415-499: the first solid placed as it is, the
+            # second by the position and rotation (inline or by reference)
+            first = second = None
+            pos, rot = (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
+            for c in el:
+                if c.tag in ("first", "second"):
+                    ref = c.get("ref", "")
+                    if ref not in self.solids:
+                        raise GdmlError(f"solid '{name}': unknown {c.tag} solid '{ref}'")
+                    if c.tag == "first":
+                        first = self.solids[ref]
+                    else:
+                        second = self.solids[ref]
+                elif c.tag == "positionref":
+                    pos = self.positions.get(c.get("ref", ""), (0.0, 0.0, 0.0))
+                elif c.tag == "rotationref":
+                    rot = self.rotations.get(c.get("ref", ""), (0.0, 0.0, 0.0))
+                elif c.tag == "position":
+                    pos = tuple(self.lengths(c, "x", "y", "z"))
+                elif c.tag == "rotation":
+                    rot = tuple(self.angles(c, "x", "y", "z"))
+            if first is None or second is None:
+                raise GdmlError(f"solid '{name}': <{tag}> needs <first> and <second>")
+            s = g.boolean(tag, first, second, None, [pos[0], pos[1], pos[2]] + _rotation(*rot))
         elif tag == "cutTube":
             # the normals read as lengths; z halved (Middleware.cpp:738-768)
             z, rmin, rmax, lx, ly, lz, hx, hy, hz = self.lengths(el, "z", "rmin", "rmax", "lowX", "lowY", "lowZ",
