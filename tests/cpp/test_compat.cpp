@@ -81,6 +81,23 @@ int main()
   EXPECT(safety[0] == 20.0 && safety[1] == 0.0);
   EXPECT(step[0] == 20.0 && state[0] == 2 && step[1] == 10.0 && state[1] == 1);
 
+  // ApproachNextDaughter: from the origin along +x the daughter's box begins
+  // at x = 20; the daughter has no daughters of its own, so its track keeps its step
+  nav.LocateGlobalPoints(G, state);
+  last[0] = last[1] = 0;
+  double appr[2] = {1e300, 1e300};
+  nav.ApproachNextDaughters(G, U, state, last, appr);
+  EXPECT(appr[0] == 20.0 && appr[1] == 1e300);
+
+  // a box minus a square bore along z
+  const int outer = geo.Box(10., 10., 10.), bore = geo.Box(5., 5., 20.);
+  PlacedVolume holed = geo.Place(geo.NewLogicalVolume(geo.Boolean(2, outer, nullptr, bore, nullptr)));
+  std::vector<double> hx{0, 7}, hy{0, 0}, hz{0, 0};
+  SOA3D<> H(hx.data(), hy.data(), hz.data(), 2);
+  bool hin[2] = {true, false};
+  holed.Contains(H, hin);
+  EXPECT(!hin[0] && hin[1]);
+
   // errors surface as exceptions with the library's message
   bool threw = false;
   try {
