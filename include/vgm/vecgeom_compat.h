@@ -14,6 +14,8 @@
 //                                                            SafetyToOut/PlacedDistanceToOut
 //   VNavigator::ComputeStepsAndPropagatedStates
 //     (VecGeom/navigation/VNavigator.h:128-132)             Navigator::ComputeStepsAndPropagatedStates
+//   VNavigator::ComputeStep (VNavigator.h:494-540)         Navigator::ComputeSteps
+//   (ComputeStepAndNextVolume, the same as Propagated)    Navigator::ComputeStepsAndNextVolumes
 //   VNavigator::ComputeStepsAndSafetiesAndPropagatedStates
 //     (VecGeom/navigation/VNavigator.h:136-148)             Navigator::ComputeStepsAndSafetiesAndPropagatedStates
 //   NavIndexTable::ValidateState (NavIndexTable.cpp:86-140),
@@ -255,6 +257,22 @@ class Navigator {
   {
     check(vgm_nav_step(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), step_max, p.size(), state, last, boundary,
                        step, flags(p)));
+  }
+  // The same call under its transport-code name: the step and, in state,
+  // the next volume.
+  void ComputeStepsAndNextVolumes(SOA3D<> const &p, SOA3D<> const &d, Precision const *step_max, uint32_t *state,
+                                  uint32_t *last, uint8_t *boundary, Precision *step) const
+  {
+    check(vgm_nav_compute_step_and_next_volume(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), step_max, p.size(),
+                                               state, last, boundary, step, flags(p)));
+  }
+  // VNavigator::ComputeStep (VNavigator.h:494-540) per track: the step and
+  // PrepareOutState's state, with no relocation after the boundary.
+  void ComputeSteps(SOA3D<> const &p, SOA3D<> const &d, Precision const *step_max, uint32_t *state, uint32_t *last,
+                    uint8_t *boundary, Precision *step) const
+  {
+    check(vgm_nav_compute_step(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), step_max, p.size(), state, last,
+                               boundary, step, flags(p)));
   }
   // ...AndSafeties...: safety[i] of the start point where calcsafety[i]
   // (calcsafety may be null: every track), else 0, then the step.

@@ -206,6 +206,24 @@ int vgm_nav_locate(vgm_ctx *ctx, const double *x, const double *y, const double 
 int vgm_nav_step(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx, const double *dy,
                  const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
                  uint8_t *boundary, double *step, unsigned flags);
+/* The same call under the name the north-star transport codes use for it
+ * ("ComputeStepAndNextVolume": the step and, in state, the next volume);
+ * identical to vgm_nav_step. */
+int vgm_nav_compute_step_and_next_volume(vgm_ctx *ctx, const double *x, const double *y, const double *z,
+                                         const double *dx, const double *dy, const double *dz,
+                                         const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                                         uint8_t *boundary, double *step, unsigned flags);
+/* VNavigator::ComputeStep (VecGeom/navigation/VNavigator.h:494-540): the
+ * step of vgm_nav_step without the relocation after the boundary.  The out
+ * state is PrepareOutState's (:192-235): on a daughter hit the daughter's
+ * node with boundary = 1 and last = the current state; on leaving the
+ * mother the current state with boundary = 1 (the caller relocates); a
+ * physics-limited step keeps the state with boundary = 0; a point whose
+ * mother DistanceToOut is kInfLength goes to the mother's mother with step
+ * kTolerance. */
+int vgm_nav_compute_step(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, const double *step_limit, size_t n, uint32_t *state,
+                         uint32_t *last, uint8_t *boundary, double *step, unsigned flags);
 /* ComputeStepsAndSafetiesAndPropagatedStates (VecGeom/navigation/
  * VNavigator.h:136-148; per track ComputeStepAndSafetyAndPropagatedState,
  * :436-492 with the safety first): as vgm_nav_step, and before the step

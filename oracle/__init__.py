@@ -62,6 +62,7 @@ def lib():
             "orc_nav_node": [_P, ctypes.c_uint, _P, _P, _P],
             "orc_nav_locate": [_P, _P, _P, _P, ctypes.c_size_t, _P, _I],
             "orc_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _P, _I],
+            "orc_nav_step2": [_P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _P, _I, _I],
             "orc_nav_transport": [_P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P, _P, _P, _I, _P, _I],
             "orc_nav_transport_limited": [_P] * 8 + [ctypes.c_size_t, _P, _P, _P, _I, _I, _P, _P, _I],
             "orc_nav_approach": [_P] * 7 + [ctypes.c_size_t, _P, _P, _P, _I],
@@ -179,6 +180,16 @@ class Geometry:
         out = np.zeros(len(x))
         self._L.orc_nav_step(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), None if sl is None else _p(sl),
                              len(x), _p(state), _p(last), _p(boundary), _p(out), nthreads)
+        return out
+
+    def nav_compute_step(self, pts, dirs, state, last, boundary, step_limit=None, nthreads=0):
+        """One ComputeStep per track (no relocation); updates the state arrays, returns steps."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        dx, dy, dz = (np.ascontiguousarray(a, dtype=np.float64) for a in dirs)
+        sl = None if step_limit is None else np.ascontiguousarray(step_limit, dtype=np.float64)
+        out = np.zeros(len(x))
+        self._L.orc_nav_step2(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), None if sl is None else _p(sl),
+                              len(x), _p(state), _p(last), _p(boundary), _p(out), 0, nthreads)
         return out
 
     def nav_transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, nthreads=0):

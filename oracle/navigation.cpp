@@ -551,7 +551,10 @@ struct Navigator {
   }
 
   // ---- one step ----------------------------------------------------------------
-  double Step(const Vec3 &gp, const Vec3 &gd, double step_limit, const State &in, State &out) const
+  // relocate = false: VNavigator::ComputeStep (VNavigator.h:494-540), the
+  // PrepareOutState result without the relocation
+  double Step(const Vec3 &gp, const Vec3 &gd, double step_limit, const State &in, State &out,
+              bool relocate = true) const
   {
     const Transform &T = nodes[in.ind].global;
     const Vec3 lp = T.Transform_(gp), ld = T.TransformDirection(gd);
@@ -578,6 +581,7 @@ struct Navigator {
     out.last = out.ind;
     if (hit >= 0) out.ind = unsigned(nodes[out.ind].children[hit]);
     if (step < 0.) step = 0.;
+    if (!relocate) return step;
     // Relocate(MovePointAfterBoundary(localpoint, localdir, step))
     const double e = step + 1E-6;
     Vec3 q(lp.x + e * ld.x, lp.y + e * ld.y, lp.z + e * ld.z);
@@ -651,9 +655,9 @@ int orc_nav_locate(orc_geom *g, const double *x, const double *y, const double *
   return 0;
 }
 
-int orc_nav_step(orc_geom *g, const double *x, const double *y, const double *z, const double *dx, const double *dy,
-                 const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
-                 uint8_t *boundary, double *step, int nthreads)
+int orc_nav_step2(orc_geom *g, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+                  const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                  uint8_t *boundary, double *step, int relocate, int nthreads)
 {
   Navigator *N = nav_of(g);
   if (!N) return -1;
@@ -666,12 +670,19 @@ int orc_nav_step(orc_geom *g, const double *x, const double *y, const double *z,
     }
     State in{state[i], last[i], boundary[i] != 0}, out;
     step[i] = N->Step(Vec3(x[i], y[i], z[i]), Vec3(dx[i], dy[i], dz[i]), step_limit ? step_limit[i] : kInfLength, in,
-                      out);
+                      out, relocate != 0);
     state[i] = out.ind;
     last[i] = out.last;
     boundary[i] = out.boundary;
   }
   return 0;
+}
+
+int orc_nav_step(orc_geom *g, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+                 const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                 uint8_t *boundary, double *step, int nthreads)
+{
+  return orc_nav_step2(g, x, y, z, dx, dy, dz, step_limit, n, state, last, boundary, step, 1, nthreads);
 }
 
 // Straight-line transport: nsteps of ComputeStepAndPropagatedState with an

@@ -62,6 +62,10 @@ int orc_nav_locate(orc_geom *g, const double *x, const double *y, const double *
 int orc_nav_step(orc_geom *g, const double *x, const double *y, const double *z, const double *dx, const double *dy,
                  const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
                  uint8_t *boundary, double *step, int nthreads);
+/* relocate = 0: VNavigator::ComputeStep, no relocation after the boundary */
+int orc_nav_step2(orc_geom *g, const double *x, const double *y, const double *z, const double *dx,
+                  const double *dy, const double *dz, const double *step_limit, size_t n, uint32_t *state,
+                  uint32_t *last, uint8_t *boundary, double *step, int relocate, int nthreads);
 int orc_nav_transport_limited(orc_geom *g, double *x, double *y, double *z, const double *dx, const double *dy,
                               const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
                               uint8_t *boundary, int nsteps, int reuse_safety, int32_t *steps, int32_t *geo_steps,

@@ -81,6 +81,28 @@ int main()
   EXPECT(safety[0] == 20.0 && safety[1] == 0.0);
   EXPECT(step[0] == 20.0 && state[0] == 2 && step[1] == 10.0 && state[1] == 1);
 
+  // ComputeStep (no relocation): the first track's state is the daughter it
+  // hits; the second leaves the daughter and keeps its state, on the boundary,
+  // with the daughter as the last exited volume (PrepareOutState)
+  nav.LocateGlobalPoints(G, state);
+  last[0] = last[1] = 0;
+  boundary[0] = boundary[1] = 0;
+  nav.ComputeSteps(G, U, nullptr, state, last, boundary, step);
+  EXPECT(step[0] == 20.0 && state[0] == 2 && boundary[0] == 1 && last[0] == 1);
+  EXPECT(step[1] == 10.0 && state[1] == 2 && boundary[1] == 1 && last[1] == 2);
+  // under a physics limit of 5 nothing is reached: states kept, off boundary
+  double limit[2] = {5.0, 5.0};
+  nav.LocateGlobalPoints(G, state);
+  last[0] = last[1] = 0;
+  nav.ComputeSteps(G, U, limit, state, last, boundary, step);
+  EXPECT(step[0] == 5.0 && state[0] == 1 && boundary[0] == 0 && step[1] == 5.0 && state[1] == 2);
+  // ComputeStepAndNextVolume is ComputeStepAndPropagatedState
+  nav.LocateGlobalPoints(G, state);
+  last[0] = last[1] = 0;
+  boundary[0] = boundary[1] = 0;
+  nav.ComputeStepsAndNextVolumes(G, U, nullptr, state, last, boundary, step);
+  EXPECT(step[0] == 20.0 && state[0] == 2 && step[1] == 10.0 && state[1] == 1);
+
   // ApproachNextDaughter: from the origin along +x the daughter's box begins
   // at x = 20; the daughter has no daughters of its own, so its track keeps its step
   nav.LocateGlobalPoints(G, state);

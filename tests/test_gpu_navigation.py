@@ -51,6 +51,48 @@ def test_step_and_safety(setup):
     assert np.array_equal(s_g, s_o) and np.array_equal(l_g, l_o) and np.array_equal(b_g, b_o)
 
 
+@pytest.mark.parametrize("limited", [False, True], ids=["unlimited", "limit5mm"])
+def test_compute_step_without_relocation(setup, limited):
+    """ComputeStep (VNavigator.h:494-540) bit-exact against the oracle's,
+    and ComputeStepAndNextVolume identical to ComputeStepAndPropagatedState.
+    Without relocation the state after a mother exit stays the current one,
+    so the states differ from the relocated step exactly on the boundary
+    steps that leave a volume."""
+    print(f"[nav] compute step limited={limited}", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    lim = np.full(N, 5.0) if limited else None
+    s_g, l_g, b_g = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step = np.zeros(N)
+    nav.compute_step(*pts, *dirs, s_g, l_g, b_g, step, step_limit=lim)
+    s_o, l_o, b_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    ref = geo.nav_compute_step(pts, dirs, s_o, l_o, b_o, step_limit=lim)
+    assert np.array_equal(step, ref)
+    assert np.array_equal(s_g, s_o) and np.array_equal(l_g, l_o) and np.array_equal(b_g, b_o)
+    # the relocating step: same step lengths, and the same state wherever no
+    # volume is left (daughter entries keep the pushed daughter on both)
+    s_r, l_r, b_r = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step_r = np.zeros(N)
+    nav.compute_step_and_next_volume(*pts, *dirs, s_r, l_r, b_r, step_r, step_limit=lim)
+    s_p, l_p, b_p = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step_p = np.zeros(N)
+    nav.step(*pts, *dirs, s_p, l_p, b_p, step_p, step_limit=lim)
+    assert np.array_equal(step_r, step_p) and np.array_equal(s_r, s_p) and np.array_equal(l_r, l_p)
+    assert np.array_equal(step_r, step)
+    # physics-limited steps relocate nothing: both keep the state
+    off = b_g == 0
+    assert np.array_equal(s_r[off], s_g[off]) and np.array_equal(s_g[off], st[off])
+    # boundary steps: ComputeStep either keeps the state (the mother is left)
+    # or pushes a daughter of it, with the current state as the last exited
+    on = ~off & (st != 0)
+    assert np.array_equal(l_g[on], st[on])
+    entered = on & (s_g != st)
+    mothers = {int(s): nav.node(int(s))[2] for s in np.unique(s_g[entered])}
+    assert all(mothers[int(s)] == int(m) for s, m in zip(s_g[entered], st[entered]))
+    if limited:
+        assert off.sum() > 0 and np.all(step[off] == 5.0)
+
+
 @pytest.mark.parametrize("mask", ["all", "half"])
 def test_step_with_safeties(setup, mask):
     """ComputeStepsAndSafetiesAndPropagatedStates: the safety of the start

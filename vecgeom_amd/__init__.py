@@ -114,6 +114,8 @@ def _declare(L):
         "vgm_nav_node": [_P, ctypes.c_uint32, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)],
         "vgm_nav_locate": [_P, _P, _P, _P, _SZ, _P, _U],
         "vgm_nav_step": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
+        "vgm_nav_compute_step": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
+        "vgm_nav_compute_step_and_next_volume": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
         "vgm_nav_step_safety": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P, _U],
         "vgm_nav_path_to_state": [_P, _P, _I, ctypes.POINTER(ctypes.c_uint32)],
         "vgm_nav_state_to_path": [_P, ctypes.c_uint32, _P, _I, ctypes.POINTER(_I)],
@@ -453,6 +455,18 @@ class Navigation:
         check(self._L.vgm_nav_step(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                    _ptr(step_limit), len(x), _ptr(state), _ptr(last), _ptr(boundary), _ptr(step),
                                    self._fl(x)))
+
+    def compute_step(self, x, y, z, dx, dy, dz, state, last, boundary, step, step_limit=None):
+        """VNavigator::ComputeStep: the step and PrepareOutState's state, no relocation."""
+        check(self._L.vgm_nav_compute_step(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy),
+                                           _ptr(dz), _ptr(step_limit), len(x), _ptr(state), _ptr(last),
+                                           _ptr(boundary), _ptr(step), self._fl(x)))
+
+    def compute_step_and_next_volume(self, x, y, z, dx, dy, dz, state, last, boundary, step, step_limit=None):
+        """vgm_nav_compute_step_and_next_volume (the same call as step())."""
+        check(self._L.vgm_nav_compute_step_and_next_volume(
+            self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz), _ptr(step_limit), len(x),
+            _ptr(state), _ptr(last), _ptr(boundary), _ptr(step), self._fl(x)))
 
     def step_safety(self, x, y, z, dx, dy, dz, state, last, boundary, step, safety, calcsafety=None, step_limit=None):
         """ComputeStepsAndSafetiesAndPropagatedStates: safety[i] before the step where calcsafety[i] (default all)."""

@@ -70,6 +70,27 @@ void **slot(T *&p)
 
 }  // namespace
 
+namespace {
+int nav_step(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx, const double *dy,
+             const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+             uint8_t *boundary, double *step, unsigned flags, bool relocate)
+{
+  if (n && (!state || !last || !boundary || !step)) return fail(VGM_EINVAL, "null state or output array");
+  NavArgs a{};
+  a.n = n;
+  a.no_relocate = relocate ? 0 : 1;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavStep, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
+                  {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit), true},
+                  {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
+                  {boundary, n, true, true, slot(a.boundary)}, {step, b8, false, true, slot(a.step)}},
+                 flags);
+}
+
+}  // namespace
+
 extern "C" {
 
 int vgm_nav_close(vgm_ctx *c, int world_pv, int *nnodes)
@@ -143,17 +164,22 @@ int vgm_nav_step(vgm_ctx *c, const double *x, const double *y, const double *z, 
                  const double *dz, const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
                  uint8_t *boundary, double *step, unsigned flags)
 {
-  if (n && (!state || !last || !boundary || !step)) return fail(VGM_EINVAL, "null state or output array");
-  NavArgs a{};
-  a.n = n;
-  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
-  return run_nav(c, kNavStep, a,
-                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
-                  {dx, b8, true, false, slot(a.dx)}, {dy, b8, true, false, slot(a.dy)},
-                  {dz, b8, true, false, slot(a.dz)}, {step_limit, b8, true, false, slot(a.step_limit), true},
-                  {state, b4, true, true, slot(a.state)}, {last, b4, true, true, slot(a.last)},
-                  {boundary, n, true, true, slot(a.boundary)}, {step, b8, false, true, slot(a.step)}},
-                 flags);
+  return nav_step(c, x, y, z, dx, dy, dz, step_limit, n, state, last, boundary, step, flags, true);
+}
+
+int vgm_nav_compute_step_and_next_volume(vgm_ctx *c, const double *x, const double *y, const double *z,
+                                         const double *dx, const double *dy, const double *dz,
+                                         const double *step_limit, size_t n, uint32_t *state, uint32_t *last,
+                                         uint8_t *boundary, double *step, unsigned flags)
+{
+  return nav_step(c, x, y, z, dx, dy, dz, step_limit, n, state, last, boundary, step, flags, true);
+}
+
+int vgm_nav_compute_step(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, const double *step_limit, size_t n, uint32_t *state,
+                         uint32_t *last, uint8_t *boundary, double *step, unsigned flags)
+{
+  return nav_step(c, x, y, z, dx, dy, dz, step_limit, n, state, last, boundary, step, flags, false);
 }
 
 int vgm_nav_step_safety(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx,

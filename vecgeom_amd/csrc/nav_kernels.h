@@ -30,6 +30,7 @@ struct NavArgs {
   int32_t *steps;  // transport: steps taken (accumulated)
   int32_t *geo_steps;  // limited transport: steps that needed the boundary distance (accumulated)
   int32_t reuse_safety;  // limited transport: carry the safety left after a safety-only move
+  int32_t no_relocate;   // step: ComputeStep, the out state without relocation
   size_t n;
   int32_t nsteps;
   uint32_t shape_mask;  // families used by the navigation tree (bit per ShapeType)

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
   if (a.safety) a.safety[i] = (!a.calcsafety || a.calcsafety[i]) ? nav.safety(p, a.state[i]) : 0.0;
   const NavState in{a.state[i], a.last[i], a.boundary[i] != 0};
   NavState out;
-  a.step[i] = nav.step(p, V3<double>{a.dx[i], a.dy[i], a.dz[i]}, a.step_limit ? a.step_limit[i] : kInfLength, in, out);
+  a.step[i] = nav.step(p, V3<double>{a.dx[i], a.dy[i], a.dz[i]}, a.step_limit ? a.step_limit[i] : kInfLength, in, out,
+                       !a.no_relocate);
   a.state[i] = out.ind;
   a.last[i] = out.last;
   a.boundary[i] = out.boundary ? 1 : 0;

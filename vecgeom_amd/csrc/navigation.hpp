@@ -339,7 +339,11 @@ struct Navigator {
   }
 
   // ---- one step -------------------------------------------------------------------
-  __host__ __device__ double step(V3<double> gp, V3<double> gd, double step_limit, const NavState &in, NavState &out) const
+  // relocate = false is ComputeStep (VNavigator.h:494-540): the out state is
+  // PrepareOutState's (the hit daughter pushed, or the current volume with
+  // its boundary flag), with no relocation after the boundary.
+  __host__ __device__ double step(V3<double> gp, V3<double> gd, double step_limit, const NavState &in, NavState &out,
+                                  bool relocate = true) const
   {
     const NavNode &N = g.nodes[in.ind];
     const V3<double> lp = to_local_point(N.global, gp), ld = to_local_dir(N.global, gd);
@@ -364,6 +368,7 @@ struct Navigator {
     out.last = out.ind;
     if (hit >= 0) out.ind = unsigned(g.children[N.child0 + hit]);
     if (st < 0.) st = 0.;
+    if (!relocate) return st;
     const double e = st + kPushTolerance;
     V3<double> q{lp.x + e * ld.x, lp.y + e * ld.y, lp.z + e * ld.z};
     if (out.ind == in.ind) {
