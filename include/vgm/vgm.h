@@ -169,6 +169,34 @@ int vgm_safety_to_in(vgm_ctx *ctx, int pv, const double *x, const double *y, con
 int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
                       double *out, unsigned flags);
 
+/* ---- allocation-free SOA3D query --------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  It is the host-pointer form of the seven queries above
+ * (method = one of VGM_Q_*; the same frames, sentinels and results as the
+ * matching vgm_* call with VGM_HOST_PTRS, VGM_MIXED allowed), but it stages
+ * the inputs and outputs through a fixed device array of
+ * VGM_NOALLOC_SCRATCH_BYTES that the library holds per device until the
+ * process ends, and calls no hipMalloc/hipFree.  Each coordinate, direction
+ * and step_max array and the output take n elements rounded up to 256 bytes;
+ * a call that needs more than the array fails with VGM_EINVAL before any
+ * work.  The geometry must already be on the device: after creating or
+ * changing volumes, run one of the allocating calls first (otherwise
+ * VGM_ESTATE).  Calls on one device share the array and overlapping calls
+ * race: serialise them (one host thread, or a lock, per device).
+ * VGM_DEVICE_PTRS is rejected: device arrays need no staging. */
+#define VGM_Q_CONTAINS 0
+#define VGM_Q_INSIDE 1
+#define VGM_Q_DISTANCE_TO_IN 2
+#define VGM_Q_DISTANCE_TO_OUT 3
+#define VGM_Q_PLACED_DISTANCE_TO_OUT 4
+#define VGM_Q_SAFETY_TO_IN 5
+#define VGM_Q_SAFETY_TO_OUT 6
+#define VGM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int vgm_query_noalloc(vgm_ctx *ctx, int pv, int method, const double *px, const double *py, const double *pz,
+                      const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                      void *out, unsigned flags);
+
 /* ---- navigation -------------------------------------------------------------
  * The drop-in for VNavigator::ComputeStepAndPropagatedState
  * (VecGeom/navigation/VNavigator.h:51-55, BVHNavigator over every logical

@@ -34,6 +34,14 @@ FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
 SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean")
 
+# vgm_query_noalloc method codes (VGM_Q_*) and output types
+NOALLOC_METHODS = {
+    "Contains": (0, np.uint8), "Inside": (1, np.int32), "DistanceToIn": (2, np.float64),
+    "DistanceToOut": (3, np.float64), "PlacedDistanceToOut": (4, np.float64), "SafetyToIn": (5, np.float64),
+    "SafetyToOut": (6, np.float64),
+}
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
 # Inside_t (VecGeom/base/Global.h:71-88)
 kInside, kSurface, kOutside = 1, 2, 3
 kInfLength = np.finfo(np.float64).max
@@ -102,6 +110,7 @@ def _declare(L):
         "vgm_distance_to_in": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
         "vgm_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
         "vgm_placed_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
+        "vgm_query_noalloc": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -427,6 +436,20 @@ class Context:
 
     def placed_distance_to_out(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
         return self._q2(self._L.vgm_placed_distance_to_out, pv, pts, dirs, step_max, out, mixed, asynchronous)
+
+    def query_noalloc(self, method: str, pv, pts: SOA3D, dirs: Optional[SOA3D] = None, step_max=None, out=None,
+                      mixed=False):
+        """vgm_query_noalloc on host arrays: the query named by `method` (a
+        key of NOALLOC_METHODS) staged through the library's fixed device
+        array, with no device allocation."""
+        code, dtype = NOALLOC_METHODS[method]
+        if out is None:
+            out = np.empty(pts.n, dtype=dtype)
+        dx, dy, dz = (None, None, None) if dirs is None else (dirs.x, dirs.y, dirs.z)
+        check(self._L.vgm_query_noalloc(
+            self.handle, pv, code, _ptr(pts.x), _ptr(pts.y), _ptr(pts.z), _ptr(dx), _ptr(dy), _ptr(dz),
+            _ptr(step_max), pts.n, _ptr(out), MIXED if mixed else HOST_PTRS))
+        return out
 
 
 class Navigation:

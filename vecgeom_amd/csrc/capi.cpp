@@ -17,7 +17,34 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 using namespace vgm;
 
+// Staging array of vgm_query_noalloc: each device holds its own copy from the
+// library's load until the process ends.  Host code never uses this name as
+// an address (on the host it is only a stand-in); it uses the per-device
+// address that hipGetSymbolAddress returns, kept in g_noalloc_ptr.
+static __device__ __align__(256) char scratch[VGM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
 namespace vgm {
+
+// Looks up the device address of `scratch` on `device` (the current device)
+// once; called by vgm_ctx_create.
+int init_noalloc_scratch(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(VGM_EINVAL, "device index above the library's limit");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return VGM_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  if (!p) return fail(VGM_EHIP, "hipGetSymbolAddress(scratch) returned a null address");
+  g_noalloc_ptr[device] = static_cast<char *>(p);
+  return VGM_OK;
+}
 
 int fail(int code, const std::string &msg)
 {
@@ -126,6 +153,43 @@ int check_pv(vgm_ctx *c, int pv)
   return VGM_OK;
 }
 
+// Bytes of device staging for nin input arrays and one output of n points.
+size_t staged_bytes(size_t nin, size_t n, size_t out_elem)
+{
+  return nin * align256(n * sizeof(double)) + align256(n * out_elem);
+}
+
+// Host-pointer query through the staging buffer `b` (staged_bytes long):
+// copy in, launch, copy out, wait.
+int run_staged(vgm_ctx *c, char *b, int method, int precision, const ShapeRec &S, const void *params,
+               const Transform &tr, bool use_tr, const double *px, const double *py, const double *pz,
+               const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
+               size_t out_elem)
+{
+  const bool with_dir = (method == kDistanceToIn || method == kDistanceToOut);
+  const size_t nin = with_dir ? (step ? 7 : 6) : 3;
+  const size_t bytes_in = align256(n * sizeof(double));
+  const double *src[7] = {px, py, pz, dx, dy, dz, step};
+  double *dst[7] = {nullptr};
+  for (size_t k = 0; k < nin; ++k) {
+    dst[k] = reinterpret_cast<double *>(b + k * bytes_in);
+    VGM_HIP(hipMemcpyAsync(dst[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  void *dout = b + nin * bytes_in;
+  SoaArgs a{};
+  a.n = n;
+  a.px = dst[0], a.py = dst[1], a.pz = dst[2];
+  if (with_dir) {
+    a.dx = dst[3], a.dy = dst[4], a.dz = dst[5];
+    a.step = step ? dst[6] : nullptr;
+  }
+  a.out = dout;
+  VGM_HIP(launch_soa(c->stream, method, precision, S, params, tr, use_tr, a));
+  VGM_HIP(hipMemcpyAsync(out, dout, n * out_elem, hipMemcpyDeviceToHost, c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  return VGM_OK;
+}
+
 // Common driver for the seven SOA queries.
 int run_soa(vgm_ctx *c, int pv, int method, bool use_tr, const double *px, const double *py, const double *pz,
             const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
@@ -156,28 +220,10 @@ int run_soa(vgm_ctx *c, int pv, int method, bool use_tr, const double *px, const
     return VGM_OK;
   }
   const size_t nin = with_dir ? (step ? 7 : 6) : 3;
-  const size_t bytes_in = align256(n * sizeof(double));
-  const size_t bytes_out = align256(n * out_elem);
-  rc = ensure_scratch(c, nin * bytes_in + bytes_out);
+  rc = ensure_scratch(c, staged_bytes(nin, n, out_elem));
   if (rc) return rc;
-  char *b = c->scratch.buf;
-  const double *src[7] = {px, py, pz, dx, dy, dz, step};
-  double *dst[7] = {nullptr};
-  for (size_t k = 0; k < nin; ++k) {
-    dst[k] = reinterpret_cast<double *>(b + k * bytes_in);
-    VGM_HIP(hipMemcpyAsync(dst[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  }
-  void *dout = b + nin * bytes_in;
-  a.px = dst[0], a.py = dst[1], a.pz = dst[2];
-  if (with_dir) {
-    a.dx = dst[3], a.dy = dst[4], a.dz = dst[5];
-    a.step = step ? dst[6] : nullptr;
-  }
-  a.out = dout;
-  VGM_HIP(launch_soa(c->stream, method, precision, S, params, P.tr, use_tr, a));
-  VGM_HIP(hipMemcpyAsync(out, dout, n * out_elem, hipMemcpyDeviceToHost, c->stream));
-  VGM_HIP(hipStreamSynchronize(c->stream));
-  return VGM_OK;
+  return run_staged(c, c->scratch.buf, method, precision, S, params, P.tr, use_tr, px, py, pz, dx, dy, dz, step, n,
+                    out, out_elem);
 }
 
 }  // namespace
@@ -207,6 +253,12 @@ int vgm_ctx_create(int device, void *stream, vgm_ctx **out)
       return hip_fail(e, "hipStreamCreate");
     }
     c->own_stream = true;
+  }
+  const int rc = init_noalloc_scratch(device);
+  if (rc) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
   }
   *out = c;
   return VGM_OK;
@@ -449,6 +501,41 @@ int vgm_safety_to_out(vgm_ctx *c, int pv, const double *x, const double *y, cons
                       unsigned flags)
 {
   return run_soa(c, pv, kSafetyToOut, false, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 8, flags);
+}
+
+int vgm_query_noalloc(vgm_ctx *c, int pv, int method, const double *px, const double *py, const double *pz,
+                      const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
+                      unsigned flags)
+{
+  // VGM_Q_* -> (kernel method, master-frame input, output element size)
+  static const int kMethod[7] = {kContains, kInside, kDistanceToIn, kDistanceToOut, kDistanceToOut, kSafetyToIn,
+                                 kSafetyToOut};
+  static const bool kMaster[7] = {true, true, true, false, true, true, false};
+  static const size_t kElem[7] = {1, 4, 8, 8, 8, 8, 8};
+  int rc = check_pv(c, pv);
+  if (rc) return rc;
+  if (method < VGM_Q_CONTAINS || method > VGM_Q_SAFETY_TO_OUT) return fail(VGM_EINVAL, "unknown query method");
+  if (flags & VGM_DEVICE_PTRS) return fail(VGM_EINVAL, "vgm_query_noalloc takes host arrays only");
+  const int m = kMethod[method];
+  const bool with_dir = (m == kDistanceToIn || m == kDistanceToOut);
+  if (n && (!px || !py || !pz || !out || (with_dir && (!dx || !dy || !dz))))
+    return fail(VGM_EINVAL, "null input or output pointer");
+  const size_t nin = with_dir ? (step ? 7 : 6) : 3;
+  const size_t need = staged_bytes(nin, n, kElem[method]);
+  if (need > size_t(VGM_NOALLOC_SCRATCH_BYTES))
+    return fail(VGM_EINVAL, "vgm_query_noalloc: " + std::to_string(n) + " points need " + std::to_string(need) +
+                                " bytes of staging, above VGM_NOALLOC_SCRATCH_BYTES (" +
+                                std::to_string(size_t(VGM_NOALLOC_SCRATCH_BYTES)) + ")");
+  if (c->dev.version != c->host.version || !c->dev.buf)
+    return fail(VGM_ESTATE, "vgm_query_noalloc: the geometry is not on the device; run an allocating query first");
+  char *buf = (c->device >= 0 && c->device < kMaxDevices) ? g_noalloc_ptr[c->device] : nullptr;
+  if (!buf) return fail(VGM_ESTATE, "vgm_query_noalloc: no staging array on this device");
+  if (n == 0) return VGM_OK;
+  VGM_HIP(hipSetDevice(c->device));
+  const PlacedRec &P = c->host.pvs[pv];
+  const ShapeRec &S = c->host.shapes[c->host.lvs[P.lv].shape];
+  return run_staged(c, buf, m, (flags & VGM_MIXED) ? kMixed : kFP64, S, c->dev.pool + S.off, P.tr, kMaster[method],
+                    px, py, pz, dx, dy, dz, step, n, out, kElem[method]);
 }
 
 }  // extern "C"
