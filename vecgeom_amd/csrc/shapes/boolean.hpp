@@ -52,11 +52,13 @@ struct BooleanShape {
   static constexpr int kWavesPerEU = 1;
   static constexpr bool kSafetyIsDistance = false;
   static constexpr bool kSafetyBelowDistance = false;
-  static constexpr bool kFloatDistanceToOut = false;
-  static constexpr bool kFloatDistanceToIn = false;
-  // the safeties are estimates (SafetyToIn of a subtraction is "very
-  // approximate"), so mixed precision evaluates every query in FP64
-  static constexpr bool kFloatQueries = false;
+  // mixed precision: Inside/Contains classified in float away from the surface
+  static constexpr bool kFloatClassify = false;
+  // mixed precision: the float safety less the guard band bounds the FP64 safety
+  static constexpr bool kFloatSafety = false;
+  // mixed precision: distance_to_in_mixed / distance_to_out_mixed (float selects
+  // the surface, FP64 evaluates it)
+  static constexpr bool kMixedDistance = false;
 
   // one operand query; `placed` transforms the point (and direction) into the operand's frame
   template <typename R>

@@ -19,13 +19,13 @@ struct CutTubeShape {
   static constexpr bool kSafetyIsDistance = false;
   // the distance-first step is not proven for this family: it takes the FP64 step
   static constexpr bool kSafetyBelowDistance = false;
-  // mixed-precision DistanceToOut may be evaluated in float
-  static constexpr bool kFloatDistanceToOut = true;
-  // mixed-precision DistanceToIn may be evaluated in float (no: the cut planes test
-  // their hit point against kTolerance, below float resolution)
-  static constexpr bool kFloatDistanceToIn = false;
-  // mixed precision may evaluate the queries in float at all
-  static constexpr bool kFloatQueries = true;
+  // mixed precision: Inside/Contains classified in float away from the surface
+  static constexpr bool kFloatClassify = true;
+  // mixed precision: the float safety less the guard band bounds the FP64 safety
+  static constexpr bool kFloatSafety = true;
+  // mixed precision: distance_to_in_mixed / distance_to_out_mixed (float selects
+  // the surface, FP64 evaluates it)
+  static constexpr bool kMixedDistance = false;
 
   // Plane::DistPlane (Plane.h:85-90): point . normal + distance
   template <typename R>

@@ -21,14 +21,13 @@ struct PolyconeShape {
   // SafetyToIn can exceed DistanceToIn (the section-wise safety of the
   // reference), so the distance-first step would change decisions
   static constexpr bool kSafetyBelowDistance = false;
-  // the section walk of DistanceToOut decides at every z-plane whether the
-  // ray continues; in float it can end on a different surface than in
-  // FP64, so mixed precision keeps this query in FP64
-  static constexpr bool kFloatDistanceToOut = false;
-  // mixed-precision DistanceToIn may be evaluated in float
-  static constexpr bool kFloatDistanceToIn = true;
-  // mixed precision may evaluate the queries in float at all
-  static constexpr bool kFloatQueries = true;
+  // mixed precision: Inside/Contains classified in float away from the surface
+  static constexpr bool kFloatClassify = true;
+  // mixed precision: the float safety less the guard band bounds the FP64 safety
+  static constexpr bool kFloatSafety = false;
+  // mixed precision: distance_to_in_mixed / distance_to_out_mixed (float selects
+  // the surface, FP64 evaluates it)
+  static constexpr bool kMixedDistance = false;
 
   // GetSectionIndex: linear search over the z-planes, compared in the
   // arithmetic type of the query

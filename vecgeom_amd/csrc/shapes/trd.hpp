@@ -16,12 +16,13 @@ struct TrdShape {
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:
   // the distance-first step takes the FP64 step's decisions
   static constexpr bool kSafetyBelowDistance = true;
-  // mixed-precision DistanceToOut may be evaluated in float
-  static constexpr bool kFloatDistanceToOut = true;
-  // mixed-precision DistanceToIn may be evaluated in float
-  static constexpr bool kFloatDistanceToIn = true;
-  // mixed precision may evaluate the queries in float at all
-  static constexpr bool kFloatQueries = true;
+  // mixed precision: Inside/Contains classified in float away from the surface
+  static constexpr bool kFloatClassify = true;
+  // mixed precision: the float safety less the guard band bounds the FP64 safety
+  static constexpr bool kFloatSafety = true;
+  // mixed precision: distance_to_in_mixed / distance_to_out_mixed (float selects
+  // the surface, FP64 evaluates it)
+  static constexpr bool kMixedDistance = false;
 
   template <typename R>
   VGM_DEV static R clamp0(R d) { return vabs(d) < R(kHalfTolerance) ? R(0) : d; }

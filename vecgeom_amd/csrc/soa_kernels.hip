@@ -53,24 +53,19 @@ __global__ __launch_bounds__(kSoaBlock) void soa_fp64_kernel(const typename S::D
   }
 }
 
-// Mixed precision (VGM_MIXED): every query is evaluated in float; FP64 is
-// kept for the decisions float cannot be trusted with.
+// Mixed precision (VGM_MIXED): float does the bulk of the work and FP64
+// decides every result, so each result equals the FP64 kernel's.
 //   Contains, Inside  the float classification stands when the float safety
 //                     puts the point farther than the guard band from the
 //                     surface; otherwise (and for float "surface") the point
-//                     is classified in FP64 - so classifications equal FP64's
-//   DistanceToIn/Out  float; FP64 when the float safety or the float distance
-//                     is below the guard band (the point may be on the surface
-//                     or the wrong side of it), or when the float hit point is
-//                     not on the surface (float safety there above 16 guards);
-//                     FP64 throughout for families whose float DistanceToOut
-//                     can end on another surface (kFloatDistanceToOut false)
-//                     or whose float DistanceToIn can miss where FP64 hits
-//                     (kFloatDistanceToIn false)
-//   SafetyToIn/Out    float; FP64 when the float safety is below the guard band
-// The guard band is a few float ulps of the largest coordinate in play.  The
-// error of the float results against FP64 is measured by the tests and the
-// benchmark, per query.
+//                     is classified in FP64 (kFloatClassify families)
+//   DistanceToIn/Out  float selects the surface, FP64 evaluates it with the
+//                     FP64 kernel's expressions; uncertain selections take the
+//                     FP64 kernel (kMixedDistance families: tube, cone); FP64
+//                     for the others
+//   SafetyToIn/Out    FP64 (a single closed form or a walk whose value is the
+//                     result: a float value would carry float error)
+// The guard band is a few float ulps of the largest coordinate in play.
 template <class S, int M>
 __global__ __launch_bounds__(kSoaBlock) void soa_mixed_kernel(const typename S::Data *__restrict__ sd, Transform tr,
                                                               int use_tr, float scale, SoaArgs a)
@@ -80,75 +75,38 @@ __global__ __launch_bounds__(kSoaBlock) void soa_mixed_kernel(const typename S::
   const typename S::Data &s = *sd;
   V3<double> p{a.px[i], a.py[i], a.pz[i]};
   if (use_tr) p = to_local_point(tr, p);
-  if constexpr (!S::kFloatQueries) {  // every query in FP64
-    V3<double> d{0.0, 0.0, 0.0};
-    if constexpr (M == kDistanceToIn || M == kDistanceToOut) {
-      d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
-      if (use_tr) d = to_local_dir(tr, d);
-    }
-    const double step = (M == kDistanceToIn || M == kDistanceToOut) && a.step ? a.step[i] : kInfLength;
-    if constexpr (M == kContains) static_cast<uint8_t *>(a.out)[i] = S::template contains<double>(s, p) ? 1 : 0;
-    else if constexpr (M == kInside) static_cast<int32_t *>(a.out)[i] = S::template inside<double>(s, p);
-    else if constexpr (M == kSafetyToIn) static_cast<double *>(a.out)[i] = S::template safety_to_in<double>(s, p);
-    else if constexpr (M == kSafetyToOut) static_cast<double *>(a.out)[i] = S::template safety_to_out<double>(s, p);
-    else if constexpr (M == kDistanceToIn) static_cast<double *>(a.out)[i] = S::template distance_to_in<double>(s, p, d, step);
-    else static_cast<double *>(a.out)[i] = S::template distance_to_out<double>(s, p, d, step);
-    return;
-  }
-  const V3<float> pf{float(p.x), float(p.y), float(p.z)};
-  const float g = 4.0e-6f * (fabsf(pf.x) + fabsf(pf.y) + fabsf(pf.z) + scale);
   if constexpr (M == kContains || M == kInside) {
-    const int r = S::template inside<float>(s, pf);
-    float sf = 0.0f;
-    if (r == kInsideV) sf = S::template safety_to_out<float>(s, pf);
-    if (r == kOutsideV) sf = S::template safety_to_in<float>(s, pf);
-    const bool firm = r != kSurfaceV && sf > g;
-    if constexpr (M == kContains)
-      static_cast<uint8_t *>(a.out)[i] = (firm ? r == kInsideV : S::template contains<double>(s, p)) ? 1 : 0;
-    else
-      static_cast<int32_t *>(a.out)[i] = firm ? r : S::template inside<double>(s, p);
-  } else if constexpr (M == kSafetyToIn || M == kSafetyToOut) {
-    float sf = (M == kSafetyToIn) ? S::template safety_to_in<float>(s, pf) : S::template safety_to_out<float>(s, pf);
-    double r = double(sf);
-    if (!(sf > g))
-      r = (M == kSafetyToIn) ? S::template safety_to_in<double>(s, p) : S::template safety_to_out<double>(s, p);
-    static_cast<double *>(a.out)[i] = r;
+    if constexpr (S::kFloatClassify) {
+      const V3<float> pf{float(p.x), float(p.y), float(p.z)};
+      const float g = 4.0e-6f * (fabsf(pf.x) + fabsf(pf.y) + fabsf(pf.z) + scale);
+      const int r = S::template inside<float>(s, pf);
+      float sf = 0.0f;
+      if (r == kInsideV) sf = S::template safety_to_out<float>(s, pf);
+      if (r == kOutsideV) sf = S::template safety_to_in<float>(s, pf);
+      const bool firm = r != kSurfaceV && sf > g;
+      if constexpr (M == kContains)
+        static_cast<uint8_t *>(a.out)[i] = (firm ? r == kInsideV : S::template contains<double>(s, p)) ? 1 : 0;
+      else
+        static_cast<int32_t *>(a.out)[i] = firm ? r : S::template inside<double>(s, p);
+    } else {
+      if constexpr (M == kContains) static_cast<uint8_t *>(a.out)[i] = S::template contains<double>(s, p) ? 1 : 0;
+      else static_cast<int32_t *>(a.out)[i] = S::template inside<double>(s, p);
+    }
+  } else if constexpr (M == kSafetyToIn) {
+    static_cast<double *>(a.out)[i] = S::template safety_to_in<double>(s, p);
+  } else if constexpr (M == kSafetyToOut) {
+    static_cast<double *>(a.out)[i] = S::template safety_to_out<double>(s, p);
   } else {
     V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
     if (use_tr) d = to_local_dir(tr, d);
-    if constexpr ((M == kDistanceToOut && !S::kFloatDistanceToOut) || (M == kDistanceToIn && !S::kFloatDistanceToIn)) {
-      const double step = a.step ? a.step[i] : kInfLength;
-      static_cast<double *>(a.out)[i] = (M == kDistanceToOut) ? S::template distance_to_out<double>(s, p, d, step)
-                                                              : S::template distance_to_in<double>(s, p, d, step);
-      return;
-    }
-    const V3<float> df{float(d.x), float(d.y), float(d.z)};
-    const double step = a.step ? a.step[i] : kInfLength;
-    const float stepf = step >= double(FLT_MAX) ? FLT_MAX : float(step);
-    // near the surface the float point may sit on the other side of it:
-    // FP64 there, and where the float distance itself is below the guard
-    const float sf = (M == kDistanceToIn) ? S::template safety_to_in<float>(s, pf)
-                                          : S::template safety_to_out<float>(s, pf);
     double r;
-    float rf = 0.0f;
-    if (sf > g)
-      rf = (M == kDistanceToIn) ? S::template distance_to_in<float>(s, pf, df, stepf)
-                                : S::template distance_to_out<float>(s, pf, df, stepf);
-    // a float hit must land on the surface: the float safety there, on the
-    // side the query ends on, must vanish within the guard band (a walk that
-    // stopped at an internal section junction lands inside the solid)
-    bool ok = sf > g && rf > g;
-    if (ok && rf < FLT_MAX) {
-      const V3<float> q{pf.x + rf * df.x, pf.y + rf * df.y, pf.z + rf * df.z};
-      const float gq = 4.0e-6f * (fabsf(q.x) + fabsf(q.y) + fabsf(q.z) + scale);
-      const float sq = (M == kDistanceToIn) ? S::template safety_to_in<float>(s, q) : S::template safety_to_out<float>(s, q);
-      ok = fabsf(sq) <= 16.0f * gq;
-    }
-    if (ok)
-      r = rf >= FLT_MAX ? kInfLength : double(rf);
-    else
+    if constexpr (S::kMixedDistance) {
+      r = (M == kDistanceToIn) ? S::distance_to_in_mixed(s, p, d, scale) : S::distance_to_out_mixed(s, p, d, scale);
+    } else {
+      const double step = a.step ? a.step[i] : kInfLength;
       r = (M == kDistanceToIn) ? S::template distance_to_in<double>(s, p, d, step)
                                : S::template distance_to_out<double>(s, p, d, step);
+    }
     static_cast<double *>(a.out)[i] = r;
   }
 }

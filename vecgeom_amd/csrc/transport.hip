@@ -13,10 +13,10 @@
 //
 // Precision modes:
 //   FP64   every operation in double, bit-identical to the CPU oracle.
-//   MIXED  safety and boundary distance evaluated in float as a filter; any
-//          step that could reach a boundary is redone with the FP64 kernels,
-//          so positions and crossings follow the FP64 trajectory except where
-//          float and double disagree by more than the guard band.
+//   MIXED  a guarded float safety settles the steps it covers, and the
+//          boundary distance comes from the mixed kernels (float selects the
+//          surface, FP64 evaluates it): the same decisions and positions as
+//          FP64, bit for bit.
 #include "vgm_internal.h"
 #include "transport.h"
 #include "stepping.hpp"
@@ -111,60 +111,62 @@ VGM_DEV bool step_fp64(const typename S::Data &s, TrackState &t, double l)
   return true;
 }
 
-// Absolute guard for float results: a few float ulps of the largest
-// coordinate magnitude in play, plus a relative margin on the distance.
+// Absolute guard for float safeties: a few float ulps of the largest
+// coordinate magnitude in play (a float safety is a short closed form or a
+// minimum of such, so its error stays well inside).
 VGM_DEV float guard_band(const V3<float> &p, float scale)
 {
   return 4.0e-6f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + scale);
 }
 
+// Mixed-precision track-step: the decisions and the position arithmetic of
+// the FP64 step, with float doing the bulk of the work.
+//  - kFloatSafety / kSafetyIsDistance families: the float safety less the
+//    guard band is a lower bound of the FP64 safety, so when it covers the
+//    step the FP64 step moves freely too, and so does this one (the side of
+//    a surface-distance family is checked in FP64, as in the FP64 step);
+//  - otherwise the boundary distance comes first (kSafetyBelowDistance
+//    families), from the mixed kernels where the family has them (float
+//    selects the surface, FP64 evaluates it: the FP64 distance bit for
+//    bit), else in FP64;
+//  - the remaining families take the FP64 step.
+// Trajectories are the FP64 ones bit for bit.  fp64_steps counts the steps
+// the float safety did not settle.
+template <class S>
+VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified,
+                                 float scale = 0.0f, bool mixed_distance = false);
+
 template <class S>
 VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, float scale, uint32_t &fp64_steps)
 {
   if constexpr (S::kSafetyIsDistance) {
-    if (l <= t.carry) return step_fp64<S>(s, t, l);  // bound carried from an FP64 safety
+    if (l <= t.carry) return step_fp64<S>(s, t, l);  // under the carried bound: moves without a query
   }
-  const V3<float> pf{float(t.p.x), float(t.p.y), float(t.p.z)};
-  const V3<float> df{float(t.d.x), float(t.d.y), float(t.d.z)};
-  const float lf = float(l);
-  const float g = guard_band(pf, scale);
-  float saf;
-  if constexpr (S::kSafetyIsDistance) {
-    saf = S::template surface_distance<float>(s, pf, t.in);
-    if (lf + g <= saf && S::template contains<float>(s, pf) != t.in) saf = -1.0f;
-  } else {
-    saf = t.in ? S::template safety_to_out<float>(s, pf) : S::template safety_to_in<float>(s, pf);
-  }
-  bool move_free = lf + g <= saf;
-  const bool by_safety = move_free;
-  if constexpr (!S::kFloatQueries) {
-    fp64_steps++;
-    return step_fp64<S>(s, t, l);
-  }
-  if constexpr (!S::kFloatDistanceToIn) {
-    // a float miss from outside is not trusted for such a family
-    if (!move_free && !t.in) {
-      fp64_steps++;
-      return step_fp64<S>(s, t, l);
+  if constexpr (S::kFloatSafety || S::kSafetyIsDistance) {
+    const V3<float> pf{float(t.p.x), float(t.p.y), float(t.p.z)};
+    const float g = guard_band(pf, scale);
+    const float lf = float(l);  // g covers its rounding
+    float saf;
+    if constexpr (S::kSafetyIsDistance) {
+      saf = S::template surface_distance<float>(s, pf, t.in);
+      if (lf + g <= saf && S::template contains<double>(s, t.p) != t.in) saf = -1.0f;
+    } else {
+      saf = t.in ? S::template safety_to_out<float>(s, pf) : S::template safety_to_in<float>(s, pf);
+    }
+    if (lf + g <= saf) {
+      t.carry = 0.0;
+      if constexpr (S::kSafetyIsDistance) t.carry = (double(saf) - double(g)) - l - carry_margin(t.p);
+      t.p.x = t.p.x + l * t.d.x;
+      t.p.y = t.p.y + l * t.d.y;
+      t.p.z = t.p.z + l * t.d.z;
+      return false;
     }
   }
-  if (!move_free) {
-    const float db = t.in ? S::template distance_to_out<float>(s, pf, df, FLT_MAX)
-                          : S::template distance_to_in<float>(s, pf, df, FLT_MAX);
-    move_free = (db < FLT_MAX) ? (lf + g + 1.0e-3f * db < db) : (!t.in);
-  }
-  if (move_free) {
-    // a guarded float safety bounds the safety as the FP64 one does, so a
-    // surface-distance family carries it like an FP64 safety
-    t.carry = 0.0;
-    if constexpr (S::kSafetyIsDistance)
-      if (by_safety) t.carry = (double(saf) - double(g)) - l - carry_margin(t.p);
-    t.p.x = t.p.x + l * t.d.x;
-    t.p.y = t.p.y + l * t.d.y;
-    t.p.z = t.p.z + l * t.d.z;
-    return false;
-  }
   fp64_steps++;
+  if constexpr (S::kSafetyBelowDistance) {
+    bool verified = false;
+    return step_distance_first<S>(s, t, l, verified, scale, S::kMixedDistance);
+  }
   return step_fp64<S>(s, t, l);
 }
 
@@ -179,10 +181,21 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
 // the carried bound differs.  `verified` is set when the side check ran and
 // agreed (see learned_bound_kernel).  Launches of other families ignore the
 // distance-first request.
+// With mixed_distance the boundary distance comes from the family's mixed
+// kernels, which return the FP64 value.
 template <class S>
-VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified)
+VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, double l, bool &verified, float scale,
+                                 bool mixed_distance)
 {
-  double db = boundary_distance<S>(s, t.p, t.d, t.in);
+  double db;
+  if constexpr (S::kMixedDistance) {
+    if (mixed_distance)
+      db = t.in ? S::distance_to_out_mixed(s, t.p, t.d, scale) : S::distance_to_in_mixed(s, t.p, t.d, scale);
+    else
+      db = boundary_distance<S>(s, t.p, t.d, t.in);
+  } else {
+    db = boundary_distance<S>(s, t.p, t.d, t.in);
+  }
   if (db < 0.0) db = 0.0;
   if (l < db) {
     t.carry = 0.0;
