@@ -10,14 +10,20 @@ Inputs are resident in HBM when timing starts.
 Runs, in one process on identical initial states (--only picks a subset):
   fp64     plain FP64 step (safety, then boundary distance), bit-identical to
            the CPU oracle
-  mixed    float filter with FP64 redo of steps near boundaries
+  mixed    a guarded float safety settles the steps it covers; boundary
+           distances from the mixed kernels (float selects the surface, FP64
+           evaluates it); the FP64 decisions
   learned  learned safety bound on the tessellated solids (networks trained on
-           the GPU first, on exact samples), FP64 fallback; mixed elsewhere
+           the GPU first, on exact samples), exact fallback; mixed elsewhere
   tuned    each family in the mode the autotuner measured fastest among FP64,
-           mixed, distance-first (same decisions as FP64) and learned
-`value` is the last run's whole-job track-steps/s (tuned by default); the
-others are reported beside it, with the number of tracks whose final state
-differs from the FP64 run and the audited safety violations of learned bounds.
+           mixed, distance-first, culled (all with the FP64 decisions) and
+           learned
+`value` is the last run's whole-job track-steps/s (tuned by default, named in
+config.run); the others are reported beside it (mixed_learned_value is the
+learned run), with the number of tracks whose final state differs from the
+FP64 run and the audited safety violations of learned bounds.  Besides the
+transport: the config-1 box queries with their HBM roofline (config1), the
+tube/cone FP64-vs-mixed comparison (precision) and the barrel navigation.
 The CPU baseline is the FP64 oracle on the host cores over a bounded sample of
 the same workload.
 """
@@ -71,7 +77,19 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
-FLOP_TABLES = {"fp64": "per_family", "distance-first": "per_family_distance_first", "culled": "per_family_culled"}
+RUN_NOTES = {
+    "fp64": "plain FP64 step (safety, then boundary distance)",
+    "mixed": "mixed precision: guarded float safety, float-selected FP64 boundary distances (FP64 decisions)",
+    "learned": "mixed precision with the learned safety bound on the tessellated solids (FP64 decisions)",
+    "tuned": "each family in the mode the autotuner timed fastest among fp64, mixed, distance-first, culled and "
+             "learned (all take the FP64 decisions)",
+}
+
+# Algorithmic FP64 work per track-step by step order.  The mixed step takes
+# the distance-first step's decisions; its FP64-equivalent work is that
+# step's (the float pre-filtering is not counted).
+FLOP_TABLES = {"fp64": "per_family", "distance-first": "per_family_distance_first", "culled": "per_family_culled",
+               "mixed": "per_family_distance_first"}
 
 
 def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights=None, launch_ms=None):
@@ -97,7 +115,7 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights
     flops = per_step * counts_by_family[fam] * S
     t = fams_ms[fam] * 1e-3
     achieved = flops / t / 1e12
-    out = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    out = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
            "kernel": f"transport_family_kernel<{fam}, FP64{'' if mode == 'fp64' else ', ' + mode}>",
            "kernel_ms": fams_ms[fam], "flops_per_track_step": per_step, "tracks": counts_by_family[fam],
@@ -114,6 +132,92 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights
         tr = json.load(open(TRAFFIC_FIXTURE))
         if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
             out["traffic"] = tr["bytes_per_launch"]
+    return out
+
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per query of the SOA3D box kernels (inputs read once,
+# output written once; DistanceToIn/Out read the step_max array too)
+C1_BYTES = {"Contains": 25, "Inside": 28, "DistanceToIn": 64, "DistanceToOut": 64, "SafetyToIn": 32,
+            "SafetyToOut": 32}
+
+
+def run_config1(args, ctx, dev, torch, stream):
+    """Box (10, 20, 30): points uniform in the box scaled x1.5, isotropic
+    directions, seed 42, step_max = 1e30; Contains, Inside, DistanceToIn,
+    DistanceToOut (on the points Inside == kInside), SafetyToIn, SafetyToOut.
+    Each query's kernel time from events on the library's stream (device
+    arrays, asynchronous calls, the average of --c1-reps launches), and its
+    HBM roofline from the algorithmic bytes per query.  At the configuration's
+    1M points a launch lasts microseconds and its fixed cost shows; the same
+    queries over --c1-stream points give the streaming rate."""
+    import vecgeom_amd as vg
+    from vecgeom_amd import sampling
+
+    half = (10.0, 20.0, 30.0)
+    box = ctx.box(*half)
+    pv = ctx.place(ctx.logical_volume(box))
+    out = {}
+    for label, n in (("config1", args.c1_points), ("streaming", args.c1_stream)):
+        if n <= 0:
+            continue
+        pts = [torch.from_numpy(a).to(dev) for a in sampling.points_in_box(42, 1, n, half, 1.5)]
+        dirs = [torch.from_numpy(a).to(dev) for a in sampling.isotropic_directions(42, 2, n)]
+        step = torch.full((n,), 1e30, dtype=torch.float64, device=dev)
+        P, D = vg.SOA3D(*pts), vg.SOA3D(*dirs)
+        ins = ctx.inside(pv, P)
+        sel = torch.nonzero(ins == vg.kInside).flatten()
+        Pin = vg.SOA3D(*[pts[k][sel].contiguous() for k in range(3)])
+        Din = vg.SOA3D(*[dirs[k][sel].contiguous() for k in range(3)])
+        stin = step[sel].contiguous()
+        o8 = torch.empty(n, dtype=torch.uint8, device=dev)
+        o32 = torch.empty(n, dtype=torch.int32, device=dev)
+        o64 = torch.empty(n, dtype=torch.float64, device=dev)
+        calls = {
+            "Contains": (n, lambda: ctx.contains(pv, P, out=o8, asynchronous=True)),
+            "Inside": (n, lambda: ctx.inside(pv, P, out=o32, asynchronous=True)),
+            "DistanceToIn": (n, lambda: ctx.distance_to_in(pv, P, D, step, out=o64, asynchronous=True)),
+            "DistanceToOut": (len(sel), lambda: ctx.distance_to_out(pv, Pin, Din, stin, out=o64[:len(sel)],
+                                                                    asynchronous=True)),
+            "SafetyToIn": (n, lambda: ctx.safety_to_in(pv, P, out=o64, asynchronous=True)),
+            "SafetyToOut": (n, lambda: ctx.safety_to_out(pv, P, out=o64, asynchronous=True)),
+        }
+        res = {}
+        for q, (m, fn) in calls.items():
+            fn()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(args.c1_reps):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / args.c1_reps
+            gbs = C1_BYTES[q] * m / (ms * 1e-3) / 1e9
+            res[q] = {"queries": m, "kernel_us": ms * 1e3, "gq_s": m / ms / 1e6,
+                      "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": gbs / HBM_PEAK_GBS, "bytes_per_query": C1_BYTES[q]}}
+        out[label] = {"points": n, "inside_points": len(sel), "reps": args.c1_reps, "queries": res}
+        log(f"[bench] box {label} ({n} points): " + ", ".join(
+            f"{q} {v['kernel_us']:.1f} us {v['roofline']['frac']:.2f}" for q, v in res.items()))
+        del pts, dirs, step, P, D, Pin, Din, stin, o8, o32, o64
+        torch.cuda.empty_cache()
+    if not args.no_cpu:
+        import oracle
+
+        geo = oracle.Geometry()
+        pvo = geo.place(geo.logical_volume(geo.box(*half)))
+        n = args.c1_points
+        pts = sampling.points_in_box(42, 1, n, half, 1.5)
+        dirs = sampling.isotropic_directions(42, 2, n)
+        nth = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        t = time.perf_counter()
+        for q in ("Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"):
+            geo.query(pvo, q, pts, dirs if "Distance" in q else None, nthreads=nth)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 6 * n / dt, "unit": "queries/s", "cores": nth, "kind": "port",
+                               "sample": f"the six queries over the {n} points (DistanceToOut on all of them), "
+                                         "FP64 oracle, OpenMP"}
     return out
 
 
@@ -288,6 +392,10 @@ def main():
     ap.add_argument("--cpu-tracks", type=int, default=400_000)
     ap.add_argument("--nav-tracks", type=int, default=4_000_000, help="navigation benchmark tracks (0: skip)")
     ap.add_argument("--nav-steps", type=int, default=200)
+    ap.add_argument("--c1-points", type=int, default=1_000_000, help="box points of the config-1 queries (0: skip)")
+    ap.add_argument("--c1-stream", type=int, default=32_000_000,
+                    help="box points of the streaming-size run of the same queries (0: skip)")
+    ap.add_argument("--c1-reps", type=int, default=20)
     ap.add_argument("--c2-points", type=int, default=16_000_000,
                     help="points per shape for the tube/cone precision comparison (0: skip)")
     ap.add_argument("--launch", default=None,
@@ -300,7 +408,8 @@ def main():
                     help="tuned run without the autotuner: family=mode,... (modes fp64, mixed, distance-first, "
                          "learned, culled); with --launch")
     ap.add_argument("--only", default=None,
-                    help="comma list of runs out of fp64,mixed,learned,tuned (default all; the headline is the last)")
+                    help="comma list out of fp64,mixed,learned,tuned (transport runs; the headline is the last), "
+                         "config1, precision, navigation (default: all)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain copy kernel of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -322,6 +431,19 @@ def main():
     # Launches go to the library's own stream; events are recorded there too.
     stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
     fams = available_families()
+    only = set(args.only.split(",")) if args.only else {"fp64", "mixed", "learned", "tuned", "config1", "precision",
+                                                        "navigation"}
+    config1 = None
+    if "config1" in only and args.c1_points > 0:
+        config1 = run_config1(args, ctx, dev, torch, stream)
+    if not only & {"fp64", "mixed", "learned", "tuned"}:
+        precision = run_precision(args, ctx, dev, torch, stream) if "precision" in only and args.c2_points else None
+        navigation = run_navigation(args, ctx, dev, torch) if "navigation" in only and args.nav_tracks else None
+        print(json.dumps({"metric": "track-steps/sec on 1 MI355X (mixed+learned vs plain FP64), 0 safety violations",
+                          "value": None, "unit": "track-steps/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "note": "no transport run requested (--only)",
+                          "config1": config1, "precision": precision, "navigation": navigation}))
+        return
     t0 = time.perf_counter()
     specs, counts = workloads.batched_solids(args.tracks, families=fams, per_family=8, seed=42)
     shapes, offsets, steps, arr = workloads.build(ctx, specs, counts, seed=42)
@@ -398,13 +520,13 @@ def main():
     counts_by_family = {}
     for (kind, _), c in zip(specs, counts):
         counts_by_family[kind] = counts_by_family.get(kind, 0) + int(c)
-    only = set(args.only.split(",")) if args.only else {"fp64", "mixed", "learned", "tuned"}
     res = {}
     if "fp64" in only:
         res["fp64"] = run(False)
     if "mixed" in only:
         res["mixed"] = run(True)
-        log(f"[bench] mixed fp64 redo fraction {res['mixed'][2]['fp64_steps'] / (n * S):.4f}")
+        log(f"[bench] mixed: fraction of steps the float safety did not settle "
+            f"{res['mixed'][2]['fp64_steps'] / (n * S):.4f}")
     tess = [i for i, (kind, _) in enumerate(specs) if kind == "tessellated"]
     train = None
     if tess and not args.no_learned and only & {"learned", "tuned"}:
@@ -462,10 +584,9 @@ def main():
         modes = res["tuned"][2].get("modes", {})
         tuned = {"value": rate("tuned"), "modes": modes, "kernel_ms": res["tuned"][1] * 1e3,
                  "family_kernel_ms": res["tuned"][4], "launch": res["tuned"][3],
-                 "exact_mode_tracks_differing_from_fp64": ndiff("tuned", family_mask(lambda k: modes.get(k) != "mixed")),
+                 "tracks_differing_from_fp64": ndiff("tuned", family_mask(lambda k: True)),
                  "audit": res["tuned"][2].get("audit"), "learned_networks": train}
-        log(f"[bench] tuned modes {modes}; exact-mode tracks differing from fp64: "
-            f"{tuned['exact_mode_tracks_differing_from_fp64']}")
+        log(f"[bench] tuned modes {modes}; tracks differing from fp64: {tuned['tracks_differing_from_fp64']}")
     head = next(k for k in ("tuned", "learned", "mixed", "fp64") if k in res)
     value, wall_v, kern_v = rate(head), res[head][0], res[head][1]
     roofline = family_roofline(res[head][4], counts_by_family, S,
@@ -502,12 +623,13 @@ def main():
                "sample": f"{cn} tracks x {S} track-steps, same solids and generator, FP64 oracle, OpenMP"}
 
     precision = None
-    if args.c2_points > 0:
+    if "precision" in only and args.c2_points > 0:
         precision = run_precision(args, ctx, dev, torch, stream)
 
     navigation = None
-    if args.nav_tracks > 0:
+    if "navigation" in only and args.nav_tracks > 0:
         navigation = run_navigation(args, ctx, dev, torch)
+    rate_ml = rate("learned")
 
     line = {
         "metric": "track-steps/sec on 1 MI355X (mixed+learned vs plain FP64), 0 safety violations",
@@ -524,11 +646,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched heterogeneous solids: {len(specs)} solids ({'/'.join(fams)}), "
                                f"{n} tracks, {S} track-steps per launch",
-                   "tracks": n, "track_steps_per_launch": S, "families": list(fams)},
+                   "tracks": n, "track_steps_per_launch": S, "families": list(fams),
+                   "run": head, "run_note": RUN_NOTES.get(head)},
         "fp64_value": rate64,
         "mixed_value": ratemx,
+        "mixed_learned_value": rate_ml,
+        "mixed_learned_over_fp64": rate_ml / rate64 if (rate_ml and rate64) else None,
         "value_over_fp64": value / rate64 if rate64 else None,
-        "fp64_redo_fraction": nfp64 / (n * S) if nfp64 is not None else None,
+        "mixed_float_unsettled_fraction": nfp64 / (n * S) if nfp64 is not None else None,
+        "mixed_tracks_differing_from_fp64": ndiff("mixed", family_mask(lambda k: True)),
         "kernel_ms": {"fp64": kern64 * 1e3 if kern64 else None, "mixed": kernmx * 1e3 if kernmx else None,
                       "value": kern_v * 1e3},
         "family_kernel_ms": {"fp64": fam64, "mixed": fammx},
@@ -539,6 +665,7 @@ def main():
         "tuned": tuned,
         "precision": precision,
         "navigation": navigation,
+        "config1": config1,
     }
     print(json.dumps(line))
 

@@ -92,6 +92,38 @@ def test_transport_mixed_follows_fp64(ctx):
     b.close()
 
 
+def test_transport_mixed_equals_fp64_long_runs(ctx):
+    """The benchmark's solids at 1M tracks, four launches of 50 track-steps:
+    the mixed run (float safeties carried across the steps of a launch) ends
+    on the FP64 run's state bit for bit.  Divergences here are rare events
+    (a float safety near a facet plane), so the run is long."""
+    import torch
+
+    print("[transport] mixed long runs", flush=True)
+    specs, counts = workloads.batched_solids(1_000_000, families=FAMILIES, per_family=8, seed=42)
+    shapes, offs, steps, arr = workloads.build(ctx, specs, counts, seed=42)
+    n = int(offs[-1])
+    b = vg.Batch(ctx, shapes, offs, steps)
+    init = [torch.from_numpy(a).cuda() for a in arr]
+    side0 = torch.zeros(n, dtype=torch.int8, device="cuda")
+    torch.cuda.synchronize()
+    b.classify(init[0], init[1], init[2], side0)
+    finals = []
+    for mixed in (False, True):
+        st = [t.clone() for t in init]
+        side, cr = side0.clone(), torch.zeros(n, dtype=torch.int32, device="cuda")
+        for k in range(4):
+            b.transport(*st, side, cr, 50, step0=50 * k, seed=7, mixed=mixed)
+        torch.cuda.synchronize()
+        finals.append(st + [side, cr])
+    diff = torch.zeros(n, dtype=torch.bool, device="cuda")
+    for x, y in zip(*finals):
+        diff |= (x != y)
+    print(f"[transport] mixed long runs: {int(diff.sum())} of {n} tracks differ", flush=True)
+    assert int(diff.sum()) == 0
+    b.close()
+
+
 @pytest.mark.parametrize("mode", ["distance_first", "cull"])
 def test_transport_distance_first_bit_exact(ctx, mode):
     """Boundary distance before the safety: the same decisions as the FP64

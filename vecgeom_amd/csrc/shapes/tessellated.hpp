@@ -122,10 +122,18 @@ struct TessellatedShape {
     R s = p.x * R(f.n[0]) + p.y * R(f.n[1]) + p.z * R(f.n[2]) + R(f.d);
     const R xx = p.x - R(f.n[0]) * s, xy = p.y - R(f.n[1]) * s, xz = p.z - R(f.n[2]) * s;
     bool within = facet_contains<R>(f, xx, xy, xz);
+    // The side test's tolerance is kTolerance in FP64.  A float evaluation
+    // widens it by its rounding error: a point within that error of the facet
+    // plane takes the plane distance, the smaller of the two values, so the
+    // float minimum never exceeds the FP64 one by more than rounding (the
+    // edge distance would jump away from the plane's).
+    R side_tol = R(kTolerance);
+    if constexpr (sizeof(R) == sizeof(float))
+      side_tol = side_tol + R(4.0e-6) * (vabs(p.x) + vabs(p.y) + vabs(p.z) + vabs(R(f.d)));
     if (to_in)
-      within &= s > R(-kTolerance);
+      within &= s > -side_tol;
     else
-      within &= s < R(kTolerance);
+      within &= s < side_tol;
     s *= s;
     if (within) return s;
     R e = inf_length<R>();
