@@ -117,7 +117,7 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights
     achieved = flops / t / 1e12
     out = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-           "kernel": f"transport_family_kernel<{fam}, FP64{'' if mode == 'fp64' else ', ' + mode}>",
+           "kernel": f"transport_family_kernel<{fam}, {mode}>",
            "kernel_ms": fams_ms[fam], "flops_per_track_step": per_step, "tracks": counts_by_family[fam],
            "track_steps": S,
            "note": "FP64 vector peak; flops counted on the step the kernel runs (+ - * / sqrt); kernel time from "
@@ -129,9 +129,14 @@ def family_roofline(fams_ms, counts_by_family, S, modes=None, streams=1, weights
                             "frac": total / (launch_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops": total,
                             "launch_ms": launch_ms, "note": "all families' flops over the whole launch"}
     if os.path.exists(TRAFFIC_FIXTURE):
-        tr = json.load(open(TRAFFIC_FIXTURE))
-        if tr.get("kernel") == out["kernel"] and tr.get("tracks") == out["tracks"] and tr.get("track_steps") == S:
-            out["traffic"] = tr["bytes_per_launch"]
+        # HBM bytes per launch of this family kernel in this mode at this
+        # workload, from FETCH_SIZE and WRITE_SIZE under rocprofv3 (scaled by a
+        # plain copy's reading of the same counters)
+        for tr in json.load(open(TRAFFIC_FIXTURE)).get("kernels", []):
+            if (tr["family"], tr["mode"], tr["tracks"], tr["track_steps"]) == (fam, mode, out["tracks"], S):
+                out["traffic"] = tr["bytes_per_launch"]
+                out["traffic_source"] = tr["source"]
+                out["algorithmic_bytes"] = tr["algorithmic_bytes"]
     return out
 
 
@@ -583,10 +588,13 @@ def main():
     if "tuned" in res:
         modes = res["tuned"][2].get("modes", {})
         tuned = {"value": rate("tuned"), "modes": modes, "kernel_ms": res["tuned"][1] * 1e3,
+                 "family_alone_ms": res["tuned"][2].get("weights"),
                  "family_kernel_ms": res["tuned"][4], "launch": res["tuned"][3],
                  "tracks_differing_from_fp64": ndiff("tuned", family_mask(lambda k: True)),
                  "audit": res["tuned"][2].get("audit"), "learned_networks": train}
         log(f"[bench] tuned modes {modes}; tracks differing from fp64: {tuned['tracks_differing_from_fp64']}")
+        log("[bench] tuned: each family alone in its mode (autotuner), ms: " + ", ".join(
+            f"{k} {v:.1f}" for k, v in (tuned["family_alone_ms"] or {}).items()))
     head = next(k for k in ("tuned", "learned", "mixed", "fp64") if k in res)
     value, wall_v, kern_v = rate(head), res[head][0], res[head][1]
     roofline = family_roofline(res[head][4], counts_by_family, S,

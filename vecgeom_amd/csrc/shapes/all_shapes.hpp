@@ -1,6 +1,7 @@
 // Every shape family known to the kernels, and the type switch that turns a
 // runtime ShapeRec into a call of the matching templated shape.
 #pragma once
+#include <type_traits>
 #include "box.hpp"
 #include "tube.hpp"
 #include "cone.hpp"
@@ -57,6 +58,29 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
   default:
     return false;
   }
+}
+
+// Distance-first eligibility of a track: the distance-first step takes the
+// FP64 step's decisions where the safety on the track's side cannot exceed
+// its boundary distance.  A family guarantees that everywhere
+// (kSafetyBelowDistance) or tells per point (distance_first_ok).
+template <class S, class = void>
+struct HasDistanceFirstOk : std::false_type {};
+template <class S>
+struct HasDistanceFirstOk<S, std::void_t<decltype(&S::template distance_first_ok<double>)>> : std::true_type {};
+
+template <class S>
+constexpr bool kDistanceFirstAny = S::kSafetyBelowDistance || HasDistanceFirstOk<S>::value;
+
+template <class S, typename R>
+VGM_DEV bool distance_first_ok(const typename S::Data &s, const V3<R> &p, bool in)
+{
+  if constexpr (S::kSafetyBelowDistance)
+    return true;
+  else if constexpr (HasDistanceFirstOk<S>::value)
+    return S::template distance_first_ok<R>(s, p, in);
+  else
+    return false;
 }
 
 // Shape-family sets, one bit per ShapeType.

@@ -396,7 +396,7 @@ struct ConeShape {
 
   // ---- mixed precision: float selects the surface, FP64 evaluates it -------
   // PointInCyclicalSector (kConeTolerance bands) at a point known to within e
-  VGM_DEV static int in_sector_cert(const Data &c, float x, float y, float e, bool includeSurface)
+  VGM_MIX static int in_sector_cert(const Data &c, float x, float y, float e, bool includeSurface)
   {
     const float startCheck = (-x * float(c.along1y)) + (y * float(c.along1x));
     const float endCheck = (-float(c.along2x) * y) + (float(c.along2y) * x);
@@ -408,7 +408,7 @@ struct ConeShape {
 
   // radius of the inner or outer surface at z (radius_at) in float, and a
   // bound on its error for z known to within e
-  VGM_DEV static float radius_at_f(const Data &c, float z, bool inner, float e, float &err)
+  VGM_MIX static float radius_at_f(const Data &c, float z, bool inner, float e, float &err)
   {
     const bool flat = inner ? (c.rmin1 == c.rmin2) : (c.orig_rmax1 == c.orig_rmax2);
     if (flat) {
@@ -427,7 +427,7 @@ struct ConeShape {
   // at most (|da| s^2 + 2 |db| s + |dcc|) / (2 sqrt(b^2 - a cc)) under
   // perturbations da, db, dcc, which bounds its float error.
   template <bool ForIn, bool Inner>
-  VGM_DEV static FCand conical_f(const Data &c, V3<double> pd, V3<float> point, V3<float> direction, float m)
+  VGM_MIX static FCand conical_f(const Data &c, V3<double> pd, V3<float> point, V3<float> direction, float m)
   {
     // a point on the conical surface takes the FP64 kernel's tolerance branch
     if (on_conical_surface<double>(c, pd, Inner)) return cand_unsure();
@@ -460,17 +460,17 @@ struct ConeShape {
     const float ed = kMixEps * (2.0f * vabs(b) * B + vabs(a) * C + vabs(cc) * A + b * b + vabs(a * cc));
     if (d2 < -ed) return cand_no();  // the FP64 discriminant test fails
     if (d2 <= ed) return cand_unsure();
-    const float delta = vsqrt(d2);
+    const float delta = mix_sqrt(d2);
     float dist;
     if (Inner) {
-      if (ForIn) dist = (b >= 0.0f) ? (cc / (-b - delta)) : (-b + delta) / a;
-      else dist = (b >= 0.0f) ? (-b - delta) / a : (cc / (-b + delta));
+      if (ForIn) dist = (b >= 0.0f) ? mix_div(cc, -b - delta) : mix_div(-b + delta, a);
+      else dist = (b >= 0.0f) ? mix_div(-b - delta, a) : mix_div(cc, -b + delta);
     } else {
-      if (ForIn) dist = (b > 0.0f) ? (-b - delta) / a : (cc / (-b + delta));
-      else dist = (b < 0.0f) ? (-b + delta) / a : (cc / (-b - delta));
+      if (ForIn) dist = (b > 0.0f) ? mix_div(-b - delta, a) : mix_div(cc, -b + delta);
+      else dist = (b < 0.0f) ? mix_div(-b + delta, a) : mix_div(cc, -b - delta);
     }
     if (!(vabs(dist) < 1.0e30f)) return cand_unsure();
-    const float err = kMixEps * ((A * dist * dist + 2.0f * B * vabs(dist) + C) / (2.0f * delta) + vabs(dist)) + m;
+    const float err = kMixEps * (mix_div(A * dist * dist + 2.0f * B * vabs(dist) + C, 2.0f * delta) + vabs(dist)) + m;
     int acc;
     const float newZ = point.z + direction.z * dist;
     if (Inner) {
@@ -487,7 +487,7 @@ struct ConeShape {
 
   // ConeUtilities::PhiPlaneTrajectoryIntersection in float, certified
   template <bool InsectorCheck>
-  VGM_DEV static FCand phi_plane_f(const Data &c, double ax_, double ay_, double nx_, double ny_, V3<float> pos,
+  VGM_MIX static FCand phi_plane_f(const Data &c, double ax_, double ay_, double nx_, double ny_, V3<float> pos,
                                    V3<float> dir, float m)
   {
     const float ax = float(ax_), ay = float(ay_), nx = float(nx_), ny = float(ny_);
@@ -496,8 +496,9 @@ struct ConeShape {
     if (acc == kCertNo) return cand_no();
     const float dirDotXY = (dir.y * ax) - (dir.x * ay);
     if (vabs(dirDotXY) < 1.0e-3f) return cand_unsure();
-    const float dist = ((ay * pos.x) - (ax * pos.y)) / dirDotXY;
-    const float err = (kMixEps * (vabs(pos.x) + vabs(pos.y)) + m) / vabs(dirDotXY) + kMixEps * vabs(dist);
+    const float rxy = __builtin_amdgcn_rcpf(dirDotXY);
+    const float dist = ((ay * pos.x) - (ax * pos.y)) * rxy;
+    const float err = (kMixEps * (vabs(pos.x) + vabs(pos.y)) + m) * vabs(rxy) + kMixEps * vabs(dist);
     acc = cert_and(acc, cert_ge(dist, float(-kConeTolerance), err));
     if (acc == kCertNo) return cand_no();
     const float hitx = pos.x + dist * dir.x;
@@ -506,7 +507,7 @@ struct ConeShape {
     if (InsectorCheck) {
       const float hitz = pos.z + dist * dir.z;
       const float r2 = (hitx * hitx) + (hity * hity);
-      const float r = vsqrt(r2);
+      const float r = mix_sqrt(r2);
       float ei, eo;
       const float ri = radius_at_f(c, hitz, true, eh, ei) + float(kTolerance);
       const float ro = radius_at_f(c, hitz, false, eh, eo) - float(kTolerance);
@@ -526,7 +527,7 @@ struct ConeShape {
   // moving away, inside, z-plane entry) in FP64, the conical and phi surfaces
   // selected in float, the selected one evaluated by the FP64 kernel's
   // expressions.  Equal to distance_to_in<double> bit for bit.
-  VGM_DEV static double distance_to_in_mixed(const Data &c, V3<double> pt, V3<double> dir, float scale)
+  VGM_MIX static double distance_to_in_mixed(const Data &c, V3<double> pt, V3<double> dir, float scale)
   {
     double distance;
     if (din_prologue<double>(c, pt, dir, distance)) return distance;
@@ -557,7 +558,7 @@ struct ConeShape {
   // DistanceToOut, mixed: wrong side, z-surface and phi-surface exits in
   // FP64, the smallest surface selected in float and evaluated by the FP64
   // kernel's expressions.
-  VGM_DEV static double distance_to_out_mixed(const Data &c, V3<double> pt, V3<double> dir, float scale)
+  VGM_MIX static double distance_to_out_mixed(const Data &c, V3<double> pt, V3<double> dir, float scale)
   {
     double distance;
     if (dout_prologue<double>(c, pt, dir, distance)) return distance;

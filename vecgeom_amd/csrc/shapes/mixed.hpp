@@ -28,6 +28,13 @@ constexpr float kMixLen = 2.0e-5f;
 // a root whose relative error bound exceeds this is not trusted
 constexpr float kMixMaxRel = 1.0e-3f;
 
+// Float square root and quotient of the selection: the hardware's 1-ulp
+// instructions (v_sqrt_f32, v_rcp_f32) rather than the correctly rounded
+// sequences; kMixEps covers their error.  Device code only.
+#define VGM_MIX __device__ __forceinline__
+VGM_MIX float mix_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+VGM_MIX float mix_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 enum Cert : int { kCertNo = 0, kCertYes = 1, kCertUnsure = 2 };
 
 // a >= b (or a > b: the same at a margin) with both known to within m
@@ -79,11 +86,12 @@ struct FSelect {
 
 // Exit through the z plane at height zt (dir.z != 0 in FP64): always taken,
 // at (zt - pz) / dz; a direction with |dz| below float range is far away.
-VGM_DEV FCand z_plane_cand(float zt, float pz, float dz, float m)
+VGM_MIX FCand z_plane_cand(float zt, float pz, float dz, float m)
 {
   if (vabs(dz) < 1.0e-20f) return FCand{kCertYes, 1.0e30f, 0.0f};
-  const float d = (zt - pz) / dz;
-  return FCand{kCertYes, d, (kMixEps * (vabs(zt) + vabs(pz)) + m) / vabs(dz) + kMixEps * vabs(d)};
+  const float r = __builtin_amdgcn_rcpf(dz);
+  const float d = (zt - pz) * r;
+  return FCand{kCertYes, d, (kMixEps * (vabs(zt) + vabs(pz)) + m) * vabs(r) + kMixEps * vabs(d)};
 }
 
 }  // namespace vgm

@@ -18,8 +18,10 @@ struct PolyconeShape {
   static constexpr int kWavesPerEU = 5;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
-  // SafetyToIn can exceed DistanceToIn (the section-wise safety of the
-  // reference), so the distance-first step would change decisions
+  // SafetyToIn can exceed DistanceToIn outside the solid beyond its
+  // z-range, where the reference takes the end section's safety alone
+  // (PolyconeImplementation.h:388-403); distance_first_ok tells the points
+  // where the distance-first step keeps the FP64 decisions
   static constexpr bool kSafetyBelowDistance = false;
   // mixed precision: Inside/Contains classified in float away from the surface
   static constexpr bool kFloatClassify = true;
@@ -39,6 +41,16 @@ struct PolyconeShape {
     for (int i = 0; i < d.nsec; ++i)
       if (zp >= R(zs[i]) && zp <= R(zs[i + 1])) return i;
     return -2;
+  }
+
+  // Inside the solid, and outside it within its z-range, the safety is the
+  // smallest section safety of the sections within reach, each a lower
+  // bound of the distance to its section: then it cannot exceed the
+  // boundary distance and the distance-first step takes the FP64 decisions
+  template <typename R>
+  VGM_DEV static bool distance_first_ok(const Data &d, V3<R> p, bool in)
+  {
+    return in || section_index<R>(d, p.z) >= 0;
   }
 
   template <typename R>

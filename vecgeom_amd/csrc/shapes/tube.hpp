@@ -326,7 +326,7 @@ struct TubeShape {
 
   // ---- mixed precision: float selects the surface, FP64 evaluates it -------
   // PointInCyclicalSector at a point known to within e, certified
-  VGM_DEV static int in_sector_cert(const Data &t, float x, float y, float e, bool includeSurface)
+  VGM_MIX static int in_sector_cert(const Data &t, float x, float y, float e, bool includeSurface)
   {
     const float startCheck = (-x * float(t.along1y) + y * float(t.along1x));
     const float endCheck = (-float(t.along2x) * y + float(t.along2y) * x);
@@ -341,16 +341,16 @@ struct TubeShape {
   // ((rsq + R^2) / (1 - dz^2)^2), P = |p|_1, R the radius, m the length
   // margin of the query
   template <bool Largest, bool InsectorCheck>
-  VGM_DEV static FCand circle_f(const Data &t, float b, float c, float mag, float P, float R, V3<float> pos,
+  VGM_MIX static FCand circle_f(const Data &t, float b, float c, float mag, float P, float R, V3<float> pos,
                                 V3<float> dir, float m)
   {
     const float d2 = b * b - c;
     const float ed = kMixEps * (mag + b * b + vabs(c));
     if (d2 < -ed) return cand_no();  // the FP64 test delta > 0 (>= 0) fails
     if (d2 <= ed) return cand_unsure();
-    const float sq = vsqrt(d2);
+    const float sq = mix_sqrt(d2);
     const float dist = Largest ? -b + sq : -b - sq;
-    const float err = ed / sq + kMixEps * (P + vabs(dist) + R) + m;
+    const float err = mix_div(ed, sq) + kMixEps * (P + vabs(dist) + R) + m;
     int acc = cert_ge(dist, float(-2.0 * kTolerance), err);
     if (InsectorCheck && acc != kCertNo) {
       const float hitz = pos.z + dist * dir.z;
@@ -363,7 +363,7 @@ struct TubeShape {
 
   // PhiPlaneTrajectoryIntersection in float, certified
   template <bool InsectorCheck>
-  VGM_DEV static FCand phi_plane_f(const Data &t, double ax_, double ay_, double nx_, double ny_, V3<float> pos,
+  VGM_MIX static FCand phi_plane_f(const Data &t, double ax_, double ay_, double nx_, double ny_, V3<float> pos,
                                    V3<float> dir, float m)
   {
     const float ax = float(ax_), ay = float(ay_), nx = float(nx_), ny = float(ny_);
@@ -372,8 +372,9 @@ struct TubeShape {
     if (acc == kCertNo) return cand_no();
     const float dirDotXY = (dir.y * ax - dir.x * ay);
     if (vabs(dirDotXY) < 1.0e-3f) return cand_unsure();  // grazing: the distance is ill-conditioned
-    const float dist = (ay * pos.x - ax * pos.y) / dirDotXY;
-    const float err = (kMixEps * (vabs(pos.x) + vabs(pos.y)) + m) / vabs(dirDotXY) + kMixEps * vabs(dist);
+    const float rxy = __builtin_amdgcn_rcpf(dirDotXY);
+    const float dist = (ay * pos.x - ax * pos.y) * rxy;
+    const float err = (kMixEps * (vabs(pos.x) + vabs(pos.y)) + m) * vabs(rxy) + kMixEps * vabs(dist);
     acc = cert_and(acc, cert_ge(dist * vabs(dirDotNorm), float(-kHalfTolerance), err));
     if (acc == kCertNo) return cand_no();
     const float hitx = pos.x + dist * dir.x;
@@ -382,7 +383,7 @@ struct TubeShape {
     if (InsectorCheck) {
       const float hitz = pos.z + dist * dir.z;
       const float r2 = hitx * hitx + hity * hity;
-      const float er2 = 2.0f * vsqrt(r2) * eh + eh * eh + kMixEps * r2;
+      const float er2 = 2.0f * mix_sqrt(r2) * eh + eh * eh + kMixEps * r2;
       acc = cert_and(acc, cert_le(vabs(hitz), float(t.tolOz), eh));
       acc = cert_and(acc, cert_ge(r2, float(t.tolOrmin2), er2));
       acc = cert_and(acc, cert_le(r2, float(t.tolOrmax2), er2));
@@ -397,7 +398,7 @@ struct TubeShape {
   // pre-step test, inside, on-surface, z-plane entry) in FP64, the curved and
   // phi surfaces selected in float, the selected one evaluated by the FP64
   // kernel's expressions.  Equal to distance_to_in<double> bit for bit.
-  VGM_DEV static double distance_to_in_mixed(const Data &t, V3<double> pt, V3<double> dir, float scale)
+  VGM_MIX static double distance_to_in_mixed(const Data &t, V3<double> pt, V3<double> dir, float scale)
   {
     const V3<float> pf{float(pt.x), float(pt.y), float(pt.z)};
     const float lim = 100.0f * float(t.maxval);
@@ -411,7 +412,7 @@ struct TubeShape {
     if (nsq < 1.0e-2f) return din_surfaces<double>(t, pt, dir, rsq, rdotn) + distToMove;  // near-axial
     const float P = vabs(pf.x) + vabs(pf.y) + vabs(pf.z);
     const float m = kMixLen * (P + scale);
-    const float invnsq = 1.0f / nsq;
+    const float invnsq = __builtin_amdgcn_rcpf(nsq);
     const float rsqf = float(rsq), b = invnsq * float(rdotn);
     int which;
     const FCand cmax = circle_f<false, true>(t, b, invnsq * (rsqf - float(t.rmax2)),
@@ -439,7 +440,7 @@ struct TubeShape {
 
   // DistanceToOut, mixed: wrong-side tests in FP64, the smallest surface
   // selected in float and evaluated by the FP64 kernel's expressions.
-  VGM_DEV static double distance_to_out_mixed(const Data &t, V3<double> pt, V3<double> dir, float scale)
+  VGM_MIX static double distance_to_out_mixed(const Data &t, V3<double> pt, V3<double> dir, float scale)
   {
     double crmax, crmin;
     if (dout_outside<double>(t, pt, crmax, crmin)) return -1.;
@@ -454,7 +455,7 @@ struct TubeShape {
       const float zt = dir.z < 0.0 ? -float(t.z) : float(t.z);
       sel.add(0, z_plane_cand(zt, pf.z, df.z, m));
     }
-    const float invnsq = 1.0f / nsq;
+    const float invnsq = __builtin_amdgcn_rcpf(nsq);
     const float rsqf = pf.x * pf.x + pf.y * pf.y;
     const float b = invnsq * (df.x * pf.x + df.y * pf.y);
     if (t.rmin_treat)

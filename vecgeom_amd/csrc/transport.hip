@@ -125,8 +125,8 @@ VGM_DEV float guard_band(const V3<float> &p, float scale)
 //    guard band is a lower bound of the FP64 safety, so when it covers the
 //    step the FP64 step moves freely too, and so does this one (the side of
 //    a surface-distance family is checked in FP64, as in the FP64 step);
-//  - otherwise the boundary distance comes first (kSafetyBelowDistance
-//    families), from the mixed kernels where the family has them (float
+//  - otherwise the boundary distance comes first (where distance_first_ok
+//    holds), from the mixed kernels where the family has them (float
 //    selects the surface, FP64 evaluates it: the FP64 distance bit for
 //    bit), else in FP64;
 //  - the remaining families take the FP64 step.
@@ -163,9 +163,11 @@ VGM_DEV bool step_mixed(const typename S::Data &s, TrackState &t, double l, floa
     }
   }
   fp64_steps++;
-  if constexpr (S::kSafetyBelowDistance) {
-    bool verified = false;
-    return step_distance_first<S>(s, t, l, verified, scale, S::kMixedDistance);
+  if constexpr (kDistanceFirstAny<S>) {
+    if (distance_first_ok<S>(s, t.p, t.in)) {
+      bool verified = false;
+      return step_distance_first<S>(s, t, l, verified, scale, S::kMixedDistance);
+    }
   }
   return step_fp64<S>(s, t, l);
 }
@@ -238,7 +240,7 @@ VGM_DEV bool step_distance_first(const typename S::Data &s, TrackState &t, doubl
 
 // Step order of an FP64 family kernel, fixed at compile time so that each
 // kernel holds the registers of one order only: the FP64 step, distance
-// first (families with kSafetyBelowDistance; the FP64 step otherwise), or
+// first (where distance_first_ok holds; the FP64 step elsewhere), or
 // distance first with the bounding-box cull.
 enum StepOrder : int { kOrderFP64 = 0, kOrderDistFirst = 1, kOrderCulled = 2 };
 
@@ -303,6 +305,9 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
         bool verified = false;  // only the learned tiles read it
         if constexpr (Order != kOrderFP64 && S::kSafetyBelowDistance)
           crossed = step_distance_first<S>(s, t, l, verified);
+        else if constexpr (Order != kOrderFP64 && kDistanceFirstAny<S>)
+          crossed = distance_first_ok<S>(s, t.p, t.in) ? step_distance_first<S>(s, t, l, verified)
+                                                       : step_fp64<S>(s, t, l);
         else
           crossed = step_fp64<S>(s, t, l);
       } else {
@@ -666,8 +671,9 @@ struct FamilyLauncher {
         return;
       }
     }
-    // distance first is the FP64 step for families without kSafetyBelowDistance
-    constexpr int kDistFirst = S::kSafetyBelowDistance ? kOrderDistFirst : kOrderFP64;
+    // distance first is the FP64 step for families whose safety may exceed
+    // their distance everywhere
+    constexpr int kDistFirst = kDistanceFirstAny<S> ? kOrderDistFirst : kOrderFP64;
     if (f.mode == kFamCulled)
       go<S, kTransportFP64, kOrderCulled>(args);
     else if (f.mode == kFamDistFirst)

@@ -186,7 +186,7 @@ V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lamb
     bc[k] = 0.5 * (rec.bmin[k] + rec.bmax[k]);
     bh[k] = 0.5 * (rec.bmax[k] - rec.bmin[k]);
   }
-  const bool distance_first = (mode & 1) && S::kSafetyBelowDistance;
+  const bool distance_first = (mode & 1) != 0;
   const bool cull = (mode & 2) != 0;
   Track<R> t;
   t.p = V3<R>{R(p0[0]), R(p0[1]), R(p0[2])};
@@ -209,7 +209,9 @@ V3<double> run_track(const typename S::Data &s, const ShapeRec &rec, double lamb
       if constexpr (S::kSafetyIsDistance) t.carry = t.carry - R(l) - m;
       crossed = false;
     } else {
-      crossed = (distance_first && !carried) ? step_distance_first<S, R>(s, t, R(l)) : step<S, R>(s, t, R(l));
+      crossed = (distance_first && !carried && distance_first_ok<S, R>(s, t.p, t.in))
+                    ? step_distance_first<S, R>(s, t, R(l))
+                    : step<S, R>(s, t, R(l));
     }
     if (!crossed) {
       const V3<double> d = new_direction(seed, i, st, dbl(t.d));
