@@ -271,6 +271,7 @@ int vgm_ctx_destroy(vgm_ctx *c)
   (void)hipStreamSynchronize(c->stream);
   if (c->dev.buf) (void)hipFree(c->dev.buf);
   if (c->scratch.buf) (void)hipFree(c->scratch.buf);
+  if (c->nav_queue) (void)hipFree(c->nav_queue);
   mlp_release(c);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

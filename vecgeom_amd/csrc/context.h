@@ -50,6 +50,7 @@ struct vgm_ctx {
   vgm::HostGeometry host;
   vgm::DeviceGeometry dev;
   vgm::Scratch scratch;
+  unsigned long long *nav_queue = nullptr;  // work-queue counter of the navigation transports
   std::vector<vgm::MlpState> mlps;  // learned safety, indexed by shape id
 };
 

@@ -32,7 +32,11 @@ int run_nav(vgm_ctx *c, int kind, NavArgs &a, std::initializer_list<Arr> arrs, u
   a.g = GeomView{d.shapes, d.pool, d.lvs, d.daughters, d.pvs, d.nav_nodes, d.nav_children, d.bvhs,
                  d.nav_ipool, d.nav_bpool, c->host.world_pv};
   a.shape_mask = c->host.nav_shape_mask;
+  a.num_cus = c->num_cus;
   if (a.n == 0) return VGM_OK;
+  if ((kind == kNavTransport || kind == kNavTransportLimited) && !c->nav_queue)
+    VGM_HIP(hipMalloc(&c->nav_queue, 256));
+  a.queue = c->nav_queue;
   if (flags & VGM_DEVICE_PTRS) {
     for (const Arr &r : arrs) *r.slot = const_cast<void *>(r.user);
     VGM_HIP(launch_nav(c->stream, kind, a));

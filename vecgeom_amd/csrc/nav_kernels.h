@@ -31,6 +31,8 @@ struct NavArgs {
   int32_t *geo_steps;  // limited transport: steps that needed the boundary distance (accumulated)
   int32_t reuse_safety;  // limited transport: carry the safety left after a safety-only move
   int32_t no_relocate;   // step: ComputeStep, the out state without relocation
+  unsigned long long *queue;  // transports: work-queue counter (device, reset by the launch)
+  int32_t num_cus;            // transports: compute units of the device (persistent grid size)
   size_t n;
   int32_t nsteps;
   uint32_t shape_mask;  // families used by the navigation tree (bit per ShapeType)

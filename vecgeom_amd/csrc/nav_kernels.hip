@@ -4,6 +4,7 @@
 //              output: ComputeStepAndSafetyAndPropagatedState)
 //   transport  nsteps of it with an unlimited physics step, moving the global
 //              point by each step; a track that leaves the world stops
+//              (persistent grid over a work queue of tracks)
 //   safety     VSafetyEstimator::ComputeSafety (mother SafetyToOut, then the
 //              daughters through the BVH)
 // States and geometry tables live in HBM; a track's point, direction and
@@ -66,31 +67,70 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
   a.boundary[i] = out.boundary ? 1 : 0;
 }
 
+// Transports run as persistent grids over a work queue of tracks: a lane
+// whose track has taken its steps (or left the world) stores it and takes
+// the next one, so a wave stays full until the queue drains instead of
+// waiting for its longest track (in the barrel, tracks leave the world after
+// about a quarter of their 200 steps).  Each track's steps are the same as
+// one lane per track would take, so the results are identical.
+//
+// The next track index for every lane of the wave that asks: one atomic per
+// wave on the queue counter.
+__device__ __forceinline__ long long pull_track(unsigned long long *queue, bool need)
+{
+  const unsigned long long mask = __ballot(need ? 1 : 0);
+  if (!mask) return -1;
+  const int lane = int(__lane_id());
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(queue, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  const unsigned long long below = mask & ((1ull << lane) - 1ull);
+  return need ? (long long)(base + (unsigned long long)__popcll(below)) : -1;
+}
+
+__global__ void nav_queue_reset_kernel(unsigned long long *queue) { *queue = 0ull; }
+
 template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_transport_kernel(NavArgs a)
 {
-  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
-  if (i >= a.n) return;
   Navigator<Mask> nav{a.g};
-  NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
-  V3<double> p{a.x[i], a.y[i], a.z[i]};
-  const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
+  const long long n = (long long)a.n;
+  long long i = -1;
+  NavState s{0u, 0u, false};
+  V3<double> p{0.0, 0.0, 0.0}, d{0.0, 0.0, 0.0};
   int k = 0;
-  for (; k < a.nsteps && s.ind; ++k) {
-    NavState out;
-    const double st = nav.step(p, d, kInfLength, s, out);
-    p.x = p.x + st * d.x;
-    p.y = p.y + st * d.y;
-    p.z = p.z + st * d.z;
-    s = out;
+  for (;;) {
+    const bool need = i < 0;
+    const long long next = pull_track(a.queue, need);
+    if (need) {
+      if (next >= n) break;  // the queue is drained
+      i = next;
+      s = NavState{a.state[i], a.last[i], a.boundary[i] != 0};
+      p = V3<double>{a.x[i], a.y[i], a.z[i]};
+      d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
+      k = 0;
+    }
+    if (k < a.nsteps && s.ind) {
+      NavState out;
+      const double st = nav.step(p, d, kInfLength, s, out);
+      p.x = p.x + st * d.x;
+      p.y = p.y + st * d.y;
+      p.z = p.z + st * d.z;
+      s = out;
+      ++k;
+    }
+    if (k >= a.nsteps || !s.ind) {
+      a.x[i] = p.x;
+      a.y[i] = p.y;
+      a.z[i] = p.z;
+      a.state[i] = s.ind;
+      a.last[i] = s.last;
+      a.boundary[i] = s.boundary ? 1 : 0;
+      a.steps[i] += k;
+      i = -1;
+    }
   }
-  a.x[i] = p.x;
-  a.y[i] = p.y;
-  a.z[i] = p.z;
-  a.state[i] = s.ind;
-  a.last[i] = s.last;
-  a.boundary[i] = s.boundary ? 1 : 0;
-  a.steps[i] += k;
 }
 
 // Transport under a physics step limit L per track: each step is
@@ -104,41 +144,58 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
 template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_transport_limited_kernel(NavArgs a)
 {
-  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
-  if (i >= a.n) return;
   Navigator<Mask> nav{a.g};
-  NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
-  V3<double> p{a.x[i], a.y[i], a.z[i]};
-  const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
-  const double L = a.step_limit[i];
-  double saf = -1.0;
+  const long long n = (long long)a.n;
+  long long i = -1;
+  NavState s{0u, 0u, false};
+  V3<double> p{0.0, 0.0, 0.0}, d{0.0, 0.0, 0.0};
+  double L = 0.0, saf = -1.0;
   int k = 0, geo = 0;
-  for (; k < a.nsteps && s.ind; ++k) {
-    if (!(a.reuse_safety && saf >= L)) saf = nav.safety(p, s.ind);
-    double st;
-    if (saf >= L) {
-      st = L;
-      s.boundary = false;
-      saf = saf - L;
-    } else {
-      NavState out;
-      st = nav.step(p, d, L, s, out);
-      s = out;
+  for (;;) {
+    const bool need = i < 0;
+    const long long next = pull_track(a.queue, need);
+    if (need) {
+      if (next >= n) break;  // the queue is drained
+      i = next;
+      s = NavState{a.state[i], a.last[i], a.boundary[i] != 0};
+      p = V3<double>{a.x[i], a.y[i], a.z[i]};
+      d = V3<double>{a.dx[i], a.dy[i], a.dz[i]};
+      L = a.step_limit[i];
       saf = -1.0;
-      ++geo;
+      k = 0;
+      geo = 0;
     }
-    p.x = p.x + st * d.x;
-    p.y = p.y + st * d.y;
-    p.z = p.z + st * d.z;
+    if (k < a.nsteps && s.ind) {
+      if (!(a.reuse_safety && saf >= L)) saf = nav.safety(p, s.ind);
+      double st;
+      if (saf >= L) {
+        st = L;
+        s.boundary = false;
+        saf = saf - L;
+      } else {
+        NavState out;
+        st = nav.step(p, d, L, s, out);
+        s = out;
+        saf = -1.0;
+        ++geo;
+      }
+      p.x = p.x + st * d.x;
+      p.y = p.y + st * d.y;
+      p.z = p.z + st * d.z;
+      ++k;
+    }
+    if (k >= a.nsteps || !s.ind) {
+      a.x[i] = p.x;
+      a.y[i] = p.y;
+      a.z[i] = p.z;
+      a.state[i] = s.ind;
+      a.last[i] = s.last;
+      a.boundary[i] = s.boundary ? 1 : 0;
+      a.steps[i] += k;
+      a.geo_steps[i] += geo;
+      i = -1;
+    }
   }
-  a.x[i] = p.x;
-  a.y[i] = p.y;
-  a.z[i] = p.z;
-  a.state[i] = s.ind;
-  a.last[i] = s.last;
-  a.boundary[i] = s.boundary ? 1 : 0;
-  a.steps[i] += k;
-  a.geo_steps[i] += geo;
 }
 
 template <unsigned Mask>
@@ -161,6 +218,14 @@ __global__ __launch_bounds__(kNavBlock) void nav_safety_kernel(NavArgs a)
 }
 
 namespace {
+// Workgroups of a persistent transport: enough to keep every SIMD of the
+// device at the kernels' occupancy, and no more than the tracks need.
+unsigned persistent_blocks(const NavArgs &a, unsigned blocks)
+{
+  const unsigned full = unsigned(a.num_cus > 0 ? a.num_cus : 256) * 16u;
+  return blocks < full ? blocks : full;
+}
+
 template <unsigned Mask>
 void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned blocks, bool &ok)
 {
@@ -169,14 +234,17 @@ void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned bl
   case kNavLocate: hipLaunchKernelGGL(nav_locate_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
   case kNavStep: hipLaunchKernelGGL(nav_step_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
   case kNavTransport:
-    hipLaunchKernelGGL(nav_transport_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    hipLaunchKernelGGL(nav_queue_reset_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+    hipLaunchKernelGGL(nav_transport_kernel<Mask>, dim3(persistent_blocks(a, blocks)), dim3(kNavBlock), 0, stream, a);
     break;
   case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
   case kNavApproach:
     hipLaunchKernelGGL(nav_approach_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
     break;
   case kNavTransportLimited:
-    hipLaunchKernelGGL(nav_transport_limited_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    hipLaunchKernelGGL(nav_queue_reset_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+    hipLaunchKernelGGL(nav_transport_limited_kernel<Mask>, dim3(persistent_blocks(a, blocks)), dim3(kNavBlock), 0,
+                       stream, a);
     break;
   default: ok = false;
   }
