@@ -317,27 +317,79 @@ def run_navigation(args, ctx, dev, torch):
     log(f"[bench] navigation: {nav.nnodes} states, {n} tracks located ({time.perf_counter() - t0:.1f}s)")
     stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
 
-    def one(timed):
-        x, y, z = (a.clone() for a in P)
-        st = st0.clone()
-        last = torch.zeros(n, dtype=torch.int32, device=dev)
-        bnd = torch.zeros(n, dtype=torch.uint8, device=dev)
-        steps = torch.zeros(n, dtype=torch.int32, device=dev)
+    # the same buffers for every run, so that a captured graph replays
+    bx, by, bz = (a.clone() for a in P)
+    bst = st0.clone()
+    blast = torch.zeros(n, dtype=torch.int32, device=dev)
+    bbnd = torch.zeros(n, dtype=torch.uint8, device=dev)
+    bsteps = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    def one(**mode):
+        for b_, a_ in zip((bx, by, bz, bst), (*P, st0)):
+            b_.copy_(a_)
+        for b_ in (blast, bbnd, bsteps):
+            b_.zero_()
         torch.cuda.synchronize()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t = time.perf_counter()
         a.record(stream)
-        nav.transport(x, y, z, *D, st, last, bnd, args.nav_steps, steps, asynchronous=True)
+        nav.transport(bx, by, bz, *D, bst, blast, bbnd, args.nav_steps, bsteps, asynchronous=True, **mode)
         b.record(stream)
         torch.cuda.synchronize()
-        return time.perf_counter() - t, a.elapsed_time(b) * 1e-3, int(steps.sum().item())
+        return time.perf_counter() - t, a.elapsed_time(b) * 1e-3, int(bsteps.sum().item())
 
-    one(False)
-    wall, kern, total = one(True)
+    one()
+    wall, kern, total = one()
     out = {"workload": f"barrel: 20 tube layers, {sum(l['n'] for l in workloads.barrel_layers())} box modules, "
                        f"{n} tracks x up to {args.nav_steps} steps", "value": total / kern, "unit": "track-steps/s",
-           "track_steps": total, "kernel_ms": kern * 1e3, "wall_ms": wall * 1e3}
+           "track_steps": total, "kernel_ms": kern * 1e3, "wall_ms": wall * 1e3,
+           "kernel": "nav_transport_kernel (persistent grid over a work queue of tracks)"}
     log(f"[bench] navigation: {total / kern / 1e9:.3f} G track-steps/s ({total} steps, kernel {kern * 1e3:.1f} ms)")
+    # the step loop as one-step kernels: launched one by one, and captured in
+    # a HIP graph (the first call captures it, the timed call replays it)
+    one(step_kernels=True)
+    w_k, k_k, t_k = one(step_kernels=True)
+    t0 = time.perf_counter()
+    one(graph=True)
+    capture_s = time.perf_counter() - t0
+    w_g, k_g, t_g = one(graph=True)
+    assert t_k == total and t_g == total
+    out["step_kernels"] = {"value": t_k / k_k, "kernel_ms": k_k * 1e3, "wall_ms": w_k * 1e3,
+                           "launches": args.nav_steps}
+    out["graph"] = {"value": t_g / k_g, "kernel_ms": k_g * 1e3, "wall_ms": w_g * 1e3,
+                    "capture_and_first_run_s": capture_s, "launch_overhead_saved_ms": (w_k - w_g) * 1e3,
+                    "note": f"the same {args.nav_steps} one-step kernels as step_kernels, replayed from a captured "
+                            "HIP graph; overhead saved = wall time launched one by one - wall time of the replay"}
+    log(f"[bench] navigation step loop: one-step kernels {t_k / k_k / 1e9:.3f} G/s (wall {w_k * 1e3:.1f} ms), "
+        f"graph {t_g / k_g / 1e9:.3f} G/s (wall {w_g * 1e3:.1f} ms, capture {capture_s:.2f} s)")
+    # the same loops on a few tracks, where each launch is short and its
+    # fixed cost shows
+    m = min(n, 20_000)
+    sx, sy, sz = (a[:m].clone() for a in P)
+    sd = [a[:m].contiguous() for a in D]
+    sst = st0[:m].clone()
+    sl = torch.zeros(m, dtype=torch.int32, device=dev)
+    sb = torch.zeros(m, dtype=torch.uint8, device=dev)
+    ss = torch.zeros(m, dtype=torch.int32, device=dev)
+
+    def small(**mode):
+        for b_, a_ in zip((sx, sy, sz, sst), (P[0][:m], P[1][:m], P[2][:m], st0[:m])):
+            b_.copy_(a_)
+        for b_ in (sl, sb, ss):
+            b_.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        nav.transport(sx, sy, sz, *sd, sst, sl, sb, args.nav_steps, ss, asynchronous=True, **mode)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    small(step_kernels=True)
+    small(graph=True)
+    wk = min(small(step_kernels=True) for _ in range(3))
+    wg = min(small(graph=True) for _ in range(3))
+    out["graph_small"] = {"tracks": m, "launches": args.nav_steps, "step_kernels_wall_ms": wk * 1e3,
+                          "graph_wall_ms": wg * 1e3, "launch_overhead_saved_ms": (wk - wg) * 1e3}
+    log(f"[bench] navigation step loop, {m} tracks: one-step kernels {wk * 1e3:.2f} ms, graph {wg * 1e3:.2f} ms")
 
     # the same tracks under a 10 mm physics step: the safety first, a boundary
     # distance only when the safety does not cover the step
@@ -502,6 +554,25 @@ def main():
         kern = float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
         fam_ms = {f: float(np.mean([m[f] for m in fms])) for f in fms[0]}
         extra = {}
+        if learned and not tuned:
+            # the same launches captured in a HIP graph (the first captures it,
+            # the rest replay it, step0 advancing on the device)
+            gs = [x.clone() for x in state]
+            gside, gcr = side.clone(), cr.clone()
+            g0 = step0
+            batch.transport(*gs, gside, gcr, S, step0=g0, seed=7, graph=True, **kw)
+            g0 += S
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                batch.transport(*gs, gside, gcr, S, step0=g0, seed=7, graph=True, asynchronous=True, **kw)
+                g0 += S
+            torch.cuda.synchronize()
+            wg = time.perf_counter() - t
+            extra["graph"] = {"ms_per_launch": wg / args.steps * 1e3, "ms_per_launch_uncaptured": wall / args.steps * 1e3,
+                              "launch_overhead_saved_ms": (wall - wg) / args.steps * 1e3,
+                              "note": "the learned families run two kernels per track-step; captured once, replayed"}
+            log(f"[bench] {tag}: graph-captured launches {wg / args.steps * 1e3:.2f} ms vs {wall / args.steps * 1e3:.2f} ms")
         if mixed and not learned:
             st2 = [x.clone() for x in init]
             s2, c2 = side0.clone(), torch.zeros(n, dtype=torch.int32, device=dev)
@@ -582,7 +653,7 @@ def main():
             "audit_sample": f"first launch ({S} track-steps of all {n} tracks), every served bound checked",
             "tessellated_tracks_differing_from_fp64": ndiff("learned", family_mask(lambda k: k == "tessellated")),
             "value": rate("learned"), "kernel_ms": res["learned"][1] * 1e3, "family_kernel_ms": res["learned"][4],
-            "launch": res["learned"][3]})
+            "launch": res["learned"][3], "graph": res["learned"][2].get("graph")})
         log(f"[bench] learned: audit {au}, tessellated tracks differing from fp64: "
             f"{learned['tessellated_tracks_differing_from_fp64']}")
     if "tuned" in res:

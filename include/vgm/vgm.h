@@ -38,6 +38,10 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
+#define VGM_GRAPH 512u /* transports, with VGM_DEVICE_PTRS: the step loop as a captured HIP graph of one-step \
+                          kernels, replayed while the arguments stay the same */
+#define VGM_STEP_KERNELS 1024u /* vgm_nav_transport: the step loop as one-step kernels launched one by one \
+                                  (what VGM_GRAPH captures) */
 
 /* Status codes. */
 #define VGM_OK 0
@@ -264,7 +268,12 @@ int vgm_nav_step_safety(vgm_ctx *ctx, const double *x, const double *y, const do
 /* nsteps straight-line steps per track (no physics limit): each step moves
  * the global point by the geometric step and takes the propagated state; a
  * track that leaves the world stops.  steps[i] is incremented by the number
- * of steps taken. */
+ * of steps taken.  By default one persistent kernel runs all steps over a
+ * work queue of tracks; with VGM_GRAPH (device pointers only) the loop is
+ * nsteps launches of a one-step kernel, captured once into a HIP graph and
+ * replayed by later calls with the same arrays, sizes and geometry;
+ * VGM_STEP_KERNELS launches those kernels one by one.  All three give the
+ * same results. */
 int vgm_nav_transport(vgm_ctx *ctx, double *x, double *y, double *z, const double *dx, const double *dy,
                       const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
                       int32_t *steps, unsigned flags);
@@ -321,7 +330,11 @@ int vgm_batch_destroy(vgm_batch *batch);
 int vgm_batch_classify(vgm_batch *batch, const double *x, const double *y, const double *z, int8_t *side);
 /* Advances every track by nsteps track-steps; crossings[i] is
  * incremented per boundary crossed.  fp64_steps (host, may be NULL) receives the
- * number of steps a VGM_MIXED run redid in FP64. */
+ * number of steps of a VGM_MIXED run that the float safety did not settle.
+ * With VGM_GRAPH the whole call (for learned families, two kernels per
+ * track-step) is captured into a HIP graph and replayed by later calls with
+ * the same arrays, step count, seed, flags and launch settings; step0 may
+ * change between replays.  Not with VGM_TIMING. */
 int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, double *dx, double *dy, double *dz,
                         int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
                         unsigned flags, unsigned long long *fp64_steps);

@@ -164,6 +164,45 @@ def test_learned_transport_bit_exact_and_ablation(ctx):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["learned", "fp64"])
+def test_graph_captured_transport(ctx, mode):
+    """VGM_GRAPH: the transport call (learned families: two kernels per
+    track-step) captured once and replayed.  Four launches of 10 track-steps,
+    step0 advancing; eager library and torch work with synchronisations
+    between the replays; after every replay the state equals the eager
+    (uncaptured) run of the same launches bit for bit."""
+    import torch
+
+    print(f"[learned] graph-captured transport mode={mode}", flush=True)
+    b, init, side0, tess, n = _batch(ctx, 200_000, seed=17)
+    kw = {}
+    if mode == "learned":
+        for s in tess:
+            ctx.mlp_train(s, samples=500_000, epochs=1, validation=200_000)
+        kw = {"learned": True}
+    runs = []
+    for graph in (False, True):
+        st = [t.clone() for t in init]
+        side = side0.clone()
+        cr = torch.zeros(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        snaps = []
+        for k in range(4):
+            b.transport(*st, side, cr, 10, step0=10 * k, seed=5, graph=graph, **kw)
+            # eager work between replays
+            tmp = torch.zeros(n, dtype=torch.int8, device="cuda")
+            b.classify(st[0], st[1], st[2], tmp)
+            assert int(tmp.sum().item()) >= 0
+            torch.cuda.synchronize()
+            snaps.append([x.cpu().numpy().copy() for x in st + [side, cr]])
+        runs.append(snaps)
+    for k, (e, g) in enumerate(zip(*runs)):
+        for j, (a, c) in enumerate(zip(e, g)):
+            bad = np.flatnonzero(a != c)
+            assert bad.size == 0, (k, j, bad.size)
+    b.close()
+
+
 def test_save_load_roundtrip(ctx):
     s = ctx.box(2.0, 3.0, 4.0)
     p = mlp.random_params(9).astype(np.float32)

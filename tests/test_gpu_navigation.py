@@ -168,6 +168,51 @@ def test_transport_200_steps(setup, device_ptrs):
     assert n_o.sum() > 20 * N
 
 
+def test_graph_captured_step_loop(setup):
+    """The step loop as one-step kernels launched one by one and as a
+    captured HIP graph: both equal the persistent transport and the oracle bit
+    for bit.  The graph is replayed three times on the same arrays (each
+    replay continues the transport by 60 steps), with eager library and torch
+    work and synchronisations between the replays, and every replay is
+    checked against the eager one-kernel transport of the same steps."""
+    print("[nav] graph-captured step loop", flush=True)
+    import torch
+
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+
+    def fresh():
+        a = [a.copy() for a in pts] + [a.copy() for a in dirs] + [st.copy(), np.zeros(N, np.uint32),
+                                                                  np.zeros(N, np.uint8), np.zeros(N, np.int32)]
+        out = [torch.from_numpy(x).cuda() for x in a]
+        torch.cuda.synchronize()
+        return out
+
+    def state(arrs):
+        return [x.cpu().numpy().copy() for x in arrs]
+
+    eager, graph, kern = fresh(), fresh(), fresh()
+    ref = [a.copy() for a in pts]
+    s_o, l_o, b_o, n_o = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8), np.zeros(N, np.int32)
+    for rep in range(3):
+        nav.transport(*eager[:6], eager[6], eager[7], eager[8], 60, eager[9])
+        nav.transport(*graph[:6], graph[6], graph[7], graph[8], 60, graph[9], graph=True)
+        nav.transport(*kern[:6], kern[6], kern[7], kern[8], 60, kern[9], step_kernels=True)
+        geo.nav_transport(*ref, *dirs, s_o, l_o, b_o, 60, n_o)
+        # eager work between replays: a locate and a torch reduction on other arrays
+        tmp = torch.zeros(N, dtype=torch.int32, device="cuda")
+        nav.locate(*[torch.from_numpy(a).cuda() for a in pts], tmp)
+        assert int((tmp > 0).sum().item()) > 0
+        torch.cuda.synchronize()
+        e, g, k = state(eager), state(graph), state(kern)
+        for j in range(10):
+            assert np.array_equal(g[j], e[j]), (rep, j)
+            assert np.array_equal(k[j], e[j]), (rep, j)
+        for j in range(3):
+            assert np.array_equal(e[j], ref[j]), (rep, j)
+        assert np.array_equal(e[6], s_o) and np.array_equal(e[9], n_o), rep
+
+
 def test_approach_next_daughter(setup):
     print("[nav] approach", flush=True)
     ctx, nav, geo, pts, dirs = setup

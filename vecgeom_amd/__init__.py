@@ -30,6 +30,8 @@ AUDIT = 32
 DISTANCE_FIRST = 64
 TUNED = 128
 CULL = 256
+GRAPH = 512
+STEP_KERNELS = 1024
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
 SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean")
@@ -511,10 +513,14 @@ class Navigation:
         check(self._L.vgm_nav_state_to_path(self.ctx.handle, int(state), _ptr(buf), len(buf), ctypes.byref(d)))
         return [int(v) for v in buf[:d.value]]
 
-    def transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, asynchronous=False):
+    def transport(self, x, y, z, dx, dy, dz, state, last, boundary, nsteps, steps, asynchronous=False,
+                  graph=False, step_kernels=False):
+        """nsteps straight-line steps per track: one persistent kernel (default),
+        a captured HIP graph of one-step kernels (graph=True) or those kernels
+        launched one by one (step_kernels=True); device arrays for the last two."""
+        fl = self._fl(x, asynchronous) | (GRAPH if graph else 0) | (STEP_KERNELS if step_kernels else 0)
         check(self._L.vgm_nav_transport(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
-                                        len(x), _ptr(state), _ptr(last), _ptr(boundary), nsteps, _ptr(steps),
-                                        self._fl(x, asynchronous)))
+                                        len(x), _ptr(state), _ptr(last), _ptr(boundary), nsteps, _ptr(steps), fl))
 
     def transport_limited(self, x, y, z, dx, dy, dz, step_limit, state, last, boundary, nsteps, steps, geo_steps,
                           reuse_safety=False):
@@ -569,8 +575,9 @@ class Batch:
 
     def transport(self, x, y, z, dx, dy, dz, side, crossings, nsteps, step0=0, seed=1, mixed=False,
                   asynchronous=False, count_fp64=False, timing=False, learned=False, audit=False,
-                  distance_first=False, tuned=False, cull=False):
+                  distance_first=False, tuned=False, cull=False, graph=False):
         flags = (MIXED if mixed else 0) | (ASYNC if asynchronous else 0) | (TIMING if timing else 0)
+        flags |= GRAPH if graph else 0
         flags |= (LEARNED if learned else 0) | (AUDIT if audit else 0) | (DISTANCE_FIRST if distance_first else 0)
         flags |= (TUNED if tuned else 0) | (CULL if cull else 0)
         cnt = ctypes.c_ulonglong(0)

@@ -52,6 +52,12 @@ struct vgm_batch {
   uint8_t *d_tflags = nullptr;
   uint64_t *d_queue = nullptr;
   int fam_mode[kNumShapeTypes];  // FamilyMode chosen by the autotuner (VGM_TUNED), -1 if none
+  // VGM_GRAPH: the captured transport of the last call and what it was
+  // captured with; the first step index lives on the device
+  int32_t *d_step0 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<char> graph_key;
   vgm_batch()
   {
     for (int &m : fam_mode) m = -1;
@@ -351,6 +357,9 @@ int vgm_batch_destroy(vgm_batch *b)
 {
   if (!b) return VGM_OK;
   (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_step0) (void)hipFree(b->d_step0);
   if (b->dbuf) (void)hipFree(b->dbuf);
   if (b->lbuf) (void)hipFree(b->lbuf);
   for (int i = 0; i < kFamilyStreams; ++i) {
@@ -469,6 +478,64 @@ int vgm_batch_classify(vgm_batch *b, const double *x, const double *y, const dou
   return VGM_OK;
 }
 
+namespace {
+// VGM_GRAPH: the whole transport call (counter reset, the families' kernels
+// on their streams, and for learned families the per-step kernel pairs)
+// captured once into a HIP graph on the context stream and replayed while
+// the arrays, step count, seed, flags, launch shape, streams, modes and
+// uploaded tables stay the same.  The first step index is the one argument
+// that changes from call to call: the kernels read it from the device, and a
+// one-thread kernel writes it before each replay.
+int graph_transport(vgm_batch *b, unsigned flags, TransportArgs a, int step0)
+{
+  hipStream_t main = b->ctx->stream;
+  if (!b->d_step0) VGM_HIP(hipMalloc(&b->d_step0, 256));
+  a.step0_dev = b->d_step0;
+  a.step0 = 0;
+  const LaunchShape ls = resolved(b);
+  // everything the captured launches depend on, byte for byte
+  std::vector<char> key;
+  auto put = [&key](const void *p, size_t n) {
+    const char *c = static_cast<const char *>(p);
+    key.insert(key.end(), c, c + n);
+  };
+  put(&a, sizeof a);
+  const unsigned f = flags & ~(VGM_GRAPH | VGM_ASYNC);
+  put(&f, sizeof f);
+  put(&ls, sizeof ls);
+  put(&b->streams, sizeof b->streams);
+  put(b->fam_mode, sizeof b->fam_mode);
+  put(b->weight, sizeof b->weight);
+  VGM_HIP(launch_set_step0(main, b->d_step0, step0));
+  if (!b->graph_exec || key != b->graph_key) {
+    if (b->graph_exec) VGM_HIP(hipGraphExecDestroy(b->graph_exec));
+    if (b->graph) VGM_HIP(hipGraphDestroy(b->graph));
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->graph_key.clear();
+    VGM_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = launch_families(b, flags, a, ls, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(main, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e != hipSuccess ? e : e2, "capturing the batched transport");
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return hip_fail(e3, "hipGraphInstantiate");
+    }
+    b->graph = g;
+    b->graph_exec = x;
+    b->graph_key = key;
+  }
+  VGM_HIP(hipGraphLaunch(b->graph_exec, main));
+  return VGM_OK;
+}
+}  // namespace
+
 int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *dx, double *dy, double *dz,
                         int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
                         unsigned flags, unsigned long long *fp64_steps)
@@ -480,7 +547,12 @@ int vgm_batch_transport(vgm_batch *b, double *x, double *y, double *z, double *d
   if (rc) return rc;
   TransportArgs a = make_args(b, x, y, z, dx, dy, dz, side, crossings, nsteps, step0, seed, flags);
   if ((rc = ensure_streams(b))) return rc;
-  VGM_HIP(launch_families(b, flags, a, resolved(b), (flags & VGM_TIMING) != 0));
+  if (flags & VGM_GRAPH) {
+    if (flags & VGM_TIMING) return fail(VGM_EINVAL, "VGM_GRAPH does not time the family kernels (VGM_TIMING)");
+    if ((rc = graph_transport(b, flags, a, step0))) return rc;
+  } else {
+    VGM_HIP(launch_families(b, flags, a, resolved(b), (flags & VGM_TIMING) != 0));
+  }
   if (fp64_steps) {
     VGM_HIP(hipMemcpyAsync(fp64_steps, b->d_fp64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->ctx->stream));

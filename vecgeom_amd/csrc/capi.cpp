@@ -272,6 +272,8 @@ int vgm_ctx_destroy(vgm_ctx *c)
   if (c->dev.buf) (void)hipFree(c->dev.buf);
   if (c->scratch.buf) (void)hipFree(c->scratch.buf);
   if (c->nav_queue) (void)hipFree(c->nav_queue);
+  if (c->nav_graph.exec) (void)hipGraphExecDestroy(c->nav_graph.exec);
+  if (c->nav_graph.graph) (void)hipGraphDestroy(c->nav_graph.graph);
   mlp_release(c);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -28,6 +28,18 @@ struct DeviceGeometry {
   double *nav_bpool = nullptr;
 };
 
+// The graph-captured navigation step loop of the last VGM_GRAPH transport
+// and the arguments it was captured with: replayed while they stay the same.
+struct NavGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *ptrs[10] = {};
+  size_t n = 0;
+  int nsteps = 0;
+  uint64_t version = 0;
+  unsigned mask = 0;
+};
+
 // Growable device scratch for host-pointer queries.
 struct Scratch {
   char *buf = nullptr;
@@ -51,6 +63,7 @@ struct vgm_ctx {
   vgm::DeviceGeometry dev;
   vgm::Scratch scratch;
   unsigned long long *nav_queue = nullptr;  // work-queue counter of the navigation transports
+  vgm::NavGraph nav_graph;
   std::vector<vgm::MlpState> mlps;  // learned safety, indexed by shape id
 };
 

@@ -19,6 +19,46 @@ struct Arr {
   bool optional = false;  // may be null
 };
 
+// The straight-line transport as nsteps launches of the one-step kernel,
+// captured into a HIP graph on the context stream and replayed; captured
+// again when the arrays, sizes, step count or geometry change.
+int nav_graph_transport(vgm_ctx *c, const NavArgs &a)
+{
+  NavGraph &G = c->nav_graph;
+  const void *ptrs[10] = {a.x, a.y, a.z, a.dx, a.dy, a.dz, a.state, a.last, a.boundary, a.steps};
+  bool same = G.exec && G.n == a.n && G.nsteps == a.nsteps && G.version == c->dev.version && G.mask == a.shape_mask;
+  for (int k = 0; k < 10 && same; ++k) same = G.ptrs[k] == ptrs[k];
+  if (!same) {
+    if (G.exec) VGM_HIP(hipGraphExecDestroy(G.exec));
+    if (G.graph) VGM_HIP(hipGraphDestroy(G.graph));
+    G = NavGraph{};
+    VGM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < a.nsteps && e == hipSuccess; ++k) e = launch_nav(c->stream, kNavStepMove, a);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return hip_fail(e != hipSuccess ? e : e2, "capturing the navigation step loop");
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+    G.graph = graph;
+    G.exec = exec;
+    for (int k = 0; k < 10; ++k) G.ptrs[k] = ptrs[k];
+    G.n = a.n;
+    G.nsteps = a.nsteps;
+    G.version = c->dev.version;
+    G.mask = a.shape_mask;
+  }
+  VGM_HIP(hipGraphLaunch(G.exec, c->stream));
+  return VGM_OK;
+}
+
 int run_nav(vgm_ctx *c, int kind, NavArgs &a, std::initializer_list<Arr> arrs, unsigned flags)
 {
   if (!c) return fail(VGM_EINVAL, "null context");
@@ -39,10 +79,22 @@ int run_nav(vgm_ctx *c, int kind, NavArgs &a, std::initializer_list<Arr> arrs, u
   a.queue = c->nav_queue;
   if (flags & VGM_DEVICE_PTRS) {
     for (const Arr &r : arrs) *r.slot = const_cast<void *>(r.user);
+    if ((flags & VGM_GRAPH) && kind == kNavTransport) {
+      if (a.nsteps > 0 && (rc = nav_graph_transport(c, a))) return rc;
+      if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
+      return VGM_OK;
+    }
+    if ((flags & VGM_STEP_KERNELS) && kind == kNavTransport) {
+      for (int k = 0; k < a.nsteps; ++k) VGM_HIP(launch_nav(c->stream, kNavStepMove, a));
+      if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
+      return VGM_OK;
+    }
     VGM_HIP(launch_nav(c->stream, kind, a));
     if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
     return VGM_OK;
   }
+  if (flags & (VGM_GRAPH | VGM_STEP_KERNELS))
+    return fail(VGM_EINVAL, "VGM_GRAPH and VGM_STEP_KERNELS take device arrays (VGM_DEVICE_PTRS)");
   size_t total = 0;
   for (const Arr &r : arrs)
     if (r.user) total += align256(r.bytes);

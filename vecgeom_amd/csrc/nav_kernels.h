@@ -15,6 +15,7 @@ enum NavKind : int32_t {
   kNavSafety = 3,
   kNavApproach = 4,           // ApproachNextDaughter per track
   kNavTransportLimited = 5,   // transport under a physics step, safety first
+  kNavStepMove = 6,           // one straight-line transport step (graph-captured loops)
 };
 
 struct NavArgs {

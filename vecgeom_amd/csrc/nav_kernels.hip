@@ -133,6 +133,33 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavW
   }
 }
 
+// One step of the straight-line transport for every track still in the
+// world: the body of the step loop as a kernel of its own, for the
+// graph-captured multi-kernel loop (one launch per step; the state goes
+// through HBM between steps).
+template <unsigned Mask>
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_step_move_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n || !a.state[i]) return;
+  Navigator<Mask> nav{a.g};
+  const NavState s{a.state[i], a.last[i], a.boundary[i] != 0};
+  V3<double> p{a.x[i], a.y[i], a.z[i]};
+  const V3<double> d{a.dx[i], a.dy[i], a.dz[i]};
+  NavState out;
+  const double st = nav.step(p, d, kInfLength, s, out);
+  p.x = p.x + st * d.x;
+  p.y = p.y + st * d.y;
+  p.z = p.z + st * d.z;
+  a.x[i] = p.x;
+  a.y[i] = p.y;
+  a.z[i] = p.z;
+  a.state[i] = out.ind;
+  a.last[i] = out.last;
+  a.boundary[i] = out.boundary ? 1 : 0;
+  a.steps[i] += 1;
+}
+
 // Transport under a physics step limit L per track: each step is
 // ComputeStepAndSafetyAndPropagatedState with calcsafety
 // (VecGeom/navigation/VNavigator.h:543-592): the safety first, and when it
@@ -240,6 +267,9 @@ void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned bl
   case kNavSafety: hipLaunchKernelGGL(nav_safety_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a); break;
   case kNavApproach:
     hipLaunchKernelGGL(nav_approach_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    break;
+  case kNavStepMove:
+    hipLaunchKernelGGL(nav_step_move_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
     break;
   case kNavTransportLimited:
     hipLaunchKernelGGL(nav_queue_reset_kernel, dim3(1), dim3(1), 0, stream, a.queue);

@@ -40,6 +40,9 @@ struct TransportArgs {
   int32_t nsteps;
   int32_t step0;  // index of the first step (RNG counter)
   int32_t pad;
+  // when set, the index of the first step is read here instead (graph
+  // replays advance it without re-capturing)
+  const int32_t *step0_dev;
   uint64_t seed;
   double *x, *y, *z;
   double *dx, *dy, *dz;
@@ -108,6 +111,8 @@ struct FamilyLaunch {
 //                  bounding box skip the queries
 enum FamilyMode : int { kFamFP64 = 0, kFamMixed = 1, kFamDistFirst = 2, kFamLearned = 3, kFamCulled = 4 };
 
+// Writes the first step index of a graph replay (TransportArgs::step0_dev).
+hipError_t launch_set_step0(hipStream_t stream, int32_t *dst, int32_t v);
 hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps,
                         unsigned long long *learned);
 // True when the family can serve learned bounds (its safety is a distance).

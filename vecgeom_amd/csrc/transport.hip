@@ -28,6 +28,9 @@ namespace vgm {
 
 namespace {
 
+// Index of the first step of this launch (the RNG counter of step 0).
+VGM_DEV int32_t first_step(const TransportArgs &a) { return a.step0_dev ? *a.step0_dev : a.step0; }
+
 struct TrackState {
   V3<double> p, d;
   bool in;
@@ -282,7 +285,7 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
     };
     int k = 0;
     while (k < a.nsteps) {
-      uint32_t step = uint32_t(a.step0 + k);
+      uint32_t step = uint32_t(first_step(a) + k);
       double l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
       // Steps that need no geometry query (under the carried bound, or clear
       // of the bounding box).  Each lane runs through its own run of them
@@ -296,7 +299,7 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
         if constexpr (S::kSafetyIsDistance) t.carry = t.carry - l - m;
         finish(step, false);
         if (++k >= a.nsteps) break;
-        step = uint32_t(a.step0 + k);
+        step = uint32_t(first_step(a) + k);
         l = physics_step(sr.lambda, a.seed, uint64_t(i), step);
       }
       if (k >= a.nsteps) break;
@@ -450,7 +453,7 @@ __global__ __launch_bounds__(kTransportMaxBlock) void learned_bound_kernel(Trans
   const bool net = sr.mlp >= 0;
   MlpDesc md{};
   if (net) md = a.mlps[sr.mlp];
-  const uint32_t step = uint32_t(a.step0 + k);
+  const uint32_t step = uint32_t(first_step(a) + k);
   const int lane = int(threadIdx.x & 63u);
   LearnedCounts cnt;
   for (int64_t base = tl.begin + int64_t(threadIdx.x & ~63u); base < tl.end; base += blockDim.x) {
@@ -536,7 +539,7 @@ learned_fallback_kernel(TransportArgs a, int k)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) a.qcount[(k + 1) & 1] = 0;
   const int nq = a.qcount[k & 1];
-  const uint32_t step = uint32_t(a.step0 + k);
+  const uint32_t step = uint32_t(first_step(a) + k);
   LearnedCounts cnt;
   for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < nq; j += int(gridDim.x * blockDim.x)) {
     const uint64_t e = a.queue[j];
@@ -699,6 +702,14 @@ bool family_learnable(int type)
 {
   LearnableQuery q;
   return dispatch_shape_type(type, q) && q.r;
+}
+
+__global__ void set_step0_kernel(int32_t *dst, int32_t v) { *dst = v; }
+
+hipError_t launch_set_step0(hipStream_t stream, int32_t *dst, int32_t v)
+{
+  hipLaunchKernelGGL(set_step0_kernel, dim3(1), dim3(1), 0, stream, dst, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long long *fp64_steps,
