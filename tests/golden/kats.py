@@ -394,6 +394,27 @@ TRD_KATS = [
      unit((-0.76165597579890043, 0.64364445891356026, -0.074515708658524193)), ("eq", 0.0)),
 ]
 
+# ---- Cone: the second test of test/unit_tests/TestCons.cpp (:996-1087), hard asserts
+# t1 = tube-like cone (0, 50, 0, 50, 50, 0, 2 pi); test10 (:1009)
+CONE_T1 = (0.0, 50.0, 0.0, 50.0, 50.0, 0.0, 2.0 * PI)
+CONE_TEST10 = (20.0, 80.0, 60.0, 140.0, 100.0, 0.17453292519943, 5.235987755983)
+CONE_T1_KATS = [
+    (CONE_T1, "Inside", (0.0, 0.0, 0.0), None, ("inside", 1)),
+    (CONE_T1, "Inside", (100.0, 0, 0), None, ("inside", 3)),
+    (CONE_T1, "SafetyToOut", (0.0, 0.0, 0.0), None, ("eq", 50)),
+] + [(CONE_T1, "DistanceToOut", (0.0, 0.0, 0.0), d, ("eq", 50)) for d in (VX, VMX, VY, VMY, VZ, VMZ, VXY)] + [
+    (CONE_T1, "SafetyToIn", p, None, ("eq", 50)) for p in ((100.0, 0, 0), (-100.0, 0, 0), (0, 100.0, 0),
+                                                            (0, -100.0, 0), (0, 0, 100.0), (0, 0, -100.0))] + [
+    (CONE_T1, "DistanceToIn", p, d, ("eq", 50)) for p, d in (((100.0, 0, 0), VMX), ((-100.0, 0, 0), VX),
+                                                             ((0, 100.0, 0), VMY), ((0, -100.0, 0), VY),
+                                                             ((0, 0, 100.0), VMZ), ((0, 0, -100.0), VZ))] + [
+    (CONE_T1, "DistanceToIn", (100.0, 0, 0), VXY, ("inf",)),
+    # a point on (rmin, z = -100) moving out of test10: DistanceToOut is 0 (:1080-1084)
+    (CONE_TEST10, "DistanceToOut", (19.218716967888, 5.5354239324172, -100.0),
+     (-0.25644483536346, -0.073799216676426, 0.96373737191901), ("eq", 0)),
+]
+
+
 # ---- Trapezoid: test/unit_tests/TestTrap.cpp (box-like trap1) ---------------------
 TRAP1 = (40.0, 0.0, 0.0, 30.0, 20.0, 20.0, 0.0, 30.0, 20.0, 20.0, 0.0)
 TRAP_KATS = [
@@ -423,6 +444,67 @@ TRAP_KATS = [
     (TRAP1, "SafetyToIn", (0, 30.11, 0), None, ("eq", 0.11)),
     (TRAP1, "SafetyToOut", (0, 30.11, 0), None, ("le", 0.0)),
     (TRAP1, "SafetyToOut", (0, 29.89, 0), None, ("eq", 0.11)),
+]
+
+# TestTrap.cpp, the rest of trap1's asserts and trap2 (the trd-like trap
+# built from 8 vertices, :63-70; its parameter form is the commented line
+# :68), trap3 and trap4 (parallelepipeds sheared by alpha = +-pi/4, :72-73).
+TRAP2 = (40.0, 0.0, 0.0, 20.0, 10.0, 10.0, 0.0, 40.0, 30.0, 30.0, 0.0)
+TRAP3 = (50.0, 0.0, 0.0, 50.0, 50.0, 50.0, PI / 4, 50.0, 50.0, 50.0, PI / 4)
+TRAP4 = (50.0, 0.0, 0.0, 50.0, 50.0, 50.0, -PI / 4, 50.0, 50.0, 50.0, -PI / 4)
+_TV = 0.11  # testValue (:447)
+VYMZ = (0.0, S2, -S2)
+_T_OUT = ((P0, VX, 20), (P0, VMX, 20), (P0, VY, 30), (P0, VMY, 30), (P0, VZ, 40), (P0, VMZ, 40),
+          (P0, VXY, math.sqrt(800.0)), (PONX1, VX, 0), (PONMX1, VMX, 0), (PONY1, VY, 0), (PONMY1, VMY, 0),
+          (PONZ1, VZ, 0), (PONMZ1, VMZ, 0))
+_T_IN = ((PBX, VMX, 80), (PBMX, VX, 80), (PBY, VMY, 70), (PBMY, VY, 70), (PBZ, VMZ, 60), (PBMZ, VZ, 60))
+TRAP_KATS += [(TRAP1, "DistanceToOut", p, d, ("eq", e)) for p, d, e in _T_OUT] + [
+    (TRAP1, "DistanceToOut", p, d, ("eq", e)) for p, d, e in ((PONX1, VMX, 40), (PONMX1, VX, 40),
+                                                             (PONY1, VMY, 60), (PONMY1, VY, 60),
+                                                             (PONZ1, VMZ, 80), (PONMZ1, VZ, 80))] + [
+    (TRAP1, "SafetyToOut", (0, -30.0 + _TV, 0), None, ("eq", _TV)),
+    (TRAP1, "SafetyToIn", (0, -30.0 - _TV, 0), None, ("eq", _TV)),
+    (TRAP1, "SafetyToOut", (0, -30.0 - _TV, 0), None, ("le", 0.0)),
+    (TRAP1, "SafetyToIn", (0, 0, 40.0 + _TV), None, ("eq", _TV)),
+    (TRAP1, "SafetyToOut", (0, 0, 40.0 + _TV), None, ("le", 0.0)),
+    (TRAP1, "SafetyToIn", (0, 0, 40.0 - _TV), None, ("le", 0.0)),
+    (TRAP1, "SafetyToOut", (0, 0, 40.0 - _TV), None, ("eq", _TV)),
+    (TRAP1, "SafetyToIn", (0, 0, -40.0 + _TV), None, ("le", 0.0)),
+    (TRAP1, "SafetyToOut", (0, 0, -40.0 + _TV), None, ("eq", _TV)),
+    (TRAP1, "SafetyToIn", (0, 0, -40.0 - _TV), None, ("eq", _TV)),
+    (TRAP1, "SafetyToOut", (0, 0, -40.0 - _TV), None, ("le", 0.0)),
+] + [(TRAP1, "DistanceToIn", p, d, ("eq", e)) for p, d, e in _T_IN] + [
+    (TRAP1, "DistanceToIn", PBX, VXY, ("inf",)),
+    (TRAP1, "DistanceToIn", PBMX, VXY, ("inf",)),
+    (TRAP1, "DistanceToIn", (0, 60.0, 0), VXMY, ("inf",)),
+    (TRAP1, "DistanceToIn", (0, 50.0, 0), VXMY, ("eq", math.sqrt(800.0))),
+    (TRAP1, "DistanceToIn", (0, 40.0, 0), VXMY, ("eq", 10.0 * math.sqrt(2.0))),
+    (TRAP1, "DistanceToIn", (0, 40.0, 50.0), VXMY, ("inf",)),
+    (TRAP1, "DistanceToIn", (40.0, 60.0, 0), VMX, ("inf",)),
+    (TRAP1, "DistanceToIn", (40.0, 60.0, 0), VMY, ("inf",)),
+    (TRAP1, "DistanceToIn", (40.0, 60.0, 50.0), VMZ, ("inf",)),
+    (TRAP1, "DistanceToIn", (0, 0, 50.0), VYMZ, ("eq", 10.0 * math.sqrt(2.0))),
+    (TRAP1, "DistanceToIn", (0, 0, 80.0), VYMZ, ("inf",)),
+    (TRAP1, "DistanceToIn", (0, 0, 70.0), VYMZ, ("eq", 30.0 * math.sqrt(2.0))),
+    (TRAP2, "Inside", P0, None, ("inside", 1)),
+    (TRAP2, "Inside", PBZ, None, ("inside", 3)),
+    (TRAP2, "Inside", PONX1, None, ("inside", 2)),
+    (TRAP2, "Inside", PONY1, None, ("inside", 2)),
+    (TRAP2, "Inside", PONZ1, None, ("inside", 2)),
+    (TRAP2, "SafetyToOut", P0, None, ("eq", 20 * COSA)),
+    (TRAP2, "SafetyToOut", VX, None, ("eq", 19 * COSA)),
+    (TRAP2, "SafetyToOut", VY, None, ("eq", 20 * COSA)),
+    (TRAP2, "SafetyToOut", VZ, None, ("eq", 20 * COSA + SINA)),
+] + [(TRAP2, "DistanceToOut", p, d, ("eq", e)) for p, d, e in _T_OUT] + [
+    (TRAP2, "SafetyToIn", p, None, ("eq", e)) for p, e in ((PBX, 80 * COSA), (PBMX, 80 * COSA), (PBY, 70 * COSA),
+                                                           (PBMY, 70 * COSA), (PBZ, 60), (PBMZ, 60))] + [
+    (TRAP2, "DistanceToIn", p, d, ("eq", e)) for p, d, e in _T_IN] + [
+    (TRAP2, "DistanceToIn", PBX, VXY, ("inf",)),
+    (TRAP2, "DistanceToIn", PBMX, VXY, ("inf",)),
+    (TRAP3, "DistanceToIn", (50.0, -50.0, 0), VY, ("eq", 50)),
+    (TRAP3, "DistanceToIn", (50.0, -50.0, 0), VMY, ("inf",)),
+    (TRAP4, "DistanceToIn", (50.0, 50.0, 0), VY, ("inf",)),
+    (TRAP4, "DistanceToIn", (50.0, 50.0, 0), VMY, ("eq", 50)),
 ]
 
 

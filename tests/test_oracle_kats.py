@@ -6,7 +6,8 @@ import oracle
 from tests.golden import kats
 
 CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_KATS]
-         + [("cone", c) for c in kats.CONE_KATS] + [("trd", c) for c in kats.TRD_KATS]
+         + [("cone", c) for c in kats.CONE_KATS] + [("cone", c) for c in kats.CONE_T1_KATS]
+         + [("trd", c) for c in kats.TRD_KATS]
          + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS]
          + [("polyhedron", c) for c in kats.POLYHEDRON_KATS]
          + [("tessellated", c) for c in kats.TESS_KATS] + [("orb", c) for c in kats.ORB_KATS])
@@ -84,3 +85,12 @@ def test_tessellated_extent_kat(brute):
     for params, lo, hi in kats.TESS_EXTENT:
         mn, mx = geo.extent(geo.tessellated(*params, brute_force=brute))
         assert np.allclose(mn, lo) and np.allclose(mx, hi)
+
+
+def test_trapezoid_extent_kat():
+    """TestTrap.cpp:640-655: trap1's extent is the box (20, 30, 40); trap2's
+    is (30, 40, 40), its wider +z face."""
+    geo = oracle.Geometry()
+    for params, lo, hi in ((kats.TRAP1, (-20, -30, -40), (20, 30, 40)), (kats.TRAP2, (-30, -40, -40), (30, 40, 40))):
+        mn, mx = geo.extent(geo.trapezoid(*params))
+        assert np.allclose(mn, lo, rtol=0, atol=1e-6) and np.allclose(mx, hi, rtol=0, atol=1e-6), (params, mn, mx)

@@ -65,6 +65,29 @@ def test_degenerate_equivalence(a, b):
         assert np.max(np.abs(da[fin] - db[fin]), initial=0.0) < 1e-9, m
 
 
+# Pairs whose safeties the reduction fixes as well: the trd's and the
+# box-like trapezoid's face distances are the box's (unit face normals), the
+# equal-radius cone's (sec = 1) and the one-section polycone's are the tube's
+# and the cone's.  (Phi sections are left out: the tube takes PhiPlaneSafety
+# and the cone the wedge normals, two different lower bounds.  The four-sided
+# polyhedron has a test of its own below.)
+SAFETY_EQUIV = [EQUIV[0], EQUIV[1], EQUIV[2], EQUIV[4]]
+
+
+@pytest.mark.parametrize("a,b", SAFETY_EQUIV, ids=[f"{a[0]}=={b[0]}" for a, b in SAFETY_EQUIV])
+def test_degenerate_safeties(a, b):
+    geo = oracle.Geometry()
+    pa, pb = _pv(geo, *a), _pv(geo, *b)
+    pts, _ = _inputs(geo, *b)
+    ia, ib = geo.query(pa, "Inside", pts), geo.query(pb, "Inside", pts)
+    firm = (ia != 2) & (ib != 2) & (ia == ib)
+    for m, side in (("SafetyToIn", 3), ("SafetyToOut", 1)):
+        sa, sb = geo.query(pa, m, pts), geo.query(pb, m, pts)
+        sel = firm & (ia == side)
+        assert sel.sum() > 1000, m
+        assert np.max(np.abs(sa[sel] - sb[sel]), initial=0.0) < 1e-9, m
+
+
 @pytest.mark.parametrize("name", sorted(SHAPES))
 def test_safety_bounds_and_surface_hits(name):
     kind, args = SHAPES[name]
@@ -142,3 +165,141 @@ def test_tessellated_box_safety_is_euclidean():
     so = geo.query(t, "SafetyToOut", pts)
     assert np.allclose(si[ins == 3], out[ins == 3], atol=1e-9)
     assert np.allclose(so[ins == 1], inn[ins == 1], atol=1e-9)
+
+
+def test_polyhedron_box_safety_is_euclidean():
+    """The four-sided polyhedron that is the box (20, 20, 40): outside, its
+    SafetyToIn is the Euclidean distance to the box (the squared distance to
+    the nearest quadrilateral, PolyhedronImplementation.h:448-560 and
+    952-995), above the box's own Chebyshev bound; inside, its SafetyToOut is
+    the distance to the nearest face, the box's."""
+    geo = oracle.Geometry()
+    t = _pv(geo, "polyhedron", EQUIV[8][0][1])
+    b = _pv(geo, "box", (20.0, 20.0, 40.0))
+    pts, _ = _inputs(geo, "box", (20.0, 20.0, 40.0))
+    h = np.array([20.0, 20.0, 40.0])
+    q = np.abs(np.stack(pts, axis=1))
+    out = np.sqrt(np.sum(np.maximum(q - h, 0.0) ** 2, axis=1))
+    inn = np.min(h - q, axis=1)
+    ins = geo.query(t, "Inside", pts)
+    si, so = geo.query(t, "SafetyToIn", pts), geo.query(t, "SafetyToOut", pts)
+    assert (ins == 3).sum() > 1000 and (ins == 1).sum() > 1000
+    assert np.allclose(si[ins == 3], out[ins == 3], rtol=0, atol=1e-9)
+    assert np.allclose(so[ins == 1], inn[ins == 1], rtol=0, atol=1e-9)
+    assert np.allclose(so[ins == 1], geo.query(b, "SafetyToOut", pts)[ins == 1], rtol=0, atol=1e-9)
+
+
+def _trap_vertices(dz, theta, phi, dy1, dx1, dx2, alpha1, dy2, dx3, dx4, alpha2):
+    """The 8 corners of a general trapezoid (G4Trap's parameterisation): the
+    -z face is centred at -dz (tan theta cos phi, tan theta sin phi), its
+    edges at y = +-dy1 have half-lengths dx1 and dx2 and are sheared by
+    y tan alpha1; likewise the +z face with dy2, dx3, dx4, alpha2."""
+    tx, ty = np.tan(theta) * np.cos(phi), np.tan(theta) * np.sin(phi)
+    v = []
+    for z, dy, dxa, dxb, al in ((-dz, dy1, dx1, dx2, alpha1), (dz, dy2, dx3, dx4, alpha2)):
+        cx, cy = z * tx, z * ty
+        for y, dx in ((-dy, dxa), (dy, dxb)):
+            for sx in (-1.0, 1.0):
+                v.append((cx + y * np.tan(al) + sx * dx, cy + y, z))
+    return np.array(v)
+
+
+def _hull_triangles(v):
+    """The 12 triangles of a convex hexahedron with corners numbered as
+    _trap_vertices, each anticlockwise seen from outside."""
+    quads = [(0, 1, 3, 2), (4, 5, 7, 6), (0, 1, 5, 4), (2, 3, 7, 6), (0, 2, 6, 4), (1, 3, 7, 5)]
+    c = v.mean(axis=0)
+    tris = []
+    for q in quads:
+        for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+            a, b, d = v[list(t)]
+            if np.dot(np.cross(b - a, d - a), (a + b + d) / 3 - c) < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(v[list(t)])
+    return np.array(tris)
+
+
+# planar side faces: the x half-lengths change along y at the same rate on
+# both z faces ((dx2 - dx1) / dy1 == (dx4 - dx3) / dy2) and alpha1 == alpha2
+@pytest.mark.parametrize("params", [
+    (40.0, 0.3, 0.7, 20.0, 10.0, 15.0, 0.2, 30.0, 18.0, 25.5, 0.2),
+    (25.0, -0.2, 2.0, 15.0, 20.0, 11.0, -0.3, 10.0, 15.0, 9.0, -0.3),
+], ids=["tilted", "tilted-narrowing"])
+def test_general_trapezoid_equals_its_tessellation(params):
+    """A general trapezoid (theta, phi, alpha != 0: every side plane
+    inclined) against the tessellated solid of its own 8 corners, whose
+    oracle is pinned by brute force and the reference's tessellated tests:
+    classification, DistanceToIn/Out and (inside, a convex solid's distance
+    to its nearest face plane) SafetyToOut agree; outside, the trapezoid's
+    SafetyToIn (largest plane distance) is a lower bound of the tessellated
+    solid's Euclidean one."""
+    geo = oracle.Geometry()
+    tz = _pv(geo, "trapezoid", params)
+    ts = geo.place(geo.logical_volume(geo.tessellated(_hull_triangles(_trap_vertices(*params)))))
+    mn, mx = geo.extent(geo.trapezoid(*params))
+    half, cen = 0.5 * (mx - mn), 0.5 * (mx + mn)
+    pts = sampling.points_in_box(11, 1, N, half, 1.3, cen)
+    dirs = sampling.isotropic_directions(11, 2, N)
+    ia, ib = geo.query(tz, "Inside", pts), geo.query(ts, "Inside", pts)
+    firm = (ia != 2) & (ib != 2)
+    assert np.array_equal(ia[firm], ib[firm])
+    inn, out = firm & (ia == 1), firm & (ia == 3)
+    assert inn.sum() > 1000 and out.sum() > 1000
+    for m, sel in (("DistanceToIn", out), ("DistanceToOut", inn)):
+        da, db = geo.query(tz, m, pts, dirs), geo.query(ts, m, pts, dirs)
+        assert np.array_equal(da[sel] >= 1e30, db[sel] >= 1e30), m
+        fin = sel & (da < 1e30)
+        assert fin.sum() > 500
+        assert np.max(np.abs(da[fin] - db[fin])) < 1e-9, m
+    so_a, so_b = geo.query(tz, "SafetyToOut", pts), geo.query(ts, "SafetyToOut", pts)
+    assert np.max(np.abs(so_a[inn] - so_b[inn])) < 1e-9
+    si_a, si_b = geo.query(tz, "SafetyToIn", pts), geo.query(ts, "SafetyToIn", pts)
+    assert np.all(si_a[out] <= si_b[out] + 1e-9)
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_first_hit_and_safety_balls(name):
+    """A returned distance is the first crossing and a safety is a ball free of
+    the surface, for every test shape:
+    * along the ray from an outside point, the points at 0.1 .. 0.9 of
+      DistanceToIn are outside (a farther root would put inside points
+      before it); from an inside point, those before DistanceToOut are inside;
+    * the points at 0.999 x SafetyToIn from an outside point, in random
+      directions, are not inside; at 0.999 x SafetyToOut from an inside
+      point, not outside (a mis-scaled safety would reach across the
+      surface).
+    Tolerance bands: steps within 1e-6 mm of a surface are left out.  Two
+    reference quirks are kept out of the checks they break: the polycone's
+    SafetyToIn beyond its z-range is the end section's alone
+    (PolyconeImplementation.h:388-403), so only points within the z-range
+    are checked; the polyhedron with a phi range above 180 degrees has an
+    Inside that disagrees with its own distances (see above), so its rays
+    and its SafetyToOut balls are not checked."""
+    kind, args = SHAPES[name]
+    geo = oracle.Geometry()
+    pv = _pv(geo, kind, args)
+    pts, dirs = _inputs(geo, kind, args, seed=23)
+    ins = geo.query(pv, "Inside", pts)
+    P = np.stack(pts, axis=1)
+    D = np.stack(dirs, axis=1)
+    push = 1e-6 if kind == "boolean" else 0.0  # subtraction entries sit kPushTolerance past the surface
+    rays = not (kind == "polyhedron" and np.pi < args[1] < 2 * np.pi)
+    for m, side, want_not in (("DistanceToIn", 3, 1), ("DistanceToOut", 1, 3)) if rays else ():
+        d = geo.query(pv, m, pts, dirs)
+        sel = (ins == side) & (d < 1e30) & (d > 1e-5)
+        for f in (0.1, 0.5, 0.9):
+            q = P[sel] + (f * (d[sel] - push))[:, None] * D[sel]
+            r = geo.query(pv, "Inside", tuple(np.ascontiguousarray(q[:, k]) for k in range(3)))
+            assert np.mean(r == want_not) == 0.0, (m, f, int(np.sum(r == want_not)))
+    rng = np.random.default_rng(5)
+    for m, side, want_not in (("SafetyToIn", 3, 1), ("SafetyToOut", 1, 3)) if rays else (("SafetyToIn", 3, 1),):
+        s = geo.query(pv, m, pts)
+        sel = (ins == side) & (s > 1e-5)
+        if kind == "polycone" and m == "SafetyToIn":
+            zs = args[2]
+            sel &= (P[:, 2] >= zs[0]) & (P[:, 2] <= zs[-1])
+        u = rng.normal(size=(int(sel.sum()), 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        q = P[sel] + (0.999 * s[sel])[:, None] * u
+        r = geo.query(pv, "Inside", tuple(np.ascontiguousarray(q[:, k]) for k in range(3)))
+        assert int(np.sum(r == want_not)) == 0, (m, int(np.sum(r == want_not)))
