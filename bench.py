@@ -46,6 +46,7 @@ os.environ["GPU_MAX_HW_QUEUES"] = "8"
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "track_step_flops.json")
 TRAFFIC_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "hbm_traffic.json")
+NAV_FLOPS_FIXTURE = os.path.join(ROOT, "vecgeom_amd", "data", "nav_step_flops.json")
 
 
 def log(msg):
@@ -345,6 +346,16 @@ def run_navigation(args, ctx, dev, torch):
            "track_steps": total, "kernel_ms": kern * 1e3, "wall_ms": wall * 1e3,
            "kernel": "nav_transport_kernel (persistent grid over a work queue of tracks)"}
     log(f"[bench] navigation: {total / kern / 1e9:.3f} G track-steps/s ({total} steps, kernel {kern * 1e3:.1f} ms)")
+    if os.path.exists(NAV_FLOPS_FIXTURE):
+        fx = json.load(open(NAV_FLOPS_FIXTURE))
+        ach = fx["flops_per_step"] * total / kern / 1e12
+        out["roofline"] = {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": ach / FP64_PEAK_TFLOPS, "kernel": "nav_transport_kernel",
+                           "flops_per_step": fx["flops_per_step"], "steps": total,
+                           "note": "algorithmic FP64 flops per step counted on the host by the same navigator "
+                                   "code (vecgeom_amd/data/nav_step_flops.json) x steps taken / kernel time; the "
+                                   "walks are latency-bound (profiles/navigation_barrel_4M.md)"}
+        log(f"[bench] navigation roofline: {ach:.3f} TF/s = {ach / FP64_PEAK_TFLOPS:.4f} of the FP64 peak")
     # the step loop as one-step kernels: launched one by one, and captured in
     # a HIP graph (the first call captures it, the timed call replays it)
     one(step_kernels=True)

@@ -61,6 +61,7 @@ VGM_DEV Counted inf_length<Counted>() { return Counted(DBL_MAX); }
 
 #include "../csrc/shapes/all_shapes.hpp"
 #include "../csrc/stepping.hpp"
+#include "../csrc/navigation.hpp"
 
 namespace vgm {
 namespace {
@@ -312,6 +313,88 @@ int cf_extent(cf_geom *h, int shape, double *mn, double *mx)
     mx[k] = r.bmax[k];
   }
   return 0;
+}
+
+// ---- navigation ---------------------------------------------------------------
+// The placement tree, as the library's vgm_lv_create / vgm_pv_create /
+// vgm_lv_add_daughter build it, and the navigation tables of close_navigation.
+int cf_lv(cf_geom *h, int shape)
+{
+  if (shape < 0 || size_t(shape) >= h->g.shapes.size()) return -1;
+  h->g.lvs.push_back(LogicalRec{shape, 0, 0, -1});
+  h->g.lv_daughters.emplace_back();
+  return int(h->g.lvs.size()) - 1;
+}
+int cf_place(cf_geom *h, int lv, const double *tr)
+{
+  if (lv < 0 || size_t(lv) >= h->g.lvs.size()) return -1;
+  PlacedRec p{};
+  p.lv = lv;
+  p.id = int32_t(h->g.pvs.size());
+  for (int k = 0; k < 3; ++k) p.tr.t[k] = tr ? tr[k] : 0.0;
+  for (int k = 0; k < 9; ++k) p.tr.r[k] = tr ? tr[3 + k] : ((k % 4 == 0) ? 1.0 : 0.0);
+  h->g.pvs.push_back(p);
+  return p.id;
+}
+int cf_add_daughter(cf_geom *h, int mother, int pv)
+{
+  if (mother < 0 || size_t(mother) >= h->g.lvs.size() || pv < 0 || size_t(pv) >= h->g.pvs.size()) return -1;
+  h->g.lv_daughters[size_t(mother)].push_back(pv);
+  return 0;
+}
+int cf_nav_close(cf_geom *h, int world_pv) { return close_navigation(h->g, world_pv, h->err); }
+
+// FP64 FLOPs of the straight-line navigation transport of n tracks (located
+// first; the location is not counted), up to nsteps ComputeStepAndPropagatedState
+// steps each with the move of the global point, counted on the counting type
+// and checked to follow the plain-double run bit for bit.  steps_out: the
+// steps taken.  Returns 0, or -2 when a counted track left the FP64 path.
+int cf_nav_count(cf_geom *h, size_t n, const double *x, const double *y, const double *z, const double *dx,
+                 const double *dy, const double *dz, int nsteps, uint64_t *flops, uint64_t *steps_out)
+{
+  HostGeometry &g = h->g;
+  if (g.nav_nodes.size() < 2) return -1;
+  std::vector<LogicalRec> lvs = g.lvs;
+  std::vector<int32_t> dl;
+  for (size_t i = 0; i < lvs.size(); ++i) {
+    lvs[i].dbegin = int32_t(dl.size());
+    lvs[i].dcount = int32_t(g.lv_daughters[i].size());
+    dl.insert(dl.end(), g.lv_daughters[i].begin(), g.lv_daughters[i].end());
+  }
+  const GeomView view{g.shapes.data(), g.pool.data(), lvs.data(), dl.data(), g.pvs.data(), g.nav_nodes.data(),
+                      g.nav_children.data(), g.bvhs.data(), g.nav_ipool.data(), g.nav_bpool.data(), g.world_pv};
+  Navigator<kAllShapes, double> nd{view};
+  Navigator<kAllShapes, Counted> nc{view};
+  uint64_t f = 0, st = 0;
+  size_t bad = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const V3<double> p0{x[i], y[i], z[i]}, d0{dx[i], dy[i], dz[i]};
+    NavState sd{nd.locate_global(p0), 0u, false}, sc = sd;
+    V3<double> pd = p0;
+    V3<Counted> pc{Counted(p0.x), Counted(p0.y), Counted(p0.z)};
+    const V3<Counted> dc{Counted(d0.x), Counted(d0.y), Counted(d0.z)};
+    for (int k = 0; k < nsteps && sd.ind; ++k) {
+      NavState od, oc;
+      const double a = nd.step(pd, d0, kInfLength, sd, od);
+      pd.x = pd.x + a * d0.x;
+      pd.y = pd.y + a * d0.y;
+      pd.z = pd.z + a * d0.z;
+      sd = od;
+      const uint64_t f0 = g_flops;
+      const Counted b = nc.step(pc, dc, Counted(kInfLength), sc, oc);
+      pc.x = pc.x + b * dc.x;
+      pc.y = pc.y + b * dc.y;
+      pc.z = pc.z + b * dc.z;
+      sc = oc;
+      f += g_flops - f0;
+      ++st;
+      if (double(b) != a || sc.ind != sd.ind) ++bad;
+    }
+    if (pd.x != double(pc.x) || pd.y != double(pc.y) || pd.z != double(pc.z)) ++bad;
+  }
+  *flops = f;
+  *steps_out = st;
+  return bad ? -2 : 0;
 }
 
 // FP64 FLOPs of `nsteps` track-steps of n sample tracks on `shape` (`mode`

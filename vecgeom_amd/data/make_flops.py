@@ -49,6 +49,12 @@ class FlopGeometry:
                                     _P, _P, _P]
         L.cf_tessellated.argtypes = [ctypes.c_void_p, ctypes.c_int, _P]
         L.cf_extent.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _P]
+        L.cf_lv.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_place.argtypes = [ctypes.c_void_p, ctypes.c_int, _P]
+        L.cf_add_daughter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.cf_nav_close.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_nav_count.argtypes = [ctypes.c_void_p, ctypes.c_size_t] + [_P] * 6 + [
+            ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
         L.cf_count.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
                                ctypes.c_size_t] + [_P] * 6 + [ctypes.c_int, ctypes.c_int,
                                                               ctypes.POINTER(ctypes.c_uint64)]
@@ -90,6 +96,30 @@ class FlopGeometry:
         self._chk(self.L.cf_extent(self.h, s, _p(mn), _p(mx)))
         return mn, mx
 
+    # the placement tree (workloads.build_barrel's calls)
+    def logical_volume(self, shape):
+        return self._chk(self.L.cf_lv(self.h, shape))
+
+    def place(self, lv, transform=None):
+        t = None if transform is None else _p(np.asarray(transform, dtype=np.float64))
+        return self._chk(self.L.cf_place(self.h, lv, t))
+
+    def add_daughter(self, mother_lv, pv):
+        return self._chk(self.L.cf_add_daughter(self.h, mother_lv, pv))
+
+    def nav_count(self, world_pv, pts, dirs, nsteps):
+        """(FP64 flops, steps) of the straight-line navigation transport of the
+        given tracks, checked to follow the plain-double run bit for bit."""
+        self._chk(self.L.cf_nav_close(self.h, world_pv))
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in list(pts) + list(dirs)]
+        f, st = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        rc = self.L.cf_nav_count(self.h, len(arrs[0]), *[a.ctypes.data_as(_P) for a in arrs], nsteps,
+                                 ctypes.byref(f), ctypes.byref(st))
+        if rc == -2:
+            raise RuntimeError("counted navigation left the FP64 path")
+        self._chk(rc)
+        return int(f.value), int(st.value)
+
     def count(self, shape, lam, seed, first, soa, nsteps, mode=0):
         """mode: 0 FP64 step, 1 distance-first, 3 culled (distance-first and bounding-box cull)."""
         out = ctypes.c_uint64(0)
@@ -108,7 +138,13 @@ def main():
     ap.add_argument("--track-steps", type=int, default=50)
     ap.add_argument("--sample", type=int, default=2000, help="tracks counted per solid")
     ap.add_argument("--out", default=os.path.join(HERE, "track_step_flops.json"))
+    ap.add_argument("--nav", action="store_true",
+                    help="count the barrel navigation instead (writes nav_step_flops.json)")
+    ap.add_argument("--nav-sample", type=int, default=20000)
+    ap.add_argument("--nav-steps", type=int, default=200)
     args = ap.parse_args()
+    if args.nav:
+        return nav_main(args)
     g = FlopGeometry()
     specs, counts = workloads.batched_solids(args.tracks, per_family=8, seed=42)
     offsets = np.concatenate([[0], np.cumsum(counts)])
@@ -160,6 +196,27 @@ def main():
     with open(args.out, "w") as fh:
         json.dump(out, fh, indent=1)
     print(f"mean {total:.1f} flops/track-step -> {args.out}")
+
+
+def nav_main(args):
+    """FP64 flops per step of the barrel navigation benchmark: the first
+    --nav-sample of its tracks (seed 42), --nav-steps straight-line steps."""
+    g = FlopGeometry()
+    world = workloads.build_barrel(g)
+    pts, dirs = workloads.barrel_tracks(args.nav_sample, seed=42)
+    t0 = time.time()
+    f, st = g.nav_count(world, pts, dirs, args.nav_steps)
+    out = {"description": "algorithmic FP64 flops (+ - * / sqrt) per step of the straight-line navigation "
+                          "transport on the barrel (ComputeStepAndPropagatedState: mother DistanceToOut, BVH "
+                          "daughter walk with its box tests, candidate DistanceToIn, relocation, and the move of "
+                          "the global point), the first `sample` tracks of the benchmark's track set",
+           "workload": {"geometry": "barrel (20 tube layers, box modules)", "sample": args.nav_sample,
+                        "max_steps": args.nav_steps, "seed": 42},
+           "steps": st, "flops": f, "flops_per_step": f / st}
+    path = os.path.join(HERE, "nav_step_flops.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(f"{f / st:.1f} flops per navigation step over {st} steps ({time.time() - t0:.0f}s) -> {path}")
 
 
 if __name__ == "__main__":
