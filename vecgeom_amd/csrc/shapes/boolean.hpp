@@ -60,9 +60,11 @@ struct BooleanShape {
   // the surface, FP64 evaluates it)
   static constexpr bool kMixedDistance = false;
 
-  // one operand query; `placed` transforms the point (and direction) into the operand's frame
+  // one operand query; `placed` transforms the point (and direction) into the
+  // operand's frame.  Not inlined: the dispatch over every operand family and
+  // query is compiled once per kernel rather than at each of its call sites.
   template <typename R>
-  VGM_DEV static R op(const Data &b, bool right, int q, V3<R> p, V3<R> d = V3<R>{R(0), R(0), R(0)},
+  __host__ __device__ __attribute__((noinline)) static R op(const Data &b, bool right, int q, V3<R> p, V3<R> d = V3<R>{R(0), R(0), R(0)},
                       R step = inf_length<R>(), bool placed = true)
   {
     const Transform &T = right ? b.rtr : b.ltr;

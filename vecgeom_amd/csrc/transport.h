@@ -118,6 +118,10 @@ hipError_t launch_reset(hipStream_t stream, int *counters, int n, unsigned long 
 // True when the family can serve learned bounds (its safety is a distance).
 bool family_learnable(int type);
 hipError_t launch_family(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block);
+// The launcher of one shape family S, instantiated in that family's own
+// translation unit (transport_family.hip).
+template <class S>
+hipError_t launch_family_of(hipStream_t stream, const TransportArgs &a, const FamilyLaunch &f, int block);
 // Initial inside flags (persistent over all tiles, uses a.counter).
 hipError_t launch_classify(hipStream_t stream, const TransportArgs &a, int grid, int block);
 
