@@ -12,10 +12,11 @@ struct PolyconeShape {
   using Data = PolyconeData;
   static constexpr bool kStage = true;  // parameter block staged in LDS
   // waves per SIMD the transport kernel asks the compiler for: the latency
-  // of the section walks hides better at 5 waves than at the 3 its
-  // unconstrained register allocation allows, although some values then
-  // spill to scratch (measured: 50 -> 42 ms for 50 steps of 5M tracks)
-  static constexpr int kWavesPerEU = 5;
+  // of the section walks hides better at 6 waves (80 VGPRs) than at the 3
+  // its unconstrained register allocation allows, although some values then
+  // spill to scratch (50 steps of 5M tracks, FP64 step: 41.5 ms at 4 waves,
+  // 37.2 at 5, 34.9 at 6)
+  static constexpr int kWavesPerEU = 6;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
   // SafetyToIn can exceed DistanceToIn outside the solid beyond its

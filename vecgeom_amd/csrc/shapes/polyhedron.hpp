@@ -18,10 +18,11 @@ struct PolyhedronShape {
   using Data = PolyhedronData;
   static constexpr bool kStage = true;  // parameter block staged in LDS
   // waves per SIMD the transport kernel asks the compiler for: the latency
-  // of the quad walks hides better at 5 waves than at the 2 its
-  // unconstrained register allocation allows, although some values then
-  // spill to scratch (measured: 152 -> 105 ms for 50 steps of 6.7M tracks)
-  static constexpr int kWavesPerEU = 5;
+  // of the quad walks hides better at 6 waves (80 VGPRs) than at the 2 its
+  // unconstrained register allocation allows, although a few values then
+  // spill to scratch (50 steps of 6.7M tracks, distance first: 87.2 ms at 4
+  // waves, 79.6 at 5, 77.5 at 6, 78.7 at 8)
+  static constexpr int kWavesPerEU = 6;
   // safety is the distance to the surface (1-Lipschitz): carried bounds are valid
   static constexpr bool kSafetyIsDistance = false;
   // SafetyToIn/Out never exceed DistanceToIn/Out beyond the surface tolerance:

@@ -265,7 +265,7 @@ VGM_DEV bool cheap_step(const SolidRec &sr, const TrackState &t, double l)
   return false;
 }
 
-template <class S, int Mode, int Order>
+template <class S, int Mode, int Order, bool Staged>
 VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename S::Data &s, const Tile &tl,
                       uint32_t &fp64_steps)
 {
@@ -310,17 +310,20 @@ VGM_DEV void run_tile(const TransportArgs &a, const SolidRec &sr, const typename
       }
       if (k >= a.nsteps) break;
       bool crossed;
+      // a block staged in LDS is re-read there each step rather than held in
+      // registers across the track loop (which spills it to scratch)
+      const typename S::Data &ss = Staged ? opaque_ref(s) : s;
       if constexpr (Mode == kTransportFP64) {
         bool verified = false;  // only the learned tiles read it
         if constexpr (Order != kOrderFP64 && S::kSafetyBelowDistance)
-          crossed = step_distance_first<S>(s, t, l, verified);
+          crossed = step_distance_first<S>(ss, t, l, verified);
         else if constexpr (Order != kOrderFP64 && kDistanceFirstAny<S>)
-          crossed = distance_first_ok<S>(s, t.p, t.in) ? step_distance_first<S>(s, t, l, verified)
-                                                       : step_fp64<S>(s, t, l);
+          crossed = distance_first_ok<S>(ss, t.p, t.in) ? step_distance_first<S>(ss, t, l, verified)
+                                                        : step_fp64<S>(ss, t, l);
         else
-          crossed = step_fp64<S>(s, t, l);
+          crossed = step_fp64<S>(ss, t, l);
       } else {
-        crossed = step_mixed<S>(s, t, l, scale, fp64_steps);
+        crossed = step_mixed<S>(ss, t, l, scale, fp64_steps);
       }
       finish(step, crossed);
       ++k;
@@ -379,7 +382,7 @@ transport_family_kernel(TransportArgs a, int t0, int t1, int *counter)
       }
       sp = reinterpret_cast<const typename S::Data *>(lds_block);
     }
-    run_tile<S, Mode, Order>(a, sr, *sp, tl, fp64_steps);
+    run_tile<S, Mode, Order, Staged>(a, sr, *sp, tl, fp64_steps);
   }
   if (Mode != kTransportFP64 && a.fp64_steps && fp64_steps) atomicAdd(a.fp64_steps, (unsigned long long)fp64_steps);
 }

@@ -44,6 +44,25 @@ VGM_DEV float vsign(float x) { return __builtin_copysignf(1.0f, x); }
 VGM_DEV double vcopysign(double a, double b) { return __builtin_copysign(a, b); }
 VGM_DEV float vcopysign(float a, float b) { return __builtin_copysignf(a, b); }
 // NonZero(x) = x + copysign(1e-30, x)  (VecGeom/base/Global.h:170-177)
+// The same reference, but opaque to the optimiser: loads through it cannot be
+// hoisted above this point or merged with loads through the original.  A
+// kernel calls it once per track-step on a parameter block in LDS, so the
+// header fields are read from LDS inside the step instead of being held in
+// registers across the whole track loop (where they spill to scratch).  The
+// offset stays wave-uniform (a scalar register), so loads from global memory
+// remain scalar loads.
+template <class T>
+VGM_DEV const T &opaque_ref(const T &r)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(&r) + z);
+#else
+  return r;
+#endif
+}
+
 VGM_DEV double nonzero(double x) { return x + __builtin_copysign(1.0e-30, x); }
 VGM_DEV float nonzero(float x) { return x + __builtin_copysignf(1.0e-30f, x); }
 // NonZeroAbs(x) = |x| + 1e-30 (VecGeom/base/Global.h:162-167)
