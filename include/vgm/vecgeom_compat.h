@@ -130,6 +130,19 @@ class PlacedVolume {
   {
     check(vgm_safety_to_out(ctx_, pv_, p.x(), p.y(), p.z(), p.size(), out, flags(p)));
   }
+  // VPlacedVolume::Normal(point, normal) -> valid (PlacedVolume.h:437-439),
+  // master-frame point and normal; the batched form takes SOA3D arrays
+  bool Normal(const double point[3], double normal[3]) const
+  {
+    uint8_t valid = 0;
+    check(vgm_normal(ctx_, pv_, &point[0], &point[1], &point[2], 1, &normal[0], &normal[1], &normal[2], &valid,
+                     VGM_HOST_PTRS));
+    return valid != 0;
+  }
+  void Normal(SOA3D<> const &p, SOA3D<> &n, uint8_t *valid) const
+  {
+    check(vgm_normal(ctx_, pv_, p.x(), p.y(), p.z(), p.size(), n.x(), n.y(), n.z(), valid, flags(p)));
+  }
 
  private:
   friend class GeoManager;
@@ -159,7 +172,7 @@ class GeoManager {
   // unplaced shapes (constructor arguments of the reference's classes)
   int Box(double dx, double dy, double dz) { return make(vgm_box_create(ctx_, dx, dy, dz, &s_)); }
   int Orb(double r) { return make(vgm_orb_create(ctx_, r, &s_)); }
-  // UnplacedBooleanVolume<op>: op 1 intersection, 2 subtraction; placements
+  // UnplacedBooleanVolume<op>: op 0 union, 1 intersection, 2 subtraction; placements
   // of 12 doubles (translation, rotation) or null for the identity
   int Boolean(int op, int left, const double *left_tr, int right, const double *right_tr)
   {

@@ -75,9 +75,10 @@ int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
 int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
 /* UnplacedBooleanVolume<op>(left placed, right placed)
  * (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:8-31; kernels
- * VecGeom/volumes/kernel/BooleanImplementation.h:39-290 and
- * BooleanIntersectionImplementation.h): op 1 = kIntersection, 2 =
- * kSubtraction (left minus right); union (0) is rejected.  The operands are
+ * VecGeom/volumes/kernel/BooleanImplementation.h:39-290,
+ * BooleanIntersectionImplementation.h and BooleanUnionImplementation.h):
+ * op 0 = kUnion, 1 = kIntersection, 2 = kSubtraction (left minus right),
+ * the reference's BooleanOperation values.  The operands are
  * existing primitive shapes (not booleans or tessellated solids) placed by
  * left_tr / right_tr (12 doubles: translation, row-major rotation; NULL =
  * identity).  The operand parameters are copied at creation. */
@@ -172,6 +173,15 @@ int vgm_safety_to_in(vgm_ctx *ctx, int pv, const double *x, const double *y, con
                      double *out, unsigned flags);
 int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n,
                       double *out, unsigned flags);
+
+/* VPlacedVolume::Normal (VecGeom/volumes/PlacedVolume.h:437-439,
+ * source/PlacedVolume.cpp:126-137) over n master-frame points: the unit
+ * normal of the surface the point is on, or of the nearest one (master
+ * frame), and valid[i] = 1 when the point is on the surface.  The reference
+ * has only the scalar form.  Every family but the tessellated solid; FP64
+ * only (VGM_MIXED is refused). */
+int vgm_normal(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n, double *nx,
+               double *ny, double *nz, uint8_t *valid, unsigned flags);
 
 /* ---- allocation-free SOA3D query --------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -73,6 +73,7 @@ def lib():
             "orc_pv_create": [_P, _I, _P],
             "orc_lv_add_daughter": [_P, _I, _I],
             "orc_query": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _I],
+            "orc_normal": [_P, _I, _P, _P, _P, _SZ, _P, _P, _P, _P, _I],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -118,7 +119,7 @@ class Geometry:
         return self._chk(self._L.orc_orb_create(self.h, r))
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None):
-        ops = {"intersection": 1, "subtraction": 2}
+        ops = {"intersection": 1, "subtraction": 2, "union": 3}
         ids = [getattr(self, s[0])(*s[1]) if isinstance(s, tuple) else s for s in (left, right)]
         trs = [None if t is None else np.ascontiguousarray(t, dtype=np.float64) for t in (left_tr, right_tr)]
         return self._chk(self._L.orc_boolean_create(self.h, ops[op], ids[0], None if trs[0] is None else _p(trs[0]),
@@ -255,3 +256,14 @@ class Geometry:
         self._L.orc_query(self.h, pv, m, _p(px), _p(py), _p(pz), _p(dx), _p(dy), _p(dz), _p(st), n,
                           out.ctypes.data, nthreads)
         return out
+
+    def normal(self, pv, pts, nthreads=0):
+        """VPlacedVolume::Normal of master-frame points: (nx, ny, nz, valid)."""
+        px, py, pz = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        n = len(px)
+        nx, ny, nz = (np.empty(n, np.float64) for _ in range(3))
+        valid = np.empty(n, np.uint8)
+        r = self._L.orc_normal(self.h, pv, _p(px), _p(py), _p(pz), n, _p(nx), _p(ny), _p(nz), _p(valid), nthreads)
+        if r != 0:
+            raise RuntimeError("no Normal for this shape family")
+        return nx, ny, nz, valid

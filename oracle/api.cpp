@@ -27,7 +27,7 @@ int orc_boolean_create(orc_geom *g, int op, int left, const double *ltr, int rig
 {
   auto *G = reinterpret_cast<Geometry *>(g);
   const int n = int(G->shapes.size());
-  if ((op != 1 && op != 2) || left < 0 || left >= n || right < 0 || right >= n) return -1;
+  if ((op < 1 || op > 3) || left < 0 || left >= n || right < 0 || right >= n) return -1;
   return add_shape(g, make_boolean(op, G->shapes[left].get(), ltr, G->shapes[right].get(), rtr));
 }
 int orc_orb_create(orc_geom *g, double r) { return r > 0 ? add_shape(g, make_orb(r)) : -1; }
@@ -154,6 +154,30 @@ int orc_query(orc_geom *g, int pv, int method, const double *px, const double *p
     }
     default: break;
     }
+  }
+  return 0;
+}
+
+// VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137): master-frame
+// point, unplaced normal in the local frame, direction back to the master.
+int orc_normal(orc_geom *g, int pv, const double *px, const double *py, const double *pz, size_t n, double *nx,
+               double *ny, double *nz, uint8_t *valid, int nthreads)
+{
+  auto *G = reinterpret_cast<Geometry *>(g);
+  const PlacedVolume &P = G->pvs[pv];
+  const Shape &S = *G->shapes[G->lvs[P.lv].shape];
+  if (!S.HasNormal()) return -1;
+  const Transform &T = P.tr;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long long i = 0; i < (long long)n; ++i) {
+    Vec3 ln;
+    const bool ok = S.Normal(T.Transform_(Vec3(px[i], py[i], pz[i])), ln);
+    const Vec3 m = T.InverseTransformDirection(ln);
+    nx[i] = m.x;
+    ny[i] = m.y;
+    nz[i] = m.z;
+    valid[i] = ok ? 1 : 0;
   }
   return 0;
 }

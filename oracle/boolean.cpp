@@ -1,7 +1,11 @@
 // Oracle: boolean solids of two placed operands.  Intersection:
 // VecGeom/volumes/kernel/BooleanIntersectionImplementation.h; subtraction:
-// VecGeom/volumes/kernel/BooleanImplementation.h:39-290; Extent:
-// source/UnplacedBooleanVolume.cpp:403-482.  Placed-volume conventions of
+// VecGeom/volumes/kernel/BooleanImplementation.h:39-290; union:
+// VecGeom/volumes/kernel/BooleanUnionImplementation.h; Extent:
+// source/UnplacedBooleanVolume.cpp:403-482.  An operand's Normal is the
+// placed volume's (source/PlacedVolume.cpp:126-137: transform the point,
+// the unplaced normal, inverse-transform the direction); the booleans call
+// it on points they have already transformed, as the reference does.  Placed-volume conventions of
 // the operands: Contains, Inside, DistanceToIn, PlacedDistanceToOut and
 // SafetyToIn transform the point into the operand's frame, DistanceToOut and
 // SafetyToOut take it as given.  The reference's unbounded search loops stop
@@ -29,6 +33,22 @@ struct Placed {  // a VPlacedVolume: unplaced shape and transformation
   double DistanceToOut(const Vec3 &p, const Vec3 &d) const { return s->DistanceToOut(p, d, kInfLength); }
   double SafetyToIn(const Vec3 &p) const { return s->SafetyToIn(tr.Transform_(p)); }
   double SafetyToOut(const Vec3 &p) const { return s->SafetyToOut(p); }
+  bool Normal(const Vec3 &p, Vec3 &n) const  // VPlacedVolume::Normal
+  {
+    Vec3 ln;
+    const bool valid = s->Normal(tr.Transform_(p), ln);
+    n = tr.InverseTransformDirection(ln);
+    return valid;
+  }
+  // the booleans' pattern: transform, placed Normal, inverse-transform
+  bool NormalOfLocal(const Vec3 &p, Vec3 &n, double sign = 1.0) const
+  {
+    Vec3 localNorm;
+    const bool valid = Normal(tr.Transform_(p), localNorm);
+    if (sign != 1.0) localNorm = Vec3(localNorm.x * sign, localNorm.y * sign, localNorm.z * sign);
+    n = tr.InverseTransformDirection(localNorm);
+    return valid;
+  }
   void TransformedExtent(Vec3 &aMin, Vec3 &aMax) const  // :403-442
   {
     Vec3 lower, upper;
@@ -52,11 +72,12 @@ struct Placed {  // a VPlacedVolume: unplaced shape and transformation
 }  // namespace
 
 struct Boolean final : Shape {
-  int op;  // 1 intersection, 2 subtraction
+  int op;  // 1 intersection, 2 subtraction, 3 union
   Placed A, B;
 
   bool Contains(const Vec3 &p) const override
   {
+    if (op == 3) return A.Contains(p) || B.Contains(p);
     if (op == 1) {
       const bool insideA = A.Contains(p), insideB = B.Contains(p);
       return insideA && insideB;
@@ -66,8 +87,28 @@ struct Boolean final : Shape {
     return inside && !B.Contains(p);
   }
 
+  // union Inside (BooleanUnionImplementation.h:71-113): on the surfaces of
+  // both, opposite normals mean touching solids (inside)
+  int UnionInside(const Vec3 &p) const
+  {
+    const int positionA = A.Inside(p);
+    if (positionA == kInside) return kInside;
+    const int positionB = B.Inside(p);
+    if (positionB == kInside) return kInside;
+    if ((positionA == kSurface) && (positionB == kSurface)) {
+      Vec3 normalA, normalB;
+      A.NormalOfLocal(p, normalA);
+      B.NormalOfLocal(p, normalB);
+      if (Dot(normalA, normalB) < 0) return kInside;
+      return kSurface;
+    }
+    if ((positionB == kSurface) || (positionA == kSurface)) return kSurface;
+    return kOutside;
+  }
+
   int Inside(const Vec3 &p) const override
   {
+    if (op == 3) return UnionInside(p);
     const int positionA = A.Inside(p);
     if (positionA == kOutside) return kOutside;
     const int positionB = B.Inside(p);
@@ -86,6 +127,11 @@ struct Boolean final : Shape {
 
   double DistanceToIn(const Vec3 &p, const Vec3 &dir, double stepMax) const override
   {
+    if (op == 3) {  // union (:118-126)
+      const double d1 = A.DistanceToIn(p, dir, stepMax);
+      const double d2 = B.DistanceToIn(p, dir, stepMax);
+      return Min(d1, d2);
+    }
     if (op == 1) {  // intersection
       Vec3 hitpoint = p;
       bool inleft = A.Contains(hitpoint), inright = B.Contains(hitpoint);
@@ -152,8 +198,44 @@ struct Boolean final : Shape {
     return kInfLength;
   }
 
+  // union DistanceToOut (:128-179): leave the operand the point is in,
+  // crossing into the other while it overlaps or connects
+  double UnionDistanceToOut(const Vec3 &point, const Vec3 &dir) const
+  {
+    double dist = 0.;
+    const double pushdist = kPushTolerance;
+    const int positionA = A.Inside(point);
+    Vec3 nextp = point;
+    bool connectingstep = false;
+    auto kernel = [&](const Placed &P, const Placed &Q) {
+      int it = 0;
+      do {
+        connectingstep = false;
+        const double disTmp = P.PlacedDistanceToOut(nextp, dir);
+        dist += (disTmp >= 0. && disTmp < kInfLength) ? disTmp : 0;
+        dist += pushdist;
+        nextp = point + dir * dist;
+        if (Q.Inside(nextp) != kOutside) {
+          const double disTmp2 = Q.PlacedDistanceToOut(nextp, dir);
+          dist += (disTmp2 >= 0. && disTmp2 < kInfLength) ? disTmp2 : 0;
+          dist += pushdist;
+          nextp = point + dir * dist;
+          connectingstep = true;
+        }
+      } while (connectingstep && (P.Inside(nextp) != kOutside) && ++it < kMaxIterations);
+    };
+    if (positionA != kOutside)
+      kernel(A, B);
+    else
+      kernel(B, A);
+    double distance = dist - pushdist;
+    if (distance < kTolerance && positionA == kOutside && B.Inside(point) == kOutside) distance = -kTolerance;
+    return distance;
+  }
+
   double DistanceToOut(const Vec3 &p, const Vec3 &dir, double stepMax) const override
   {
+    if (op == 3) return UnionDistanceToOut(p, dir);
     if (op == 1) return Min(A.DistanceToOut(p, dir), B.PlacedDistanceToOut(p, dir));
     const double distancel = A.PlacedDistanceToOut(p, dir, stepMax);
     const double dinright = B.DistanceToIn(p, dir, stepMax);
@@ -162,6 +244,7 @@ struct Boolean final : Shape {
 
   double SafetyToIn(const Vec3 &p) const override
   {
+    if (op == 3) return Min(A.SafetyToIn(p), B.SafetyToIn(p));  // :184-194
     if (op == 1) {
       const bool insideA = A.Contains(p), insideB = B.Contains(p);
       if (!insideA && insideB) return A.SafetyToIn(p);
@@ -174,6 +257,15 @@ struct Boolean final : Shape {
 
   double SafetyToOut(const Vec3 &p) const override
   {
+    if (op == 3) {  // :199-223
+      const int insideA = A.Inside(p), insideB = B.Inside(p);
+      if (insideA == kOutside && insideB == kOutside) return -kTolerance;
+      if (insideA != kOutside && insideB != kOutside)
+        return Max(A.SafetyToOut(p), B.SafetyToOut(B.tr.Transform_(p)));
+      if (insideA == kSurface || insideB == kSurface) return -kTolerance;
+      if (insideA == kOutside) return B.SafetyToOut(B.tr.Transform_(p));
+      return A.SafetyToOut(p);
+    }
     if (op == 1) {
       double safety = Min(A.SafetyToOut(p), B.SafetyToOut(B.tr.Transform_(p)));
       if (safety < 0.) safety = 0.;
@@ -181,6 +273,33 @@ struct Boolean final : Shape {
     }
     return Min(A.SafetyToOut(p), B.SafetyToIn(p));
   }
+
+  // NormalKernel of each operation: union BooleanUnionImplementation.h:228-279,
+  // subtraction BooleanImplementation.h:239-288, intersection
+  // BooleanIntersectionImplementation.h:234-274
+  bool Normal(const Vec3 &point, Vec3 &normal) const override
+  {
+    if (op == 3) {
+      if (A.Contains(point)) return A.NormalOfLocal(point, normal);
+      if (B.Contains(point)) return B.NormalOfLocal(point, normal);
+      const double safetyA = A.SafetyToIn(point), safetyB = B.SafetyToIn(point);
+      if (safetyA < safetyB) return A.NormalOfLocal(point, normal);
+      return B.NormalOfLocal(point, normal);
+    }
+    if (op == 2) {
+      if (B.Contains(point)) return B.NormalOfLocal(point, normal, -1.);
+      if (!A.Contains(point)) return A.NormalOfLocal(point, normal);
+      const double safetyA = A.SafetyToOut(A.tr.Transform_(point));
+      const double safetyB = B.SafetyToIn(point);
+      if (safetyA < safetyB) return A.NormalOfLocal(point, normal);
+      return B.NormalOfLocal(point, normal, -1.);
+    }
+    const double safetyA = A.Contains(point) ? A.SafetyToOut(A.tr.Transform_(point)) : A.SafetyToIn(point);
+    const double safetyB = B.Contains(point) ? B.SafetyToOut(B.tr.Transform_(point)) : B.SafetyToIn(point);
+    if (safetyA < safetyB) return A.NormalOfLocal(point, normal);
+    return B.NormalOfLocal(point, normal);
+  }
+  bool HasNormal() const override { return A.s->HasNormal() && B.s->HasNormal(); }
 
   void Extent(Vec3 &aMin, Vec3 &aMax) const override  // :444-482
   {
@@ -190,6 +309,11 @@ struct Boolean final : Shape {
     if (op == 2) {
       aMin = minLeft;
       aMax = maxLeft;
+      return;
+    }
+    if (op == 3) {
+      aMin = Vec3(Min(minLeft.x, minRight.x), Min(minLeft.y, minRight.y), Min(minLeft.z, minRight.z));
+      aMax = Vec3(Max(maxLeft.x, maxRight.x), Max(maxLeft.y, maxRight.y), Max(maxLeft.z, maxRight.z));
       return;
     }
     aMin = Vec3(Max(minLeft.x, minRight.x), Max(minLeft.y, minRight.y), Max(minLeft.z, minRight.z));

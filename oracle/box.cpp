@@ -50,6 +50,23 @@ struct Box final : Shape {
     mn = -h;
     mx = h;
   }
+
+  // NormalKernel (:172-189): the faces within tolerance of the nearest one,
+  // averaged on edges and corners
+  bool Normal(const Vec3 &p, Vec3 &n) const override
+  {
+    const Vec3 safety = (p.Abs() - h).Abs();
+    const double safmin = safety.MinC();
+    const bool valid = safmin < kHalfTolerance;
+    Vec3 normal(0., 0., 0.);
+    if (safety.x - safmin < kHalfTolerance) normal.x = Sign(p.x);
+    if (safety.y - safmin < kHalfTolerance) normal.y = Sign(p.y);
+    if (safety.z - safmin < kHalfTolerance) normal.z = Sign(p.z);
+    if (normal.Mag2() > 1.0) normal = normal.Normalized();
+    n = normal;
+    return valid;
+  }
+  bool HasNormal() const override { return true; }
 };
 
 Shape *make_box(double dx, double dy, double dz) { return new Box(dx, dy, dz); }

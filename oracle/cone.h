@@ -494,6 +494,113 @@ struct Cone final : Shape {
     return safety;
   }
 
+  // ConeStruct::Normal (ConeStruct.h:177-306).  _frmin/_frmax are the
+  // constructor's radii: rmin unchanged, rmax before the kConeTolerance bump.
+  bool Normal(const Vec3 &p, Vec3 &norm) const override
+  {
+    const double _frmin1 = fRmin1, _frmin2 = fRmin2, _frmax1 = fOriginalRmax1, _frmax2 = fOriginalRmax2;
+    int noSurfaces = 0;
+    double distSPhi = kInfLength, distEPhi = kInfLength;
+    Vec3 sumnorm(0., 0., 0.), nZ(0., 0., 1.);
+    Vec3 nR, nr(0., 0., 0.), nPs, nPe;
+    norm = sumnorm;
+    double distZ = Abs(p.z) - fDz;
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
+    double distRMin, distRMax;
+    const double pRMin = rho - p.z * fTanRMin;
+    const double widRMin = fRmin2 - fDz * fTanRMin;
+    if (Abs(_frmin1 - _frmin2) < fInnerTolerance)
+      distRMin = (rho - _frmin2);
+    else
+      distRMin = (pRMin - widRMin) / fSecRMin;
+    const double pRMax = rho - p.z * fTanRMax;
+    const double widRMax = fRmax2 - fDz * fTanRMax;
+    if (Abs(_frmax1 - _frmax2) < fOuterTolerance)
+      distRMax = (rho - _frmax2);
+    else
+      distRMax = (pRMax - widRMax) / fSecRMax;
+    bool inside = distZ < kTolerance && distRMax < fOuterTolerance;
+    if (fRmin1 != 0. || fRmin2 != 0.) inside &= distRMin > -fInnerTolerance;
+    distZ = std::fabs(distZ);
+    distRMax = std::fabs(distRMax);
+    distRMin = std::fabs(distRMin);
+    double distNearest = distZ;
+    Vec3 normNearest = nZ;
+    if (p.z < 0.) normNearest = Vec3(0, 0, -1.);
+    const bool fullPhi = fDPhi == kTwoPi;
+    if (!fullPhi) {
+      if (rho != 0.) {
+        double pPhi = std::atan2(p.y, p.x);
+        if (pPhi < fSPhi - kHalfTolerance)
+          pPhi += 2 * kPi;
+        else if (pPhi > fSPhi + fDPhi + kHalfTolerance)
+          pPhi -= 2 * kPi;
+        distSPhi = rho * (pPhi - fSPhi);
+        distEPhi = rho * (pPhi - fSPhi - fDPhi);
+        inside = inside && (distSPhi > -kTolerance) && (distEPhi < kTolerance);
+        distSPhi = Abs(distSPhi);
+        distEPhi = Abs(distEPhi);
+      } else if (!(fRmin1 != 0.) || !(fRmin2 != 0.)) {
+        distSPhi = 0.;
+        distEPhi = 0.;
+      }
+      nPs = Vec3(std::sin(fSPhi), -std::cos(fSPhi), 0);
+      nPe = Vec3(-std::sin(fSPhi + fDPhi), std::cos(fSPhi + fDPhi), 0);
+    }
+    if (rho > kHalfTolerance) {
+      nR = Vec3(p.x / rho / fSecRMax, p.y / rho / fSecRMax, -fTanRMax / fSecRMax);
+      if (fRmin1 != 0. || fRmin2 != 0.) nr = Vec3(-p.x / rho / fSecRMin, -p.y / rho / fSecRMin, fTanRMin / fSecRMin);
+    }
+    if (inside && distZ <= kHalfTolerance) {
+      noSurfaces++;
+      if (p.z >= 0.)
+        sumnorm += nZ;
+      else
+        sumnorm = Vec3(0, 0, -1.);
+    }
+    if (inside && distRMax <= fOuterTolerance) {
+      noSurfaces++;
+      sumnorm += nR;
+    } else if (noSurfaces == 0 && distRMax < distNearest) {
+      distNearest = distRMax;
+      normNearest = nR;
+    }
+    if (fRmin1 != 0. || fRmin2 != 0.) {
+      if (inside && distRMin <= fInnerTolerance) {
+        noSurfaces++;
+        sumnorm += nr;
+      } else if (noSurfaces == 0 && distRMin < distNearest) {
+        distNearest = distRMin;
+        normNearest = nr;
+      }
+    }
+    if (!fullPhi) {
+      if (inside && distSPhi <= kHalfTolerance) {
+        noSurfaces++;
+        sumnorm += nPs;
+      } else if (noSurfaces == 0 && distSPhi < distNearest) {
+        distNearest = distSPhi;
+        normNearest = nPs;
+      }
+      if (inside && distEPhi <= kHalfTolerance) {
+        noSurfaces++;
+        sumnorm += nPe;
+      } else if (noSurfaces == 0 && distEPhi < distNearest) {
+        normNearest = nPe;
+      }
+    }
+    if (noSurfaces == 0)
+      norm = normNearest;
+    else if (noSurfaces == 1)
+      norm = sumnorm;
+    else
+      norm = sumnorm.Unit();
+    bool valid = noSurfaces != 0;
+    if (noSurfaces > 2) valid = false;
+    return valid;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     double r = Max(fOriginalRmax1, fOriginalRmax2);

@@ -63,6 +63,26 @@ struct Vec3 {
   double Mag() const { return std::sqrt(Mag2()); }
   double Perp2() const { return x * x + y * y; }
   double Perp() const { return std::sqrt(Perp2()); }
+  Vec3 operator/(double s) const { return Vec3(x / s, y / s, z / s); }
+  Vec3 &operator+=(const Vec3 &o)
+  {
+    x += o.x;
+    y += o.y;
+    z += o.z;
+    return *this;
+  }
+  // Vector3D::Normalize (VecGeom/base/Vector3D.h:198): times 1/|v|
+  Vec3 Normalized() const
+  {
+    const double s = 1.0 / Mag();
+    return Vec3(x * s, y * s, z * s);
+  }
+  // Vector3D::Unit (Vector3D.h:284-290): divided by sqrt(|v|^2 + DBL_MIN)
+  Vec3 Unit() const
+  {
+    const double m = std::sqrt(Mag2() + DBL_MIN);
+    return Vec3(x / m, y / m, z / m);
+  }
 };
 
 // Vector3D dot: x*x' + y*y' + z*z' in that order.
@@ -136,11 +156,19 @@ struct Shape {
   virtual double SafetyToIn(const Vec3 &p) const = 0;
   virtual double SafetyToOut(const Vec3 &p) const = 0;
   virtual void Extent(Vec3 &mn, Vec3 &mx) const = 0;
+  // VUnplacedVolume::Normal (UnplacedVolume.h:252): the surface normal at p
+  // and whether p is on the surface.  Families without one return false.
+  virtual bool Normal(const Vec3 & /*p*/, Vec3 &n) const
+  {
+    n = Vec3();
+    return false;
+  }
+  virtual bool HasNormal() const { return false; }
 };
 
 Shape *make_box(double dx, double dy, double dz);
 Shape *make_orb(double r);
-// op 1 intersection, 2 subtraction; ltr/rtr: 12 doubles (translation, rotation) or null
+// op 1 intersection, 2 subtraction, 3 union; ltr/rtr: 12 doubles (translation, rotation) or null
 Shape *make_boolean(int op, const Shape *left, const double *ltr, const Shape *right, const double *rtr);
 Shape *make_cut_tube(double rmin, double rmax, double z, double sphi, double dphi, const double bot[3],
                      const double top[3]);

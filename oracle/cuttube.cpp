@@ -194,6 +194,19 @@ struct CutTube final : Shape {
     return (fDz - (r / n.z) * (n.x * std::cos(phi) + n.y * std::sin(phi)));
   }
 
+  // NormalKernel (CutTubeImplementation.h:340-360): the tube's normal where
+  // the tube is nearer than the cut planes, else the cut plane of the point's
+  // half
+  bool Normal(const Vec3 &point, Vec3 &normal) const override
+  {
+    const double safcut = PlanesSafetyToOut(point);
+    const double saftube = fTube.SafetyToOut(point);
+    if (Abs(saftube) < Abs(safcut)) return fTube.Normal(point, normal);
+    normal = point.z < 0 ? fCut[0].fNormal : fCut[1].fNormal;
+    return true;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &aMin, Vec3 &aMax) const override  // source/UnplacedCutTube.cpp:128-172
   {
     const Vec3 &tn = fCut[1].fNormal, &bn = fCut[0].fNormal;

@@ -27,6 +27,7 @@ void orc_geom_destroy(orc_geom *g);
 
 int orc_box_create(orc_geom *g, double dx, double dy, double dz);
 int orc_orb_create(orc_geom *g, double r);
+/* op: 1 intersection, 2 subtraction, 3 union */
 int orc_boolean_create(orc_geom *g, int op, int left, const double *ltr, int right, const double *rtr);
 int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
                         double bz, double tx, double ty, double tz);
@@ -51,6 +52,12 @@ int orc_lv_add_daughter(orc_geom *g, int mother, int pv);
 int orc_query(orc_geom *g, int pv, int method, const double *px, const double *py, const double *pz,
               const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
               int nthreads);
+
+/* VPlacedVolume::Normal over a batch of master-frame points: the unit
+ * normal of the nearest surface (master frame) and whether the point is on
+ * the surface.  Returns -1 for a family without Normal (tessellated). */
+int orc_normal(orc_geom *g, int pv, const double *px, const double *py, const double *pz, size_t n, double *nx,
+               double *ny, double *nz, uint8_t *valid, int nthreads);
 
 /* Navigation (navigation.cpp): states are node numbers of a depth-first
  * walk of the placement tree from the world (1 = world, 0 = outside).

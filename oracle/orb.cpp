@@ -80,6 +80,17 @@ struct Orb final : Shape {
     return r - rad;
   }
 
+  // NormalKernel (:227-240)
+  bool Normal(const Vec3 &p, Vec3 &n) const override
+  {
+    const double rad2 = mag2(p);
+    const double invRadius = 1. / Sqrt(rad2);
+    n = Vec3(p.x * invRadius, p.y * invRadius, p.z * invRadius);
+    const double tolRMaxO = r + kTolerance, tolRMaxI = r - kTolerance;
+    return (rad2 <= tolRMaxO * tolRMaxO) && (rad2 >= tolRMaxI * tolRMaxI);
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     mn = Vec3(-r, -r, -r);

@@ -294,6 +294,46 @@ struct Polycone final : Shape {
     return minSafety;
   }
 
+  // UnplacedPolycone::Normal (source/UnplacedPolycone.cpp:782-832): the
+  // section's cone normal; on the plane between two sections the next
+  // section's too
+  bool Normal(const Vec3 &point, Vec3 &norm) const override
+  {
+    bool valid = true;
+    const int index = SectionIndex(point.z - kTolerance);
+    if (index < 0) {
+      valid = false;
+      if (index == -1) norm = Vec3(0., 0., -1.);
+      if (index == -2) norm = Vec3(0., 0., 1.);
+      return valid;
+    }
+    const Section &sec = secs[index];
+    valid = sec.solid->Normal(Shifted(point, sec.shift), norm);
+    if (index + 1 < NSec() && std::abs(point.z - zs[index + 1]) < kTolerance) {
+      const Section &sec2 = secs[index + 1];
+      Vec3 norm2;
+      const bool valid2 = sec2.solid->Normal(Shifted(point, sec2.shift), norm2);
+      if (!valid && valid2) {
+        norm = norm2;
+        valid = valid2;
+      }
+      if (valid && valid2) {
+        const double s = kTolerance * 10;
+        const Vec3 q = point + Vec3(s * norm2.x, s * norm2.y, s * norm2.z);
+        const bool c2 = sec2.solid->Contains(Shifted(q, sec2.shift));
+        if (c2) {
+          norm = norm2;
+        } else {
+          norm = norm + norm2;
+          if (norm.Mag2() < kTolerance) norm = Vec3(point.x, point.y, 0);
+        }
+      }
+    }
+    if (valid) norm = norm / norm.Mag();
+    return valid;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     mn = Vec3(-rmaxExtent, -rmaxExtent, zmin);

@@ -749,6 +749,88 @@ struct Polyhedron final : Shape {
 
   // UnplacedPolyhedron.cpp:273-303: bounding box over every corner of every
   // z plane (radii are side distances, corners sit 1/cos(sidePhi/2) further).
+  // ScalarSafetyToEndcapsSquared with the endcap index (:620-650): iz = -1
+  // or +1 when the endcap is the nearest surface so far
+  void SafetyToEndcapsSquaredIz(const Vec3 &p, double &distanceSquared, int &iz) const
+  {
+    const double firstDistance = fZPlanes[0] - p[2];
+    const double lastDistance = fZPlanes[NSeg()] - p[2];
+    const bool isFirst = Abs(firstDistance) < Abs(lastDistance);
+    iz = 0;
+    const Seg &s = isFirst ? seg[0] : seg[NSeg() - 1];
+    const double dt = isFirst ? firstDistance : lastDistance;
+    const double dt2 = dt * dt;
+    if (dt2 >= distanceSquared) return;
+    const Vec3 x(p[0], p[1], p[2] + dt);
+    if (!s.outer.Contains(x)) return;
+    if (s.hasInner() && s.inner.Contains(x)) return;
+    if (fHasPhiCutout && InPhiCutoutWedge(s, x)) return;
+    iz = isFirst ? -1 : 1;
+    distanceSquared = dt2;
+  }
+
+  // ScalarNormalKernel (:999-1068): the normal of the nearest quadrilateral
+  // or endcap.  The reference indexes the outer shell with phiIndex even when
+  // no surface was recorded (iSurf == -1, phiIndex possibly -1); such a point
+  // gets the zero vector here.
+  bool Normal(const Vec3 &point, Vec3 &normal) const override
+  {
+    double safety = kInfLength;
+    const int zMax = NSeg();
+    const int zIndex = FindZSegment(point[2]);
+    if (zIndex < 0) {
+      normal = Vec3(0, 0, -1);
+      return true;
+    }
+    if (zIndex >= zMax) {
+      normal = Vec3(0, 0, 1);
+      return true;
+    }
+    int iSeg = zIndex, iSurf = -1, iz = 0;
+    int phiIndex = FindPhiSegment(point);
+    for (int z = zIndex; z < zMax;) {
+      int iSurfCrt = -1;
+      const double safetySeg = SafetyToZSegmentSquared(z, phiIndex, point, true, iSurfCrt);
+      if (safetySeg < safety) {
+        safety = safetySeg;
+        iSeg = z;
+        iSurf = iSurfCrt;
+      }
+      ++z;
+      const double dz = fZPlanes[z] - point[2];
+      if (dz * dz > safety) break;
+    }
+    for (int z = zIndex - 1; z >= 0; --z) {
+      int iSurfCrt = -1;
+      const double safetySeg = SafetyToZSegmentSquared(z, phiIndex, point, true, iSurfCrt);
+      if (safetySeg < safety) {
+        safety = safetySeg;
+        iSeg = z;
+        iSurf = iSurfCrt;
+      }
+      const double dz = point[2] - fZPlanes[z];
+      if (dz * dz > safety) break;
+    }
+    SafetyToEndcapsSquaredIz(point, safety, iz);
+    if (iz != 0) {
+      normal = Vec3(0, 0, iz);
+      return true;
+    }
+    const Seg &segment = seg[iSeg];
+    if (iSurf >= 0 && iSurf < 2) {
+      normal = segment.phi.normal[iSurf];
+    } else if (iSurf == 2) {
+      const Vec3 n = segment.inner.normal[phiIndex];
+      normal = Vec3(-1. * n.x, -1. * n.y, -1. * n.z);
+    } else if (phiIndex >= 0 && phiIndex < segment.outer.n) {
+      normal = segment.outer.normal[phiIndex];
+    } else {
+      normal = Vec3();
+    }
+    return true;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     mn = Vec3(kInfLength, kInfLength, kInfLength);

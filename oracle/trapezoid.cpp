@@ -159,6 +159,32 @@ struct Trapezoid final : Shape {
     return safety;
   }
 
+  // NormalKernel (TrapezoidImplementation.h:406-462) over
+  // PlaneShell::NormalKernel (VecGeom/base/PlaneShell.h:342-370)
+  bool Normal(const Vec3 &point, Vec3 &normal) const override
+  {
+    const double delta = 1000. * kTolerance;
+    normal = Vec3(0., 0., 0.);
+    double safety = -kInfLength;
+    double dist[4];
+    for (int i = 0; i < 4; ++i) dist[i] = PlaneDist(i, point);
+    for (int i = 0; i < 4; ++i) {
+      const double saf_i = dist[i] - safety;
+      if (Abs(saf_i) < kHalfTolerance && dist[i] >= -kHalfTolerance) normal = normal + Vec3(fA[i], fB[i], fC[i]);
+      if (saf_i > 0.) normal = Vec3(fA[i], fB[i], fC[i]);
+      if (saf_i > 0.) safety = dist[i];
+    }
+    const double safz = Sign(point[2]) * point[2] - fDz;
+    if (Abs(safz - safety) < kHalfTolerance && safz >= -kHalfTolerance)
+      normal = normal + Vec3(0, 0, Sign(point.z));
+    if (safz > safety && safz >= -kHalfTolerance) normal = Vec3(0, 0, Sign(point.z));
+    if (safz > safety) safety = safz;
+    const bool valid = Abs(safety) <= delta;
+    if (normal.Mag2() > 1.0) normal = normal.Normalized();
+    return valid;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     mn = corners[0];

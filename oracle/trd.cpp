@@ -245,6 +245,60 @@ struct Trd final : Shape {
   double SafetyToIn(const Vec3 &p) const override { return Safety(p, false); }
   double SafetyToOut(const Vec3 &p) const override { return Safety(p, true); }
 
+  // UnplacedTrd::Normal (source/UnplacedTrd.cpp:277-352)
+  bool Normal(const Vec3 &point, Vec3 &norm) const override
+  {
+    int noSurfaces = 0;
+    Vec3 sumnorm(0., 0., 0.), vecnorm(0., 0., 0.);
+    const double distz = Abs(Abs(point[2]) - fDZ);
+    const double xnorm = 1.0 / std::sqrt(4 * fDZ * fDZ + (fDX2 - fDX1) * (fDX2 - fDX1));
+    const double ynorm = 1.0 / std::sqrt(4 * fDZ * fDZ + (fDY2 - fDY1) * (fDY2 - fDY1));
+    double distmx = -2.0 * fDZ * point[0] - (fDX2 - fDX1) * point[2] - fDZ * (fDX1 + fDX2);
+    distmx *= xnorm;
+    double distpx = 2.0 * fDZ * point[0] - (fDX2 - fDX1) * point[2] - fDZ * (fDX1 + fDX2);
+    distpx *= xnorm;
+    double distmy = -2.0 * fDZ * point[1] - (fDY2 - fDY1) * point[2] - fDZ * (fDY1 + fDY2);
+    distmy *= ynorm;
+    double distpy = 2.0 * fDZ * point[1] - (fDY2 - fDY1) * point[2] - fDZ * (fDY1 + fDY2);
+    distpy *= ynorm;
+    double safmin = Min(Abs(distmx), Abs(distpx));
+    safmin = Min(safmin, Min(Abs(distmy), Abs(distpy)));
+    safmin = Min(safmin, Abs(distz));
+    const bool valid = safmin <= kHalfTolerance;
+    if (Abs(distmx) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      sumnorm += Vec3(-2.0 * fDZ, 0.0, -(fDX2 - fDX1)) * xnorm;
+    }
+    if (Abs(distpx) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      sumnorm += Vec3(2.0 * fDZ, 0.0, -(fDX2 - fDX1)) * xnorm;
+    }
+    if (Abs(distpy) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      sumnorm += Vec3(0.0, 2.0 * fDZ, -(fDY2 - fDY1)) * ynorm;
+    }
+    if (Abs(distmy) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      sumnorm += Vec3(0.0, -2.0 * fDZ, -(fDY2 - fDY1)) * ynorm;
+    }
+    if (Abs(distz) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      if (point[2] >= 0.)
+        sumnorm += Vec3(0., 0., 1.);
+      else
+        sumnorm += Vec3(0., 0., -1.);
+    }
+    if (noSurfaces == 0)
+      vecnorm = Vec3(0., 0., 1.);
+    else if (noSurfaces == 1)
+      vecnorm = sumnorm;
+    else
+      vecnorm = sumnorm.Unit();
+    norm = vecnorm;
+    return valid;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     double x = Max(fDX1, fDX2), y = Max(fDY1, fDY2);

@@ -344,6 +344,122 @@ struct Tube final : Shape {
     return safety;
   }
 
+  // ApproxSurfaceNormalKernel (:969-1015): the nearest surface of a point
+  // that is not on one
+  Vec3 ApproxSurfaceNormal(const Vec3 &p) const
+  {
+    Vec3 norm(0., 0., 0.);
+    const double radius = p.Perp();
+    double distRMax = Abs(radius - fRmax);
+    double distRMin = kInfLength;
+    if (distRMax < 0.) distRMax = kInfLength;
+    if (fRmin != 0.) {
+      distRMin = Abs(fRmin - radius);
+      if (distRMin < 0.) distRMin = kInfLength;
+    }
+    double distMin = Min(distRMin, distRMax);
+    double distPhi1 = kInfLength, distPhi2 = kInfLength;
+    const Vec3 n1 = fPhiWedge.GetNormal1(), n2 = fPhiWedge.GetNormal2();
+    if (fDphi != kTwoPi) {
+      distPhi1 = p.x * n1.x + p.y * n1.y;
+      distPhi2 = p.x * n2.x + p.y * n2.y;
+      if (distPhi1 < 0.) distPhi1 = kInfLength;
+      if (distPhi2 < 0.) distPhi2 = kInfLength;
+      distMin = Min(distMin, Min(distPhi1, distPhi2));
+    }
+    double distZ = kInfLength;
+    if (p.z < 0.) distZ = Abs(p.z + fZ);
+    if (p.z >= 0.) distZ = Abs(p.z - fZ);
+    distMin = Min(distMin, distZ);
+    if (fDphi != 0.) {
+      if (distMin == distPhi1) norm = -n1;
+      if (distMin == distPhi2) norm = -n2;
+    }
+    if ((distMin == distZ) && (p.z < 0.)) norm = Vec3(0., 0., -1.);
+    if ((distMin == distZ) && (p.z >= 0.)) norm = Vec3(0., 0., 1.);
+    if (Abs(p.z) < (fZ + kTolerance)) {
+      const Vec3 temp(p.x, p.y, 0.);
+      if (distMin == distRMax) norm = temp.Unit();
+      if (fRmin != 0.)
+        if (distMin == distRMin) norm = -temp.Unit();
+    }
+    return norm;
+  }
+
+  // NormalKernel (:1019-1100): the sum of the normals of the surfaces the
+  // point is on, over the square root of their number
+  bool Normal(const Vec3 &p, Vec3 &norm) const override
+  {
+    bool ci = false, co = false;
+    Generic(p, true, ci, co);
+    if (ci || co) {
+      norm = ApproxSurfaceNormal(p);
+      return false;
+    }
+    norm = Vec3();
+    int nosurface = 0;
+    const double x2y2 = Sqrt(p.x * p.x + p.y * p.y);
+    const bool inZ = ((p.z < fZ + kTolerance) && (p.z > -fZ - kTolerance));
+    const bool inR = ((x2y2 >= fRmin - kTolerance) && (x2y2 <= fRmax + kTolerance));
+    if (inR && (Abs(p.z - fZ) <= kTolerance)) {
+      norm = Vec3(0., 0., 1.);
+      nosurface++;
+    }
+    if (inR && (Abs(p.z + fZ) <= kTolerance)) {
+      if (nosurface > 0)
+        norm.z += -1.;
+      else
+        norm = Vec3(0., 0., -1.);
+      nosurface++;
+    }
+    if (fRmin > 0.) {
+      if (inZ && (Abs(x2y2 - fRmin) <= kTolerance)) {
+        const double inv = 1. / x2y2;
+        if (nosurface == 0) {
+          norm.x = -p.x * inv;
+          norm.y = -p.y * inv;
+          norm.z = 0.;
+        } else {
+          norm.x += -p.x * inv;
+          norm.y += -p.y * inv;
+        }
+        nosurface++;
+      }
+    }
+    if (inZ && (Abs(x2y2 - fRmax) <= kTolerance)) {
+      const double inv = 1. / x2y2;
+      if (nosurface > 0) {
+        norm.x += p.x * inv;
+        norm.y += p.y * inv;
+      } else {
+        norm.x = p.x * inv;
+        norm.y = p.y * inv;
+        norm.z = 0.;
+      }
+      nosurface++;
+    }
+    if (fDphi < kTwoPi) {
+      const Vec3 n1 = fPhiWedge.GetNormal1(), n2 = fPhiWedge.GetNormal2();
+      if (inR && Wedge::IsOnSurfaceGeneric(fPhiWedge.fAlong1, n1, p)) {
+        if (nosurface == 0)
+          norm = -n1;
+        else
+          norm += -n1;
+        nosurface++;
+      }
+      if (inR && Wedge::IsOnSurfaceGeneric(fPhiWedge.fAlong2, n2, p)) {
+        if (nosurface == 0)
+          norm = -n2;
+        else
+          norm += -n2;
+        nosurface++;
+      }
+    }
+    if (nosurface > 1) norm = norm / std::sqrt(1. * nosurface);
+    return nosurface != 0;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &mn, Vec3 &mx) const override
   {
     mn = Vec3(-fRmax, -fRmax, -fZ);

@@ -120,6 +120,19 @@ int main()
   holed.Contains(H, hin);
   EXPECT(!hin[0] && hin[1]);
 
+  // a rod in a sleeve: touching along r = 5, so the shared cylinder is
+  // inside the union; the outer wall's normal is radial
+  const int rod = geo.Tube(0., 5., 10., 0., 2 * M_PI), sleeve = geo.Tube(5., 8., 10., 0., 2 * M_PI);
+  PlacedVolume both = geo.Place(geo.NewLogicalVolume(geo.Boolean(0, rod, nullptr, sleeve, nullptr)));
+  std::vector<double> qx{5, 8}, qy{0, 0}, qz{0, 0};
+  SOA3D<> UP(qx.data(), qy.data(), qz.data(), 2);
+  Inside_t uin[2] = {0, 0};
+  both.Inside(UP, uin);
+  EXPECT(uin[0] == 1 && uin[1] == 2);
+  const double wall[3] = {0., 8., 0.};
+  double nrm[3] = {0., 0., 0.};
+  EXPECT(both.Normal(wall, nrm) && nrm[0] == 0. && nrm[1] == 1. && nrm[2] == 0.);
+
   // errors surface as exceptions with the library's message
   bool threw = false;
   try {

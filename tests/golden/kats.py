@@ -806,3 +806,49 @@ ORB_KATS = (
        (ORB_B3, "DistanceToIn", (-1.704652541027918744, 0.92532982039202238411, -2.2886512267834184797),
         (0.55992127966252225324, -0.73207465430510332283, 0.38801399601708530529), ("eq", 0.0))]
 )
+
+
+# ---- Normal (VPlacedVolume::Normal): (family, shape, point, expected normal, expected valid or None) ----
+_P2, _P3 = 1.0 / math.sqrt(2.0), 1.0 / math.sqrt(3.0)
+_BOX1 = (20.0, 30.0, 40.0)
+_FACES = [((20, 0, 0), (1, 0, 0)), ((-20, 0, 0), (-1, 0, 0)), ((0, 30, 0), (0, 1, 0)), ((0, -30, 0), (0, -1, 0)),
+          ((0, 0, 40), (0, 0, 1)), ((0, 0, -40), (0, 0, -1)), ((0, 25, 40), (0, 0, 1)), ((0, 25, -40), (0, 0, -1))]
+_EDGES = [((20, 30, 0), (_P2, _P2, 0)), ((-20, -30, 0), (-_P2, -_P2, 0)), ((20, -30, 0), (_P2, -_P2, 0)),
+          ((-20, 30, 0), (-_P2, _P2, 0)), ((20, 0, 40), (_P2, 0, _P2)), ((-20, 0, -40), (-_P2, 0, -_P2)),
+          ((20, 0, -40), (_P2, 0, -_P2)), ((-20, 0, 40), (-_P2, 0, _P2)), ((0, 30, 40), (0, _P2, _P2)),
+          ((0, -30, -40), (0, -_P2, -_P2)), ((0, 30, -40), (0, _P2, -_P2)), ((0, -30, 40), (0, -_P2, _P2))]
+_CORNERS = [((sx * 20, sy * 30, sz * 40), (sx * _P3, sy * _P3, sz * _P3))
+            for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)]
+_COSA, _SINA = 4 / math.sqrt(17.0), 1 / math.sqrt(17.0)
+_T4 = (45.0, 50.0, 50.0, 0.5 * PI, 0.5 * PI)
+_T4_CASES = [((0., 50., 0.), (_P2, _P2, 0.)), ((0., 45., 0.), (_P2, -_P2, 0.)), ((0., 45., 50.), (_P3, -_P3, _P3)),
+             ((0., 45., -50.), (_P3, -_P3, -_P3)), ((-50., 0., -50.), (-_P3, -_P3, -_P3)),
+             ((-50., 0., 0.), (-_P2, -_P2, 0.))]
+NORMAL_KATS = (
+    # TestBox.cpp:81-167
+    [("box", _BOX1, p, n, True) for p, n in _FACES + _EDGES + _CORNERS]
+    # TestTrd.cpp:115-226: trd1 is the box, trd2 has inclined x and y faces
+    + [("trd", (20.0, 20.0, 30.0, 30.0, 40.0), p, n, True) for p, n in _FACES + _EDGES + _CORNERS]
+    + [("trd", (10.0, 30.0, 20.0, 40.0, 40.0), p, n, None) for p, n in [
+        ((20, 0, 0), (_COSA, 0, -_SINA)), ((-20, 0, 0), (-_COSA, 0, -_SINA)), ((0, 30, 0), (0, _COSA, -_SINA)),
+        ((0, -30, 0), (0, -_COSA, -_SINA)), ((0, 0, 40), (0, 0, 1)), ((0, 0, -40), (0, 0, -1)),
+        ((0, 25, 40), (0, 0, 1)), ((0, 25, -40), (0, 0, -1))]]
+    # TestTube.cpp:160-175
+    + [("tube", (0.0, 50.0, 50.0, 0.0, 2 * PI), (50, 0, 0), (1, 0, 0), None)]
+    + [("tube", _T4, p, n, None) for p, n in _T4_CASES]
+    + [("tube", (0.0, 50.0, 50.0, 0.5 * PI, 1.5 * PI), (0., 0., 0.), (_P2, _P2, 0.), None)]
+    # TestCons.cpp:277-316 (printed checks): cn1 is the tube t4 as a cone
+    + [("cone", (45.0, 50.0, 45.0, 50.0, 50.0, 0.5 * PI, 0.5 * PI), p, n, None) for p, n in _T4_CASES]
+    + [("cone", (45.0, 50.0, 45.0, 50.0, 50.0, 0.5 * PI, 1.5 * PI), (50., 0., 0.), (_P2, _P2, 0.), None),
+       ("cone", (0.0, 150.0, 0.0, 150.0, 50.0, 0.0, 2 * PI), (0., 0., 50.), (0., 0., 1.), None)]
+    # TestOrb.cpp:87-108
+    + [("orb", (9.0,), p, n, True) for p, n in [((9, 0, 0), (1, 0, 0)), ((0, 9, 0), (0, 1, 0)), ((0, 0, 9), (0, 0, 1)),
+                                               ((-9, 0, 0), (-1, 0, 0)), ((0, -9, 0), (0, -1, 0)),
+                                               ((0, 0, -9), (0, 0, -1))]]
+    # TestPolycone.cpp:190-199
+    + [("polycone", PSIMPLE, p, n, None) for p, n in [((70, 0, -5), (1, 0, 0)), ((-70, 0, -5), (-1, 0, 0)),
+                                                      ((0, 70, -5), (0, 1, 0)), ((0, 0, 10), (0, 0, 1)),
+                                                      ((0, 0, -10), (0, 0, -1))]]
+)
+# TestPolycone.cpp:171-186: away from the surface Normal is invalid, with a unit vector
+NORMAL_OFF_SURFACE = [("polycone", PSIMPLE, p) for p in [(70, 70, -5), (70, 70, -10), (0, 0, 100), (0, 0, -100)]]

@@ -84,6 +84,17 @@ SHAPES.update({
                                       ("box", (20.0, 20.0, 40.0)), None, transform12((3.0, 0.0, 0.0)))),
     "bool_cone_and_orb": ("boolean", ("intersection", ("cone", (0.0, 10.0, 5.0, 25.0, 40.0, 0.0, 2 * PI)),
                                       ("orb", (30.0,)), None, transform12((0.0, 0.0, 15.0), angle=0.0))),
+    # unions: a rod in a sleeve (they touch along r = 10: the Normal branch of
+    # Inside), a box with a rotated box through one face, a sector and a
+    # polycone side by side
+    "bool_rod_or_sleeve": ("boolean", ("union", ("tube", (0.0, 10.0, 20.0, 0.0, 2 * PI)),
+                                       ("tube", (10.0, 20.0, 15.0, 0.0, 2 * PI)), None, None)),
+    "bool_box_or_rot_box": ("boolean", ("union", ("box", (20.0, 15.0, 10.0)), ("box", (8.0, 8.0, 8.0)),
+                                        None, transform12((20.0, 5.0, 0.0), angle=0.4))),
+    "bool_sector_or_pcon": ("boolean", ("union", ("tube", (5.0, 25.0, 30.0, 0.3, 1.2 * PI)),
+                                        ("polycone", (0.0, 2 * PI, (-10.0, 0.0, 15.0), (0.0, 2.0, 0.0),
+                                                      (12.0, 18.0, 9.0))),
+                                        None, transform12((-10.0, 4.0, 0.0), angle=0.0))),
 })
 
 SHAPES.update({

@@ -123,7 +123,7 @@ def test_numbers_read_like_stod():
     assert r.of("box")[0][1] == 0.0
 
 
-@pytest.mark.parametrize("name, what", [("scaledPhysvol", "scale"), ("solidUnion", "union")])
+@pytest.mark.parametrize("name, what", [("scaledPhysvol", "scale")])
 def test_unsupported_is_an_error(name, what):
     with pytest.raises(gdml.GdmlError, match=what):
         read(name)
@@ -147,6 +147,13 @@ def test_boolean_operands_and_placement():
     R = np.array(rtr[3:]).reshape(3, 3)
     assert np.allclose(R @ R.T, np.eye(3), atol=1e-15) and not np.allclose(R, np.eye(3))
     assert r.of("box")[0] == ("box", 50.0, 50.0, 50.0)
+
+
+def test_union_operands_and_placement():
+    r, world, solids, _ = read("solidUnion")
+    (_, op, first, second, ltr, rtr), = r.of("boolean")
+    assert (op, first, second, ltr) == ("union", solids["firstHalf"], solids["secondHalf"], None)
+    assert [b[1:] for b in r.of("box")] == [(50.0, 50.0, 50.0), (35.0, 35.0, 35.0)]
 
 
 def test_orb_radius_and_constant():

@@ -90,3 +90,21 @@ def test_shape_parity(ctx, name, placed):
         got = gpu_query(ctx, pv_g, m, pts, dirs)
         bad = compare(ref, got)
         assert bad.size == 0, (m, bad.size, bad[:5], ref[bad[:5]], got[bad[:5]])
+    if kind != "tessellated":  # VPlacedVolume::Normal: every family but the tessellated solid
+        # the inputs above plus the rays' surface points without the jitter
+        # (on the surface to rounding, where Normal is valid)
+        half, cen = 0.5 * (mx - mn), 0.5 * (mx + mn)
+        p0 = sampling.points_in_box(7, 1, 50_000, half, 1.5, cen)
+        d0 = sampling.isotropic_directions(7, 2, 50_000)
+        din = geo.query(pv_local, "DistanceToIn", p0, d0)
+        dout = geo.query(pv_local, "DistanceToOut", p0, d0)
+        t = np.where((din > 0) & (din < 1e30), din, np.where((dout > 0) & (dout < 1e30), dout, 0.0))
+        on = np.stack([p0[k] + t * d0[k] for k in range(3)])
+        if placed:
+            on = SHIFT[:, None] + transform12(SHIFT)[3:].reshape(3, 3) @ on
+        npts = tuple(np.ascontiguousarray(np.concatenate([pts[k], on[k]])) for k in range(3))
+        ref = geo.normal(pv_o, npts)
+        got = ctx.normal(pv_g, vg.SOA3D(*npts))
+        for k, c in enumerate("xyzv"):
+            bad = compare(ref[k], got[k])
+            assert bad.size == 0, ("Normal", c, bad.size, bad[:5], ref[k][bad[:5]], got[k][bad[:5]])

@@ -1,0 +1,177 @@
+"""Oracle Normal (VPlacedVolume::Normal) against the reference's known answers,
+and properties the geometry fixes: unit length, the gradient of the
+surface's implicit function on smooth parts, and the boolean rules."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import kats
+
+
+def _normal(geo, kind, params, p, tr=None):
+    pv = geo.place(geo.logical_volume(getattr(geo, kind)(*params)), tr)
+    nx, ny, nz, valid = geo.normal(pv, [np.array([float(c)]) for c in p])
+    return (nx[0], ny[0], nz[0]), bool(valid[0])
+
+
+@pytest.mark.parametrize("case", kats.NORMAL_KATS, ids=[f"{c[0]}-{i}" for i, c in enumerate(kats.NORMAL_KATS)])
+def test_normal_kat(case):
+    kind, params, p, want, want_valid = case
+    n, valid = _normal(oracle.Geometry(), kind, params, p)
+    assert all(kats.approx_equal(a, b) for a, b in zip(n, want)), (n, want)
+    if want_valid is not None:
+        assert valid == want_valid
+
+
+@pytest.mark.parametrize("case", kats.NORMAL_OFF_SURFACE)
+def test_normal_off_surface(case):
+    kind, params, p = case
+    n, valid = _normal(oracle.Geometry(), kind, params, p)
+    assert not valid and kats.approx_equal(n[0] ** 2 + n[1] ** 2 + n[2] ** 2, 1.0)
+
+
+def _surface_points(rng, n, r):
+    u = rng.normal(size=(n, 3))
+    return u / np.linalg.norm(u, axis=1)[:, None] * r
+
+
+def test_orb_normal_is_radial():
+    geo = oracle.Geometry()
+    pv = geo.place(geo.logical_volume(geo.orb(7.0)))
+    p = _surface_points(np.random.default_rng(3), 500, 7.0)
+    nx, ny, nz, valid = geo.normal(pv, p.T)
+    assert valid.all()
+    np.testing.assert_allclose(np.stack([nx, ny, nz], 1), p / 7.0, rtol=0, atol=1e-15)
+
+
+def test_tube_and_cone_side_normals():
+    """On the curved side away from edges the normal is the gradient of the
+    surface: radial for the tube, tilted by the half-angle for the cone."""
+    geo = oracle.Geometry()
+    rng = np.random.default_rng(5)
+    phi = rng.uniform(0, 2 * np.pi, 300)
+    z = rng.uniform(-20, 20, 300)
+    tube = geo.place(geo.logical_volume(geo.tube(0.0, 12.0, 30.0, 0.0, 2 * np.pi)))
+    nx, ny, nz, valid = geo.normal(tube, (12 * np.cos(phi), 12 * np.sin(phi), z))
+    assert valid.all()
+    np.testing.assert_allclose(np.stack([nx, ny, nz], 1), np.stack([np.cos(phi), np.sin(phi), 0 * phi], 1),
+                               atol=1e-12)
+    # cone rmax 10 at z=-30 to 20 at z=+30: r(z) = 15 + z/6, outward normal ~ (cos, sin, -1/6)
+    cone = geo.place(geo.logical_volume(geo.cone(0.0, 10.0, 0.0, 20.0, 30.0, 0.0, 2 * np.pi)))
+    r = 15 + z / 6
+    nx, ny, nz, valid = geo.normal(cone, (r * np.cos(phi), r * np.sin(phi), z))
+    want = np.stack([np.cos(phi), np.sin(phi), -np.full_like(phi, 1 / 6)], 1)
+    want /= np.linalg.norm(want, axis=1)[:, None]
+    assert valid.all()
+    np.testing.assert_allclose(np.stack([nx, ny, nz], 1), want, atol=1e-9)
+
+
+def test_placed_normal_is_rotated():
+    """A placed box's normal is the local normal turned by the placement."""
+    geo = oracle.Geometry()
+    c, s = math.cos(0.3), math.sin(0.3)
+    # rotation about z by 0.3; Transformation3D takes master to local as
+    # local = R^T (p - t) with the row-major R of its nine rotation entries
+    tr = np.array([1.0, 2.0, 3.0, c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0])
+    pv = geo.place(geo.logical_volume(geo.box(5.0, 6.0, 7.0)), tr)
+    # the local +x face centre (5, 0, 0) in the master frame
+    local = np.array([5.0, 0.0, 0.0])
+    R = tr[3:].reshape(3, 3)
+    master = R @ local + tr[:3]
+    nx, ny, nz, valid = geo.normal(pv, [np.array([v]) for v in master])
+    assert valid[0]
+    np.testing.assert_allclose([nx[0], ny[0], nz[0]], R @ np.array([1.0, 0.0, 0.0]), atol=1e-14)
+
+
+def test_polyhedron_normals():
+    """A hexagonal prism: side faces have the face normals, endcaps +-z."""
+    geo = oracle.Geometry()
+    sphi, dphi, ns, z, rmin, rmax = kats.HEXPRISM
+    pv = geo.place(geo.logical_volume(geo.polyhedron(sphi, dphi, ns, z, rmin, rmax)))
+    r = rmax[0]
+    for k in range(6):
+        a = (k + 0.5) * np.pi / 3  # face centre angle (first side centred on phi 30 degrees)
+        p = (r * math.cos(a), r * math.sin(a), 1.0)
+        nx, ny, nz, valid = geo.normal(pv, [np.array([v]) for v in p])
+        np.testing.assert_allclose([nx[0], ny[0], nz[0]], [math.cos(a), math.sin(a), 0.0], atol=1e-12)
+    for zz, want in ((10.0, 1.0), (-10.0, -1.0)):
+        nx, ny, nz, valid = geo.normal(pv, [np.array([v]) for v in (1.0, 2.0, zz)])
+        np.testing.assert_allclose([nx[0], ny[0], nz[0]], [0.0, 0.0, want], atol=0)
+
+
+def test_trapezoid_normal_box_like():
+    """A trapezoid with parallel faces is a box: its face normals are the box's."""
+    geo = oracle.Geometry()
+    trap = (40.0, 0.0, 0.0, 30.0, 20.0, 20.0, 0.0, 30.0, 20.0, 20.0, 0.0)
+    pts = [p for p, _ in kats._FACES + kats._EDGES + kats._CORNERS]
+    pa = [np.array([float(p[k]) for p in pts]) for k in range(3)]
+    box = geo.place(geo.logical_volume(geo.box(*kats._BOX1)))
+    tp = geo.place(geo.logical_volume(geo.trapezoid(*trap)))
+    a, b = geo.normal(box, pa), geo.normal(tp, pa)
+    for k in range(3):
+        np.testing.assert_allclose(a[k], b[k], atol=1e-15)
+    assert (b[3] == 1).all()
+
+
+def test_union_of_touching_solids():
+    """Union Inside: on the cylinder shared by a rod and the sleeve around it
+    the normals are opposite, so the point is inside; on an outer face it is
+    on the surface."""
+    geo = oracle.Geometry()
+    rod = geo.tube(0.0, 10.0, 20.0, 0.0, 2 * np.pi)
+    sleeve = geo.tube(10.0, 20.0, 20.0, 0.0, 2 * np.pi)
+    u = geo.place(geo.logical_volume(geo.boolean("union", rod, sleeve)))
+    pts = [np.array([10.0, 0.0, 20.0, 15.0, 25.0]), np.array([0.0, 20.0, 0.0, 0.0, 0.0]),
+           np.array([0.0, 0.0, 0.0, 0.0, 0.0])]
+    assert list(geo.query(u, "Inside", pts)) == [1, 2, 2, 1, 3]
+    # the union's Normal on its outer faces is the operand's
+    nx, ny, nz, valid = geo.normal(u, [np.array([0.0, 20.0]), np.array([0.0, 0.0]), np.array([20.0, 0.0])])
+    np.testing.assert_allclose(nx, [0.0, 1.0])
+    np.testing.assert_allclose(nz, [1.0, 0.0])
+    assert valid.all()
+
+
+def test_union_normal_transforms_twice():
+    """The reference's union Inside transforms the point into an operand's
+    frame and then calls that operand's placed Normal, which transforms it
+    again (BooleanUnionImplementation.h:94-100, PlacedVolume.cpp:126-137).
+    For two boxes touching along x, with the right one translated, the right
+    normal is taken 20 mm further off and is not the face's, so the shared
+    face stays a surface point."""
+    geo = oracle.Geometry()
+    a = geo.box(10.0, 10.0, 10.0)
+    b = geo.box(10.0, 10.0, 10.0)
+    shift = np.array([20.0, 0.0, 0.0, 1, 0, 0, 0, 1, 0, 0, 0, 1])
+    u = geo.place(geo.logical_volume(geo.boolean("union", a, b, None, shift)))
+    pts = [np.array([10.0, -10.0, 25.0, 35.0]), np.array([0.0, 0.0, 0.0, 0.0]), np.array([0.0, 0.0, 0.0, 0.0])]
+    assert list(geo.query(u, "Inside", pts)) == [2, 2, 1, 3]
+
+
+def test_union_distances_and_safeties():
+    """Two overlapping boxes along x: the union's distances are those of the
+    combined box where it is one, and SafetyToIn is the nearer operand's."""
+    geo = oracle.Geometry()
+    a = geo.box(10.0, 5.0, 5.0)
+    b = geo.box(10.0, 5.0, 5.0)
+    shift = np.array([15.0, 0.0, 0.0, 1, 0, 0, 0, 1, 0, 0, 0, 1])
+    u = geo.place(geo.logical_volume(geo.boolean("union", a, b, None, shift)))
+    big = geo.place(geo.logical_volume(geo.box(17.5, 5.0, 5.0)), np.array([7.5, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1.0]))
+    rng = np.random.default_rng(9)
+    p = [rng.uniform(-30, 40, 2000), rng.uniform(-9, 9, 2000), rng.uniform(-9, 9, 2000)]
+    d = rng.normal(size=(3, 2000))
+    d /= np.linalg.norm(d, axis=0)
+    assert (geo.query(u, "Inside", p) == geo.query(big, "Inside", p)).all()
+    din_u, din_b = geo.query(u, "DistanceToIn", p, list(d)), geo.query(big, "DistanceToIn", p, list(d))
+    hit = (din_b < 1e30) & (geo.query(u, "Inside", p) == 3)
+    np.testing.assert_allclose(din_u[hit], din_b[hit], atol=1e-9)
+    inside = geo.query(u, "Inside", p) == 1
+    pin = [c[inside] for c in p]
+    dout_u = geo.query(u, "DistanceToOut", pin, [c[inside] for c in d])
+    # the union pushes through the overlap: one kPushTolerance per operand crossed, less one at the end
+    dout_b = geo.query(big, "PlacedDistanceToOut", pin, [c[inside] for c in d])
+    assert np.all(np.abs(dout_u - dout_b) <= 2e-6 + 1e-9)
+    sa = geo.query(geo.place(geo.logical_volume(a)), "SafetyToIn", p)
+    sb = geo.query(geo.place(geo.logical_volume(b), shift), "SafetyToIn", p)
+    np.testing.assert_array_equal(geo.query(u, "SafetyToIn", p), np.minimum(sa, sb))

@@ -109,6 +109,7 @@ def _declare(L):
         "vgm_inside": [_P, _I, _P, _P, _P, _SZ, _P, _U],
         "vgm_safety_to_in": [_P, _I, _P, _P, _P, _SZ, _P, _U],
         "vgm_safety_to_out": [_P, _I, _P, _P, _P, _SZ, _P, _U],
+        "vgm_normal": [_P, _I, _P, _P, _P, _SZ, _P, _P, _P, _P, _U],
         "vgm_distance_to_in": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
         "vgm_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
         "vgm_placed_distance_to_out": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _U],
@@ -271,9 +272,9 @@ class Context:
         return self._new(self._L.vgm_orb_create, r)
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None) -> int:
-        """UnplacedBooleanVolume: op "intersection" or "subtraction"; left and
-        right are shape ids or (kind, args) pairs built here first."""
-        ops = {"intersection": 1, "subtraction": 2}
+        """UnplacedBooleanVolume: op "union", "intersection" or "subtraction";
+        left and right are shape ids or (kind, args) pairs built here first."""
+        ops = {"union": 0, "intersection": 1, "subtraction": 2}
         ids = [getattr(self, s[0])(*s[1]) if isinstance(s, tuple) else s for s in (left, right)]
         trs = [None if t is None else np.ascontiguousarray(t, dtype=np.float64) for t in (left_tr, right_tr)]
         return self._new(self._L.vgm_boolean_create, ops[op], ids[0], _ptr(trs[0]), ids[1], _ptr(trs[1]))
@@ -429,6 +430,14 @@ class Context:
 
     def safety_to_out(self, pv, pts, out=None, mixed=False, asynchronous=False):
         return self._q1(self._L.vgm_safety_to_out, pv, pts, np.float64, out, mixed, asynchronous)
+
+    def normal(self, pv, pts: SOA3D, asynchronous=False):
+        """VPlacedVolume::Normal of master-frame points: (nx, ny, nz, valid)."""
+        nx, ny, nz = (self._alloc(pts.x, pts.n, np.float64) for _ in range(3))
+        valid = self._alloc(pts.x, pts.n, np.uint8)
+        check(self._L.vgm_normal(self.handle, pv, _ptr(pts.x), _ptr(pts.y), _ptr(pts.z), pts.n, _ptr(nx), _ptr(ny),
+                                 _ptr(nz), _ptr(valid), self._flags(pts.x, False, asynchronous)))
+        return nx, ny, nz, valid
 
     def distance_to_in(self, pv, pts, dirs, step_max=None, out=None, mixed=False, asynchronous=False):
         return self._q2(self._L.vgm_distance_to_in, pv, pts, dirs, step_max, out, mixed, asynchronous)

@@ -506,6 +506,47 @@ int vgm_safety_to_out(vgm_ctx *c, int pv, const double *x, const double *y, cons
   return run_soa(c, pv, kSafetyToOut, false, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 8, flags);
 }
 
+int vgm_normal(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, double *nx,
+               double *ny, double *nz, uint8_t *valid, unsigned flags)
+{
+  int rc = check_pv(c, pv);
+  if (rc) return rc;
+  if (n && (!x || !y || !z || !nx || !ny || !nz || !valid)) return fail(VGM_EINVAL, "null input or output pointer");
+  const PlacedRec &P = c->host.pvs[pv];
+  const ShapeRec &S = c->host.shapes[c->host.lvs[P.lv].shape];
+  if (!family_has_normal(S.type)) return fail(VGM_EINVAL, "this shape family has no Normal");
+  if (flags & VGM_MIXED) return fail(VGM_EINVAL, "Normal is FP64 only");
+  if (n == 0) return VGM_OK;
+  VGM_HIP(hipSetDevice(c->device));
+  rc = sync_geometry(c);
+  if (rc) return rc;
+  const void *params = c->dev.pool + S.off;
+  if (flags & VGM_DEVICE_PTRS) {
+    NormalArgs a{x, y, z, n, nx, ny, nz, valid};
+    VGM_HIP(launch_normal(c->stream, S, params, P.tr, a));
+    if (!(flags & VGM_ASYNC)) VGM_HIP(hipStreamSynchronize(c->stream));
+    return VGM_OK;
+  }
+  const size_t col = align256(n * sizeof(double));
+  rc = ensure_scratch(c, 6 * col + align256(n));
+  if (rc) return rc;
+  char *b = c->scratch.buf;
+  double *d[6];
+  for (int k = 0; k < 6; ++k) d[k] = reinterpret_cast<double *>(b + k * col);
+  uint8_t *dv = reinterpret_cast<uint8_t *>(b + 6 * col);
+  const double *src[3] = {x, y, z};
+  for (int k = 0; k < 3; ++k)
+    VGM_HIP(hipMemcpyAsync(d[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  NormalArgs a{d[0], d[1], d[2], n, d[3], d[4], d[5], dv};
+  VGM_HIP(launch_normal(c->stream, S, params, P.tr, a));
+  double *dst[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; ++k)
+    VGM_HIP(hipMemcpyAsync(dst[k], d[3 + k], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  VGM_HIP(hipMemcpyAsync(valid, dv, n, hipMemcpyDeviceToHost, c->stream));
+  VGM_HIP(hipStreamSynchronize(c->stream));
+  return VGM_OK;
+}
+
 int vgm_query_noalloc(vgm_ctx *c, int pv, int method, const double *px, const double *py, const double *pz,
                       const double *dx, const double *dy, const double *dz, const double *step, size_t n, void *out,
                       unsigned flags)

@@ -201,15 +201,15 @@ static void transformed_extent(const ShapeRec &s, const Transform &T, double mn[
       }
 }
 
-// UnplacedBooleanVolume<kIntersection / kSubtraction>(left, right)
+// UnplacedBooleanVolume<kUnion / kIntersection / kSubtraction>(left, right)
 // (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:19-31) and its
 // Extent (source/UnplacedBooleanVolume.cpp:444-482): the operand blocks are
 // copied after the header so that the boolean's block is self-contained
 int build_boolean(HostGeometry &g, int op, int left, const double *ltr, int right, const double *rtr,
                   std::string &err)
 {
-  if (op != kBoolIntersection && op != kBoolSubtraction) {
-    err = "boolean operation must be intersection (1) or subtraction (2); union is not implemented";
+  if (op != kBoolUnion && op != kBoolIntersection && op != kBoolSubtraction) {
+    err = "boolean operation must be union (0), intersection (1) or subtraction (2)";
     return -1;
   }
   const int n = int(g.shapes.size());
@@ -246,6 +246,9 @@ int build_boolean(HostGeometry &g, int op, int left, const double *ltr, int righ
   for (int k = 0; k < 3; ++k) {
     if (op == kBoolSubtraction) {
       lo[k] = mnL[k], hi[k] = mxL[k];
+    } else if (op == kBoolUnion) {
+      lo[k] = mnL[k] < mnR[k] ? mnL[k] : mnR[k];
+      hi[k] = mxL[k] > mxR[k] ? mxL[k] : mxR[k];
     } else {
       lo[k] = mnL[k] > mnR[k] ? mnL[k] : mnR[k];
       hi[k] = mxL[k] < mxR[k] ? mxL[k] : mxR[k];

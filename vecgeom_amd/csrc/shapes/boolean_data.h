@@ -10,7 +10,7 @@ namespace vgm {
 enum BooleanOp : int32_t { kBoolUnion = 0, kBoolIntersection = 1, kBoolSubtraction = 2 };  // BooleanOperation
 
 struct BooleanData {
-  int32_t op;            // BooleanOp (union is not built)
+  int32_t op;            // BooleanOp
   int32_t ltype, rtype;  // operand shape types
   int32_t loff, roff;    // operand parameter blocks: offsets in doubles from the start of this block
   int32_t pad;

@@ -87,6 +87,22 @@ struct BoxShape {
   {
     return vmin3(R(b.dim[0]) - vabs(p.x), R(b.dim[1]) - vabs(p.y), R(b.dim[2]) - vabs(p.z));
   }
+
+  // NormalKernel (BoxImplementation.h:172-189): the faces within tolerance of
+  // the nearest one, averaged on edges and corners
+  VGM_DEV static bool normal(const Data &b, V3<double> p, V3<double> &n)
+  {
+    const double sx = vabs(vabs(p.x) - b.dim[0]), sy = vabs(vabs(p.y) - b.dim[1]), sz = vabs(vabs(p.z) - b.dim[2]);
+    const double safmin = vmin3(sx, sy, sz);
+    const bool valid = safmin < kHalfTolerance;
+    V3<double> v{0.0, 0.0, 0.0};
+    if (sx - safmin < kHalfTolerance) v.x = vsign(p.x);
+    if (sy - safmin < kHalfTolerance) v.y = vsign(p.y);
+    if (sz - safmin < kHalfTolerance) v.z = vsign(p.z);
+    if (v3_mag2(v) > 1.0) v = v3_normalized(v);
+    n = v;
+    return valid;
+  }
 };
 
 }  // namespace vgm

@@ -640,6 +640,120 @@ struct ConeShape {
     if (vabs(safety) < R(kTolerance)) safety = R(0);
     return safety;
   }
+
+  // ConeStruct::Normal (ConeStruct.h:177-306).  The reference's _frmin /
+  // _frmax are the constructor's radii: rmin as given, rmax before the
+  // kConeTolerance bump.  The phi-plane normals come from the normalised
+  // start and end angles (along1/along2 hold their cosines and sines).
+  VGM_DEV static bool normal(const Data &c, V3<double> p, V3<double> &norm)
+  {
+    const double frmin1 = c.rmin1, frmin2 = c.rmin2, frmax2 = c.orig_rmax2, frmax1 = c.orig_rmax1;
+    int noSurfaces = 0;
+    double distSPhi = kInfLength, distEPhi = kInfLength;
+    V3<double> sumnorm{0., 0., 0.};
+    V3<double> nR{0., 0., 0.}, nr{0., 0., 0.}, nPs{0., 0., 0.}, nPe{0., 0., 0.};
+    norm = sumnorm;
+    double distZ = vabs(p.z) - c.dz;
+    const double rho = __builtin_sqrt(p.x * p.x + p.y * p.y);
+    double distRMin, distRMax;
+    const double pRMin = rho - p.z * c.tan_rmin;
+    const double widRMin = c.rmin2 - c.dz * c.tan_rmin;
+    if (vabs(frmin1 - frmin2) < c.inner_tol)
+      distRMin = (rho - frmin2);
+    else
+      distRMin = (pRMin - widRMin) / c.sec_rmin;
+    const double pRMax = rho - p.z * c.tan_rmax;
+    const double widRMax = c.rmax2 - c.dz * c.tan_rmax;
+    if (vabs(frmax1 - frmax2) < c.outer_tol)
+      distRMax = (rho - frmax2);
+    else
+      distRMax = (pRMax - widRMax) / c.sec_rmax;
+    const bool hasRmin = c.rmin1 != 0. || c.rmin2 != 0.;
+    bool inside = distZ < kTolerance && distRMax < c.outer_tol;
+    if (hasRmin) inside &= distRMin > -c.inner_tol;
+    distZ = vabs(distZ);
+    distRMax = vabs(distRMax);
+    distRMin = vabs(distRMin);
+    double distNearest = distZ;
+    V3<double> normNearest{0., 0., 1.};
+    if (p.z < 0.) normNearest = V3<double>{0., 0., -1.};
+    const bool fullPhi = c.dphi == kTwoPi;
+    if (!fullPhi) {
+      if (rho != 0.) {
+        double pPhi = atan2(p.y, p.x);
+        if (pPhi < c.sphi - kHalfTolerance)
+          pPhi += 2 * kPi;
+        else if (pPhi > c.sphi + c.dphi + kHalfTolerance)
+          pPhi -= 2 * kPi;
+        distSPhi = rho * (pPhi - c.sphi);
+        distEPhi = rho * (pPhi - c.sphi - c.dphi);
+        inside = inside && (distSPhi > -kTolerance) && (distEPhi < kTolerance);
+        distSPhi = vabs(distSPhi);
+        distEPhi = vabs(distEPhi);
+      } else if (!(c.rmin1 != 0.) || !(c.rmin2 != 0.)) {
+        distSPhi = 0.;
+        distEPhi = 0.;
+      }
+      nPs = V3<double>{c.along1y, -c.along1x, 0.};
+      nPe = V3<double>{-c.along2y, c.along2x, 0.};
+    }
+    if (rho > kHalfTolerance) {
+      nR = V3<double>{p.x / rho / c.sec_rmax, p.y / rho / c.sec_rmax, -c.tan_rmax / c.sec_rmax};
+      if (hasRmin) nr = V3<double>{-p.x / rho / c.sec_rmin, -p.y / rho / c.sec_rmin, c.tan_rmin / c.sec_rmin};
+    }
+    auto acc = [&](const V3<double> &v) {
+      sumnorm.x += v.x;
+      sumnorm.y += v.y;
+      sumnorm.z += v.z;
+    };
+    if (inside && distZ <= kHalfTolerance) {
+      noSurfaces++;
+      if (p.z >= 0.)
+        acc(V3<double>{0., 0., 1.});
+      else
+        sumnorm = V3<double>{0., 0., -1.};
+    }
+    if (inside && distRMax <= c.outer_tol) {
+      noSurfaces++;
+      acc(nR);
+    } else if (noSurfaces == 0 && distRMax < distNearest) {
+      distNearest = distRMax;
+      normNearest = nR;
+    }
+    if (hasRmin) {
+      if (inside && distRMin <= c.inner_tol) {
+        noSurfaces++;
+        acc(nr);
+      } else if (noSurfaces == 0 && distRMin < distNearest) {
+        distNearest = distRMin;
+        normNearest = nr;
+      }
+    }
+    if (!fullPhi) {
+      if (inside && distSPhi <= kHalfTolerance) {
+        noSurfaces++;
+        acc(nPs);
+      } else if (noSurfaces == 0 && distSPhi < distNearest) {
+        distNearest = distSPhi;
+        normNearest = nPs;
+      }
+      if (inside && distEPhi <= kHalfTolerance) {
+        noSurfaces++;
+        acc(nPe);
+      } else if (noSurfaces == 0 && distEPhi < distNearest) {
+        normNearest = nPe;
+      }
+    }
+    if (noSurfaces == 0)
+      norm = normNearest;
+    else if (noSurfaces == 1)
+      norm = sumnorm;
+    else
+      norm = v3_unit(sumnorm);
+    bool valid = noSurfaces != 0;
+    if (noSurfaces > 2) valid = false;
+    return valid;
+  }
 };
 
 }  // namespace vgm

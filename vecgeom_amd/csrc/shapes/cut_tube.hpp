@@ -206,6 +206,18 @@ struct CutTubeShape {
     if (saftube < safety) safety = saftube;
     return safety;
   }
+
+  // NormalKernel (CutTubeImplementation.h:340-360): the tube's normal where
+  // the tube is nearer than the cut planes, else the cut plane of the
+  // point's half
+  VGM_DEV static bool normal(const Data &c, V3<double> p, V3<double> &n)
+  {
+    const double safcut = vmin(-dist0<double>(c, p), -dist1<double>(c, p));
+    const double saftube = TubeShape::safety_to_out<double>(c.tube, p);
+    if (vabs(saftube) < vabs(safcut)) return TubeShape::normal(c.tube, p, n);
+    n = p.z < 0 ? V3<double>{c.n0x, c.n0y, c.n0z} : V3<double>{c.n1x, c.n1y, c.n1z};
+    return true;
+  }
 };
 
 }  // namespace vgm

@@ -123,6 +123,16 @@ struct OrbShape {
     if ((rad > R(o.r) - R(kTolerance)) && (rad < R(o.r) + R(kTolerance))) safety = R(0);
     return safety;
   }
+
+  // NormalKernel (OrbImplementation.h:227-240): radial
+  VGM_DEV static bool normal(const Data &o, V3<double> p, V3<double> &n)
+  {
+    const double rad2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    const double invRadius = 1. / __builtin_sqrt(rad2);
+    n = V3<double>{p.x * invRadius, p.y * invRadius, p.z * invRadius};
+    const double tolRMaxO = o.r + kTolerance, tolRMaxI = o.r - kTolerance;
+    return (rad2 <= tolRMaxO * tolRMaxO) && (rad2 >= tolRMaxI * tolRMaxI);
+  }
 };
 
 }  // namespace vgm

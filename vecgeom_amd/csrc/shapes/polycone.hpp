@@ -251,6 +251,46 @@ struct PolyconeShape {
     if (safety < R(kTolerance)) return R(0);
     return neighbour_safety<R>(d, index, point, safety);
   }
+
+  // UnplacedPolycone::Normal (source/UnplacedPolycone.cpp:782-832): the
+  // section's cone normal; on the plane between two sections the next
+  // section's too
+  VGM_DEV static bool normal(const Data &d, V3<double> point, V3<double> &norm)
+  {
+    bool valid = true;
+    const int index = section_index<double>(d, point.z - kTolerance);
+    if (index < 0) {
+      valid = false;
+      if (index == -1) norm = V3<double>{0., 0., -1.};
+      if (index == -2) norm = V3<double>{0., 0., 1.};
+      return valid;
+    }
+    valid = ConeShape::normal(d.sec()[index], shifted<double>(d, index, point), norm);
+    if (index + 1 < d.nsec && vabs(point.z - d.zs()[index + 1]) < kTolerance) {
+      V3<double> norm2{0., 0., 0.};
+      const bool valid2 = ConeShape::normal(d.sec()[index + 1], shifted<double>(d, index + 1, point), norm2);
+      if (!valid && valid2) {
+        norm = norm2;
+        valid = valid2;
+      }
+      if (valid && valid2) {
+        const double s = kTolerance * 10;
+        const V3<double> q{point.x + s * norm2.x, point.y + s * norm2.y, point.z + s * norm2.z};
+        const bool c2 = ConeShape::contains<double>(d.sec()[index + 1], shifted<double>(d, index + 1, q));
+        if (c2) {
+          norm = norm2;
+        } else {
+          norm = V3<double>{norm.x + norm2.x, norm.y + norm2.y, norm.z + norm2.z};
+          if (v3_mag2(norm) < kTolerance) norm = V3<double>{point.x, point.y, 0};
+        }
+      }
+    }
+    if (valid) {
+      const double m = __builtin_sqrt(v3_mag2(norm));
+      norm = V3<double>{norm.x / m, norm.y / m, norm.z / m};
+    }
+    return valid;
+  }
 };
 
 }  // namespace vgm

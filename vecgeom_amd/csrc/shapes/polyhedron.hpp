@@ -259,33 +259,38 @@ struct PolyhedronShape {
 
   // ScalarSafetyToZSegmentSquared (:448-560).  The reference's branches pick
   // at most four quads and keep the smallest squared distance (some of them
-  // also record the phi index of the winner).  Here the branches only fill a
-  // list of (quad, phi index to record) and one loop evaluates it, so the
-  // quad distance is compiled once; the comparisons and their order are the
-  // reference's.
+  // also record the phi index of the winner, and the surface it lies on:
+  // 0/1 the phi planes, 2 the inner shell, 3 the outer).  Here the branches
+  // only fill a list of (quad, phi index to record, surface) and one loop
+  // evaluates it, so the quad distance is compiled once; the comparisons and
+  // their order are the reference's.  `iSurf` (the Normal kernel's) receives
+  // the winner's surface; the safeties pass none.
   template <typename R>
-  VGM_DEV static R safety_zseg2(const Data &h, int si, int &phiIndex, V3<R> p, bool pt_inside)
+  VGM_DEV static R safety_zseg2(const Data &h, int si, int &phiIndex, V3<R> p, bool pt_inside,
+                                int *iSurf = nullptr)
   {
     const PolyhedronSeg &s = h.segs()[si];
     const bool in_cutout = phiIndex < 0;
     uint64_t cq = 0;   // quad indices, 16 bits each
     uint32_t cp = 0;   // phi index to record + 1, 8 bits each (0: none)
+    uint32_t cs = 0;   // surface, 2 bits each
     int nc = 0;
     bool first_assigns = false;  // the first candidate is assigned, not compared
-    auto add = [&](int quad, int phi_rec) {
+    auto add = [&](int quad, int phi_rec, int surf) {
       cq |= uint64_t(uint32_t(quad) & 0xffffu) << (16 * nc);
       cp |= uint32_t(phi_rec + 1) << (8 * nc);
+      cs |= uint32_t(surf) << (2 * nc);
       ++nc;
     };
     bool done = false;
     if (h.phi_cutout && s.n_phi == 2 && (pt_inside || in_cutout)) {
       first_assigns = true;
-      add(s.phi, -1);
-      add(s.phi + 1, -1);
+      add(s.phi, -1, 0);
+      add(s.phi + 1, -1, 1);
       if (in_cutout) {
         if (h.large_cutout && s.n_inner > 0) {
-          add(s.inner, 0);
-          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1);
+          add(s.inner, 0, 2);
+          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1, 2);
         }
         done = true;
       }
@@ -293,19 +298,19 @@ struct PolyhedronShape {
     if (!done) {
       if (in_cutout && si > 0 && si < h.nseg - 1 && h.zplanes()[si] == h.zplanes()[si + 1]) {
         if (s.n_outer > 0) {
-          add(s.outer, 0);
-          if (s.n_outer > 1) add(s.outer + s.n_outer - 1, s.n_outer - 1);
+          add(s.outer, 0, 3);
+          if (s.n_outer > 1) add(s.outer + s.n_outer - 1, s.n_outer - 1, 3);
         }
         if (s.n_inner > 0) {
-          add(s.inner, 0);
-          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1);
+          add(s.inner, 0, 2);
+          if (s.n_inner > 1) add(s.inner + s.n_inner - 1, s.n_inner - 1, 2);
         }
       } else if (phiIndex >= 0) {
         // A phi-cutout point reaches the shells only for a zero-height end
         // segment, where the reference indexes them with -1 (out of bounds);
         // it is skipped here.  Inner is compared before outer.
-        if (s.n_inner > 0) add(s.inner + phiIndex, -1);
-        if (s.n_outer > 0) add(s.outer + phiIndex, -1);
+        if (s.n_inner > 0) add(s.inner + phiIndex, -1, 2);
+        if (s.n_outer > 0) add(s.outer + phiIndex, -1, 3);
       }
     }
     const Quad *q = h.quads();
@@ -313,12 +318,11 @@ struct PolyhedronShape {
 #pragma unroll 1
     for (int c = 0; c < nc; ++c) {
       const R v = quad_dist2<R>(q[int((cq >> (16 * c)) & 0xffffu)], p);
-      if (c == 0 && first_assigns) {
-        safetySquared = v;
-      } else if (v < safetySquared) {
+      if ((c == 0 && first_assigns) || v < safetySquared) {
         safetySquared = v;
         const int rec = int((cp >> (8 * c)) & 0xffu) - 1;
         if (rec >= 0) phiIndex = rec;
+        if (iSurf) *iSurf = int((cs >> (2 * c)) & 3u);
       }
     }
     return safetySquared;
@@ -342,9 +346,10 @@ struct PolyhedronShape {
     distance = dt;
   }
 
-  // ScalarSafetyToEndcapsSquared (:620-650)
+  // ScalarSafetyToEndcapsSquared (:620-650); `iz` (the Normal kernel's)
+  // receives -1 or +1 when an endcap is the nearer surface
   template <typename R>
-  VGM_DEV static void safety_to_endcaps2(const Data &h, V3<R> p, R &distanceSquared)
+  VGM_DEV static void safety_to_endcaps2(const Data &h, V3<R> p, R &distanceSquared, int *iz = nullptr)
   {
     const R firstDistance = R(h.zplanes()[0]) - p.z;
     const R lastDistance = R(h.zplanes()[h.nseg]) - p.z;
@@ -358,6 +363,7 @@ struct PolyhedronShape {
     if (!shell_contains<R>(q + s.outer, s.n_outer, x)) return;
     if (s.n_inner > 0 && shell_contains<R>(q + s.inner, s.n_inner, x)) return;
     if (h.phi_cutout && in_phi_cutout_wedge<R>(h, s, x)) return;
+    if (iz) *iz = isFirst ? -1 : 1;
     distanceSquared = dt2;
   }
 
@@ -590,6 +596,71 @@ struct PolyhedronShape {
   VGM_DEV static R distance_to_out(const Data &h, V3<R> p, V3<R> d, R /*step_max*/)
   {
     return boundary_distance<R>(h, p, d, true);
+  }
+
+  // ScalarNormalKernel (:999-1068): the normal of the nearest quadrilateral
+  // or endcap.  The reference indexes the outer shell with phiIndex even when
+  // no surface was recorded (phiIndex possibly -1); such a point gets the
+  // zero vector here.
+  VGM_DEV static bool normal(const Data &h, V3<double> p, V3<double> &n)
+  {
+    double safety = kInfLength;
+    const int zMax = h.nseg;
+    const int zIndex = find_z_segment<double>(h, p.z);
+    if (zIndex < 0) {
+      n = V3<double>{0, 0, -1};
+      return true;
+    }
+    if (zIndex >= zMax) {
+      n = V3<double>{0, 0, 1};
+      return true;
+    }
+    const double *zp = h.zplanes();
+    int iSeg = zIndex, iSurf = -1, iz = 0;
+    int phiIndex = find_phi_segment<double>(h, p);
+    for (int z = zIndex; z < zMax;) {
+      int iSurfCrt = -1;
+      const double safetySeg = safety_zseg2<double>(h, z, phiIndex, p, true, &iSurfCrt);
+      if (safetySeg < safety) {
+        safety = safetySeg;
+        iSeg = z;
+        iSurf = iSurfCrt;
+      }
+      ++z;
+      const double dz = zp[z] - p.z;
+      if (dz * dz > safety) break;
+    }
+    for (int z = zIndex - 1; z >= 0; --z) {
+      int iSurfCrt = -1;
+      const double safetySeg = safety_zseg2<double>(h, z, phiIndex, p, true, &iSurfCrt);
+      if (safetySeg < safety) {
+        safety = safetySeg;
+        iSeg = z;
+        iSurf = iSurfCrt;
+      }
+      const double dz = p.z - zp[z];
+      if (dz * dz > safety) break;
+    }
+    safety_to_endcaps2<double>(h, p, safety, &iz);
+    if (iz != 0) {
+      n = V3<double>{0, 0, double(iz)};
+      return true;
+    }
+    const PolyhedronSeg &s = h.segs()[iSeg];
+    const Quad *q = h.quads();
+    if (iSurf >= 0 && iSurf < 2) {
+      const Quad &f = q[s.phi + iSurf];
+      n = V3<double>{f.n[0], f.n[1], f.n[2]};
+    } else if (iSurf == 2) {
+      const Quad &f = q[s.inner + phiIndex];
+      n = V3<double>{-1. * f.n[0], -1. * f.n[1], -1. * f.n[2]};
+    } else if (phiIndex >= 0 && phiIndex < s.n_outer) {
+      const Quad &f = q[s.outer + phiIndex];
+      n = V3<double>{f.n[0], f.n[1], f.n[2]};
+    } else {
+      n = V3<double>{0., 0., 0.};
+    }
+    return true;
   }
 };
 

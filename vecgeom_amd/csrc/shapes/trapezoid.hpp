@@ -141,6 +141,34 @@ struct TrapezoidShape {
     }
     return safety;
   }
+
+  // NormalKernel (TrapezoidImplementation.h:406-462) over
+  // PlaneShell::NormalKernel (VecGeom/base/PlaneShell.h:342-370)
+  VGM_DEV static bool normal(const Data &t, V3<double> p, V3<double> &n)
+  {
+    const double delta = 1000. * kTolerance;
+    V3<double> normal{0., 0., 0.};
+    double safety = -kInfLength;
+    double dist[4];
+    for (int i = 0; i < 4; ++i) dist[i] = pdist<double>(t, i, p);
+    for (int i = 0; i < 4; ++i) {
+      const double saf_i = dist[i] - safety;
+      if (vabs(saf_i) < kHalfTolerance && dist[i] >= -kHalfTolerance)
+        normal = V3<double>{normal.x + t.a[i], normal.y + t.b[i], normal.z + t.c[i]};
+      if (saf_i > 0.) normal = V3<double>{t.a[i], t.b[i], t.c[i]};
+      if (saf_i > 0.) safety = dist[i];
+    }
+    const double sz = vsign(p.z);
+    const double safz = sz * p.z - t.dz;
+    if (vabs(safz - safety) < kHalfTolerance && safz >= -kHalfTolerance)
+      normal = V3<double>{normal.x + 0, normal.y + 0, normal.z + sz};
+    if (safz > safety && safz >= -kHalfTolerance) normal = V3<double>{0, 0, sz};
+    if (safz > safety) safety = safz;
+    const bool valid = vabs(safety) <= delta;
+    if (v3_mag2(normal) > 1.0) normal = v3_normalized(normal);
+    n = normal;
+    return valid;
+  }
 };
 
 }  // namespace vgm

@@ -212,6 +212,69 @@ struct TrdShape {
   VGM_DEV static R safety_to_in(const Data &t, V3<R> p) { return safety<R, false>(t, p); }
   template <typename R>
   VGM_DEV static R safety_to_out(const Data &t, V3<R> p) { return safety<R, true>(t, p); }
+
+  // UnplacedTrd::Normal (source/UnplacedTrd.cpp:277-352): the faces within
+  // tolerance of the nearest one, averaged
+  VGM_DEV static bool normal(const Data &t, V3<double> p, V3<double> &norm)
+  {
+    int noSurfaces = 0;
+    V3<double> sumnorm{0., 0., 0.};
+    const double distz = vabs(vabs(p.z) - t.dz);
+    const double xnorm = 1.0 / __builtin_sqrt(4 * t.dz * t.dz + (t.dx2 - t.dx1) * (t.dx2 - t.dx1));
+    const double ynorm = 1.0 / __builtin_sqrt(4 * t.dz * t.dz + (t.dy2 - t.dy1) * (t.dy2 - t.dy1));
+    double distmx = -2.0 * t.dz * p.x - (t.dx2 - t.dx1) * p.z - t.dz * (t.dx1 + t.dx2);
+    distmx *= xnorm;
+    double distpx = 2.0 * t.dz * p.x - (t.dx2 - t.dx1) * p.z - t.dz * (t.dx1 + t.dx2);
+    distpx *= xnorm;
+    double distmy = -2.0 * t.dz * p.y - (t.dy2 - t.dy1) * p.z - t.dz * (t.dy1 + t.dy2);
+    distmy *= ynorm;
+    double distpy = 2.0 * t.dz * p.y - (t.dy2 - t.dy1) * p.z - t.dz * (t.dy1 + t.dy2);
+    distpy *= ynorm;
+    double safmin = vmin(vabs(distmx), vabs(distpx));
+    safmin = vmin(safmin, vmin(vabs(distmy), vabs(distpy)));
+    safmin = vmin(safmin, vabs(distz));
+    const bool valid = safmin <= kHalfTolerance;
+    auto acc = [&](double a, double b, double c, double s) {
+      sumnorm.x += a * s;
+      sumnorm.y += b * s;
+      sumnorm.z += c * s;
+    };
+    if (vabs(distmx) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      acc(-2.0 * t.dz, 0.0, -(t.dx2 - t.dx1), xnorm);
+    }
+    if (vabs(distpx) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      acc(2.0 * t.dz, 0.0, -(t.dx2 - t.dx1), xnorm);
+    }
+    if (vabs(distpy) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      acc(0.0, 2.0 * t.dz, -(t.dy2 - t.dy1), ynorm);
+    }
+    if (vabs(distmy) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      acc(0.0, -2.0 * t.dz, -(t.dy2 - t.dy1), ynorm);
+    }
+    if (vabs(distz) - safmin <= kHalfTolerance) {
+      noSurfaces++;
+      if (p.z >= 0.) {
+        sumnorm.x += 0.;
+        sumnorm.y += 0.;
+        sumnorm.z += 1.;
+      } else {
+        sumnorm.x += 0.;
+        sumnorm.y += 0.;
+        sumnorm.z += -1.;
+      }
+    }
+    if (noSurfaces == 0)
+      norm = V3<double>{0., 0., 1.};
+    else if (noSurfaces == 1)
+      norm = sumnorm;
+    else
+      norm = v3_unit(sumnorm);
+    return valid;
+  }
 };
 
 }  // namespace vgm

@@ -524,6 +524,138 @@ struct TubeShape {
     }
     return safety;
   }
+
+  // IsOnSurfaceGeneric (Wedge_Evolution.h:343-354)
+  VGM_DEV static bool on_phi_plane(double ax, double ay, double nx, double ny, V3<double> p)
+  {
+    if (!(ax * p.x + ay * p.y >= 0.)) return false;
+    return vabs(nx * p.x + ny * p.y) < kTolerance;
+  }
+
+  // ApproxSurfaceNormalKernel (TubeImplementation.h:969-1015): the nearest
+  // surface of a point that is not on one
+  VGM_DEV static V3<double> approx_normal(const Data &t, V3<double> p)
+  {
+    V3<double> norm{0., 0., 0.};
+    const double radius = __builtin_sqrt(p.x * p.x + p.y * p.y);
+    double distRMax = vabs(radius - t.rmax);
+    double distRMin = kInfLength;
+    if (distRMax < 0.) distRMax = kInfLength;
+    if (t.rmin != 0.) {
+      distRMin = vabs(t.rmin - radius);
+      if (distRMin < 0.) distRMin = kInfLength;
+    }
+    double distMin = vmin(distRMin, distRMax);
+    double distPhi1 = kInfLength, distPhi2 = kInfLength;
+    if (t.dphi != kTwoPi) {
+      distPhi1 = p.x * t.normal1x + p.y * t.normal1y;
+      distPhi2 = p.x * t.normal2x + p.y * t.normal2y;
+      if (distPhi1 < 0.) distPhi1 = kInfLength;
+      if (distPhi2 < 0.) distPhi2 = kInfLength;
+      distMin = vmin(distMin, vmin(distPhi1, distPhi2));
+    }
+    double distZ = kInfLength;
+    if (p.z < 0.) distZ = vabs(p.z + t.z);
+    if (p.z >= 0.) distZ = vabs(p.z - t.z);
+    distMin = vmin(distMin, distZ);
+    if (t.dphi != 0.) {
+      if (distMin == distPhi1) norm = V3<double>{-t.normal1x, -t.normal1y, -0.0};
+      if (distMin == distPhi2) norm = V3<double>{-t.normal2x, -t.normal2y, -0.0};
+    }
+    if ((distMin == distZ) && (p.z < 0.)) norm = V3<double>{0., 0., -1.};
+    if ((distMin == distZ) && (p.z >= 0.)) norm = V3<double>{0., 0., 1.};
+    if (vabs(p.z) < (t.z + kTolerance)) {
+      const V3<double> temp{p.x, p.y, 0.};
+      if (distMin == distRMax) norm = v3_unit(temp);
+      if (t.rmin != 0.)
+        if (distMin == distRMin) {
+          const V3<double> u = v3_unit(temp);
+          norm = V3<double>{-u.x, -u.y, -u.z};
+        }
+    }
+    return norm;
+  }
+
+  // NormalKernel (TubeImplementation.h:1019-1100): the normals of the
+  // surfaces the point is on, summed and divided by the root of their number
+  VGM_DEV static bool normal(const Data &t, V3<double> p, V3<double> &norm)
+  {
+    bool ci = false, co = false;
+    generic<double, true>(t, p, ci, co);
+    if (ci || co) {
+      norm = approx_normal(t, p);
+      return false;
+    }
+    norm = V3<double>{0., 0., 0.};
+    int nosurface = 0;
+    const double x2y2 = __builtin_sqrt(p.x * p.x + p.y * p.y);
+    const bool inZ = ((p.z < t.z + kTolerance) && (p.z > -t.z - kTolerance));
+    const bool inR = ((x2y2 >= t.rmin - kTolerance) && (x2y2 <= t.rmax + kTolerance));
+    if (inR && (vabs(p.z - t.z) <= kTolerance)) {
+      norm = V3<double>{0., 0., 1.};
+      nosurface++;
+    }
+    if (inR && (vabs(p.z + t.z) <= kTolerance)) {
+      if (nosurface > 0)
+        norm.z += -1.;
+      else
+        norm = V3<double>{0., 0., -1.};
+      nosurface++;
+    }
+    if (t.rmin > 0.) {
+      if (inZ && (vabs(x2y2 - t.rmin) <= kTolerance)) {
+        const double inv = 1. / x2y2;
+        if (nosurface == 0) {
+          norm.x = -p.x * inv;
+          norm.y = -p.y * inv;
+          norm.z = 0.;
+        } else {
+          norm.x += -p.x * inv;
+          norm.y += -p.y * inv;
+        }
+        nosurface++;
+      }
+    }
+    if (inZ && (vabs(x2y2 - t.rmax) <= kTolerance)) {
+      const double inv = 1. / x2y2;
+      if (nosurface > 0) {
+        norm.x += p.x * inv;
+        norm.y += p.y * inv;
+      } else {
+        norm.x = p.x * inv;
+        norm.y = p.y * inv;
+        norm.z = 0.;
+      }
+      nosurface++;
+    }
+    if (t.dphi < kTwoPi) {
+      if (inR && on_phi_plane(t.along1x, t.along1y, t.normal1x, t.normal1y, p)) {
+        if (nosurface == 0) {
+          norm = V3<double>{-t.normal1x, -t.normal1y, -0.0};
+        } else {
+          norm.x += -t.normal1x;
+          norm.y += -t.normal1y;
+          norm.z += -0.0;
+        }
+        nosurface++;
+      }
+      if (inR && on_phi_plane(t.along2x, t.along2y, t.normal2x, t.normal2y, p)) {
+        if (nosurface == 0) {
+          norm = V3<double>{-t.normal2x, -t.normal2y, -0.0};
+        } else {
+          norm.x += -t.normal2x;
+          norm.y += -t.normal2y;
+          norm.z += -0.0;
+        }
+        nosurface++;
+      }
+    }
+    if (nosurface > 1) {
+      const double s = __builtin_sqrt(1. * nosurface);
+      norm = V3<double>{norm.x / s, norm.y / s, norm.z / s};
+    }
+    return nosurface != 0;
+  }
 };
 
 }  // namespace vgm

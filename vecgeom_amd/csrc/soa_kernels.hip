@@ -111,6 +111,87 @@ __global__ __launch_bounds__(kSoaBlock) void soa_mixed_kernel(const typename S::
   }
 }
 
+// VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137) over a batch:
+// the master point into the local frame, the family's unplaced Normal, the
+// normal back to the master frame.  Outputs: the three components and
+// whether the point is on the surface.
+template <class S, bool Staged>
+__global__ __launch_bounds__(kSoaBlock) void soa_normal_kernel(const typename S::Data *__restrict__ sd, int nd,
+                                                               Transform tr, NormalArgs a)
+{
+  extern __shared__ double lds_block[];
+  const typename S::Data *sp = sd;
+  if constexpr (Staged) {
+    stage_block(lds_block, reinterpret_cast<const double *>(sd), nd);
+    __syncthreads();
+    sp = reinterpret_cast<const typename S::Data *>(lds_block);
+  }
+  const size_t i = size_t(blockIdx.x) * kSoaBlock + threadIdx.x;
+  if (i >= a.n) return;
+  V3<double> ln{0.0, 0.0, 0.0};
+  const bool valid = S::normal(*sp, to_local_point(tr, V3<double>{a.px[i], a.py[i], a.pz[i]}), ln);
+  const V3<double> m = to_master_dir(tr, ln);
+  a.nx[i] = m.x;
+  a.ny[i] = m.y;
+  a.nz[i] = m.z;
+  a.valid[i] = valid ? 1 : 0;
+}
+
+namespace {
+template <class S, class = void>
+struct HasNormal : std::false_type {};
+template <class S>
+struct HasNormal<S, std::void_t<decltype(&S::normal)>> : std::true_type {};
+
+struct NormalLaunch {
+  hipStream_t stream;
+  int nd;
+  const void *params;
+  const Transform &tr;
+  const NormalArgs &a;
+  hipError_t err;
+  template <class S>
+  void operator()()
+  {
+    if constexpr (HasNormal<S>::value) {
+      const unsigned blocks = unsigned((a.n + kSoaBlock - 1) / kSoaBlock);
+      const auto *p = static_cast<const typename S::Data *>(params);
+      if (S::kStage && nd > 0 && size_t(nd) * sizeof(double) <= size_t(kMaxStageBytes))
+        hipLaunchKernelGGL((soa_normal_kernel<S, S::kStage>), dim3(blocks), dim3(kSoaBlock),
+                           size_t(nd) * sizeof(double), stream, p, nd, tr, a);
+      else
+        hipLaunchKernelGGL((soa_normal_kernel<S, false>), dim3(blocks), dim3(kSoaBlock), 0, stream, p, nd, tr, a);
+      err = hipGetLastError();
+    } else {
+      err = hipErrorNotSupported;
+    }
+  }
+};
+struct NormalQuery {
+  bool r = false;
+  template <class S>
+  void operator()()
+  {
+    r = HasNormal<S>::value;
+  }
+};
+}  // namespace
+
+bool family_has_normal(int type)
+{
+  NormalQuery q;
+  return dispatch_shape_type(type, q) && q.r;
+}
+
+hipError_t launch_normal(hipStream_t stream, const ShapeRec &rec, const void *params, const Transform &tr,
+                         const NormalArgs &a)
+{
+  if (a.n == 0) return hipSuccess;
+  NormalLaunch L{stream, rec.nd, params, tr, a, hipSuccess};
+  if (!dispatch_shape_type(rec.type, L)) return hipErrorInvalidValue;
+  return L.err;
+}
+
 namespace {
 struct SoaLaunch {
   hipStream_t stream;

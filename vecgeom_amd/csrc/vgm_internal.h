@@ -27,6 +27,18 @@ enum Precision : int32_t { kFP64 = 0, kMixed = 1 };
 hipError_t launch_soa(hipStream_t stream, int method, int precision, const ShapeRec &rec, const void *params,
                       const Transform &tr, bool use_transform, const SoaArgs &a);
 
+// VPlacedVolume::Normal over a batch of master-frame points.
+struct NormalArgs {
+  const double *px, *py, *pz;
+  size_t n;
+  double *nx, *ny, *nz;
+  uint8_t *valid;
+};
+// True when the family has a Normal kernel (every family but the tessellated solid).
+bool family_has_normal(int type);
+hipError_t launch_normal(hipStream_t stream, const ShapeRec &rec, const void *params, const Transform &tr,
+                         const NormalArgs &a);
+
 void set_error(const std::string &msg);
 
 }  // namespace vgm

@@ -63,6 +63,20 @@ VGM_DEV const T &opaque_ref(const T &r)
 #endif
 }
 
+// Vector3D::Unit (VecGeom/base/Vector3D.h:284-290): v / sqrt(|v|^2 + DBL_MIN)
+VGM_DEV V3<double> v3_unit(const V3<double> &v)
+{
+  const double m = __builtin_sqrt(v.x * v.x + v.y * v.y + v.z * v.z + DBL_MIN);
+  return V3<double>{v.x / m, v.y / m, v.z / m};
+}
+// Vector3D::Normalize (Vector3D.h:198): v * (1 / |v|)
+VGM_DEV V3<double> v3_normalized(const V3<double> &v)
+{
+  const double s = 1.0 / __builtin_sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  return V3<double>{v.x * s, v.y * s, v.z * s};
+}
+VGM_DEV double v3_mag2(const V3<double> &v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+
 VGM_DEV double nonzero(double x) { return x + __builtin_copysign(1.0e-30, x); }
 VGM_DEV float nonzero(float x) { return x + __builtin_copysignf(1.0e-30f, x); }
 // NonZeroAbs(x) = |x| + 1e-30 (VecGeom/base/Global.h:162-167)

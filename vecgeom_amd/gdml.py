@@ -44,14 +44,14 @@ from typing import Dict, List, Tuple
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
 SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb",
-                    "subtraction", "intersection")
+                    "subtraction", "intersection", "union")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
 _REFERENCE_SOLIDS = SUPPORTED_SOLIDS + (
     "sphere", "torus", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
     "genericPolycone", "genericPolyhedra", "hype", "paraboloid", "tet", "twistedbox", "twistedtrap",
-    "twistedtrd", "twistedtubs", "scaledSolid", "union", "multiUnion")
+    "twistedtrd", "twistedtubs", "scaledSolid", "multiUnion")
 
 _STOD = re.compile(r"\s*[+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan)", re.IGNORECASE)
 _KTOL = 1e-9  # kTolerance, the facet validity threshold
@@ -193,9 +193,9 @@ class _Reader:
             z, rmin, rmax = self.lengths(el, "z", "rmin", "rmax")
             sphi, dphi = self.angles(el, "startphi", "deltaphi")
             s = g.tube(rmin, rmax, z / 2.0, sphi, dphi)
-        elif tag in ("subtraction", "intersection"):
-            # Middleware.cpp:415-499: the first solid placed as it is, the
-            # second by the position and rotation (inline or by reference)
+        elif tag in ("subtraction", "intersection", "union"):
+            # Middleware.cpp:415-499, 622-623: the first solid placed as it is,
+            # the second by the position and rotation (inline or by reference)
             first = second = None
             pos, rot = (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
             for c in el:
