@@ -32,6 +32,7 @@
 #pragma once
 #include <stdexcept>
 #include <string>
+#include <cmath>
 #include <vector>
 #include "vgm.h"
 
@@ -172,6 +173,11 @@ class GeoManager {
   // unplaced shapes (constructor arguments of the reference's classes)
   int Box(double dx, double dy, double dz) { return make(vgm_box_create(ctx_, dx, dy, dz, &s_)); }
   int Orb(double r) { return make(vgm_orb_create(ctx_, r, &s_)); }
+  int Sphere(double rmin, double rmax, double sphi = 0., double dphi = 2 * M_PI, double stheta = 0.,
+             double dtheta = M_PI)
+  {
+    return make(vgm_sphere_create(ctx_, rmin, rmax, sphi, dphi, stheta, dtheta, &s_));
+  }
   // UnplacedBooleanVolume<op>: op 0 union, 1 intersection, 2 subtraction; placements
   // of 12 doubles (translation, rotation) or null for the identity
   int Boolean(int op, int left, const double *left_tr, int right, const double *right_tr)

@@ -33,8 +33,8 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
 /* Shape types, as indices of the per-family arrays of the batch calls:
  * 0 box, 1 tube, 2 cone, 3 trd, 4 trapezoid, 5 polycone, 6 polyhedron,
- * 7 tessellated, 8 orb, 9 cut tube, 10 boolean. */
-#define VGM_NUM_SHAPE_TYPES 11
+ * 7 tessellated, 8 orb, 9 cut tube, 10 boolean, 11 sphere. */
+#define VGM_NUM_SHAPE_TYPES 12
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
@@ -73,6 +73,12 @@ int vgm_box_create(vgm_ctx *ctx, double dx, double dy, double dz, int *shape);
 /* UnplacedOrb(r) (VecGeom/volumes/UnplacedOrb.h; kernels
  * VecGeom/volumes/kernel/OrbImplementation.h:64-225): a full sphere. */
 int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
+/* UnplacedSphere(rmin, rmax, sphi, dphi, stheta, dtheta)
+ * (VecGeom/volumes/UnplacedSphere.h:40-41, SphereStruct.h; kernels
+ * VecGeom/volumes/kernel/SphereImplementation.h with the theta cone of
+ * ThetaCone_Evolution.h): a spherical shell cut in phi and theta. */
+int vgm_sphere_create(vgm_ctx *ctx, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta,
+                      int *shape);
 /* UnplacedBooleanVolume<op>(left placed, right placed)
  * (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:8-31; kernels
  * VecGeom/volumes/kernel/BooleanImplementation.h:39-290,

@@ -50,6 +50,7 @@ def lib():
         sig = {
             "orc_box_create": [_P, _D, _D, _D],
             "orc_orb_create": [_P, _D],
+            "orc_sphere_create": [_P, _D, _D, _D, _D, _D, _D],
             "orc_cut_tube_create": [_P] + [_D] * 11,
             "orc_boolean_create": [_P, _I, _I, _P, _I, _P],
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
@@ -117,6 +118,9 @@ class Geometry:
 
     def orb(self, r):
         return self._chk(self._L.orc_orb_create(self.h, r))
+
+    def sphere(self, rmin, rmax, sphi=0.0, dphi=2 * np.pi, stheta=0.0, dtheta=np.pi):
+        return self._chk(self._L.orc_sphere_create(self.h, rmin, rmax, sphi, dphi, stheta, dtheta))
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None):
         ops = {"intersection": 1, "subtraction": 2, "union": 3}

@@ -31,6 +31,11 @@ int orc_boolean_create(orc_geom *g, int op, int left, const double *ltr, int rig
   return add_shape(g, make_boolean(op, G->shapes[left].get(), ltr, G->shapes[right].get(), rtr));
 }
 int orc_orb_create(orc_geom *g, double r) { return r > 0 ? add_shape(g, make_orb(r)) : -1; }
+int orc_sphere_create(orc_geom *g, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta)
+{
+  if (!(rmin >= 0 && rmin < rmax && dphi > 0 && stheta >= 0 && stheta <= kPi && dtheta > 0)) return -1;
+  return add_shape(g, make_sphere(rmin, rmax, sphi, dphi, stheta, dtheta));
+}
 int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
                         double bz, double tx, double ty, double tz)
 {

@@ -133,6 +133,14 @@ int main()
   double nrm[3] = {0., 0., 0.};
   EXPECT(both.Normal(wall, nrm) && nrm[0] == 0. && nrm[1] == 1. && nrm[2] == 0.);
 
+  // a hollow sphere: the shell's inner wall is 6 mm from the origin along +x
+  PlacedVolume shell = geo.Place(geo.NewLogicalVolume(geo.Sphere(6., 8.)));
+  std::vector<double> ox{0}, oy{0}, oz{0}, ex{1}, ey{0}, ez{0};
+  SOA3D<> O(ox.data(), oy.data(), oz.data(), 1), E(ex.data(), ey.data(), ez.data(), 1);
+  double din_shell = 0.;
+  shell.DistanceToIn(O, E, nullptr, &din_shell);
+  EXPECT(din_shell == 6.);
+
   // errors surface as exceptions with the library's message
   bool threw = false;
   try {

@@ -852,3 +852,114 @@ NORMAL_KATS = (
 )
 # TestPolycone.cpp:171-186: away from the surface Normal is invalid, with a unit vector
 NORMAL_OFF_SURFACE = [("polycone", PSIMPLE, p) for p in [(70, 70, -5), (70, 70, -10), (0, 0, 100), (0, 0, -100)]]
+
+
+# ---- Sphere (TestSphere.cpp:43-320): (shape, method, point, direction, expectation) ----
+_SB4 = (6.0, 8.0, 0.0, 2 * PI, 0.0, PI)
+_SB5 = (6.0, 8.0, 0.0, PI, 0.0, PI)
+_SB6 = (6.0, 8.0, PI / 6, PI / 3, 0.0, PI)
+_SB7 = (6.0, 8.0, PI / 6, PI / 6, 0.0, PI / 6)
+_SB8 = (6.0, 8.0, PI / 6, PI / 6, 0.0, PI)
+_SB9 = (6.0, 8.0, PI / 6, PI / 6, PI / 6, PI / 6)
+_BIG = [(100, 0, 0), (0, 100, 0), (0, 0, 100), (-100, 0, 0), (0, -100, 0), (0, 0, -100)]
+_PO_31_45 = (3.038887886090341, 3.240358815458723, 5.409735221140894)
+SPHERE_KATS = [
+    (_SB4, "Inside", (0.337317535387103, 6.265557026812151, 3.103064940359008), None, ("inside", 1)),
+    (_SB4, "Inside", (0.162722224118177, -8.624868144449298, -2.566158796458579), None, ("inside", 3)),
+    (_SB4, "Inside", (-2.393620784732096, -4.753290873255242, 5.973006396542139), None, ("inside", 2)),
+    (_SB4, "Inside", (-4.231374871926247, -4.183095200116632, -0.772775025466741), None, ("inside", 2)),
+    (_SB4, "SafetyToIn", (100, 0, 0), None, ("eq", 92.0)),
+    (_SB4, "SafetyToOut", (0, 0, 0), None, ("le", 0.0)),
+    (_SB4, "SafetyToOut", (0.162722224118177, -8.624868144449298, -2.566158796458579), None, ("le", 0.0)),
+    (_SB4, "SafetyToOut", (9, 0, 0), None, ("le", 0.0)),
+    (_SB4, "SafetyToOut", (0, 9, 0), None, ("le", 0.0)),
+    (_SB4, "SafetyToOut", (0, 0, 9), None, ("le", 0.0)),
+] + [(_SB4, "SafetyToOut", p, None, ("eq", 0.5)) for p in [(6.5, 0, 0), (0, 6.5, 0), (0, 0, 6.5), (7.5, 0, 0),
+                                                           (0, 7.5, 0), (0, 0, 7.5)]] + [
+    (_SB4, "SafetyToIn", p, None, ("le", 0.0)) for p in [(7.5, 0, 0), (0, 7.5, 0), (0, 0, 7.5),
+                                                         (3.796560684305335, -6.207283535497058, 2.519078815824183)]
+] + [
+    (_SB5, "Inside", (-6.354024677915919, 2.724956501738179, -1.095893451057325), None, ("inside", 1)),
+    (_SB5, "Inside", (-2.523183636323228, -3.188902592924771, -5.697757856405303), None, ("inside", 3)),
+    (_SB5, "Inside", (-6.975035234148935, 0.000000000583791, -0.590663594934467), None, ("inside", 2)),
+    (_SB6, "Inside", (5.343263338886987, 3.536603816131425, -2.818150162612158), None, ("inside", 1)),
+    (_SB6, "Inside", (6.043467293054153, 0.461937628581358, 3.501873313683026), None, ("inside", 3)),
+    (_SB6, "Inside", (0.139868946641446, 0.080753374000628, 6.998136578429498), None, ("inside", 2)),
+    (_SB6, "Inside", (4.589551141500899, 2.649778586424563, 4.573258549707597), None, ("inside", 2)),
+    (_SB8, "SafetyToIn", (0, 5, 0), None, ("eq", 2.5)),
+] + [(_SB8, "SafetyToIn", p, None, ("eq", 92.0)) for p in _BIG] + [
+    (_SB8, "SafetyToOut", p, None, ("le", 0.0)) for p in _BIG] + [
+    (_SB8, "SafetyToOut", (0.041433182037376, 3.902131470711286, -6.637895244481554), None, ("le", 0.0)),
+    (_SB8, "SafetyToIn", (0.041433182037376, 3.902131470711286, -6.637895244481554), None, ("eq", 1.91518354715165)),
+    (_SB7, "Inside", (2.051950506806283, 1.301790637821730, 6.564666042754737), None, ("inside", 1)),
+    (_SB7, "Inside", _PO_31_45, None, ("inside", 3)),
+    (_SB7, "Inside", (2.421058285775014, 2.527543625149363, 6.062177826478448), None, ("inside", 2)),
+    (_SB7, "Inside", (2.983684745081379, 1.829651698111631, 6.062177826950138), None, ("inside", 2)),
+] + [(_SB9, "SafetyToIn", p, None, ("eq", 92.0)) for p in _BIG] + [
+    (_SB9, "SafetyToOut", p, None, ("le", 0.0)) for p in _BIG] + [
+    (_SB7, "SafetyToIn", (2.051950506806283, 1.301790637821730, 6.564666042754737), None, ("le", 0.0)),
+    (_SB7, "SafetyToIn", _PO_31_45, None, ("eq", 1.142348737370068)),
+]
+# TestSphere.cpp:188-205, 290-293, 313-314 (the last two "verified with Geant4", to 6 digits)
+SPHERE_NORMAL_KATS = [
+    (_SB4, (8, 0, 0), (1, 0, 0), True), (_SB4, (0, 8, 0), (0, 1, 0), True), (_SB4, (0, 0, 8), (0, 0, 1), True),
+    (_SB4, (-8, 0, 0), (-1, 0, 0), True), (_SB4, (0, -8, 0), (0, -1, 0), True),
+    (_SB4, (0, 0, -8), (0, 0, -1), True),
+]
+SPHERE_NORMAL_APPROX = [
+    ((6.0, 8.0, PI / 3, PI / 3, 0.0, PI), (2.821923096621196, 6.085843652625694, 5.999938089059868),
+     (0.866025, -0.5, 0.0)),
+    (_SB7, _PO_31_45, (-0.434127, -0.462908, -0.772819)),
+]
+
+# TestSphere.cpp:460-573 (the "Solid Sphere_t" family)
+_S1 = (0.0, 50.0, 0.0, 2 * PI, 0.0, PI)
+_SN1 = (0.0, 50.0, 0.5 * PI, 1.5 * PI, 0.0, PI)
+_SN11 = (0.0, 50.0, 0.0, 2 * PI, 0.0, 0.5 * PI)
+_SN12 = (0.0, 50.0, 0.0, 2 * PI, 0.0, 0.25 * PI)
+_SN13 = (0.0, 50.0, 0.0, 2 * PI, 0.75 * PI, 0.25 * PI)
+_S2 = (45.0, 50.0, 0.0, 2 * PI, 0.0, PI)
+_SN2 = (45.0, 50.0, 0.5 * PI, 0.5 * PI, 0.0, PI)
+_S3 = (45.0, 50.0, 0.0, 2 * PI, PI / 4, 0.5 * PI)
+_S4 = (45.0, 50.0, -PI / 4, 0.5 * PI, 0.0, 2 * PI)
+_S41 = (5.0, 50.0, -PI, 1.5 * PI, 0.0, 2 * PI)
+_S42 = (5.0, 50.0, -PI / 2, 1.5 * PI, 0.0, 2 * PI)
+_S5 = (45.0, 50.0, -PI / 4, 0.5 * PI, PI / 4, 0.5 * PI)
+_S9 = (0.0, 410.0, 0.0, 2 * PI, 0.5 * PI, 0.5 * PI)
+_R2 = math.sqrt(2.0)
+_PONRMIN2, _PONRMAX2 = (45 / _R2, 45 / _R2, 0.0), (50 / _R2, 50 / _R2, 0.0)
+_PONTHETA1 = (0.0, 48 * math.sin(PI / 4), 48 * math.cos(PI / 4))
+_PONTHETA2 = (0.0, 48 * math.sin(PI / 4), -48 * math.cos(PI / 4))
+_PONPHI1, _POVERPHI, _PINPHI = (48 / _R2, -48 / _R2, 0.0), (-48 / _R2, 48 / _R2, 0.0), (48 * 0.866, -24.0, 0.0)
+SPHERE_KATS += [
+    (_S1, "Inside", (0, 0, 0), None, ("inside", 1)), (_S2, "Inside", (0, 0, 0), None, ("inside", 3)),
+    (_S2, "Inside", _PONRMIN2, None, ("inside", 2)), (_S2, "Inside", _PONRMAX2, None, ("inside", 2)),
+    (_S3, "Inside", _PONTHETA1, None, ("inside", 2)), (_S3, "Inside", _PONTHETA2, None, ("inside", 2)),
+    (_S4, "Inside", _PONPHI1, None, ("inside", 2)), (_S4, "Inside", _POVERPHI, None, ("inside", 3)),
+    (_S4, "Inside", _PINPHI, None, ("inside", 1)), (_S5, "Inside", (0, 0, 100), None, ("inside", 3)),
+    (_S41, "Inside", (-30, 0, 0), None, ("inside", 2)), (_S42, "Inside", (-30, 0, 0), None, ("inside", 2)),
+    (_SN11, "Inside", (0, 0, 0), None, ("inside", 2)), (_SN12, "Inside", (0, 0, 0), None, ("inside", 2)),
+    (_SN13, "Inside", (0, 0, 0), None, ("inside", 2)), (_SN13, "Inside", (0, 0, -30), None, ("inside", 1)),
+    (_SN12, "Inside", (0, 0, 30), None, ("inside", 1)), (_SN12, "Inside", (30, 0, 0), None, ("inside", 3)),
+    (_SN11, "Inside", (30, 0, 0), None, ("inside", 2)),
+    (_S1, "SafetyToOut", (0, 0, 0), None, ("eq", 50.0)), (_S1, "SafetyToOut", (50, 0, 0), None, ("eq", 0.0)),
+    (_S1, "SafetyToOut", (30, 0, 0), None, ("eq", 20.0)), (_S2, "SafetyToOut", (0, 0, 0), None, ("le", 0.0)),
+    (_S2, "SafetyToOut", (50, 0, 0), None, ("eq", 0.0)), (_S2, "SafetyToOut", (45, 0, 0), None, ("eq", 0.0)),
+    (_SN11, "SafetyToOut", (0, 0, 0), None, ("eq", 0.0)), (_SN11, "SafetyToOut", (30, 0, 0), None, ("eq", 0.0)),
+    (_SN12, "SafetyToOut", (0, 0, 0), None, ("eq", 0.0)), (_SN12, "SafetyToOut", (0, 0, 30), None, ("eq", 20.0)),
+    (_SN13, "SafetyToOut", (0, 0, -30), None, ("eq", 20.0)), (_S3, "SafetyToOut", (0, 0, 0), None, ("le", 0.0)),
+    (_S9, "SafetyToOut", (0, 0, 0), None, ("eq", 0.0)), (_S9, "SafetyToOut", (30, 0, 0), None, ("eq", 0.0)),
+    (_S41, "SafetyToOut", (0, 0, 0), None, ("le", 0.0)), (_S41, "SafetyToOut", (50, 0, 0), None, ("eq", 0.0)),
+    (_S1, "DistanceToOut", (0, 0, 30), VZ, ("eq", 20.0)), (_S1, "DistanceToOut", (50, 0, 0), VX, ("eq", 0.0)),
+    (_S2, "DistanceToOut", (45, 0, 0), VX, ("eq", 5.0)), (_S2, "DistanceToOut", _PONRMAX2, (1 / _R2, 1 / _R2, 0.0),
+                                                          ("eq", 0.0)),
+    (_S1, "DistanceToOut", (0, 0, 0), VX, ("eq", 50.0)), (_S1, "DistanceToOut", (0, 0, 0), VMX, ("eq", 50.0)),
+    (_S1, "DistanceToOut", (0, 0, 0), VY, ("eq", 50.0)), (_S1, "DistanceToOut", (0, 0, 0), VMY, ("eq", 50.0)),
+]
+_P3S = 1 / math.sqrt(3.0)
+SPHERE_NORMAL_KATS += [
+    (_SN1, (0.0, 0.0, 50.0), (_P3S, _P3S, _P3S), None), (_SN1, (0.0, 0.0, 0.0), (1 / _R2, 1 / _R2, 0.0), None),
+    (_SN11, (0.0, 0.0, 0.0), (0.0, 0.0, -1.0), None), (_SN12, (0.0, 0.0, 0.0), (0.0, 0.0, -1.0), None),
+    (_SN13, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), None), (_SN2, (-45.0, 0.0, 0.0), (1 / _R2, -1 / _R2, 0.0), None),
+    (_S1, (50.0, 0.0, 0.0), (1.0, 0.0, 0.0), None),
+]

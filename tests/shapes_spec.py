@@ -49,6 +49,21 @@ SHAPES = {
 }
 
 SHAPES.update({
+    # spheres: TestSphere.cpp's b4, b8, b9, s3, s41, sn13 and cut spheres on both
+    # sides of the equator (the theta cone's branches)
+    "sph_b4": ("sphere", (6.0, 8.0, 0.0, 2 * PI, 0.0, PI)),
+    "sph_b8": ("sphere", (6.0, 8.0, PI / 6, PI / 6, 0.0, PI)),
+    "sph_b9": ("sphere", (6.0, 8.0, PI / 6, PI / 6, PI / 6, PI / 6)),
+    "sph_s3": ("sphere", (45.0, 50.0, 0.0, 2 * PI, PI / 4, 0.5 * PI)),
+    "sph_s41": ("sphere", (5.0, 50.0, -PI, 1.5 * PI, 0.0, 2 * PI)),
+    "sph_sn13": ("sphere", (0.0, 50.0, 0.0, 2 * PI, 0.75 * PI, 0.25 * PI)),
+    "sph_solid": ("sphere", (0.0, 25.0, 0.0, 2 * PI, 0.0, PI)),
+    "sph_band": ("sphere", (5.0, 40.0, 0.0, 2 * PI, 1.2, 1.5)),
+    "sph_patch_low": ("sphere", (20.0, 40.0, 0.4, 1.1, 1.9, 0.8)),
+    "sph_wide_phi": ("sphere", (0.0, 50.0, 0.3, 1.2 * PI, 0.3, 0.9)),
+})
+
+SHAPES.update({
     # TestOrb.cpp radii
     "orb_9": ("orb", (9.0,)),
     "orb_6": ("orb", (6.0,)),

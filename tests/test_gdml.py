@@ -29,7 +29,8 @@ class Recorder:
         return len(self.calls) - 1
 
     def __getattr__(self, name):
-        if name in ("box", "orb", "tube", "cut_tube", "boolean", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated",
+        if name in ("box", "orb", "sphere", "tube", "cut_tube", "boolean", "cone", "trd", "trapezoid", "polycone",
+                    "polyhedron", "tessellated",
                     "logical_volume", "place", "add_daughter"):
             return lambda *a: self._rec(name, *a)
         raise AttributeError(name)
@@ -127,8 +128,8 @@ def test_numbers_read_like_stod():
 def test_unsupported_is_an_error(name, what):
     with pytest.raises(gdml.GdmlError, match=what):
         read(name)
-    text = open(path("oneOrb")).read().replace("<orb ", "<sphere ")
-    with pytest.raises(gdml.GdmlError, match="sphere"):
+    text = open(path("oneOrb")).read().replace("<orb ", "<ellipsoid ")
+    with pytest.raises(gdml.GdmlError, match="ellipsoid"):
         gdml.load(text, Recorder())
 
 
@@ -196,3 +197,11 @@ def test_oracle_from_gdml_equals_oracle_by_hand(name, build):
     dirs = sampling.isotropic_directions(3, 2, 2000)
     for q in ("Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn"):
         assert np.array_equal(geo.query(world, q, pts, dirs), geo2.query(pv2, q, pts, dirs)), q
+
+
+def test_sphere_lengths_and_angles():
+    r, *_ = read("oneSphere")  # lunit cm, aunit deg
+    (_, rmin, rmax, sphi, dphi, sth, dth), = r.of("sphere")
+    assert (rmin, rmax) == (0.0, 100.0)
+    assert (sphi, sth) == (0.0, 0.0)
+    assert abs(dphi - 2 * np.pi) < 1e-15 and abs(dth - np.pi) < 1e-15

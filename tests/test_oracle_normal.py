@@ -175,3 +175,20 @@ def test_union_distances_and_safeties():
     sa = geo.query(geo.place(geo.logical_volume(a)), "SafetyToIn", p)
     sb = geo.query(geo.place(geo.logical_volume(b), shift), "SafetyToIn", p)
     np.testing.assert_array_equal(geo.query(u, "SafetyToIn", p), np.minimum(sa, sb))
+
+
+@pytest.mark.parametrize("case", kats.SPHERE_NORMAL_KATS)
+def test_sphere_normal_kat(case):
+    params, p, want, want_valid = case
+    n, valid = _normal(oracle.Geometry(), "sphere", params, p)
+    assert all(kats.approx_equal(a, b) for a, b in zip(n, want)), (n, valid)
+    if want_valid is not None:
+        assert valid == want_valid
+
+
+@pytest.mark.parametrize("case", kats.SPHERE_NORMAL_APPROX)
+def test_sphere_normal_geant4_values(case):
+    """Normals the reference's test states to six digits (checked against Geant4 there)."""
+    params, p, want = case
+    n, _ = _normal(oracle.Geometry(), "sphere", params, p)
+    np.testing.assert_allclose(n, want, atol=1e-6)

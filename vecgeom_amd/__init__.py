@@ -34,7 +34,8 @@ GRAPH = 512
 STEP_KERNELS = 1024
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
-SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean")
+SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean",
+               "sphere")
 
 # vgm_query_noalloc method codes (VGM_Q_*) and output types
 NOALLOC_METHODS = {
@@ -92,6 +93,7 @@ def _declare(L):
         "vgm_ctx_synchronize": [_P],
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_orb_create": [_P, _D, ctypes.POINTER(_I)],
+        "vgm_sphere_create": [_P, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cut_tube_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_boolean_create": [_P, _I, _I, _P, _I, _P, ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
@@ -270,6 +272,10 @@ class Context:
 
     def orb(self, r) -> int:
         return self._new(self._L.vgm_orb_create, r)
+
+    def sphere(self, rmin, rmax, sphi=0.0, dphi=2 * np.pi, stheta=0.0, dtheta=np.pi) -> int:
+        """UnplacedSphere(rmin, rmax, sphi, dphi, stheta, dtheta)."""
+        return self._new(self._L.vgm_sphere_create, rmin, rmax, sphi, dphi, stheta, dtheta)
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None) -> int:
         """UnplacedBooleanVolume: op "union", "intersection" or "subtraction";

@@ -506,6 +506,17 @@ int vgm_safety_to_out(vgm_ctx *c, int pv, const double *x, const double *y, cons
   return run_soa(c, pv, kSafetyToOut, false, x, y, z, nullptr, nullptr, nullptr, nullptr, n, out, 8, flags);
 }
 
+int vgm_sphere_create(vgm_ctx *c, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta,
+                      int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_sphere(c->host, rmin, rmax, sphi, dphi, stheta, dtheta, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_normal(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, double *nx,
                double *ny, double *nz, uint8_t *valid, unsigned flags)
 {

@@ -7,6 +7,7 @@
 #include "shapes/orb_data.h"
 #include "shapes/cut_tube_data.h"
 #include "shapes/boolean_data.h"
+#include "shapes/sphere_data.h"
 #include "transform.hpp"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
@@ -98,6 +99,131 @@ int build_orb(HostGeometry &g, double r, std::string &err)
   OrbData o{r};
   const double lo[3] = {-r, -r, -r}, hi[3] = {r, r, r};
   return g.add_shape(kOrb, &o, sizeof(o) / sizeof(double), 0, 0, lo, hi);
+}
+
+// UnplacedSphere(rmin, rmax, sphi, dphi, stheta, dtheta): SphereStruct's
+// angle checks (VecGeom/volumes/SphereStruct.h:120-210), the wedge and theta
+// cone of the raw angles (Wedge_Evolution.h, ThetaCone_Evolution.h:40-60),
+// and the Extent of source/UnplacedSphere.cpp:76-164
+int build_sphere(HostGeometry &g, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta,
+                 std::string &err)
+{
+  const double kPi = 3.14159265358979323846, kAngTol = 1e-9;
+  if (!(rmin >= 0 && rmin < rmax && dphi > 0 && stheta >= 0 && stheta <= kPi && dtheta > 0)) {
+    err = "sphere needs 0 <= rmin < rmax, dphi > 0, 0 <= stheta <= pi, dtheta > 0";
+    return -1;
+  }
+  SphereData s{};
+  s.rmin = rmin;
+  s.rmax = rmax;
+  s.sphi = sphi;
+  s.dphi = dphi;
+  s.stheta = stheta;
+  s.dtheta = dtheta;
+  bool fullPhi = true;
+  if (dphi >= 2 * kPi - kAngTol * 0.5) {
+    s.dphi = 2 * kPi;
+    s.sphi = 0;
+  } else {
+    fullPhi = false;
+    s.sphi = sphi < 0 ? 2 * kPi - std::fmod(std::fabs(sphi), 2 * kPi) : std::fmod(sphi, 2 * kPi);
+    if (s.sphi + s.dphi > 2 * kPi) s.sphi -= 2 * kPi;
+  }
+  if (dtheta + stheta >= kPi) s.dtheta = kPi - stheta;
+  const bool fullTheta = !(s.dtheta - s.stheta < kPi);
+  s.etheta = s.stheta + s.dtheta;
+  s.full_phi = fullPhi;
+  s.full_theta = fullTheta;
+  s.full_sphere = fullPhi && fullTheta;
+  // Wedge(dphi, sphi) of the raw angles
+  s.w_a1x = std::cos(sphi);
+  s.w_a1y = std::sin(sphi);
+  s.w_a2x = std::cos(sphi + dphi);
+  s.w_a2y = std::sin(sphi + dphi);
+  s.w_n1x = -std::sin(sphi);
+  s.w_n1y = std::cos(sphi);
+  s.w_n2x = std::sin(sphi + dphi);
+  s.w_n2y = -std::cos(sphi + dphi);
+  s.w_dphi = dphi;
+  // ThetaCone(stheta, dtheta) of the raw angles
+  s.tc_s = stheta;
+  s.tc_e = stheta + dtheta;
+  double ts = s.tc_s, te = s.tc_e;
+  if (s.tc_s > kPi / 2) ts = kPi - s.tc_s;
+  if (s.tc_e > kPi / 2) te = kPi - s.tc_e;
+  s.tan_s = std::tan(ts);
+  s.tan_s2 = s.tan_s * s.tan_s;
+  s.tan_e = std::tan(te);
+  s.tan_e2 = s.tan_e * s.tan_e;
+  s.tan_bis = std::tan(ts + (dtheta / 2));
+  if (s.tc_s > kPi / 2 && s.tc_e > kPi / 2) s.tan_bis = std::tan(ts - (dtheta / 2));
+  s.slope1 = std::tan(kPi / 2 - s.tc_s);
+  s.slope2 = std::tan(kPi / 2 - s.tc_e);
+  // Extent
+  double lo[3] = {-rmax, -rmax, -rmax}, hi[3] = {rmax, rmax, rmax};
+  const double eT = s.stheta + s.dtheta;
+  if (!s.full_sphere) {
+    double st1 = 0., st2 = 0.;
+    if (!fullTheta) {
+      hi[2] = ((s.stheta <= kPi / 2.) ? rmax : rmin) * std::cos(s.stheta);
+      lo[2] = ((eT <= kPi / 2.) ? rmin : rmax) * std::cos(eT);
+      st1 = std::sin(s.stheta);
+      st2 = std::sin(s.etheta);
+      if (s.stheta <= kPi / 2.) {
+        if (eT < kPi / 2.) {
+          hi[0] = rmax * st2;
+          lo[0] = -rmax * st2;
+          hi[1] = rmax * st2;
+          lo[1] = -rmax * st2;
+        }
+      } else {
+        hi[0] = rmax * st1;
+        lo[0] = -rmax * st1;
+        hi[1] = rmax * st1;
+        lo[1] = -rmax * st1;
+      }
+    }
+    if (!fullPhi) {
+      double Rmax = rmax;
+      if (s.stheta > kPi / 2.) Rmax *= st1;
+      if (eT < kPi / 2.) Rmax *= st2;
+      const double Rmin = rmin * (st1 < st2 ? st1 : st2);
+      const double Rin = 0.5 * (Rmax + Rmin);
+      auto wedge_has = [&](double x, double y) {
+        const double startCheck = (-x * s.w_a1y + y * s.w_a1x);
+        const double endCheck = (-s.w_a2x * y + s.w_a2y * x);
+        bool out = startCheck < 0.;
+        if (s.w_dphi < kPi)
+          out |= endCheck < 0.;
+        else
+          out &= endCheck < 0.;
+        return !out;
+      };
+      const bool phi0out = !wedge_has(Rin, 0), phi90out = !wedge_has(0, Rin);
+      const bool phi180out = !wedge_has(-Rin, 0), phi270out = !wedge_has(0, -Rin);
+      if (phi0out || phi90out || phi180out || phi270out) {
+        const double Cx = Rmax * std::cos(s.sphi), Dx = Rmax * std::cos(s.sphi + s.dphi);
+        const double Cy = Rmax * std::sin(s.sphi), Dy = Rmax * std::sin(s.sphi + s.dphi);
+        if (phi0out) hi[0] = Cx > Dx ? Cx : Dx;
+        if (phi90out) hi[1] = Cy > Dy ? Cy : Dy;
+        if (phi180out) lo[0] = Cx < Dx ? Cx : Dx;
+        if (phi270out) lo[1] = Cy < Dy ? Cy : Dy;
+        if (s.dphi < kPi) {
+          const double Ax = Rmin * std::cos(s.sphi), Bx = Rmin * std::cos(s.sphi + s.dphi);
+          const double Ay = Rmin * std::sin(s.sphi), By = Rmin * std::sin(s.sphi + s.dphi);
+          double t = Ax > Bx ? Ax : Bx;
+          hi[0] = t > hi[0] ? t : hi[0];
+          t = Ay > By ? Ay : By;
+          hi[1] = t > hi[1] ? t : hi[1];
+          t = Ax < Bx ? Ax : Bx;
+          lo[0] = t < lo[0] ? t : lo[0];
+          t = Ay < By ? Ay : By;
+          lo[1] = t < lo[1] ? t : lo[1];
+        }
+      }
+    }
+  }
+  return g.add_shape(kSphere, &s, sizeof(s) / sizeof(double), 0, 0, lo, hi);
 }
 
 // Plane::Set(normal, origin) (VecGeom/volumes/Plane.h:30-37) of the unit

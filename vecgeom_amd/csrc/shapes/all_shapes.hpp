@@ -13,6 +13,7 @@
 #include "orb.hpp"
 #include "cut_tube.hpp"
 #include "boolean.hpp"
+#include "sphere.hpp"
 
 namespace vgm {
 
@@ -54,6 +55,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kBoolean:
     f.template operator()<BooleanShape>();
+    return true;
+  case kSphere:
+    f.template operator()<SphereShape>();
     return true;
   default:
     return false;
@@ -126,6 +130,9 @@ __host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
     return true;
   case kBoolean:
     if constexpr ((Mask & (1u << kBoolean)) != 0) f.template operator()<BooleanShape>();
+    return true;
+  case kSphere:
+    if constexpr ((Mask & (1u << kSphere)) != 0) f.template operator()<SphereShape>();
     return true;
   default:
     return false;

@@ -18,7 +18,7 @@ namespace vgm {
 constexpr int kBoolMaxIterations = 1000;
 constexpr unsigned kBoolOperandMask = (1u << kBox) | (1u << kTube) | (1u << kCone) | (1u << kTrd) |
                                       (1u << kTrapezoid) | (1u << kPolycone) | (1u << kPolyhedron) | (1u << kOrb) |
-                                      (1u << kCutTube);
+                                      (1u << kCutTube) | (1u << kSphere);
 
 enum BoolOperandQuery : int { kOpContains, kOpInside, kOpDistIn, kOpDistOut, kOpSafIn, kOpSafOut };
 

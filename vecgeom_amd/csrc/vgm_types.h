@@ -20,7 +20,8 @@ enum ShapeType : int32_t {
   kOrb = 8,
   kCutTube = 9,
   kBoolean = 10,
-  kNumShapeTypes = 11
+  kSphere = 11,
+  kNumShapeTypes = 12
 };
 
 enum Method : int32_t {

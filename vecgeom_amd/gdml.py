@@ -44,12 +44,12 @@ from typing import Dict, List, Tuple
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
 SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb",
-                    "subtraction", "intersection", "union")
+                    "sphere", "subtraction", "intersection", "union")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
 _REFERENCE_SOLIDS = SUPPORTED_SOLIDS + (
-    "sphere", "torus", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
+    "torus", "para", "arb8", "ellipsoid", "elcone", "eltube", "xtru",
     "genericPolycone", "genericPolyhedra", "hype", "paraboloid", "tet", "twistedbox", "twistedtrap",
     "twistedtrd", "twistedtubs", "scaledSolid", "multiUnion")
 
@@ -218,6 +218,11 @@ class _Reader:
             if first is None or second is None:
                 raise GdmlError(f"solid '{name}': <{tag}> needs <first> and <second>")
             s = g.boolean(tag, first, second, None, [pos[0], pos[1], pos[2]] + _rotation(*rot))
+        elif tag == "sphere":
+            # Middleware.cpp:885-904: lengths and angles as given (no halving)
+            rmin, rmax = self.lengths(el, "rmin", "rmax")
+            sphi, dphi, sth, dth = self.angles(el, "startphi", "deltaphi", "starttheta", "deltatheta")
+            s = g.sphere(rmin, rmax, sphi, dphi, sth, dth)
         elif tag == "cutTube":
             # the normals read as lengths; z halved (Middleware.cpp:738-768)
             z, rmin, rmax, lx, ly, lz, hx, hy, hz = self.lengths(el, "z", "rmin", "rmax", "lowX", "lowY", "lowZ",
