@@ -178,6 +178,10 @@ class GeoManager {
   {
     return make(vgm_sphere_create(ctx_, rmin, rmax, sphi, dphi, stheta, dtheta, &s_));
   }
+  int Torus(double rmin, double rmax, double rtor, double sphi = 0., double dphi = 2 * M_PI)
+  {
+    return make(vgm_torus_create(ctx_, rmin, rmax, rtor, sphi, dphi, &s_));
+  }
   // UnplacedBooleanVolume<op>: op 0 union, 1 intersection, 2 subtraction; placements
   // of 12 doubles (translation, rotation) or null for the identity
   int Boolean(int op, int left, const double *left_tr, int right, const double *right_tr)

@@ -33,8 +33,8 @@ typedef struct vgm_ctx vgm_ctx;
 #define VGM_DISTANCE_FIRST 64u /* transport: boundary distance before the safety (same decisions as FP64) */
 /* Shape types, as indices of the per-family arrays of the batch calls:
  * 0 box, 1 tube, 2 cone, 3 trd, 4 trapezoid, 5 polycone, 6 polyhedron,
- * 7 tessellated, 8 orb, 9 cut tube, 10 boolean, 11 sphere. */
-#define VGM_NUM_SHAPE_TYPES 12
+ * 7 tessellated, 8 orb, 9 cut tube, 10 boolean, 11 sphere, 12 torus. */
+#define VGM_NUM_SHAPE_TYPES 13
 #define VGM_TUNED 128u     /* transport: each family in the mode vgm_batch_autotune chose for it */
 #define VGM_CULL 256u /* transport: distance first, and outside moves clear of the solid's bounding box skip \
                           the queries (same decisions as FP64) */
@@ -79,6 +79,13 @@ int vgm_orb_create(vgm_ctx *ctx, double r, int *shape);
  * ThetaCone_Evolution.h): a spherical shell cut in phi and theta. */
 int vgm_sphere_create(vgm_ctx *ctx, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta,
                       int *shape);
+/* UnplacedTorus2(rmin, rmax, rtor, sphi, dphi) (VecGeom/volumes/UnplacedTorus2.h:64-75,
+ * TorusStruct2.h; kernels VecGeom/volumes/kernel/TorusImplementation2.h,
+ * Normal source/UnplacedTorus2.cpp:201-285): a torus of tube radii
+ * rmin..rmax around a circle of radius rtor, cut in phi.  FP64; its
+ * distances solve a quartic (cbrt, pow, cos, atan2, then Newton) and agree
+ * with a host evaluation to the refinement's tolerance. */
+int vgm_torus_create(vgm_ctx *ctx, double rmin, double rmax, double rtor, double sphi, double dphi, int *shape);
 /* UnplacedBooleanVolume<op>(left placed, right placed)
  * (VecGeom/volumes/UnplacedBooleanVolume.h, BooleanStruct.h:8-31; kernels
  * VecGeom/volumes/kernel/BooleanImplementation.h:39-290,
@@ -355,7 +362,8 @@ int vgm_batch_transport(vgm_batch *batch, double *x, double *y, double *z, doubl
                         int8_t *side, int32_t *crossings, int nsteps, int step0, uint64_t seed,
                         unsigned flags, unsigned long long *fp64_steps);
 /* Kernel time in ms of each shape family (index = shape type: box, tube, cone,
- * trd, trapezoid, polycone, polyhedron, tessellated, orb; VGM_NUM_SHAPE_TYPES
+ * trd, trapezoid, polycone, polyhedron, tessellated, orb, cut tube, boolean,
+ * sphere, torus; VGM_NUM_SHAPE_TYPES
  * entries) in the last
  * vgm_batch_transport called with VGM_TIMING, measured with events on the
  * stream that family ran on; -1 for families not run.  Waits for them. */

@@ -51,6 +51,7 @@ def lib():
             "orc_box_create": [_P, _D, _D, _D],
             "orc_orb_create": [_P, _D],
             "orc_sphere_create": [_P, _D, _D, _D, _D, _D, _D],
+            "orc_torus_create": [_P, _D, _D, _D, _D, _D],
             "orc_cut_tube_create": [_P] + [_D] * 11,
             "orc_boolean_create": [_P, _I, _I, _P, _I, _P],
             "orc_tube_create": [_P, _D, _D, _D, _D, _D],
@@ -121,6 +122,9 @@ class Geometry:
 
     def sphere(self, rmin, rmax, sphi=0.0, dphi=2 * np.pi, stheta=0.0, dtheta=np.pi):
         return self._chk(self._L.orc_sphere_create(self.h, rmin, rmax, sphi, dphi, stheta, dtheta))
+
+    def torus(self, rmin, rmax, rtor, sphi=0.0, dphi=2 * np.pi):
+        return self._chk(self._L.orc_torus_create(self.h, rmin, rmax, rtor, sphi, dphi))
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None):
         ops = {"intersection": 1, "subtraction": 2, "union": 3}

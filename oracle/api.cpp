@@ -36,6 +36,11 @@ int orc_sphere_create(orc_geom *g, double rmin, double rmax, double sphi, double
   if (!(rmin >= 0 && rmin < rmax && dphi > 0 && stheta >= 0 && stheta <= kPi && dtheta > 0)) return -1;
   return add_shape(g, make_sphere(rmin, rmax, sphi, dphi, stheta, dtheta));
 }
+int orc_torus_create(orc_geom *g, double rmin, double rmax, double rtor, double sphi, double dphi)
+{
+  if (!(rmin >= 0 && rmin < rmax && rtor > 0 && dphi > 0)) return -1;
+  return add_shape(g, make_torus(rmin, rmax, rtor, sphi, dphi));
+}
 int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,
                         double bz, double tx, double ty, double tz)
 {

@@ -169,6 +169,7 @@ struct Shape {
 Shape *make_box(double dx, double dy, double dz);
 Shape *make_orb(double r);
 Shape *make_sphere(double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta);
+Shape *make_torus(double rmin, double rmax, double rtor, double sphi, double dphi);
 // op 1 intersection, 2 subtraction, 3 union; ltr/rtr: 12 doubles (translation, rotation) or null
 Shape *make_boolean(int op, const Shape *left, const double *ltr, const Shape *right, const double *rtr);
 Shape *make_cut_tube(double rmin, double rmax, double z, double sphi, double dphi, const double bot[3],

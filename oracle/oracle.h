@@ -29,6 +29,8 @@ int orc_box_create(orc_geom *g, double dx, double dy, double dz);
 int orc_orb_create(orc_geom *g, double r);
 /* UnplacedSphere(rmin, rmax, sphi, dphi, stheta, dtheta) */
 int orc_sphere_create(orc_geom *g, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta);
+/* UnplacedTorus2(rmin, rmax, rtor, sphi, dphi) */
+int orc_torus_create(orc_geom *g, double rmin, double rmax, double rtor, double sphi, double dphi);
 /* op: 1 intersection, 2 subtraction, 3 union */
 int orc_boolean_create(orc_geom *g, int op, int left, const double *ltr, int right, const double *rtr);
 int orc_cut_tube_create(orc_geom *g, double rmin, double rmax, double z, double sphi, double dphi, double bx, double by,

@@ -10,7 +10,7 @@ FILES = ["aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "one
          "onePolyhedron", "oneTesselatedSolid", "oneTrapezoid", "oneTube", "twoBoxes",
          # refused: a solid outside the implemented families, a scaled physvol
          "oneOrb", "scaledPhysvol", "aConstant", "oneCutTube", "solidSubtraction",
-         "solidIntersection", "solidUnion", "oneSphere"]
+         "solidIntersection", "solidUnion", "oneSphere", "oneTorus"]
 
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))

@@ -963,3 +963,51 @@ SPHERE_NORMAL_KATS += [
     (_SN13, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), None), (_SN2, (-45.0, 0.0, 0.0), (1 / _R2, -1 / _R2, 0.0), None),
     (_S1, (50.0, 0.0, 0.0), (1.0, 0.0, 0.0), None),
 ]
+
+# TestTorus2.cpp:17-255 (rmin, rmax, rtor, sphi, dphi); Rtor 100, Rmax 90, Rmin 10
+_T1 = (0.0, 90.0, 100.0, 0.0, 2 * PI)
+_T2 = (10.0, 90.0, 100.0, 0.0, PI / 2)
+_TN2 = (10.0, 90.0, 100.0, PI / 2, PI / 2)
+_TN3 = (10.0, 90.0, 100.0, PI / 2, 3 * PI / 2)
+_T3 = (40.0, 90.0, 100.0, PI / 2 - PI / 24, PI / 12)
+_T6 = (100.0, 150.0, 200.0, 0.0, PI / 3)
+_CMS_EE_COOL3 = (0.0, 6.0, 1640.0, 0.09599, 1.4312)
+_TBX, _TBY, _TBZ = (240.0, 0.0, 0.0), (0.0, 240.0, 0.0), (0.0, 0.0, 240.0)
+_TBMX, _TBMY, _TBMZ = (-240.0, 0.0, 0.0), (0.0, -240.0, 0.0), (0.0, 0.0, -240.0)
+_TONRMAX, _TONRMIN, _TONRTOR = (190.0, 0.0, 0.0), (0.0, 110.0, 0.0), (0.0, 100.0, 0.0)
+_TONPHI1, _TONPHI2 = (100 / _R2, 100 / _R2, 0.0), (-100 / _R2, 100 / _R2, 0.0)
+_TONPHI12 = (190 / _R2, 190 / _R2, 0.0)
+_VXY = (1 / _R2, 1 / _R2, 0.0)
+TORUS_KATS = [
+    (_CMS_EE_COOL3, "Inside", (1230.993171, 1078.075896, -2.947036), None, ("inside", 1)),
+    (_T1, "Inside", O, None, ("inside", 3)), (_T1, "Inside", _TBX, None, ("inside", 3)),
+    (_T1, "Inside", _TONRMAX, None, ("inside", 2)), (_T2, "Inside", _TONRMIN, None, ("inside", 2)),
+    (_T2, "Inside", _TBX, None, ("inside", 3)), (_T2, "Inside", _TBY, None, ("inside", 3)),
+    (_T2, "Inside", _TONPHI1, None, ("inside", 3)), (_T2, "Inside", _TONPHI2, None, ("inside", 3)),
+    (_T2, "Inside", (20.0, 0.0, 0.0), None, ("inside", 2)), (_T2, "Inside", (0.0, 20.0, 0.0), None, ("inside", 2)),
+    (_T6, "Inside", (70.75950555416668, -3.552713678800501e-15, 22.37458414788935), None, ("inside", 2)),
+    (_T1, "SafetyToOut", _TONRMIN, None, ("eq", 80.0)), (_T1, "SafetyToOut", _TONRMAX, None, ("eq", 0.0)),
+    (_T2, "SafetyToOut", (20.0, 0.0, 0.0), None, ("eq", 0.0)), (_T2, "SafetyToOut", (0.0, 20.0, 0.0), None, ("eq", 0.0)),
+    (_T1, "DistanceToOut", _TONRMAX, VX, ("eq", 0.0)), (_T1, "DistanceToOut", (130.0, 0.0, 0.0), VX, ("eq", 60.0)),
+    (_T1, "DistanceToOut", _TONRMIN, VY, ("eq", 80.0)), (_T1, "DistanceToOut", _TONRMIN, VMY, ("eq", 100.0)),
+    (_T1, "DistanceToOut", (100.0, 0.0, 0.0), unit((0.00001, 0.000001, 1.0)), ("eq", 90.0)),
+    (_T2, "DistanceToOut", (7.07106781186547524400844362, 7.07106781186547524400844362, 0.0), VMX, ("eq", 0.0)),
+] + [(_T1, "SafetyToIn", p, None, ("eq", 50.0)) for p in (_TBX, _TBMX, _TBY, _TBMY)] + [
+    (_T1, "DistanceToIn", _TBX, VMX, ("eq", 50.0)), (_T1, "DistanceToIn", _TBMX, VX, ("eq", 50.0)),
+    (_T1, "DistanceToIn", _TBY, VMY, ("eq", 50.0)), (_T1, "DistanceToIn", _TBMY, VY, ("eq", 50.0)),
+    (_T1, "DistanceToIn", _TBZ, VMZ, ("inf",)), (_T1, "DistanceToIn", _TBMZ, VZ, ("inf",)),
+    (_T1, "DistanceToIn", _TBX, _VXY, ("inf",)),
+    (_T2, "DistanceToIn", O, VY, ("eq", 10.0)), (_T2, "DistanceToIn", _TONPHI12, VY, ("inf",)),
+    (_T2, "DistanceToIn", _TONPHI12, VMY, ("eq", 0.0)),
+    (_T2, "DistanceToIn", _TONPHI1, VY, ("eq", 13.550819613108856743)),
+    (_T2, "DistanceToIn", _TONRMIN, VY, ("inf",)), (_T2, "DistanceToIn", _TONRMIN, VMY, ("inf",)),
+] + [(_T3, "DistanceToIn", _TONRTOR, v, ("eq", 40.0)) for v in (VY, VMY, VZ, VMZ)] + [
+    (_T3, "DistanceToIn", _TONRTOR, v, ("inf",)) for v in (VX, VMX)]
+# TestTorus2.cpp:108-139: (params, point, normal)
+TORUS_NORMAL_KATS = [
+    (_T1, _TONRMAX, VX), (_T1, (0.0, 190.0, 0.0), VY),
+    (_TN2, (0.0, 190.0, 0.0), (1 / _R2, 1 / _R2, 0.0)), (_TN2, (0.0, 110.0, 0.0), (1 / _R2, -1 / _R2, 0.0)),
+    (_TN2, (0.0, 90.0, 0.0), (1 / _R2, 1 / _R2, 0.0)), (_TN2, (0.0, 10.0, 0.0), (1 / _R2, -1 / _R2, 0.0)),
+    (_TN3, (100.0, 0.0, 90.0), (0.0, 1 / _R2, 1 / _R2)), (_TN3, (0.0, 100.0, 90.0), (1 / _R2, 0.0, 1 / _R2)),
+    (_T2, _TONRMIN, (-1 / _R2, -1 / _R2, 0.0)), (_T2, (20.0, 0.0, 0.0), VMY), (_T2, (0.0, 20.0, 0.0), VMX),
+]

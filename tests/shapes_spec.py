@@ -64,6 +64,17 @@ SHAPES.update({
 })
 
 SHAPES.update({
+    # tori: TestTorus2.cpp's t1, t2, tn3, t3, t6, the ring and cmsEECool3
+    "torus_t1": ("torus", (0.0, 90.0, 100.0, 0.0, 2 * PI)),
+    "torus_t2": ("torus", (10.0, 90.0, 100.0, 0.0, PI / 2)),
+    "torus_tn3": ("torus", (10.0, 90.0, 100.0, PI / 2, 3 * PI / 2)),
+    "torus_t3": ("torus", (40.0, 90.0, 100.0, PI / 2 - PI / 24, PI / 12)),
+    "torus_t6": ("torus", (100.0, 150.0, 200.0, 0.0, PI / 3)),
+    "torus_ring": ("torus", (0.0, 100.0, 1000.0, 0.0, 2 * PI)),
+    "torus_cms": ("torus", (0.0, 6.0, 1640.0, 0.09599, 1.4312)),
+})
+
+SHAPES.update({
     # TestOrb.cpp radii
     "orb_9": ("orb", (9.0,)),
     "orb_6": ("orb", (6.0,)),

@@ -29,7 +29,7 @@ class Recorder:
         return len(self.calls) - 1
 
     def __getattr__(self, name):
-        if name in ("box", "orb", "sphere", "tube", "cut_tube", "boolean", "cone", "trd", "trapezoid", "polycone",
+        if name in ("box", "orb", "sphere", "torus", "tube", "cut_tube", "boolean", "cone", "trd", "trapezoid", "polycone",
                     "polyhedron", "tessellated",
                     "logical_volume", "place", "add_daughter"):
             return lambda *a: self._rec(name, *a)
@@ -197,6 +197,13 @@ def test_oracle_from_gdml_equals_oracle_by_hand(name, build):
     dirs = sampling.isotropic_directions(3, 2, 2000)
     for q in ("Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn"):
         assert np.array_equal(geo.query(world, q, pts, dirs), geo2.query(pv2, q, pts, dirs)), q
+
+
+def test_torus_lengths_and_angles():
+    r, *_ = read("oneTorus")  # lunit cm, aunit rad
+    (_, rmin, rmax, rtor, sphi, dphi), = r.of("torus")
+    assert (rmin, rmax, rtor) == (10.0, 100.0, 20.0)
+    assert (sphi, dphi) == (1.0, 3.0)
 
 
 def test_sphere_lengths_and_angles():

@@ -1,4 +1,11 @@
 """GPU parity: every SOA3D query of every shape vs the FP64 oracle, bit-exact.
+The torus's distances are the one exception: they solve a quartic through
+cbrt, pow, cos and atan2 (SolveCubic, TorusImplementation2.h:24-64), whose
+device and host versions may round differently, and then refine the root by
+Newton steps until one is below kTolerance; they are held to
+|gpu - oracle| <= 1e-9 * max(1, |oracle|) (the refinement's tolerance),
+with the same hit/miss and sign everywhere, and the share that is bit-exact
+is printed.
 
 Inputs: uniform points in 1.5x the bounding box plus a surface band (entry and
 exit points of random rays, pushed by U(-1e-6, 1e-6) mm along the ray), with
@@ -54,6 +61,20 @@ def make_inputs(geo, pv_local, mn, mx, n, seed=42):
     return P, D
 
 
+TORUS_REL_TOL = 1e-9
+
+
+def compare_refined(ref, got, rel=TORUS_REL_TOL):
+    """Indices where two distance arrays disagree beyond the root
+    refinement's tolerance, in hit/miss or in sign."""
+    ref = np.asarray(ref, dtype=np.float64)
+    got = np.asarray(got, dtype=np.float64)
+    big_r, big_g = ref >= 1e300, got >= 1e300
+    close = np.abs(ref - got) <= rel * np.maximum(1.0, np.abs(ref))
+    ok = (ref == got) | ((big_r == big_g) & ~big_r & close & (np.sign(ref) == np.sign(got)))
+    return np.flatnonzero(~ok)
+
+
 def compare(ref, got):
     ref = np.asarray(ref)
     got = np.asarray(got)
@@ -88,7 +109,14 @@ def test_shape_parity(ctx, name, placed):
     for m in METHODS:
         ref = geo.query(pv_o, m, pts, dirs if "Distance" in m else None)
         got = gpu_query(ctx, pv_g, m, pts, dirs)
-        bad = compare(ref, got)
+        if kind == "torus" and "Distance" in m:
+            inexact = np.flatnonzero(np.asarray(ref) != np.asarray(got))
+            err = np.abs(ref[inexact] - got[inexact]) if inexact.size else np.zeros(1)
+            print(f"[parity] torus {m}: {inexact.size} of {len(ref)} not bit-exact, max |diff| {err.max():.3g}",
+                  flush=True)
+            bad = compare_refined(ref, got)
+        else:
+            bad = compare(ref, got)
         assert bad.size == 0, (m, bad.size, bad[:5], ref[bad[:5]], got[bad[:5]])
     if kind != "tessellated":  # VPlacedVolume::Normal: every family but the tessellated solid
         # the inputs above plus the rays' surface points without the jitter

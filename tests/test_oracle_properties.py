@@ -263,6 +263,20 @@ def test_general_trapezoid_equals_its_tessellation(params):
     assert np.all(si_a[out] <= si_b[out] + 1e-9)
 
 
+def _crosses_phi_half_planes(P, D, dist, sphi, dphi):
+    """Rays whose segment [0, dist] crosses the half-plane phi = sphi or
+    phi = sphi + dphi."""
+    hit = np.zeros(len(dist), dtype=bool)
+    for a in (sphi, sphi + dphi):
+        n = np.array([-np.sin(a), np.cos(a)])
+        along = np.array([np.cos(a), np.sin(a)])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = -(P[:, :2] @ n) / (D[:, :2] @ n)
+        q = P[:, :2] + np.nan_to_num(t)[:, None] * D[:, :2]
+        hit |= (t > 0) & (t < np.minimum(dist, 1e30)) & (q @ along >= 0)
+    return hit
+
+
 @pytest.mark.parametrize("name", sorted(SHAPES))
 def test_first_hit_and_safety_balls(name):
     """A returned distance is the first crossing and a safety is a ball free of
@@ -280,7 +294,10 @@ def test_first_hit_and_safety_balls(name):
     (PolyconeImplementation.h:388-403), so only points within the z-range
     are checked; the polyhedron with a phi range above 180 degrees has an
     Inside that disagrees with its own distances (see above), so its rays
-    and its SafetyToOut balls are not checked."""
+    and its SafetyToOut balls are not checked; the torus's DistanceToIn
+    takes a phi-plane entry only within kTolerance of the start
+    (TorusImplementation2.h:600-640), so its rays that cross a phi
+    half-plane before the returned distance are not checked."""
     kind, args = SHAPES[name]
     geo = oracle.Geometry()
     pv = _pv(geo, kind, args)
@@ -293,6 +310,8 @@ def test_first_hit_and_safety_balls(name):
     for m, side, want_not in (("DistanceToIn", 3, 1), ("DistanceToOut", 1, 3)) if rays else ():
         d = geo.query(pv, m, pts, dirs)
         sel = (ins == side) & (d < 1e30) & (d > 1e-5)
+        if kind == "torus" and m == "DistanceToIn" and args[4] < 2 * np.pi:
+            sel &= ~_crosses_phi_half_planes(P, D, d, args[3], args[4])
         for f in (0.1, 0.5, 0.9):
             q = P[sel] + (f * (d[sel] - push))[:, None] * D[sel]
             r = geo.query(pv, "Inside", tuple(np.ascontiguousarray(q[:, k]) for k in range(3)))

@@ -35,7 +35,7 @@ STEP_KERNELS = 1024
 FAMILY_MODES = ("fp64", "mixed", "distance-first", "learned", "culled")
 MLP_PARAMS = 56321
 SHAPE_TYPES = ("box", "tube", "cone", "trd", "trapezoid", "polycone", "polyhedron", "tessellated", "orb", "cut_tube", "boolean",
-               "sphere")
+               "sphere", "torus")
 
 # vgm_query_noalloc method codes (VGM_Q_*) and output types
 NOALLOC_METHODS = {
@@ -94,6 +94,7 @@ def _declare(L):
         "vgm_box_create": [_P, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_orb_create": [_P, _D, ctypes.POINTER(_I)],
         "vgm_sphere_create": [_P, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
+        "vgm_torus_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
         "vgm_cut_tube_create": [_P] + [_D] * 11 + [ctypes.POINTER(_I)],
         "vgm_boolean_create": [_P, _I, _I, _P, _I, _P, ctypes.POINTER(_I)],
         "vgm_tube_create": [_P, _D, _D, _D, _D, _D, ctypes.POINTER(_I)],
@@ -276,6 +277,10 @@ class Context:
     def sphere(self, rmin, rmax, sphi=0.0, dphi=2 * np.pi, stheta=0.0, dtheta=np.pi) -> int:
         """UnplacedSphere(rmin, rmax, sphi, dphi, stheta, dtheta)."""
         return self._new(self._L.vgm_sphere_create, rmin, rmax, sphi, dphi, stheta, dtheta)
+
+    def torus(self, rmin, rmax, rtor, sphi=0.0, dphi=2 * np.pi) -> int:
+        """UnplacedTorus2(rmin, rmax, rtor, sphi, dphi)."""
+        return self._new(self._L.vgm_torus_create, rmin, rmax, rtor, sphi, dphi)
 
     def boolean(self, op, left, right, left_tr=None, right_tr=None) -> int:
         """UnplacedBooleanVolume: op "union", "intersection" or "subtraction";

@@ -517,6 +517,16 @@ int vgm_sphere_create(vgm_ctx *c, double rmin, double rmax, double sphi, double 
   return VGM_OK;
 }
 
+int vgm_torus_create(vgm_ctx *c, double rmin, double rmax, double rtor, double sphi, double dphi, int *shape)
+{
+  if (!c || !shape) return fail(VGM_EINVAL, "null argument");
+  std::string err;
+  int id = build_torus(c->host, rmin, rmax, rtor, sphi, dphi, err);
+  if (id < 0) return fail(VGM_EINVAL, err);
+  *shape = id;
+  return VGM_OK;
+}
+
 int vgm_normal(vgm_ctx *c, int pv, const double *x, const double *y, const double *z, size_t n, double *nx,
                double *ny, double *nz, uint8_t *valid, unsigned flags)
 {

@@ -8,6 +8,7 @@
 #include "shapes/cut_tube_data.h"
 #include "shapes/boolean_data.h"
 #include "shapes/sphere_data.h"
+#include "shapes/torus_data.h"
 #include "transform.hpp"
 #include "shapes/tube_data.h"
 #include "shapes/cone_data.h"
@@ -224,6 +225,45 @@ int build_sphere(HostGeometry &g, double rmin, double rmax, double sphi, double 
     }
   }
   return g.add_shape(kSphere, &s, sizeof(s) / sizeof(double), 0, 0, lo, hi);
+}
+
+// UnplacedTorus2(rmin, rmax, rtor, sphi, dphi): TorusStruct2's cached squares,
+// Wedge(dphi, sphi) and the bounding tube (VecGeom/volumes/TorusStruct2.h:37-43)
+// of the constructor's angles as given.  The extent is the bounding tube's
+// full box (UnplacedTorus2.h:210 asks the tube, which trims it by phi; the
+// full box contains the trimmed one).
+int build_torus(HostGeometry &g, double rmin, double rmax, double rtor, double sphi, double dphi, std::string &err)
+{
+  const double kPi = 3.14159265358979323846, kTol = 1e-9;
+  if (!(rmin >= 0 && rmin < rmax && rtor > 0 && dphi > 0)) {
+    err = "torus needs 0 <= rmin < rmax, rtor > 0, dphi > 0";
+    return -1;
+  }
+  TorusData t{};
+  t.rmin = rmin;
+  t.rmax = rmax;
+  t.rtor = rtor;
+  t.sphi = sphi;
+  t.dphi = dphi;
+  t.rmin2 = rmin * rmin;
+  t.rmax2 = rmax * rmax;
+  t.rtor2 = rtor * rtor;
+  t.w_a1x = std::cos(sphi);
+  t.w_a1y = std::sin(sphi);
+  t.w_a2x = std::cos(sphi + dphi);
+  t.w_a2y = std::sin(sphi + dphi);
+  t.w_n1x = -std::sin(sphi);
+  t.w_n1y = std::cos(sphi);
+  t.w_n2x = std::sin(sphi + dphi);
+  t.w_n2y = -std::cos(sphi + dphi);
+  const double band = 1.0e-8 * (rtor + rmax);
+  t.normal_delta = 10.0 * kTol > band ? 10.0 * kTol : band;
+  t.has_phi = dphi < 2 * kPi;
+  t.w_lt_pi = dphi < kPi;
+  make_tube_data(t.tube, rtor - rmax - kTol, rtor + rmax + kTol, rmax, sphi, dphi);
+  const double r = t.tube.rmax;
+  const double lo[3] = {-r, -r, -rmax}, hi[3] = {r, r, rmax};
+  return g.add_shape(kTorus, &t, sizeof(t) / sizeof(double), 0, 0, lo, hi);
 }
 
 // Plane::Set(normal, origin) (VecGeom/volumes/Plane.h:30-37) of the unit

@@ -46,6 +46,7 @@ int close_navigation(HostGeometry &g, int world_pv, std::string &err);
 // Shape builders.  Return the new shape id, or -1 with `err` set.
 int build_box(HostGeometry &g, double dx, double dy, double dz, std::string &err);
 int build_orb(HostGeometry &g, double r, std::string &err);
+int build_torus(HostGeometry &g, double rmin, double rmax, double rtor, double sphi, double dphi, std::string &err);
 int build_sphere(HostGeometry &g, double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta,
                  std::string &err);
 int build_boolean(HostGeometry &g, int op, int left, const double *ltr, int right, const double *rtr,

@@ -14,6 +14,7 @@
 #include "cut_tube.hpp"
 #include "boolean.hpp"
 #include "sphere.hpp"
+#include "torus.hpp"
 
 namespace vgm {
 
@@ -58,6 +59,9 @@ __host__ __device__ inline bool dispatch_shape_type(int type, F &&f)
     return true;
   case kSphere:
     f.template operator()<SphereShape>();
+    return true;
+  case kTorus:
+    f.template operator()<TorusShape>();
     return true;
   default:
     return false;
@@ -133,6 +137,9 @@ __host__ __device__ inline bool dispatch_shape_mask(int type, F &&f)
     return true;
   case kSphere:
     if constexpr ((Mask & (1u << kSphere)) != 0) f.template operator()<SphereShape>();
+    return true;
+  case kTorus:
+    if constexpr ((Mask & (1u << kTorus)) != 0) f.template operator()<TorusShape>();
     return true;
   default:
     return false;

@@ -21,7 +21,8 @@ enum ShapeType : int32_t {
   kCutTube = 9,
   kBoolean = 10,
   kSphere = 11,
-  kNumShapeTypes = 12
+  kTorus = 12,
+  kNumShapeTypes = 13
 };
 
 enum Method : int32_t {

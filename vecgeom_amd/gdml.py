@@ -44,7 +44,7 @@ from typing import Dict, List, Tuple
 __all__ = ["GdmlError", "load", "SUPPORTED_SOLIDS"]
 
 SUPPORTED_SOLIDS = ("box", "tube", "cutTube", "cone", "trd", "trap", "polycone", "polyhedra", "tessellated", "orb",
-                    "sphere", "subtraction", "intersection", "union")
+                    "sphere", "torus", "subtraction", "intersection", "union")
 
 # everything the reference reads as a solid (Middleware.cpp:64-103); the
 # others are reported as unsupported rather than skipped
@@ -223,6 +223,11 @@ class _Reader:
             rmin, rmax = self.lengths(el, "rmin", "rmax")
             sphi, dphi, sth, dth = self.angles(el, "startphi", "deltaphi", "starttheta", "deltatheta")
             s = g.sphere(rmin, rmax, sphi, dphi, sth, dth)
+        elif tag == "torus":
+            # Middleware.cpp:865-883: lengths and angles as given
+            rmin, rmax, rtor = self.lengths(el, "rmin", "rmax", "rtor")
+            sphi, dphi = self.angles(el, "startphi", "deltaphi")
+            s = g.torus(rmin, rmax, rtor, sphi, dphi)
         elif tag == "cutTube":
             # the normals read as lengths; z halved (Middleware.cpp:738-768)
             z, rmin, rmax, lx, ly, lz, hx, hy, hz = self.lengths(el, "z", "rmin", "rmax", "lowX", "lowY", "lowZ",
