@@ -598,7 +598,8 @@ def main():
             extra["audit"] = batch.learned_stats()
         if tuned or not args.launch:
             extra["modes"] = batch.modes()
-        extra["weights"] = batch.weights()
+        if not args.launch:  # the autotuner's per-family times (without it the weights are track counts)
+            extra["weights"] = batch.weights()
         final = state + [side, cr]
         log(f"[bench] {tag}: {args.steps * n * S / wall / 1e9:.2f} G track-steps/s (kernel {kern * 1e3:.2f} ms/launch)")
         log(f"[bench] {tag} family kernel ms: " + ", ".join(f"{k} {v:.1f}" for k, v in fam_ms.items()))
