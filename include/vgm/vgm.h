@@ -92,7 +92,8 @@ int vgm_torus_create(vgm_ctx *ctx, double rmin, double rmax, double rtor, double
  * BooleanIntersectionImplementation.h and BooleanUnionImplementation.h):
  * op 0 = kUnion, 1 = kIntersection, 2 = kSubtraction (left minus right),
  * the reference's BooleanOperation values.  The operands are
- * existing primitive shapes (not booleans or tessellated solids) placed by
+ * existing primitive shapes (not booleans, tessellated solids or tori:
+ * VGM_EINVAL) placed by
  * left_tr / right_tr (12 doubles: translation, row-major rotation; NULL =
  * identity).  The operand parameters are copied at creation. */
 int vgm_boolean_create(vgm_ctx *ctx, int op, int left, const double *left_tr, int right, const double *right_tr,

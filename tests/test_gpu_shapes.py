@@ -136,3 +136,17 @@ def test_shape_parity(ctx, name, placed):
         for k, c in enumerate("xyzv"):
             bad = compare(ref[k], got[k])
             assert bad.size == 0, ("Normal", c, bad.size, bad[:5], ref[k][bad[:5]], got[k][bad[:5]])
+
+
+def test_boolean_operand_families(ctx):
+    """Operands outside the families the boolean kernels dispatch to (booleans,
+    tessellated solids, tori) are refused when the solid is built, with the
+    message saying so; a primitive pair builds."""
+    box = ctx.box(10.0, 10.0, 10.0)
+    torus = ctx.torus(0.0, 5.0, 20.0)
+    tess = ctx.tessellated(*SHAPES["tess_box"][1])
+    ok = ctx.boolean("union", box, ctx.sphere(0.0, 12.0))
+    for bad in (torus, tess, ok):
+        for left, right in ((box, bad), (bad, box)):
+            with pytest.raises(vg.VgmError, match="primitive solids"):
+                ctx.boolean("subtraction", left, right)

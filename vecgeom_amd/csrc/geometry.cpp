@@ -385,8 +385,8 @@ int build_boolean(HostGeometry &g, int op, int left, const double *ltr, int righ
   }
   const ShapeRec L = g.shapes[left], R = g.shapes[right];
   for (const ShapeRec *s : {&L, &R})
-    if (s->type == kBoolean || s->type == kTessellated) {
-      err = "boolean operands must be primitive solids (no booleans, no tessellated solids)";
+    if (!(kBoolOperandMask & (1u << s->type))) {
+      err = "boolean operands must be primitive solids (no booleans, no tessellated solids, no tori)";
       return -1;
     }
   BooleanData b{};

@@ -7,7 +7,8 @@
 // boolean's frame and transform it; DistanceToOut and SafetyToOut take it as
 // given.  The reference's unbounded search loops are capped at
 // kBoolMaxIterations passes, after which the ray counts as a miss.
-// Operands are primitive families (no booleans, no tessellated solids).
+// Operands are the families of kBoolOperandMask (no booleans, no tessellated
+// solids, no tori); the builder refuses the others.
 #pragma once
 #include "../vgm_math.hpp"
 #include "../transform.hpp"
@@ -16,9 +17,6 @@
 namespace vgm {
 
 constexpr int kBoolMaxIterations = 1000;
-constexpr unsigned kBoolOperandMask = (1u << kBox) | (1u << kTube) | (1u << kCone) | (1u << kTrd) |
-                                      (1u << kTrapezoid) | (1u << kPolycone) | (1u << kPolyhedron) | (1u << kOrb) |
-                                      (1u << kCutTube) | (1u << kSphere);
 
 enum BoolOperandQuery : int { kOpContains, kOpInside, kOpDistIn, kOpDistOut, kOpSafIn, kOpSafOut };
 

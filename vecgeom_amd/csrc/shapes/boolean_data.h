@@ -9,6 +9,12 @@ namespace vgm {
 
 enum BooleanOp : int32_t { kBoolUnion = 0, kBoolIntersection = 1, kBoolSubtraction = 2 };  // BooleanOperation
 
+// Shape families an operand may be: the primitive solids whose queries the
+// boolean kernels dispatch to (not booleans, tessellated solids or tori).
+constexpr unsigned kBoolOperandMask = (1u << kBox) | (1u << kTube) | (1u << kCone) | (1u << kTrd) |
+                                      (1u << kTrapezoid) | (1u << kPolycone) | (1u << kPolyhedron) | (1u << kOrb) |
+                                      (1u << kCutTube) | (1u << kSphere);
+
 struct BooleanData {
   int32_t op;            // BooleanOp
   int32_t ltype, rtype;  // operand shape types
