@@ -191,9 +191,10 @@ int vgm_safety_to_out(vgm_ctx *ctx, int pv, const double *x, const double *y, co
 /* VPlacedVolume::Normal (VecGeom/volumes/PlacedVolume.h:437-439,
  * source/PlacedVolume.cpp:126-137) over n master-frame points: the unit
  * normal of the surface the point is on, or of the nearest one (master
- * frame), and valid[i] = 1 when the point is on the surface.  The reference
- * has only the scalar form.  Every family but the tessellated solid; FP64
- * only (VGM_MIXED is refused). */
+ * frame), and valid[i] = 1 when the point is on the surface (the families'
+ * own rules: the tessellated solid's nearest-facet normal is always valid,
+ * a torus point off every surface gets a zero normal).  The reference has
+ * only the scalar form.  Every family; FP64 only (VGM_MIXED is refused). */
 int vgm_normal(vgm_ctx *ctx, int pv, const double *x, const double *y, const double *z, size_t n, double *nx,
                double *ny, double *nz, uint8_t *valid, unsigned flags);
 

@@ -435,6 +435,26 @@ struct Tessellated final : Shape {
     return Sqrt(safety2<false>(p));
   }
 
+  // UnplacedTessellated::Normal (source/UnplacedTessellated.cpp:83-89,
+  // TessellatedImplementation.h:163-172): the normal of the facet nearest to
+  // p by SafetySq<ToIn = false>, valid everywhere; over all facets in
+  // definition order with strict '<', so ties go to the facet defined first.
+  bool Normal(const Vec3 &p, Vec3 &n) const override
+  {
+    double best = kInfLength;
+    int isurf = -1;
+    for (size_t k = 0; k < f.size(); ++k) {
+      const double s = facet_safety2<false>(f[k], p);
+      if (s < best) {
+        best = s;
+        isurf = int(k);
+      }
+    }
+    n = f[size_t(isurf)].n;
+    return true;
+  }
+  bool HasNormal() const override { return true; }
+
   void Extent(Vec3 &a, Vec3 &b) const override
   {
     a = mn;

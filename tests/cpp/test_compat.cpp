@@ -141,6 +141,26 @@ int main()
   shell.DistanceToIn(O, E, nullptr, &din_shell);
   EXPECT(din_shell == 6.);
 
+  // TestTorus2.cpp's t1 (tube radius 90 around a circle of radius 100),
+  // entered along -x from (240, 0, 0): the outer surface at x = 190
+  PlacedVolume ring = geo.Place(geo.NewLogicalVolume(geo.Torus(0., 90., 100.)));
+  std::vector<double> tx{240}, ty{0}, tz{0}, mx{-1}, my{0}, mz{0};
+  SOA3D<> TP(tx.data(), ty.data(), tz.data(), 1), TD(mx.data(), my.data(), mz.data(), 1);
+  double din_ring = 0.;
+  ring.DistanceToIn(TP, TD, nullptr, &din_ring);
+  EXPECT(std::fabs(din_ring - 50.) < 1e-9);
+  double n_ring[3];
+  const double on_ring[3] = {190., 0., 0.};
+  EXPECT(ring.Normal(on_ring, n_ring) && std::fabs(n_ring[0] - 1.) < 1e-12 && n_ring[1] == 0. && n_ring[2] == 0.);
+  // a torus is not a boolean operand of this library
+  bool torus_refused = false;
+  try {
+    geo.Boolean(2, geo.Box(300., 300., 100.), nullptr, geo.Torus(0., 90., 100.), nullptr);
+  } catch (const Error &e) {
+    torus_refused = e.code == VGM_EINVAL;
+  }
+  EXPECT(torus_refused);
+
   // errors surface as exceptions with the library's message
   bool threw = false;
   try {

@@ -192,3 +192,28 @@ def test_sphere_normal_geant4_values(case):
     params, p, want = case
     n, _ = _normal(oracle.Geometry(), "sphere", params, p)
     np.testing.assert_allclose(n, want, atol=1e-6)
+
+
+def test_tessellated_normal_is_the_nearest_facets():
+    """A tessellated box (TestTessellated.cpp's trd-like construction with
+    equal ends) has the box's face normals at points inside its faces, and a
+    hull's normal at a facet's centroid is that facet's normal; off the
+    surface the nearest facet still gives it, and it is always valid."""
+    geo = oracle.Geometry()
+    tri = kats.trd_like_triangles(20.0, 20.0, 30.0, 30.0, 40.0)
+    pv = geo.place(geo.logical_volume(geo.tessellated(tri)))
+    pts = [(20.0, 3.0, -7.0), (-20.0, 1.0, 2.0), (4.0, 30.0, 5.0), (-3.0, -30.0, 1.0), (2.0, 5.0, 40.0),
+           (-6.0, 2.0, -40.0), (25.0, 1.0, 2.0), (1.0, 2.0, 44.0)]
+    want = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 0, 0), (0, 0, 1)]
+    nx, ny, nz, valid = geo.normal(pv, [np.array([p[k] for p in pts]) for k in range(3)])
+    np.testing.assert_allclose(np.stack([nx, ny, nz], 1), np.array(want, dtype=float), atol=1e-15)
+    assert valid.all()
+    from vecgeom_amd import workloads
+    hull = workloads.ellipsoid_hull(200, (30.0, 20.0, 45.0), 3)
+    pv = geo.place(geo.logical_volume(geo.tessellated(hull)))
+    c = hull.mean(axis=1)
+    e1, e2 = hull[:, 1] - hull[:, 0], hull[:, 2] - hull[:, 0]
+    n = np.cross(e1, e2)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    nx, ny, nz, valid = geo.normal(pv, c.T.copy())
+    np.testing.assert_allclose(np.stack([nx, ny, nz], 1), n, atol=1e-12)

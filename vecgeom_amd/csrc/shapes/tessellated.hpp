@@ -282,6 +282,61 @@ struct TessellatedShape {
     return best;
   }
 
+  // Storage slot of the facet nearest to p by TessellatedCluster::SafetySq
+  // with ToIn = false; equal distances go to the facet defined first.  The
+  // same walk as safety2, which prunes only boxes strictly beyond the best
+  // distance, so every facet that ties with it is visited.
+  VGM_DEV static int nearest_facet(const Data &t, V3<double> p)
+  {
+    const TessFacet *fa = t.facets();
+    const TessNode *nodes = t.nodes();
+    double best = kInfLength;
+    int slot = -1, orig = 0;
+    TessWalkStack stack;
+    int sp = 0;
+    stack.set(sp++, tess_walk_entry<double>(0, -1.0));
+    while (sp) {
+      const uint32_t e = stack.get(--sp);
+      if (tess_walk_key<double>(e) > best) continue;
+      const TessNode &nd = nodes[e & 0xffffu];
+      const double d0 = box_dist2<double>(nd, 0, p), d1 = box_dist2<double>(nd, 1, p);
+      const int near = (d1 < d0) ? 1 : 0;
+#pragma unroll 1
+      for (int o = 0; o < 2; ++o) {
+        const int i = near ^ o;
+        if (nd.child[i] >= 0 || (i ? d1 : d0) > best) continue;
+        const int f0 = nd.first[i], f1 = f0 + nd.count[i];
+        for (int k = f0; k < f1; ++k) {
+          const double s = facet_safety2<double>(fa[k], p, false);
+          if (s < best || (s == best && fa[k].orig < orig)) {
+            best = s;
+            slot = k;
+            orig = fa[k].orig;
+          }
+        }
+      }
+#pragma unroll 1
+      for (int o = 1; o >= 0; --o) {
+        const int i = near ^ o;
+        const double di = i ? d1 : d0;
+        if (nd.child[i] < 0 || di > best) continue;
+        stack.set(sp++, tess_walk_entry<double>(nd.child[i], di));
+      }
+    }
+    return slot;
+  }
+
+  // UnplacedTessellated::Normal (source/UnplacedTessellated.cpp:83-89,
+  // TessellatedImplementation.h:163-172): the normal of the nearest facet,
+  // valid everywhere.
+  VGM_DEV static bool normal(const Data &t, V3<double> p, V3<double> &n)
+  {
+    const int k = nearest_facet(t, p);
+    const TessFacet &f = t.facets()[k];
+    n = V3<double>{f.n[0], f.n[1], f.n[2]};
+    return true;
+  }
+
   // BoxImplementation::IntersectCachedKernel2 over the extent, t0 = -kTolerance,
   // t1 = kInfLength (BoxImplementation.h:326-368)
   template <typename R>
