@@ -335,6 +335,18 @@ class Navigator {
   {
     check(vgm_nav_approach(ctx_, p.x(), p.y(), p.z(), d.x(), d.y(), d.z(), p.size(), state, last, step, flags(p)));
   }
+  // VLevelLocator::LevelLocateExclVol (BVHLevelLocator.h:73-95) per point,
+  // in the frame of the volume of state[i]: the daughter's state (0: none)
+  // and the point in its frame; with a direction (dir non-null) a surface
+  // point is taken only when the direction enters the daughter; excl may be
+  // null
+  void LevelLocate(SOA3D<> const &p, SOA3D<> const *dir, uint32_t const *state, uint32_t const *excl,
+                   uint32_t *child, SOA3D<> &daughter_point) const
+  {
+    check(vgm_nav_level_locate(ctx_, p.x(), p.y(), p.z(), dir ? dir->x() : nullptr, dir ? dir->y() : nullptr,
+                               dir ? dir->z() : nullptr, p.size(), state, excl, child, daughter_point.x(),
+                               daughter_point.y(), daughter_point.z(), flags(p)));
+  }
   void ComputeSafeties(SOA3D<> const &p, uint32_t const *state, Precision *safety) const
   {
     check(vgm_nav_safety(ctx_, p.x(), p.y(), p.z(), p.size(), state, safety, flags(p)));

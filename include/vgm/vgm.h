@@ -322,6 +322,19 @@ int vgm_nav_transport_limited(vgm_ctx *ctx, double *x, double *y, double *z, con
 int vgm_nav_approach(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx,
                      const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *last,
                      double *step, unsigned flags);
+/* VLevelLocator::LevelLocateExclVol through the BVH
+ * (VecGeom/navigation/BVHLevelLocator.h:73-95, VecGeom/base/BVH.h:219-344):
+ * for point i in the frame of the volume of state[i], the first daughter in
+ * BVH order, other than the daughter whose state is excl[i] (excl NULL or
+ * excl[i] = 0: none), that holds it: by its Contains without a direction
+ * (dx, dy, dz NULL), or with one, by its Inside being kInside, or kSurface
+ * with the direction entering it (its Normal against the direction < 0).
+ * child[i] = that daughter's state and (qx, qy, qz)[i] the point in its
+ * frame; child[i] = 0 and the point unchanged when none holds it or
+ * state[i] = 0. */
+int vgm_nav_level_locate(vgm_ctx *ctx, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *excl,
+                         uint32_t *child, double *qx, double *qy, double *qz, unsigned flags);
 /* out[i] = safety of global point i in its state (mother SafetyToOut, then
  * the daughters through the BVH); 0 for state 0. */
 int vgm_nav_safety(vgm_ctx *ctx, const double *x, const double *y, const double *z, size_t n,

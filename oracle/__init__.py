@@ -69,6 +69,7 @@ def lib():
             "orc_nav_transport_limited": [_P] * 8 + [ctypes.c_size_t, _P, _P, _P, _I, _I, _P, _P, _I],
             "orc_nav_approach": [_P] * 7 + [ctypes.c_size_t, _P, _P, _P, _I],
             "orc_nav_safety": [_P, _P, _P, _P, ctypes.c_size_t, _P, _P, _I],
+            "orc_nav_level_locate": [_P] * 7 + [ctypes.c_size_t] + [_P] * 6 + [_I],
             "orc_trapezoid_create": [_P] + [_D] * 11,
             "orc_shape_extent": [_P, _I, _P, _P],
             "orc_lv_create": [_P, _I],
@@ -218,6 +219,18 @@ class Geometry:
         dx, dy, dz = (np.ascontiguousarray(a, dtype=np.float64) for a in dirs)
         self._L.orc_nav_approach(self.h, _p(x), _p(y), _p(z), _p(dx), _p(dy), _p(dz), len(x), _p(state), _p(last),
                                  _p(step), nthreads)
+
+    def nav_level_locate(self, pts, state, dirs=None, excl=None, nthreads=0):
+        """LevelLocate in the volume of each state (directional with dirs): (child state, daughter-frame point)."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)
+        st = np.ascontiguousarray(state, dtype=np.uint32)
+        d = [None] * 3 if dirs is None else [np.ascontiguousarray(a, dtype=np.float64) for a in dirs]
+        ex = None if excl is None else np.ascontiguousarray(excl, dtype=np.uint32)
+        child = np.zeros(len(x), np.uint32)
+        q = [np.zeros(len(x)) for _ in range(3)]
+        self._L.orc_nav_level_locate(self.h, _p(x), _p(y), _p(z), _p(d[0]), _p(d[1]), _p(d[2]), len(x), _p(st),
+                                     _p(ex), _p(child), _p(q[0]), _p(q[1]), _p(q[2]), nthreads)
+        return child, q
 
     def nav_safety(self, pts, state, nthreads=0):
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in pts)

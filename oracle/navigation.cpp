@@ -344,6 +344,79 @@ struct Navigator {
     return -1;
   }
 
+  // BVH::LevelLocate(exclvol, localpoint, localdirection, pvol,
+  // daughterlocalpoint) (VecGeom/base/BVH.h:297-344): the first daughter
+  // (skipping `excl`) whose unplaced Inside is kInside, or kSurface with
+  // the direction entering it (the unplaced Normal against the daughter-local
+  // direction < 0); a linear scan in daughter order without the BVH.
+  int LevelLocateDir(const LogicalVolume &lv, int lvid, int excl, const Vec3 &p, const Vec3 &d, Vec3 &dp) const
+  {
+    auto holds = [&](int v, Vec3 &q) {
+      const Transform &T = G.pvs[v].tr;
+      q = T.Transform_(p);
+      const Shape &u = ShapeOf(v);
+      const int inside = u.Inside(q);
+      if (inside == kInside) return true;
+      if (inside != kSurface) return false;
+      const Vec3 dir = T.TransformDirection(d);
+      Vec3 normal;
+      u.Normal(q, normal);
+      return Dot(normal, dir) < 0.0;
+    };
+    if (!use_bvh) {
+      for (size_t k = 0; k < lv.daughters.size(); ++k) {
+        const int v = lv.daughters[k];
+        Vec3 q;
+        if (v != excl && holds(v, q)) {
+          dp = q;
+          return int(k);
+        }
+      }
+      return -1;
+    }
+    const Bvh &B = bvh[lvid];
+    unsigned stack[40], *ptr = &stack[1];
+    stack[0] = 0;
+    do {
+      const unsigned id = *--ptr;
+      if (B.nchild[id] >= 0) {
+        for (int i = 0; i < B.nchild[id]; ++i) {
+          const int k = B.prim[B.offset[id] + i];
+          if (!B.aabbs[k].Contains(p)) continue;
+          const int v = lv.daughters[k];
+          if (v == excl) continue;
+          Vec3 q;
+          if (holds(v, q)) {
+            dp = q;
+            return k;
+          }
+        }
+      } else {
+        const unsigned L = 2 * id + 1, R = 2 * id + 2;
+        if (B.nodes[L].Contains(p)) *ptr++ = L;
+        if (B.nodes[R].Contains(p)) *ptr++ = R;
+      }
+    } while (ptr > stack);
+    return -1;
+  }
+
+  // LevelLocate in the volume of state `node` (directional when d is given),
+  // skipping the daughter of state `excl`: the daughter's state or 0
+  unsigned LevelLocateState(unsigned node, unsigned excl, const Vec3 &p, const Vec3 *d, Vec3 &q) const
+  {
+    q = p;
+    if (!node) return 0u;
+    const int pv = nodes[node].pv;
+    const LogicalVolume &lv = LvOf(pv);
+    if (lv.daughters.empty()) return 0u;
+    const int ex = excl ? nodes[excl].pv : -1;
+    Vec3 dp;
+    const int k = d ? LevelLocateDir(lv, G.pvs[pv].lv, ex, p, *d, dp) : LevelLocate(lv, G.pvs[pv].lv, ex, p, dp);
+    if (k < 0) return 0u;
+    q = dp;
+    return unsigned(nodes[node].children[size_t(k)]);
+  }
+
   // LocateGlobalPoint(vol, point, path, top = false) with `node` the node of vol
   unsigned Descend(unsigned node, Vec3 p, int excl) const
   {
@@ -780,6 +853,25 @@ int orc_nav_approach(orc_geom *g, const double *x, const double *y, const double
   for (long i = 0; i < long(n); ++i) {
     if (!state[i]) continue;
     step[i] = N->Approach(Vec3(x[i], y[i], z[i]), Vec3(dx[i], dy[i], dz[i]), step[i], State{state[i], last[i], false});
+  }
+  return 0;
+}
+
+int orc_nav_level_locate(orc_geom *g, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *excl,
+                         uint32_t *child, double *qx, double *qy, double *qz, int nthreads)
+{
+  Navigator *N = nav_of(g);
+  if (!N) return -1;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (long i = 0; i < long(n); ++i) {
+    const Vec3 d = dx ? Vec3(dx[i], dy[i], dz[i]) : Vec3();
+    Vec3 q;
+    child[i] = N->LevelLocateState(state[i], excl ? excl[i] : 0u, Vec3(x[i], y[i], z[i]), dx ? &d : nullptr, q);
+    qx[i] = q.x;
+    qy[i] = q.y;
+    qz[i] = q.z;
   }
   return 0;
 }

@@ -87,6 +87,10 @@ int orc_nav_approach(orc_geom *g, const double *x, const double *y, const double
 int orc_nav_transport(orc_geom *g, double *x, double *y, double *z, const double *dx, const double *dy,
                       const double *dz, size_t n, uint32_t *state, uint32_t *last, uint8_t *boundary, int nsteps,
                       int32_t *steps, int nthreads);
+/* LevelLocate in the volume of each state, directional when dx is given */
+int orc_nav_level_locate(orc_geom *g, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *excl,
+                         uint32_t *child, double *qx, double *qy, double *qz, int nthreads);
 int orc_nav_safety(orc_geom *g, const double *x, const double *y, const double *z, size_t n, const uint32_t *state,
                    double *out, int nthreads);
 

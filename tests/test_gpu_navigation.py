@@ -248,3 +248,33 @@ def test_transport_limited(setup, L, reuse):
     for k, (r, g) in enumerate(zip(*outs)):
         assert np.array_equal(g, r), k
     assert outs[0][7].sum() < outs[0][6].sum()
+
+
+@pytest.mark.parametrize("directional", [False, True])
+def test_level_locate(setup, directional):
+    """LevelLocate in the world and in the layers (placed at the identity, so
+    global points are their local ones), on random points and on the surface
+    points where one step stops, with and without the direction and with an
+    excluded daughter: the daughter's state and its local point bit-exact
+    against the oracle."""
+    print(f"[nav] level locate directional={directional}", flush=True)
+    ctx, nav, geo, pts, dirs = setup
+    st = geo.nav_locate(pts)
+    s, last, bnd = st.copy(), np.zeros(N, np.uint32), np.zeros(N, np.uint8)
+    step = geo.nav_step(pts, dirs, s, last, bnd)
+    on = tuple(np.ascontiguousarray(pts[k] + step * dirs[k]) for k in range(3))
+    layer = np.array([n if geo.nav_node(int(n))[1] <= 1 else geo.nav_node(int(n))[2] for n in st], np.uint32)
+    world = np.where(st != 0, 1, 0).astype(np.uint32)
+    d = dirs if directional else None
+    found = 0
+    for p, state, excl in ((pts, world, None), (pts, layer, None), (on, layer, None), (on, layer, s)):
+        ex = None if excl is None else np.ascontiguousarray(excl, dtype=np.uint32)
+        c_o, q_o = geo.nav_level_locate(p, state, d, ex)
+        c_g = np.zeros(N, np.uint32)
+        q_g = [np.zeros(N) for _ in range(3)]
+        nav.level_locate(*p, state, c_g, *q_g, dirs=d, excl=ex)
+        assert np.array_equal(c_g, c_o)
+        for k in range(3):
+            assert np.array_equal(q_g[k], q_o[k])
+        found += int(np.count_nonzero(c_g))
+    assert found > N

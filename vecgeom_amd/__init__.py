@@ -138,6 +138,7 @@ def _declare(L):
         "vgm_nav_transport_limited": [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _I, _I, _P, _P, _U],
         "vgm_nav_approach": [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _U],
         "vgm_nav_safety": [_P, _P, _P, _P, _SZ, _P, _P, _U],
+        "vgm_nav_level_locate": [_P] * 7 + [_SZ] + [_P] * 6 + [_U],
     }
     for name, args in nsig.items():
         f = getattr(L, name)
@@ -554,6 +555,15 @@ class Navigation:
         """BVH::ApproachNextDaughter: step shortened to the nearest daughter bounding box."""
         check(self._L.vgm_nav_approach(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(dx), _ptr(dy), _ptr(dz),
                                        len(x), _ptr(state), _ptr(last), _ptr(step), self._fl(x)))
+
+    def level_locate(self, x, y, z, state, child, qx, qy, qz, dirs=None, excl=None):
+        """LevelLocate in the volume of each state, points in that volume's frame
+        (VLevelLocator::LevelLocateExclVol; directional with dirs = (dx, dy, dz)):
+        child = the daughter's state or 0, (qx, qy, qz) = the point in its frame."""
+        d = (None, None, None) if dirs is None else dirs
+        check(self._L.vgm_nav_level_locate(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), _ptr(d[0]), _ptr(d[1]),
+                                           _ptr(d[2]), len(x), _ptr(state), _ptr(excl), _ptr(child), _ptr(qx),
+                                           _ptr(qy), _ptr(qz), self._fl(x)))
 
     def safety(self, x, y, z, state, out):
         check(self._L.vgm_nav_safety(self.ctx.handle, _ptr(x), _ptr(y), _ptr(z), len(x), _ptr(state), _ptr(out),

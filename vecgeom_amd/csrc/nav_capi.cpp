@@ -315,6 +315,25 @@ int vgm_nav_approach(vgm_ctx *c, const double *x, const double *y, const double 
                  flags);
 }
 
+int vgm_nav_level_locate(vgm_ctx *c, const double *x, const double *y, const double *z, const double *dx,
+                         const double *dy, const double *dz, size_t n, const uint32_t *state, const uint32_t *excl,
+                         uint32_t *child, double *qx, double *qy, double *qz, unsigned flags)
+{
+  if (n && (!state || !child || !qx || !qy || !qz)) return fail(VGM_EINVAL, "null state or output array");
+  if ((dx || dy || dz) && !(dx && dy && dz)) return fail(VGM_EINVAL, "direction needs all three components");
+  NavArgs a{};
+  a.n = n;
+  const size_t b8 = n * sizeof(double), b4 = n * sizeof(uint32_t);
+  return run_nav(c, kNavLevelLocate, a,
+                 {{x, b8, true, false, slot(a.x)}, {y, b8, true, false, slot(a.y)}, {z, b8, true, false, slot(a.z)},
+                  {dx, b8, true, false, slot(a.dx), true}, {dy, b8, true, false, slot(a.dy), true},
+                  {dz, b8, true, false, slot(a.dz), true}, {state, b4, true, false, slot(a.state)},
+                  {excl, b4, true, false, slot(a.last), true}, {child, b4, false, true, slot(a.child)},
+                  {qx, b8, false, true, slot(a.ox)}, {qy, b8, false, true, slot(a.oy)},
+                  {qz, b8, false, true, slot(a.oz)}},
+                 flags);
+}
+
 int vgm_nav_safety(vgm_ctx *c, const double *x, const double *y, const double *z, size_t n, const uint32_t *state,
                    double *out, unsigned flags)
 {

@@ -16,6 +16,7 @@ enum NavKind : int32_t {
   kNavApproach = 4,           // ApproachNextDaughter per track
   kNavTransportLimited = 5,   // transport under a physics step, safety first
   kNavStepMove = 6,           // one straight-line transport step (graph-captured loops)
+  kNavLevelLocate = 7,        // LevelLocate in the volume of a state (with a direction: the entering rule)
 };
 
 struct NavArgs {
@@ -30,6 +31,8 @@ struct NavArgs {
   double *safety;             // step with safeties: output safety (null: plain step)
   int32_t *steps;  // transport: steps taken (accumulated)
   int32_t *geo_steps;  // limited transport: steps that needed the boundary distance (accumulated)
+  uint32_t *child;              // level locate: the daughter's state, 0 if none (excluded one: `last`)
+  double *ox, *oy, *oz;         // level locate: the point in the daughter's frame
   int32_t reuse_safety;  // limited transport: carry the safety left after a safety-only move
   int32_t no_relocate;   // step: ComputeStep, the out state without relocation
   unsigned long long *queue;  // transports: work-queue counter (device, reset by the launch)

@@ -91,6 +91,39 @@ __device__ __forceinline__ long long pull_track(unsigned long long *queue, bool 
 
 __global__ void nav_queue_reset_kernel(unsigned long long *queue) { *queue = 0ull; }
 
+// LevelLocate in the logical volume of state[i] for a point (and, when dx is
+// given, a direction) in that volume's frame, skipping the daughter whose
+// state is last[i] (0: none): the daughter's state and the point in its
+// frame, or 0 and the point itself.
+template <unsigned Mask>
+__global__ __launch_bounds__(kNavBlock) void nav_level_locate_kernel(NavArgs a)
+{
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const V3<double> p{a.x[i], a.y[i], a.z[i]};
+  const unsigned node = a.state[i];
+  uint32_t child = 0u;
+  V3<double> q = p;
+  if (node) {
+    Navigator<Mask> nav{a.g};
+    const NavNode &N = a.g.nodes[node];
+    const int lv = a.g.pvs[N.pv].lv;
+    const unsigned ex = a.last ? a.last[i] : 0u;
+    const int excl_pv = ex ? a.g.nodes[ex].pv : -1;
+    V3<double> dp;
+    const int k = a.dx ? nav.level_locate_dir(lv, excl_pv, p, V3<double>{a.dx[i], a.dy[i], a.dz[i]}, dp)
+                       : nav.level_locate(lv, excl_pv, p, dp);
+    if (k >= 0) {
+      child = uint32_t(a.g.children[N.child0 + k]);
+      q = dp;
+    }
+  }
+  a.child[i] = child;
+  a.ox[i] = q.x;
+  a.oy[i] = q.y;
+  a.oz[i] = q.z;
+}
+
 template <unsigned Mask>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kNavWavesPerEU<Mask>))) void nav_transport_kernel(NavArgs a)
 {
@@ -270,6 +303,9 @@ void launch_nav_mask(hipStream_t stream, int kind, const NavArgs &a, unsigned bl
     break;
   case kNavStepMove:
     hipLaunchKernelGGL(nav_step_move_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
+    break;
+  case kNavLevelLocate:
+    hipLaunchKernelGGL(nav_level_locate_kernel<Mask>, dim3(blocks), dim3(kNavBlock), 0, stream, a);
     break;
   case kNavTransportLimited:
     hipLaunchKernelGGL(nav_queue_reset_kernel, dim3(1), dim3(1), 0, stream, a.queue);

@@ -38,7 +38,18 @@ struct GeomView {
   int32_t world_pv;
 };
 
-enum ShapeQuery : int32_t { kQContains = 0, kQDistIn = 1, kQDistOut = 2, kQSafIn = 3, kQSafOut = 4 };
+// kQInside returns the Inside code; kQNormalDot the dot product of the
+// unplaced Normal at p with d (FP64 only), the entering test of the
+// directional LevelLocate.
+enum ShapeQuery : int32_t {
+  kQContains = 0,
+  kQDistIn = 1,
+  kQDistOut = 2,
+  kQSafIn = 3,
+  kQSafOut = 4,
+  kQInside = 5,
+  kQNormalDot = 6
+};
 
 template <typename R = double>
 struct ShapeCall {
@@ -56,6 +67,16 @@ struct ShapeCall {
     case kQDistIn: result = S::template distance_to_in<R>(s, p, d, step); break;
     case kQDistOut: result = S::template distance_to_out<R>(s, p, d, step); break;
     case kQSafIn: result = S::template safety_to_in<R>(s, p); break;
+    case kQInside: result = R(double(S::template inside<R>(s, p))); break;
+    case kQNormalDot:
+      if constexpr (std::is_same_v<R, double>) {
+        V3<double> n{0.0, 0.0, 0.0};
+        S::normal(s, p, n);
+        result = n.x * d.x + n.y * d.y + n.z * d.z;  // Vector3D::Dot
+      } else {
+        result = R(0.0);
+      }
+      break;
     default: result = S::template safety_to_out<R>(s, p); break;
     }
   }
@@ -312,6 +333,51 @@ struct Navigator {
           if (v == excl_pv) continue;
           const V3<R> q = to_local_point(g.pvs[v].tr, p);
           if (shape_query<Mask, R>(g.shapes, g.pool, shape_of_pv(v), kQContains, q, q, 0.0) != 0.0) {
+            dp = q;
+            return k;
+          }
+        }
+      } else {
+        const unsigned cl = 2 * id + 1, cr = 2 * id + 2;
+        if (box_contains(nbox + 6 * cl, p)) stack.set(sp++, cl);
+        if (box_contains(nbox + 6 * cr, p)) stack.set(sp++, cr);
+      }
+    } while (sp > 0);
+    return -1;
+  }
+
+  // BVH::LevelLocate(exclvol, localpoint, localdirection, pvol,
+  // daughterlocalpoint) (VecGeom/base/BVH.h:297-344): first daughter in tree
+  // order (skipping excl_pv) whose unplaced Inside at the daughter-local
+  // point is kInside, or kSurface with the direction entering it (the
+  // unplaced Normal there against the daughter-local direction, < 0)
+  __host__ __device__ int level_locate_dir(int lv, int excl_pv, V3<R> p, V3<R> d, V3<R> &dp) const
+  {
+    const LogicalRec &L = g.lvs[lv];
+    if (L.bvh < 0) return -1;
+    const BvhRec &B = g.bvhs[L.bvh];
+    const int32_t *prim = g.ipool + B.prim, *offset = g.ipool + B.offset, *nchild = g.ipool + B.nchild;
+    const double *nbox = g.bpool + B.node_box, *pbox = g.bpool + B.prim_box;
+    NavWalkStack stack;
+    int sp = 0;
+    stack.set(sp++, 0);
+    do {
+      const unsigned id = stack.get(--sp);
+      const int nc = nchild[id];
+      if (nc >= 0) {
+        for (int i = 0; i < nc; ++i) {
+          const int k = prim[offset[id] + i];
+          if (!box_contains(pbox + 6 * k, p)) continue;
+          const int v = g.daughters[L.dbegin + k];
+          if (v == excl_pv) continue;
+          const V3<R> q = to_local_point(g.pvs[v].tr, p);
+          const int inside = int(shape_query<Mask, R>(g.shapes, g.pool, shape_of_pv(v), kQInside, q, q, R(0.0)));
+          bool take = inside == kInsideV;
+          if (inside == kSurfaceV) {
+            const V3<R> qd = to_local_dir(g.pvs[v].tr, d);
+            take = shape_query<Mask, R>(g.shapes, g.pool, shape_of_pv(v), kQNormalDot, q, qd, R(0.0)) < R(0.0);
+          }
+          if (take) {
             dp = q;
             return k;
           }
