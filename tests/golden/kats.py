@@ -1011,3 +1011,78 @@ TORUS_NORMAL_KATS = [
     (_TN3, (100.0, 0.0, 90.0), (0.0, 1 / _R2, 1 / _R2)), (_TN3, (0.0, 100.0, 90.0), (1 / _R2, 0.0, 1 / _R2)),
     (_T2, _TONRMIN, (-1 / _R2, -1 / _R2, 0.0)), (_T2, (20.0, 0.0, 0.0), VMY), (_T2, (0.0, 20.0, 0.0), VMX),
 ]
+
+# test/core/PhiWedgeTest.cpp and test/core/ThetaConeTest.cpp, through a
+# sphere whose radial range holds the points (rmin 0, rmax 1000): its Inside
+# and Contains are then those of its phi wedge (Wedge_Evolution.h, the same
+# kernels as the Wedge these tests construct) or its theta cone
+# (ThetaCone_Evolution.h, the same kernels as ThetaCone).  Sphere parameters
+# (rmin, rmax, sphi, dphi, stheta, dtheta).
+_KT = 1e-9  # kTolerance
+_KA = 1e-9  # kAngTolerance
+
+
+def _wedge(angle, zero=0.0):
+    return (0.0, 1000.0, zero, angle, 0.0, PI)
+
+
+def _theta(st, dt):
+    return (0.0, 1000.0, 0.0, 2 * PI, st, dt)
+
+
+def _cs(a):
+    return (math.cos(a), math.sin(a), 0.0)
+
+
+_W1 = _wedge(1.5 * PI, PI / 6)
+_W2 = _wedge(PI / 3)
+_A3 = PI + PI / 3
+_W3 = _wedge(_A3)
+_IN, _SURF, _OUT = ("inside", 1), ("inside", 2), ("inside", 3)
+_YES, _NO = ("inside", 1), ("inside", 0)
+WEDGE_THETA_KATS = [
+    (_W1, "Inside", (4.589551141500899, 2.649778586424563, 4.573258549707597), None, _SURF),
+    (_W1, "Contains", (-4.589551141500899, -2.649778586424563, 4.573258549707597), None, _YES),
+    (_W1, "Inside", (-4.589551141500899, -2.649778586424563, 4.573258549707597), None, _IN),
+    (_W1, "Inside", (0.0, 0.0, 0.0), None, _SURF),
+] + [(_W2, "Contains", p, None, e) for p, e in [
+    ((1.0, -2 * _KT, 0.0), _NO), ((1.0, 0.0, 0.0), _YES), ((1.0, -_KT, 0.0), _NO), ((-1.0, -1.0, 0.0), _NO),
+    (_cs(PI / 3), _YES), (_cs(PI / 3 + _KA), _NO), (_cs(PI / 6), _YES), (_cs(PI / 2), _NO), ((-1.0, 0.0, 0.0), _NO)]
+] + [(_W2, "Inside", p, None, e) for p, e in [
+    ((1.0, 0.0, 0.0), _SURF), ((1.0, -0.5 * _KT, 0.0), _SURF), ((1.0, 0.5 * _KT, 0.0), _SURF), (_cs(PI / 3), _SURF),
+    (_cs(PI / 3 + _KA), _OUT), (_cs(PI / 6), _IN), (_cs(PI / 2), _OUT)]
+] + [(_W3, "Contains", p, None, e) for p, e in [
+    ((1.0, -2 * _KT, 0.0), _NO), ((1.0, 0.0, 0.0), _YES), ((1.0, -_KT, 0.0), _NO), (_cs(_A3 / 3), _YES),
+    (_cs(_A3 + _KA), _NO), (_cs(_A3 / 6), _YES), (_cs(_A3 + 0.1), _NO)]
+] + [(_W3, "Inside", p, None, e) for p, e in [
+    ((1.0, 0.0, 0.0), _SURF), ((1.0, -0.5 * _KT, 0.0), _SURF), ((1.0, 0.5 * _KT, 0.0), _SURF), (_cs(_A3), _SURF),
+    (_cs(_A3 + _KA), _OUT), (_cs(_A3 / 6), _IN), (_cs(_A3 + 0.1), _OUT)]
+]
+_R3 = math.sqrt(3.0)
+_TC1, _TC2, _TC3 = _theta(0.0, PI / 3), _theta(2 * PI / 3, PI / 3), _theta(0.0, 2 * PI / 3)
+_TC4, _TC5, _TC6 = _theta(PI / 3, 2 * PI / 3), _theta(0.0, 0.5 * PI), _theta(0.5 * PI, 0.5 * PI)
+_PX, _PY, _PZ = (30.0, 0.0, 0.0), (0.0, 30.0, 0.0), (0.0, 0.0, 30.0)
+
+
+def _neg(p):
+    return tuple(-c for c in p)
+
+
+WEDGE_THETA_KATS += [(_TC1, "Inside", p, None, e) for p, e in [
+    (VZ, _IN), (VMZ, _OUT), (VX, _OUT), (VMX, _OUT), (VY, _OUT), (VMY, _OUT), ((1.0, 0.0, 1.0), _IN),
+    ((0.0, 1.0, 1.0), _IN), ((0.0, _R3, 1.0), _SURF), ((-_R3, 0.0, 1.0), _SURF), ((-_R3, 0.0, -1.0), _OUT)]
+] + [(_TC1, "Contains", p, None, e) for p, e in [
+    (VX, _NO), (VMX, _NO), (VY, _NO), (VMY, _NO), ((1.0, 0.0, 1.0), _YES), ((-1.0, 0.0, 1.0), _YES),
+    ((0.0, 2.0, 1.0), _NO), ((0.0, -1.0, 1.0), _YES)]
+] + [(_TC2, "Inside", p, None, e) for p, e in [
+    (VZ, _OUT), (VMZ, _IN), (VX, _OUT), (VMX, _OUT), (VY, _OUT), (VMY, _OUT), ((1.0, 0.0, -1.0), _IN),
+    ((0.0, 1.0, -1.0), _IN), ((0.0, _R3, -1.0), _SURF), ((-_R3, 0.0, -1.0), _SURF), ((-_R3, 0.0, 1.0), _OUT)]
+] + [(_TC3, "Inside", p, None, e) for p, e in [(VZ, _IN), (VX, _IN), (VY, _IN), (VMZ, _OUT)]
+] + [(_TC4, "Inside", p, None, e) for p, e in [(VZ, _OUT), (VX, _IN), (VY, _IN), (VMZ, _IN)]
+] + [(_TC5, "Inside", p, None, e) for p, e in [
+    (VZ, _IN), (VMZ, _OUT), (VX, _SURF), (VMX, _SURF), (VY, _SURF), (VMY, _SURF), (_PZ, _IN), (_neg(_PZ), _OUT),
+    (_PX, _SURF), (_neg(_PX), _SURF), (_PY, _SURF), (_neg(_PY), _SURF)]
+] + [(_TC6, "Inside", p, None, e) for p, e in [
+    (VZ, _OUT), (VMZ, _IN), (VX, _SURF), (VMX, _SURF), (VY, _SURF), (VMY, _SURF), (_PZ, _OUT), (_neg(_PZ), _IN),
+    (_PX, _SURF), (_neg(_PX), _SURF), (_PY, _SURF), (_neg(_PY), _SURF)]
+]

@@ -11,7 +11,8 @@ CASES = ([("box", c) for c in kats.BOX_KATS] + [("tube", c) for c in kats.TUBE_K
          + [("trapezoid", c) for c in kats.TRAP_KATS] + [("polycone", c) for c in kats.POLYCONE_KATS]
          + [("polyhedron", c) for c in kats.POLYHEDRON_KATS]
          + [("tessellated", c) for c in kats.TESS_KATS] + [("orb", c) for c in kats.ORB_KATS]
-         + [("sphere", c) for c in kats.SPHERE_KATS] + [("torus", c) for c in kats.TORUS_KATS])
+         + [("sphere", c) for c in kats.SPHERE_KATS] + [("torus", c) for c in kats.TORUS_KATS]
+         + [("sphere", c) for c in kats.WEDGE_THETA_KATS])
 
 
 def _run(geo, kind, params, method, p, d):

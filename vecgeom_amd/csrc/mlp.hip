@@ -183,17 +183,23 @@ __global__ __launch_bounds__(256) void mlp_loss_kernel(const __bf16 *H4, const f
   if ((threadIdx.x & 63u) == 0 && loss) atomicAdd(loss, l);
 }
 
-// Backward through one SiLU layer for rows [64 blockIdx.x, +64), one thread
-// per unit: g = dH[i][r] (or dy[i] w5[r] for the last hidden layer),
-// dZ = g SiLU'(Z); dZ is stored in BF16 for the products and summed into the
-// bias gradient.  The last hidden layer also accumulates dw5 and db5.
+// Backward through one SiLU layer for rows [kBwdRows blockIdx.x, +kBwdRows),
+// one thread per unit: g = dH[i][r] (or dy[i] w5[r] for the last hidden
+// layer), dZ = g SiLU'(Z); dZ is stored in BF16 for the products and summed
+// into the bias gradient.  The last hidden layer also accumulates dw5 and
+// db5.  Eight rows per workgroup: a batch of 8192 rows makes 1024 workgroups,
+// enough waves to keep the loads of every compute unit in flight (64 rows per
+// workgroup left one wave per compute unit walking its rows one load at a
+// time).
+constexpr int kBwdRows = 8;
 __global__ __launch_bounds__(128) void mlp_bwd_act_kernel(const float *dH, const float *dy, const float *w5,
                                                           const __bf16 *H4, const float *Z, __bf16 *dZ, float *db,
                                                           float *dw5, float *db5, int B)
 {
   const int r = int(threadIdx.x);
-  const int i0 = int(blockIdx.x) * 64, i1 = min(B, i0 + 64);
+  const int i0 = int(blockIdx.x) * kBwdRows, i1 = min(B, i0 + kBwdRows);
   float sb = 0.0f, sw = 0.0f, sb5 = 0.0f;
+#pragma unroll
   for (int i = i0; i < i1; ++i) {
     const size_t idx = size_t(i) * kH + r;
     const float g = dH ? dH[idx] : dy[i] * w5[r];
@@ -413,7 +419,7 @@ hipError_t forward_backward(hipStream_t st, Work &w, const float *params, const 
   hipLaunchKernelGGL(mlp_loss_kernel, dim3(blocks_for(size_t(B), 256)), dim3(256), 0, st, w.H[3],
                      params + kIW5, params + kIB5, target, B, over_weight, w.y, w.dy, w.loss);
   if ((e = hipMemsetAsync(w.grad, 0, size_t(kMlpInternal) * sizeof(float), st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + 63) / 64)), dim3(kH), 0, st, nullptr, w.dy,
+  hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + kBwdRows - 1) / kBwdRows)), dim3(kH), 0, st, nullptr, w.dy,
                      params + kIW5, w.H[3], w.Z[3], w.dZ[1], w.grad + kIB[3], w.grad + kIW5, w.grad + kIB5, B);
   const int kchunk = std::min(B, 512);
   for (int l = 3; l >= 0; --l) {
@@ -426,7 +432,7 @@ hipError_t forward_backward(hipStream_t st, Work &w, const float *params, const 
     if (l == 0) break;
     // dH = dZ W: A(i, k = unit) = dZ[i][k]; B(k = unit, n) = W[k][n]
     if ((e = gemm(st, dz, kH, 1, w.pb + kIW[l], kH, 1, w.dH, kH, B, kH, kH, kH, false)) != hipSuccess) return e;
-    hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + 63) / 64)), dim3(kH), 0, st, w.dH, nullptr,
+    hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + kBwdRows - 1) / kBwdRows)), dim3(kH), 0, st, w.dH, nullptr,
                        nullptr, nullptr, w.Z[l - 1], w.dZ[(l - 1) & 1], w.grad + kIB[l - 1], nullptr, nullptr, B);
   }
   return hipGetLastError();
