@@ -121,6 +121,13 @@ SHAPES.update({
                                         ("polycone", (0.0, 2 * PI, (-10.0, 0.0, 15.0), (0.0, 2.0, 0.0),
                                                       (12.0, 18.0, 9.0))),
                                         None, transform12((-10.0, 4.0, 0.0), angle=0.0))),
+    # TestBooleans.cpp's two holes (tubes of radius 2.25, 5 mm apart along x
+    # and y), shifted so that the left one sits at the origin: the union's
+    # SafetyToOut evaluates the left operand at the untransformed point
+    # (BooleanUnionImplementation.h:214), which holds for an unplaced left one
+    "bool_two_holes": ("boolean", ("union", ("tube", (0.0, 2.25, 10.0, 0.0, 2 * PI)),
+                                   ("tube", (0.0, 2.25, 10.0, 0.0, 2 * PI)), None,
+                                   transform12((5.0, 5.0, 0.0), angle=0.0))),
 })
 
 SHAPES.update({
