@@ -101,9 +101,13 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(const float *u0, cons
 }
 
 // C[M][N] (row stride ldc) = sum_k A(m,k) B(k,n), A(m,k) = A[m sam + k sak],
-// B(k,n) = B[k sbk + n sbn].  M, N multiples of 32, K and kchunk of 16.  Wave
-// w of workgroup b computes tile 4b + w; blockIdx.y is the K chunk.  With
-// `atomic` the tile is added to C (split K), otherwise stored.
+// B(k,n) = B[k sbk + n sbn], one of each pair of strides 1.  M, N multiples
+// of 64, K and kchunk of 32.  Workgroup b computes the 64 x 64 tile b of C
+// over the K chunk blockIdx.y: per 32-deep step the A and B tiles are read
+// from global memory along their contiguous dimension (16-byte loads) into
+// LDS, k-contiguous, and each of the four waves takes its 32 x 32 quarter in
+// two 32x32x16 BF16 MFMAs from LDS.  With `atomic` the tile is added to C
+// (split K), otherwise stored.
 struct GemmArgs {
   const __bf16 *A, *B;
   float *C;
@@ -112,37 +116,54 @@ struct GemmArgs {
   int atomic;
 };
 
+constexpr int kGemmTile = 64, kGemmK = 32, kGemmRow = kGemmK + 8;  // LDS rows padded by 16 bytes
+
 __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g)
 {
-  const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
-  const int tiles_n = g.N / 32;
-  const int tile = int(blockIdx.x) * 4 + wave;
-  if (tile >= (g.M / 32) * tiles_n) return;
-  const int m0 = (tile / tiles_n) * 32, n0 = (tile % tiles_n) * 32;
+  __shared__ __attribute__((aligned(16))) __bf16 As[kGemmTile][kGemmRow];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[kGemmTile][kGemmRow];
+  const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+  const int tiles_n = g.N / kGemmTile;
+  const int m0 = (int(blockIdx.x) / tiles_n) * kGemmTile, n0 = (int(blockIdx.x) % tiles_n) * kGemmTile;
   const int kb = int(blockIdx.y) * g.kchunk, ke = min(g.K, kb + g.kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // the wave's quarter of the tile
   const int r = lane & 31, h = lane >> 5;
   mlp_f32x16 acc = {};
-  for (int k0 = kb; k0 < ke; k0 += 16) {
-    mlp_bf16x8 a, b;
-    const __bf16 *pa = g.A + size_t(m0 + r) * size_t(g.sam) + size_t(k0 + 8 * h) * size_t(g.sak);
-    if (g.sak == 1) {
-      a = *reinterpret_cast<const mlp_bf16x8 *>(pa);
-    } else {
+  for (int k0 = kb; k0 < ke; k0 += kGemmK) {
+    if (g.sak == 1) {  // rows of A along k: thread t reads row t / 4, k (t % 4) * 8
+      const int m = tid >> 2, kc = (tid & 3) * 8;
+      *reinterpret_cast<mlp_bf16x8 *>(&As[m][kc]) =
+          *reinterpret_cast<const mlp_bf16x8 *>(g.A + size_t(m0 + m) * size_t(g.sam) + size_t(k0 + kc));
+    } else {  // A contiguous along m: thread t reads k = t / 8, m (t % 8) * 8
+      const int k = tid >> 3, mc = (tid & 7) * 8;
+      const mlp_bf16x8 v =
+          *reinterpret_cast<const mlp_bf16x8 *>(g.A + size_t(k0 + k) * size_t(g.sak) + size_t(m0 + mc));
 #pragma unroll
-      for (int j = 0; j < 8; ++j) a[j] = pa[size_t(j) * size_t(g.sak)];
+      for (int j = 0; j < 8; ++j) As[mc + j][k] = v[j];
     }
-    const __bf16 *pb = g.B + size_t(k0 + 8 * h) * size_t(g.sbk) + size_t(n0 + r) * size_t(g.sbn);
-    if (g.sbk == 1) {
-      b = *reinterpret_cast<const mlp_bf16x8 *>(pb);
-    } else {
+    if (g.sbk == 1) {  // columns of B along k
+      const int n = tid >> 2, kc = (tid & 3) * 8;
+      *reinterpret_cast<mlp_bf16x8 *>(&Bs[n][kc]) =
+          *reinterpret_cast<const mlp_bf16x8 *>(g.B + size_t(n0 + n) * size_t(g.sbn) + size_t(k0 + kc));
+    } else {  // B contiguous along n
+      const int k = tid >> 3, nc = (tid & 7) * 8;
+      const mlp_bf16x8 v =
+          *reinterpret_cast<const mlp_bf16x8 *>(g.B + size_t(k0 + k) * size_t(g.sbk) + size_t(n0 + nc));
 #pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = pb[size_t(j) * size_t(g.sbk)];
+      for (int j = 0; j < 8; ++j) Bs[nc + j][k] = v[j];
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kGemmK; kk += 16) {
+      const mlp_bf16x8 a = *reinterpret_cast<const mlp_bf16x8 *>(&As[wm + r][kk + 8 * h]);
+      const mlp_bf16x8 b = *reinterpret_cast<const mlp_bf16x8 *>(&Bs[wn + r][kk + 8 * h]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
   }
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
-    float *c = g.C + size_t(m0 + 8 * (v / 4) + 4 * h + (v % 4)) * size_t(g.ldc) + size_t(n0 + r);
+    float *c = g.C + size_t(m0 + wm + 8 * (v / 4) + 4 * h + (v % 4)) * size_t(g.ldc) + size_t(n0 + wn + r);
     if (g.atomic)
       atomicAdd(c, acc[v]);
     else
@@ -183,24 +204,26 @@ __global__ __launch_bounds__(256) void mlp_loss_kernel(const __bf16 *H4, const f
   if ((threadIdx.x & 63u) == 0 && loss) atomicAdd(loss, l);
 }
 
-// Backward through one SiLU layer for rows [kBwdRows blockIdx.x, +kBwdRows),
-// one thread per unit: g = dH[i][r] (or dy[i] w5[r] for the last hidden
-// layer), dZ = g SiLU'(Z); dZ is stored in BF16 for the products and summed
-// into the bias gradient.  The last hidden layer also accumulates dw5 and
-// db5.  Eight rows per workgroup: a batch of 8192 rows makes 1024 workgroups,
-// enough waves to keep the loads of every compute unit in flight (64 rows per
-// workgroup left one wave per compute unit walking its rows one load at a
-// time).
-constexpr int kBwdRows = 8;
-__global__ __launch_bounds__(128) void mlp_bwd_act_kernel(const float *dH, const float *dy, const float *w5,
-                                                          const __bf16 *H4, const float *Z, __bf16 *dZ, float *db,
-                                                          float *dw5, float *db5, int B)
+// Backward through one SiLU layer: g = dH[i][r] (or dy[i] w5[r] for the
+// last hidden layer), dZ = g SiLU'(Z), stored in BF16 for the products.  A
+// workgroup takes kBwdRows rows, two halves of 128 threads (one per unit)
+// over half of them each; the per-unit sums of dZ (the bias gradient) and,
+// for the last hidden layer, of dy H4 and dy (dw5, db5) are combined in LDS
+// and written as the workgroup's partial, which mlp_bwd_reduce_kernel adds
+// up: atomics of every workgroup on the same 128 addresses serialise.
+constexpr int kBwdRows = 32;
+__global__ __launch_bounds__(256) void mlp_bwd_act_kernel(const float *dH, const float *dy, const float *w5,
+                                                          const __bf16 *H4, const float *Z, __bf16 *dZ,
+                                                          float *part, int B)
 {
-  const int r = int(threadIdx.x);
-  const int i0 = int(blockIdx.x) * kBwdRows, i1 = min(B, i0 + kBwdRows);
+  __shared__ float red[3][kH];
+  const int r = int(threadIdx.x) & (kH - 1), half = int(threadIdx.x) >> 7;
+  const int i0 = int(blockIdx.x) * kBwdRows + half * (kBwdRows / 2);
   float sb = 0.0f, sw = 0.0f, sb5 = 0.0f;
 #pragma unroll
-  for (int i = i0; i < i1; ++i) {
+  for (int j = 0; j < kBwdRows / 2; ++j) {
+    const int i = i0 + j;
+    if (i >= B) break;
     const size_t idx = size_t(i) * kH + r;
     const float g = dH ? dH[idx] : dy[i] * w5[r];
     const float z = Z[idx];
@@ -213,10 +236,57 @@ __global__ __launch_bounds__(128) void mlp_bwd_act_kernel(const float *dH, const
       sb5 += dy[i];
     }
   }
-  atomicAdd(db + r, sb);
-  if (!dH) {
-    atomicAdd(dw5 + r, sw);
-    if (r == 0) atomicAdd(db5, sb5);
+  if (half) {
+    red[0][r] = sb;
+    red[1][r] = sw;
+    red[2][r] = sb5;
+  }
+  __syncthreads();
+  if (!half) {
+    float *pb = part + size_t(blockIdx.x) * (2 * kH + 1);
+    pb[r] = sb + red[0][r];
+    if (!dH) {
+      pb[kH + r] = sw + red[1][r];
+      if (r == 0) pb[2 * kH] = sb5 + red[2][0];
+    }
+  }
+}
+
+// The bias gradient (and for the last hidden layer dw5, db5) as the sum of
+// the nblk workgroup partials of mlp_bwd_act_kernel: eight groups of 128
+// threads (one per unit) each add every eighth partial, with independent
+// loads in flight, and the groups are combined in LDS.
+constexpr int kBwdReduceGroups = 8;
+__global__ __launch_bounds__(kBwdReduceGroups * kH) void mlp_bwd_reduce_kernel(const float *part, int nblk,
+                                                                               float *db, float *dw5, float *db5)
+{
+  __shared__ float red[kBwdReduceGroups][3][kH];
+  const int r = int(threadIdx.x) & (kH - 1), grp = int(threadIdx.x) / kH;
+  float sb = 0.0f, sw = 0.0f, sb5 = 0.0f;
+#pragma unroll 4
+  for (int b = grp; b < nblk; b += kBwdReduceGroups) {
+    const float *pb = part + size_t(b) * (2 * kH + 1);
+    sb += pb[r];
+    if (dw5) {
+      sw += pb[kH + r];
+      sb5 += pb[2 * kH];
+    }
+  }
+  red[grp][0][r] = sb;
+  red[grp][1][r] = sw;
+  red[grp][2][r] = sb5;
+  __syncthreads();
+  if (grp == 0) {
+    for (int k = 1; k < kBwdReduceGroups; ++k) {
+      sb += red[k][0][r];
+      sw += red[k][1][r];
+      sb5 += red[k][2][r];
+    }
+    db[r] = sb;
+    if (dw5) {
+      dw5[r] = sw;
+      if (r == 0) *db5 = sb5;
+    }
   }
 }
 
@@ -352,9 +422,9 @@ hipError_t gemm(hipStream_t st, const __bf16 *A, int sam, int sak, const __bf16 
                 int ldc, int M, int N, int K, int kchunk, bool atomic)
 {
   GemmArgs g{A, B, C, M, N, K, kchunk, sam, sak, sbk, sbn, ldc, atomic ? 1 : 0};
-  const int tiles = (M / 32) * (N / 32);
-  hipLaunchKernelGGL(mlp_gemm_kernel, dim3(unsigned((tiles + 3) / 4), unsigned((K + kchunk - 1) / kchunk)),
-                     dim3(256), 0, st, g);
+  const int tiles = (M / kGemmTile) * (N / kGemmTile);
+  hipLaunchKernelGGL(mlp_gemm_kernel, dim3(unsigned(tiles), unsigned((K + kchunk - 1) / kchunk)), dim3(256), 0, st,
+                     g);
   return hipGetLastError();
 }
 
@@ -364,6 +434,7 @@ struct Work {
   __bf16 *F = nullptr, *H[4] = {}, *dZ[2] = {}, *pb = nullptr;
   float *Z[4] = {}, *dH = nullptr, *y = nullptr, *dy = nullptr, *loss = nullptr;
   float *grad = nullptr, *m = nullptr, *v = nullptr;
+  float *part = nullptr;  // mlp_bwd_act_kernel partials: per workgroup 2 kH + 1
   ~Work()
   {
     if (buf) (void)hipFree(buf);
@@ -373,7 +444,8 @@ struct Work {
     const size_t bh = align256(size_t(B) * kH * 2), bz = align256(size_t(B) * kH * 4);
     const size_t bf = align256(size_t(B) * kMlpInPad * 2), bb = align256(size_t(B) * 4);
     const size_t bp = align256(size_t(kMlpInternal) * 4), bpb = align256(size_t(kMlpInternal) * 2);
-    const size_t total = bf + 4 * bh + 2 * bh + 4 * bz + bz + 2 * bb + 256 + 3 * bp + bpb;
+    const size_t bpart = align256(size_t((B + kBwdRows - 1) / kBwdRows) * (2 * kH + 1) * 4);
+    const size_t total = bf + 4 * bh + 2 * bh + 4 * bz + bz + 2 * bb + 256 + 3 * bp + bpb + bpart;
     hipError_t e = hipMalloc(&buf, total);
     if (e != hipSuccess) return e;
     char *q = buf;
@@ -390,6 +462,7 @@ struct Work {
     y = reinterpret_cast<float *>(take(bb));
     dy = reinterpret_cast<float *>(take(bb));
     loss = reinterpret_cast<float *>(take(256));
+    part = reinterpret_cast<float *>(take(bpart));
     grad = reinterpret_cast<float *>(take(bp));
     m = reinterpret_cast<float *>(take(bp));
     v = reinterpret_cast<float *>(take(bp));
@@ -419,8 +492,12 @@ hipError_t forward_backward(hipStream_t st, Work &w, const float *params, const 
   hipLaunchKernelGGL(mlp_loss_kernel, dim3(blocks_for(size_t(B), 256)), dim3(256), 0, st, w.H[3],
                      params + kIW5, params + kIB5, target, B, over_weight, w.y, w.dy, w.loss);
   if ((e = hipMemsetAsync(w.grad, 0, size_t(kMlpInternal) * sizeof(float), st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + kBwdRows - 1) / kBwdRows)), dim3(kH), 0, st, nullptr, w.dy,
-                     params + kIW5, w.H[3], w.Z[3], w.dZ[1], w.grad + kIB[3], w.grad + kIW5, w.grad + kIB5, B);
+  const int nblk = (B + kBwdRows - 1) / kBwdRows;
+  hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned(nblk)), dim3(2 * kH), 0, st, nullptr, w.dy, params + kIW5,
+                     w.H[3], w.Z[3], w.dZ[1], w.part, B);
+  hipLaunchKernelGGL(mlp_bwd_reduce_kernel, dim3(1), dim3(kBwdReduceGroups * kH), 0, st, w.part, nblk, w.grad + kIB[3],
+                     w.grad + kIW5,
+                     w.grad + kIB5);
   const int kchunk = std::min(B, 512);
   for (int l = 3; l >= 0; --l) {
     const __bf16 *dz = w.dZ[l & 1];
@@ -432,8 +509,10 @@ hipError_t forward_backward(hipStream_t st, Work &w, const float *params, const 
     if (l == 0) break;
     // dH = dZ W: A(i, k = unit) = dZ[i][k]; B(k = unit, n) = W[k][n]
     if ((e = gemm(st, dz, kH, 1, w.pb + kIW[l], kH, 1, w.dH, kH, B, kH, kH, kH, false)) != hipSuccess) return e;
-    hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned((B + kBwdRows - 1) / kBwdRows)), dim3(kH), 0, st, w.dH, nullptr,
-                       nullptr, nullptr, w.Z[l - 1], w.dZ[(l - 1) & 1], w.grad + kIB[l - 1], nullptr, nullptr, B);
+    hipLaunchKernelGGL(mlp_bwd_act_kernel, dim3(unsigned(nblk)), dim3(2 * kH), 0, st, w.dH, nullptr, nullptr, nullptr,
+                       w.Z[l - 1], w.dZ[(l - 1) & 1], w.part, B);
+    hipLaunchKernelGGL(mlp_bwd_reduce_kernel, dim3(1), dim3(kBwdReduceGroups * kH), 0, st, w.part, nblk,
+                       w.grad + kIB[l - 1], nullptr, nullptr);
   }
   return hipGetLastError();
 }
