@@ -124,15 +124,22 @@ def test_transport_mixed_equals_fp64_long_runs(ctx):
     b.close()
 
 
-@pytest.mark.parametrize("mode", ["distance_first", "cull"])
-def test_transport_distance_first_bit_exact(ctx, mode):
+@pytest.mark.parametrize("mode,launch", [("distance_first", None), ("cull", None),
+                                         ("distance_first", (64, 0, 2048)), ("cull", (128, 64, 512)),
+                                         ("cull", (256, 1024, 8192))],
+                         ids=["distance-first", "cull", "distance-first-per-tile", "cull-small", "cull-large"])
+def test_transport_distance_first_bit_exact(ctx, mode, launch):
     """Boundary distance before the safety: the same decisions as the FP64
     step (SafetyToIn/Out never exceed DistanceToIn/Out beyond the window), so
     trajectories match the oracle bit for bit for every family.  The culled
     mode also moves outside tracks whose step stays clear of the bounding box
-    without any query: still bit for bit."""
-    print(f"[transport] {mode}", flush=True)
+    without any query: still bit for bit.  The launch shapes give a thread
+    one track of a tile or many (the tessellated kernel's lanes run on from
+    one track to their next without waiting for the wave)."""
+    print(f"[transport] {mode} launch={launch}", flush=True)
     b, dev, side, cr, ref, side_o, cr_o, _ = _setup(ctx, 120_000, 30)
+    if launch:
+        b.set_launch(*launch)
     b.transport(*dev, side, cr, 30, step0=0, seed=5, **{mode: True})
     for k in range(6):
         got = dev[k].cpu().numpy()
