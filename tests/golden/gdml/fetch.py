@@ -10,7 +10,9 @@ FILES = ["aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "one
          "onePolyhedron", "oneTesselatedSolid", "oneTrapezoid", "oneTube", "twoBoxes",
          # refused: a solid outside the implemented families, a scaled physvol
          "oneOrb", "scaledPhysvol", "aConstant", "oneCutTube", "solidSubtraction",
-         "solidIntersection", "solidUnion", "oneSphere", "oneTorus"]
+         "solidIntersection", "solidUnion", "oneSphere", "oneTorus",
+         # materials (read and ignored), a rotated daughter in a boolean world
+         "compositeMaterial", "mixtureElement", "mixtureMaterial", "simpleMaterial", "unionWIthDetector"]
 
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))

@@ -164,6 +164,32 @@ def test_orb_radius_and_constant():
         assert r.of("orb") == [("orb", 100.0)]
 
 
+@pytest.mark.parametrize("name", ["compositeMaterial", "mixtureElement", "mixtureMaterial", "simpleMaterial"])
+def test_materials_are_read_and_ignored(name):
+    """Elements, isotopes, composites and mixtures carry no geometry: each
+    file is one orb of radius 100 cm, as oneOrb.gdml."""
+    r, world, solids, volumes = read(name)
+    assert r.of("orb") == read("oneOrb")[0].of("orb") == [("orb", 100.0)]
+    assert list(volumes) == ["World"] and not r.of("add_daughter")
+
+
+def test_union_world_with_a_rotated_detector():
+    """unionWIthDetector.gdml: a union of two boxes as the world's solid,
+    holding the second box placed with the union's own position and
+    rotation."""
+    r, world, solids, volumes = read("unionWIthDetector")
+    assert set(solids) == {"firstHalf", "secondHalf", "world"}
+    assert [c[1:] for c in r.of("box")] == [(50.0, 50.0, 50.0), (35.0, 35.0, 35.0)]
+    (_, mother, pv), = r.of("add_daughter")
+    assert mother == volumes["World"]
+    (_, op, left, right, ltr, rtr), = r.of("boolean")
+    assert (op, left, right, ltr) == ("union", solids["firstHalf"], solids["secondHalf"], None)
+    # the detector sits exactly where the union puts its second box
+    placed = [c for c in r.of("place") if c[1] == volumes["Detector"]]
+    assert len(placed) == 1 and placed[0][2] == rtr
+    assert not np.allclose(np.asarray(rtr)[3:12], np.eye(3).ravel())  # a real rotation
+
+
 def test_two_boxes_structure():
     r, world, solids, volumes = read("twoBoxes")
     assert set(solids) == {"detector", "world"} and set(volumes) == {"Detector", "World"}

@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gdml")
 FILES = ["aConstant", "aPosition", "aRotation", "auxiliary", "oneBox", "oneConeSegment", "oneCutTube", "oneOrb", "oneGeneralTrapezoid", "onePolycone",
          "onePolyhedron", "oneTesselatedSolid", "oneTrapezoid", "oneTube", "solidIntersection", "solidSubtraction", "solidUnion", "twoBoxes",
-         "oneSphere", "oneTorus"]
+         "oneSphere", "oneTorus", "compositeMaterial", "mixtureElement", "mixtureMaterial", "simpleMaterial",
+         "unionWIthDetector"]
 QUERIES = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 
 
