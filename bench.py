@@ -596,6 +596,21 @@ def main():
             torch.cuda.synchronize()
             batch.transport(*st2, s2, c2, S, step0=0, seed=7, audit=True, **kw)
             extra["audit"] = batch.learned_stats()
+        if tuned and batch.streams() > 1:
+            # each family kernel's own duration: one more launch of the same
+            # modes and launch shape, the families back to back on one stream
+            # (run concurrently, a kernel's interval also holds the time it
+            # shares the machine, and that share changes from run to run)
+            st2 = [x.clone() for x in state]
+            s2, c2 = side.clone(), cr.clone()
+            nstreams = batch.streams()
+            batch.set_streams(1)
+            torch.cuda.synchronize()
+            batch.transport(*st2, s2, c2, S, step0=step0, seed=7, timing=True, **kw)
+            extra["family_ms_one_stream"] = batch.family_ms()
+            batch.set_streams(nstreams)
+            log(f"[bench] {tag} family kernel ms, one stream: " + ", ".join(
+                f"{k} {v:.1f}" for k, v in extra["family_ms_one_stream"].items()))
         if tuned or not args.launch:
             extra["modes"] = batch.modes()
         if not args.launch:  # the autotuner's per-family times (without it the weights are track counts)
@@ -685,6 +700,13 @@ def main():
                                res[head][2].get("weights") if not args.launch else None, kern_v * 1e3)
     if roofline:
         roofline["run"] = head
+        alone = res[head][2].get("family_ms_one_stream")
+        if alone:
+            fam = roofline["kernel"].split("<")[1].split(",")[0]
+            a_ach = roofline["achieved"] * roofline["kernel_ms"] / alone[fam]
+            roofline["alone"] = {"kernel_ms": alone[fam], "achieved": a_ach, "frac": a_ach / roofline["peak"],
+                                 "note": "the same kernel's own duration: one more launch of the same modes, the "
+                                         "families back to back on one stream, after the timed launches"}
     rate64, ratemx = rate("fp64"), rate("mixed")
     used = set(res[head][2].get("modes", {}).values()) if head == "tuned" else {head}
     dtype = "f64" + ("+f32" if used & {"mixed"} else "") + ("+bf16" if used & {"learned"} else "")
