@@ -218,42 +218,80 @@ struct Builder {
     return id;
   }
 
-  // The device layout: one record per internal node, preorder, holding the
-  // boxes of both children; leaves are folded into their parents.  A tree
-  // that is a single leaf gets a root whose second child is empty.
-  std::vector<TessNode> wide() const
+  // Children of the node record of binary node o: each child of o, or, for
+  // an internal child, its two children (grandchildren pulled up).
+  void record_slots(int o, int *slot, int &ns) const
+  {
+    ns = 0;
+    const Node &nd = nodes[size_t(o)];
+    for (const int c : {nd.left, nd.right}) {
+      const Node &cn = nodes[size_t(c)];
+      if (cn.left >= 0) {
+        slot[ns++] = cn.left;
+        slot[ns++] = cn.right;
+      } else {
+        slot[ns++] = c;
+      }
+    }
+  }
+
+  // The device layout: one record per node of the four-wide tree, preorder,
+  // holding the boxes of its children; leaves are folded into their parents.
+  // A tree that is a single leaf gets a root with one leaf child.  `depth`
+  // is the number of records on the longest root-to-leaf path.
+  std::vector<TessNode> wide(int &depth) const
   {
     std::vector<TessNode> out;
-    std::vector<int> map(nodes.size(), -1);
-    std::vector<int> todo{0};
-    while (!todo.empty()) {  // preorder numbering of the internal nodes
+    depth = 1;
+    if (nodes[0].left < 0) {
+      TessNode w{};
+      for (int k = 0; k < 3; ++k) {
+        w.lo[0][k] = nodes[0].lo[k];
+        w.hi[0][k] = nodes[0].hi[k];
+      }
+      for (int i = 0; i < kTessWidth; ++i) w.child[i] = -1;
+      w.first[0] = nodes[0].first;
+      w.count[0] = nodes[0].count;
+      out.push_back(w);
+      return out;
+    }
+    std::vector<int> map(nodes.size(), -1), level(nodes.size(), 0);
+    std::vector<int> todo{0}, order;
+    level[0] = 1;
+    while (!todo.empty()) {  // preorder numbering of the records
       const int o = todo.back();
       todo.pop_back();
-      map[size_t(o)] = int(out.size());
-      out.push_back(TessNode{});
-      const Node &nd = nodes[size_t(o)];
-      if (nd.left < 0) break;  // single-leaf tree
-      if (nodes[size_t(nd.right)].left >= 0) todo.push_back(nd.right);
-      if (nodes[size_t(nd.left)].left >= 0) todo.push_back(nd.left);
+      map[size_t(o)] = int(order.size());
+      order.push_back(o);
+      depth = std::max(depth, level[size_t(o)]);
+      int slot[kTessWidth], ns;
+      record_slots(o, slot, ns);
+      for (int i = ns - 1; i >= 0; --i)
+        if (nodes[size_t(slot[i])].left >= 0) {
+          level[size_t(slot[i])] = level[size_t(o)] + 1;
+          todo.push_back(slot[i]);
+        }
     }
-    for (size_t o = 0; o < nodes.size(); ++o) {
-      if (map[o] < 0) continue;
-      TessNode &w = out[size_t(map[o])];
-      const Node &nd = nodes[o];
-      const int kids[2] = {nd.left, nd.right};
-      for (int i = 0; i < 2; ++i) {
-        const Node &c = kids[0] < 0 ? nd : nodes[size_t(kids[i])];
-        const bool empty = kids[0] < 0 && i == 1;
+    for (const int o : order) {
+      TessNode w{};
+      int slot[kTessWidth], ns;
+      record_slots(o, slot, ns);
+      for (int i = 0; i < kTessWidth; ++i) {
+        w.child[i] = -1;
+        if (i >= ns) continue;  // unused: an empty leaf with a point box
+        const Node &c = nodes[size_t(slot[i])];
         for (int k = 0; k < 3; ++k) {
           w.lo[i][k] = c.lo[k];
           w.hi[i][k] = c.hi[k];
         }
-        const bool leaf = kids[0] < 0 || c.left < 0;
-        w.child[i] = leaf ? -1 : map[size_t(kids[i])];
-        w.first[i] = leaf ? c.first : 0;
-        w.count[i] = (leaf && !empty) ? c.count : 0;
+        if (c.left >= 0) {
+          w.child[i] = map[size_t(slot[i])];
+        } else {
+          w.first[i] = c.first;
+          w.count[i] = c.count;
+        }
       }
-      w.pad[0] = w.pad[1] = 0;
+      out.push_back(w);
     }
     return out;
   }
@@ -294,27 +332,30 @@ int build_tessellated(HostGeometry &g, int nfacets, const double *tri, const dou
   // tests); larger solids use the smallest leaf size whose tree keeps node
   // indices within 16 bits.  The input facet order is kept so that every
   // rebuild starts from the same sequence.
+  // The device walks keep 16-bit node indices on a stack of kTessWalkDepth
+  // entries; a nearest-first walk defers at most kTessWidth - 1 siblings per
+  // level, so the four-wide depth D needs (kTessWidth - 1) D + 1 entries.
   const std::vector<TessFacet> input = facets;
   Builder B{facets, {}, 1e-7 + 1e-12 * big};
+  std::vector<TessNode> wide;
+  int wdepth = 0;
+  auto fits = [&]() { return (kTessWidth - 1) * wdepth + 1 <= kTessWalkDepth; };
   for (;;) {
     facets = input;
     B.nodes.clear();
     B.max_depth = 0;
     B.nodes.reserve(2 * size_t(n) / size_t(B.leaf) + 1);
     B.build(0, n);
-    if (B.max_depth > kTessWalkDepth - 2 && B.sah) {
+    wide = B.wide(wdepth);
+    if (!fits() && B.sah) {
       B.sah = false;  // median splits: depth log2(n / leaf)
       continue;
     }
-    // internal nodes: half of the binary tree's nodes
-    if (B.nodes.size() / 2 <= 65535 || B.leaf >= n) break;
+    if (wide.size() <= 65535 || B.leaf >= n) break;
     B.leaf *= 2;
   }
-  const std::vector<TessNode> wide = B.wide();
   const int nn = int(wide.size());
-  // the device walks keep 16-bit node indices, at most one deferred sibling
-  // per level, on a stack of kTessWalkDepth entries
-  if (nn > 65535 || B.max_depth > kTessWalkDepth - 2) {
+  if (nn > 65535 || !fits()) {
     err = "tessellated solid too large for the facet tree walk (" + std::to_string(n) + " facets)";
     return -1;
   }

@@ -5,9 +5,9 @@
 //
 // The reference walks float boxes of facet clusters in distance order and
 // keeps the strict minimum; its results are the minimum over all facets.
-// Here a bounding-volume tree over the facets (boxes enlarged by a margin,
-// so pruning is conservative) is walked with a per-lane stack; equal
-// distances go to the facet defined first.  Facet data stay in global
+// Here a four-wide bounding-volume tree over the facets (boxes enlarged by
+// a margin, so pruning is conservative) is walked with a per-lane stack;
+// equal distances go to the facet defined first.  Facet data stay in global
 // memory: a 20k-facet solid is 3.8 MB, served from L2 / infinity cache.
 #pragma once
 #include "../vgm_math.hpp"
@@ -22,8 +22,9 @@ namespace vgm {
 // device the stack lives in LDS, one column per thread of the workgroup (a
 // private array would be spilled to scratch memory, whose traffic dominated
 // the walks); on the host it is an ordinary local array.  The tree builder
-// keeps node indices below 65536 and the depth below kTessWalkDepth - 2, so a
-// nearest-first walk (one deferred sibling per level) never overflows.
+// keeps node indices below 65536 and (kTessWidth - 1) x depth + 1 within
+// kTessWalkDepth, so a nearest-first walk (at most kTessWidth - 1 deferred
+// siblings per level) never overflows.
 constexpr int kTessWalkDepth = 32;
 constexpr int kTessWalkThreads = 256;  // largest workgroup of any kernel that walks
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -175,6 +176,36 @@ struct TessellatedShape {
     return ex * ex + ey * ey + ez * ez;
   }
 
+  // The keys of a node's children and the children in increasing key order:
+  // a sorting network of compare-swaps on (key, child) pairs, in registers.
+  template <typename R>
+  struct SlotOrder {
+    R k0, k1, k2, k3;
+    int s0, s1, s2, s3;
+    VGM_DEV static void cswap(R &ka, int &sa, R &kb, int &sb)
+    {
+      if (kb < ka) {
+        const R t = ka;
+        ka = kb;
+        kb = t;
+        const int u = sa;
+        sa = sb;
+        sb = u;
+      }
+    }
+    VGM_DEV SlotOrder(R a, R b, R c, R d) : k0(a), k1(b), k2(c), k3(d), s0(0), s1(1), s2(2), s3(3)
+    {
+      cswap(k0, s0, k1, s1);
+      cswap(k2, s2, k3, s3);
+      cswap(k0, s0, k2, s2);
+      cswap(k1, s1, k3, s3);
+      cswap(k1, s1, k2, s2);
+    }
+    VGM_DEV R key(int o) const { return o == 0 ? k0 : (o == 1 ? k1 : (o == 2 ? k2 : k3)); }
+    VGM_DEV int slot(int o) const { return o == 0 ? s0 : (o == 1 ? s1 : (o == 2 ? s2 : s3)); }
+  };
+  static_assert(kTessWidth == 4, "SlotOrder sorts four children");
+
   // Minimum entering and exiting facet hits (DistanceToCluster over all
   // facets).  The walk is bounded by the minimum of the kind selected by
   // to_in; the other kind is only needed below that bound.
@@ -206,12 +237,12 @@ struct TessellatedShape {
       const uint32_t e = stack.get(--sp);
       if (tess_walk_key<R>(e) > bound()) continue;
       const TessNode &nd = nodes[e & 0xffffu];
-      const R t0 = box_entry<R>(nd, 0, p, inv), t1 = box_entry<R>(nd, 1, p, inv);
-      const int near = (t1 < t0) ? 1 : 0;
+      const SlotOrder<R> ord(box_entry<R>(nd, 0, p, inv), box_entry<R>(nd, 1, p, inv), box_entry<R>(nd, 2, p, inv),
+                             box_entry<R>(nd, 3, p, inv));
 #pragma unroll 1
-      for (int o = 0; o < 2; ++o) {
-        const int i = near ^ o;
-        const R te = i ? t1 : t0;
+      for (int o = 0; o < kTessWidth; ++o) {
+        const int i = ord.slot(o);
+        const R te = ord.key(o);
         if (nd.child[i] >= 0 || te == inf_length<R>() || te > bound()) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
@@ -237,9 +268,9 @@ struct TessellatedShape {
         }
       }
 #pragma unroll 1
-      for (int o = 1; o >= 0; --o) {
-        const int i = near ^ o;
-        const R te = i ? t1 : t0;
+      for (int o = kTessWidth - 1; o >= 0; --o) {
+        const int i = ord.slot(o);
+        const R te = ord.key(o);
         if (nd.child[i] < 0 || te == inf_length<R>() || te > bound()) continue;
         stack.set(sp++, tess_walk_entry<R>(nd.child[i], te));
       }
@@ -259,12 +290,12 @@ struct TessellatedShape {
       const uint32_t e = stack.get(--sp);
       if (tess_walk_key<R>(e) > best) continue;
       const TessNode &nd = nodes[e & 0xffffu];
-      const R d0 = box_dist2<R>(nd, 0, p), d1 = box_dist2<R>(nd, 1, p);
-      const int near = (d1 < d0) ? 1 : 0;
+      const SlotOrder<R> ord(box_dist2<R>(nd, 0, p), box_dist2<R>(nd, 1, p), box_dist2<R>(nd, 2, p),
+                             box_dist2<R>(nd, 3, p));
 #pragma unroll 1
-      for (int o = 0; o < 2; ++o) {
-        const int i = near ^ o;
-        if (nd.child[i] >= 0 || (i ? d1 : d0) > best) continue;
+      for (int o = 0; o < kTessWidth; ++o) {
+        const int i = ord.slot(o);
+        if (nd.child[i] >= 0 || ord.key(o) > best) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const R s = facet_safety2<R>(fa[k], p, to_in);
@@ -272,9 +303,9 @@ struct TessellatedShape {
         }
       }
 #pragma unroll 1
-      for (int o = 1; o >= 0; --o) {
-        const int i = near ^ o;
-        const R di = i ? d1 : d0;
+      for (int o = kTessWidth - 1; o >= 0; --o) {
+        const int i = ord.slot(o);
+        const R di = ord.key(o);
         if (nd.child[i] < 0 || di > best) continue;
         stack.set(sp++, tess_walk_entry<R>(nd.child[i], di));
       }
@@ -299,12 +330,12 @@ struct TessellatedShape {
       const uint32_t e = stack.get(--sp);
       if (tess_walk_key<double>(e) > best) continue;
       const TessNode &nd = nodes[e & 0xffffu];
-      const double d0 = box_dist2<double>(nd, 0, p), d1 = box_dist2<double>(nd, 1, p);
-      const int near = (d1 < d0) ? 1 : 0;
+      const SlotOrder<double> ord(box_dist2<double>(nd, 0, p), box_dist2<double>(nd, 1, p),
+                                  box_dist2<double>(nd, 2, p), box_dist2<double>(nd, 3, p));
 #pragma unroll 1
-      for (int o = 0; o < 2; ++o) {
-        const int i = near ^ o;
-        if (nd.child[i] >= 0 || (i ? d1 : d0) > best) continue;
+      for (int o = 0; o < kTessWidth; ++o) {
+        const int i = ord.slot(o);
+        if (nd.child[i] >= 0 || ord.key(o) > best) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const double s = facet_safety2<double>(fa[k], p, false);
@@ -316,9 +347,9 @@ struct TessellatedShape {
         }
       }
 #pragma unroll 1
-      for (int o = 1; o >= 0; --o) {
-        const int i = near ^ o;
-        const double di = i ? d1 : d0;
+      for (int o = kTessWidth - 1; o >= 0; --o) {
+        const int i = ord.slot(o);
+        const double di = ord.key(o);
         if (nd.child[i] < 0 || di > best) continue;
         stack.set(sp++, tess_walk_entry<double>(nd.child[i], di));
       }

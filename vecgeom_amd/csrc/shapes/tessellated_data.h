@@ -18,16 +18,20 @@ struct TessFacet {   // 24 doubles
 };
 static_assert(sizeof(TessFacet) == 24 * sizeof(double), "facet is 24 doubles");
 
-// Internal node of the facet tree, holding the boxes of both children so a
-// walk reads one node per level.  A child is another node (child >= 0) or a
-// leaf: the facets [first, first + count) of the leaf order (child < 0).
-struct TessNode {      // 16 doubles
-  double lo[2][3], hi[2][3];  // child boxes, enlarged by a margin (conservative pruning)
-  int32_t child[2];
-  int32_t first[2], count[2];
-  int32_t pad[2];
+// Node of the facet tree: up to four children, whose boxes it holds so that
+// a walk reads one node per level (a binary tree with each node's
+// grandchildren pulled up: half the levels, so half the dependent loads on a
+// walk's path).  A child is another node (child >= 0) or a leaf: the facets
+// [first, first + count) of the leaf order (child < 0; count 0 for an
+// unused slot).
+constexpr int kTessWidth = 4;
+struct TessNode {      // 32 doubles
+  double lo[kTessWidth][3], hi[kTessWidth][3];  // child boxes, enlarged by a margin (conservative pruning)
+  int32_t child[kTessWidth];
+  int32_t first[kTessWidth], count[kTessWidth];
+  int32_t pad[kTessWidth];
 };
-static_assert(sizeof(TessNode) == 16 * sizeof(double), "node is 16 doubles");
+static_assert(sizeof(TessNode) == 32 * sizeof(double), "node is 32 doubles");
 
 struct TessData {
   int32_t nfacets, nnodes;
