@@ -176,33 +176,54 @@ struct TessellatedShape {
     return ex * ex + ey * ey + ez * ez;
   }
 
-  // The keys of a node's children and the children in increasing key order:
-  // a sorting network of compare-swaps on (key, child) pairs, in registers.
+  // A node's children in increasing key order.  A child's sort entry holds
+  // the upper 16 bits of a float at or below its key, mapped so that the
+  // keys order as unsigned integers (a key below 0, a box holding the point,
+  // as -infinity; a box the ray misses last), and the child's slot in the
+  // low bits: a sorting network of unsigned min/max orders them in four
+  // registers.  The decoded key bounds the exact one from below, so it
+  // prunes conservatively.
   template <typename R>
   struct SlotOrder {
-    R k0, k1, k2, k3;
-    int s0, s1, s2, s3;
-    VGM_DEV static void cswap(R &ka, int &sa, R &kb, int &sb)
+    uint32_t e0, e1, e2, e3;
+    VGM_DEV static uint32_t pack(R x, int slot)
     {
-      if (kb < ka) {
-        const R t = ka;
-        ka = kb;
-        kb = t;
-        const int u = sa;
-        sa = sb;
-        sb = u;
+      uint32_t m;
+      if (!(x < inf_length<R>())) {
+        m = 0xffffu;  // the ray misses the box
+      } else if (x < R(0)) {
+        m = 0x007fu;  // -infinity
+      } else {
+        const float f = float(x);
+        uint32_t bits = __builtin_bit_cast(uint32_t, f);
+        if (R(f) > x) bits -= 1u;  // the next float below
+        m = 0x8000u | (bits >> 16);
       }
+      return (m << 16) | uint32_t(slot);
     }
-    VGM_DEV SlotOrder(R a, R b, R c, R d) : k0(a), k1(b), k2(c), k3(d), s0(0), s1(1), s2(2), s3(3)
+    VGM_DEV static void cswap(uint32_t &a, uint32_t &b)
     {
-      cswap(k0, s0, k1, s1);
-      cswap(k2, s2, k3, s3);
-      cswap(k0, s0, k2, s2);
-      cswap(k1, s1, k3, s3);
-      cswap(k1, s1, k2, s2);
+      const uint32_t lo = a < b ? a : b;
+      b = a < b ? b : a;
+      a = lo;
     }
-    VGM_DEV R key(int o) const { return o == 0 ? k0 : (o == 1 ? k1 : (o == 2 ? k2 : k3)); }
-    VGM_DEV int slot(int o) const { return o == 0 ? s0 : (o == 1 ? s1 : (o == 2 ? s2 : s3)); }
+    VGM_DEV SlotOrder(R a, R b, R c, R d) : e0(pack(a, 0)), e1(pack(b, 1)), e2(pack(c, 2)), e3(pack(d, 3))
+    {
+      cswap(e0, e1);
+      cswap(e2, e3);
+      cswap(e0, e2);
+      cswap(e1, e3);
+      cswap(e1, e2);
+    }
+    VGM_DEV uint32_t entry(int o) const { return o == 0 ? e0 : (o == 1 ? e1 : (o == 2 ? e2 : e3)); }
+    VGM_DEV static int slot(uint32_t e) { return int(e & 3u); }
+    VGM_DEV static bool miss(uint32_t e) { return (e >> 16) == 0xffffu; }
+    VGM_DEV static R key(uint32_t e)
+    {
+      const uint32_t m = e >> 16;
+      if (!(m & 0x8000u)) return -inf_length<R>();
+      return R(__builtin_bit_cast(float, (m & 0x7fffu) << 16));
+    }
   };
   static_assert(kTessWidth == 4, "SlotOrder sorts four children");
 
@@ -241,9 +262,10 @@ struct TessellatedShape {
                              box_entry<R>(nd, 3, p, inv));
 #pragma unroll 1
       for (int o = 0; o < kTessWidth; ++o) {
-        const int i = ord.slot(o);
-        const R te = ord.key(o);
-        if (nd.child[i] >= 0 || te == inf_length<R>() || te > bound()) continue;
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<R>::slot(se);
+        const R te = SlotOrder<R>::key(se);
+        if (nd.child[i] >= 0 || SlotOrder<R>::miss(se) || te > bound()) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const TessFacet &f = fa[k];
@@ -269,9 +291,10 @@ struct TessellatedShape {
       }
 #pragma unroll 1
       for (int o = kTessWidth - 1; o >= 0; --o) {
-        const int i = ord.slot(o);
-        const R te = ord.key(o);
-        if (nd.child[i] < 0 || te == inf_length<R>() || te > bound()) continue;
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<R>::slot(se);
+        const R te = SlotOrder<R>::key(se);
+        if (nd.child[i] < 0 || SlotOrder<R>::miss(se) || te > bound()) continue;
         stack.set(sp++, tess_walk_entry<R>(nd.child[i], te));
       }
     }
@@ -294,8 +317,9 @@ struct TessellatedShape {
                              box_dist2<R>(nd, 3, p));
 #pragma unroll 1
       for (int o = 0; o < kTessWidth; ++o) {
-        const int i = ord.slot(o);
-        if (nd.child[i] >= 0 || ord.key(o) > best) continue;
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<R>::slot(se);
+        if (nd.child[i] >= 0 || SlotOrder<R>::key(se) > best) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const R s = facet_safety2<R>(fa[k], p, to_in);
@@ -304,8 +328,9 @@ struct TessellatedShape {
       }
 #pragma unroll 1
       for (int o = kTessWidth - 1; o >= 0; --o) {
-        const int i = ord.slot(o);
-        const R di = ord.key(o);
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<R>::slot(se);
+        const R di = SlotOrder<R>::key(se);
         if (nd.child[i] < 0 || di > best) continue;
         stack.set(sp++, tess_walk_entry<R>(nd.child[i], di));
       }
@@ -334,8 +359,9 @@ struct TessellatedShape {
                                   box_dist2<double>(nd, 2, p), box_dist2<double>(nd, 3, p));
 #pragma unroll 1
       for (int o = 0; o < kTessWidth; ++o) {
-        const int i = ord.slot(o);
-        if (nd.child[i] >= 0 || ord.key(o) > best) continue;
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<double>::slot(se);
+        if (nd.child[i] >= 0 || SlotOrder<double>::key(se) > best) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const double s = facet_safety2<double>(fa[k], p, false);
@@ -348,8 +374,9 @@ struct TessellatedShape {
       }
 #pragma unroll 1
       for (int o = kTessWidth - 1; o >= 0; --o) {
-        const int i = ord.slot(o);
-        const double di = ord.key(o);
+        const uint32_t se = ord.entry(o);
+        const int i = SlotOrder<double>::slot(se);
+        const double di = SlotOrder<double>::key(se);
         if (nd.child[i] < 0 || di > best) continue;
         stack.set(sp++, tess_walk_entry<double>(nd.child[i], di));
       }
