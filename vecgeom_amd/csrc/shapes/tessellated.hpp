@@ -265,7 +265,8 @@ struct TessellatedShape {
         const uint32_t se = ord.entry(o);
         const int i = SlotOrder<R>::slot(se);
         const R te = SlotOrder<R>::key(se);
-        if (nd.child[i] >= 0 || SlotOrder<R>::miss(se) || te > bound()) continue;
+        if (SlotOrder<R>::miss(se) || te > bound()) break;  // so are the children after it
+        if (nd.child[i] >= 0) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const TessFacet &f = fa[k];
@@ -319,7 +320,8 @@ struct TessellatedShape {
       for (int o = 0; o < kTessWidth; ++o) {
         const uint32_t se = ord.entry(o);
         const int i = SlotOrder<R>::slot(se);
-        if (nd.child[i] >= 0 || SlotOrder<R>::key(se) > best) continue;
+        if (SlotOrder<R>::key(se) > best) break;  // so are the children after it
+        if (nd.child[i] >= 0) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const R s = facet_safety2<R>(fa[k], p, to_in);
@@ -361,7 +363,8 @@ struct TessellatedShape {
       for (int o = 0; o < kTessWidth; ++o) {
         const uint32_t se = ord.entry(o);
         const int i = SlotOrder<double>::slot(se);
-        if (nd.child[i] >= 0 || SlotOrder<double>::key(se) > best) continue;
+        if (SlotOrder<double>::key(se) > best) break;  // so are the children after it
+        if (nd.child[i] >= 0) continue;
         const int f0 = nd.first[i], f1 = f0 + nd.count[i];
         for (int k = f0; k < f1; ++k) {
           const double s = facet_safety2<double>(fa[k], p, false);
